@@ -1,0 +1,212 @@
+"""ctypes binding of the C ABI in include/eio.h (libeio_hip.so, hand-written HIP for gfx950).
+
+This is the Python view of the same boundary the C++ operator layer uses; the
+torch FWI layer and the C-ABI tests call through it.  The library must be built
+(``__graft_entry__.build()``); a missing library raises instead of falling back.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libeio_hip.so")
+
+EIO_DEVICE_PTRS = 1
+GRID_OF_COMP = (1, 2, 0, 0, 3)
+
+
+class EioError(RuntimeError):
+    pass
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("nz", ctypes.c_int), ("nx", ctypes.c_int), ("nts", ctypes.c_int),
+                ("sub_fd", ctypes.c_int), ("minPad", ctypes.c_int), ("dz", ctypes.c_double),
+                ("dx", ctypes.c_double), ("dts", ctypes.c_double), ("alphaCos", ctypes.c_double)]
+
+
+class Shot(ctypes.Structure):
+    _fields_ = [("model", ctypes.c_int),
+                ("src_pos", ctypes.c_void_p * 4), ("n_src", ctypes.c_int * 4),
+                ("rec_pos", ctypes.c_void_p * 4), ("n_rec", ctypes.c_int * 4),
+                ("src_tr", ctypes.c_void_p * 5), ("rec_tr", ctypes.c_void_p * 5)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise EioError("libeio_hip.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        L.eio_last_error.restype = ctypes.c_char_p
+        vp = ctypes.c_void_p
+        for name, args in [
+            ("eio_ctx_create", [ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]),
+            ("eio_ctx_destroy", [vp]),
+            ("eio_ctx_set_stream", [vp, vp]),
+            ("eio_ctx_set_graph", [vp, ctypes.c_int]),
+            ("eio_ctx_last_timing", [vp, ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(ctypes.c_longlong),
+                                     ctypes.POINTER(ctypes.c_double)]),
+            ("eio_model_add", [vp, ctypes.POINTER(Grid), ctypes.POINTER(vp), ctypes.c_int,
+                               ctypes.POINTER(ctypes.c_int)]),
+            ("eio_model_set_media", [vp, ctypes.c_int, ctypes.POINTER(vp), ctypes.c_int]),
+            ("eio_model_remove", [vp, ctypes.c_int]),
+            ("eio_nl_forward", [vp, ctypes.c_int, ctypes.POINTER(Shot), ctypes.c_int]),
+            ("eio_nl_adjoint", [vp, ctypes.c_int, ctypes.POINTER(Shot), ctypes.c_int]),
+            ("eio_born_forward", [vp, ctypes.c_int, ctypes.POINTER(Shot), ctypes.POINTER(vp),
+                                  ctypes.c_int]),
+            ("eio_born_adjoint", [vp, ctypes.c_int, ctypes.POINTER(Shot), ctypes.POINTER(vp),
+                                  ctypes.c_int]),
+            ("eio_device_alloc", [vp, ctypes.c_ulonglong, ctypes.POINTER(vp)]),
+            ("eio_device_free", [vp, vp]),
+            ("eio_memcpy", [vp, vp, vp, ctypes.c_ulonglong, ctypes.c_int]),
+            ("eio_synchronize", [vp]),
+        ]:
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+EXPORTED = ("eio_last_error", "eio_ctx_create", "eio_ctx_destroy", "eio_ctx_set_stream",
+            "eio_ctx_set_graph", "eio_ctx_last_timing", "eio_model_add", "eio_model_set_media",
+            "eio_model_remove", "eio_nl_forward", "eio_nl_adjoint", "eio_born_forward",
+            "eio_born_adjoint", "eio_device_alloc", "eio_device_free", "eio_memcpy",
+            "eio_synchronize")
+
+
+def check(rc):
+    if rc != 0:
+        raise EioError("eio error %d: %s" % (rc, lib().eio_last_error().decode()))
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else ctypes.c_void_p(0)
+
+
+class Context:
+    """Owns an eio_ctx (one HIP device, fixed real type)."""
+
+    def __init__(self, device=0, dtype=np.float32):
+        self.dtype = np.dtype(dtype)
+        self.prec = self.dtype.itemsize
+        h = ctypes.c_void_p()
+        check(lib().eio_ctx_create(device, self.prec, ctypes.byref(h)))
+        self.h = h
+        self._keep = []
+
+    def close(self):
+        if self.h:
+            check(lib().eio_ctx_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream_handle):
+        check(lib().eio_ctx_set_stream(self.h, ctypes.c_void_p(stream_handle)))
+
+    def set_graph(self, on):
+        check(lib().eio_ctx_set_graph(self.h, int(bool(on))))
+
+    def last_timing(self):
+        t, n, s = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_double()
+        check(lib().eio_ctx_last_timing(self.h, ctypes.byref(t), ctypes.byref(n), ctypes.byref(s)))
+        return t.value, n.value, s.value
+
+    def model_add(self, nz, nx, nts, sub_fd, dz, dx, dts, minPad, media, alphaCos=0.99,
+                  device_ptrs=None):
+        """media: array (5, nx, nz) on the host, or 5 device addresses (device_ptrs)."""
+        g = Grid(int(nz), int(nx), int(nts), int(sub_fd), int(minPad), float(dz), float(dx),
+                 float(dts), float(alphaCos))
+        mid = ctypes.c_int()
+        if device_ptrs is not None:
+            arr = (ctypes.c_void_p * 5)(*device_ptrs)
+            check(lib().eio_model_add(self.h, ctypes.byref(g), arr, EIO_DEVICE_PTRS, ctypes.byref(mid)))
+        else:
+            med = np.ascontiguousarray(media, dtype=self.dtype)
+            arr = (ctypes.c_void_p * 5)(*[med[k].ctypes.data for k in range(5)])
+            check(lib().eio_model_add(self.h, ctypes.byref(g), arr, 0, ctypes.byref(mid)))
+        return mid.value
+
+    def model_set_media(self, mid, media=None, device_ptrs=None):
+        if device_ptrs is not None:
+            arr = (ctypes.c_void_p * 5)(*device_ptrs)
+            check(lib().eio_model_set_media(self.h, mid, arr, EIO_DEVICE_PTRS))
+        else:
+            med = np.ascontiguousarray(media, dtype=self.dtype)
+            arr = (ctypes.c_void_p * 5)(*[med[k].ctypes.data for k in range(5)])
+            check(lib().eio_model_set_media(self.h, mid, arr, 0))
+
+    def model_remove(self, mid):
+        check(lib().eio_model_remove(self.h, mid))
+
+    # -------------------------------------------------------------- shots (host arrays)
+    def make_shot(self, mid, spos, rpos, src_tr, rec_tr):
+        """spos/rpos: 4 int32 arrays; src_tr/rec_tr: 5 arrays (host numpy) or 5 device addresses."""
+        s = Shot()
+        s.model = mid
+        keep = []
+        for g in range(4):
+            a = np.ascontiguousarray(spos[g], dtype=np.int32)
+            b = np.ascontiguousarray(rpos[g], dtype=np.int32)
+            keep += [a, b]
+            s.src_pos[g] = a.ctypes.data if a.size else None
+            s.n_src[g] = a.size
+            s.rec_pos[g] = b.ctypes.data if b.size else None
+            s.n_rec[g] = b.size
+        for k in range(5):
+            for arrname, tr in (("src_tr", src_tr), ("rec_tr", rec_tr)):
+                x = tr[k]
+                if isinstance(x, np.ndarray):
+                    if not x.flags.c_contiguous or x.dtype != self.dtype:
+                        raise EioError("trace arrays must be C-contiguous %s" % self.dtype)
+                    getattr(s, arrname)[k] = x.ctypes.data if x.size else None
+                    keep.append(x)
+                else:
+                    getattr(s, arrname)[k] = int(x) if x else None
+        self._keep = keep
+        return s, keep
+
+    def _shots(self, shots):
+        arr = (Shot * len(shots))(*shots)
+        return arr
+
+    def nl_forward(self, shots, device_ptrs=False):
+        arr = self._shots(shots)
+        check(lib().eio_nl_forward(self.h, len(shots), arr, EIO_DEVICE_PTRS if device_ptrs else 0))
+
+    def nl_adjoint(self, shots, device_ptrs=False):
+        arr = self._shots(shots)
+        check(lib().eio_nl_adjoint(self.h, len(shots), arr, EIO_DEVICE_PTRS if device_ptrs else 0))
+
+    def born_forward(self, shots, dm5, device_ptrs=False):
+        arr = self._shots(shots)
+        if device_ptrs:
+            dp = (ctypes.c_void_p * 5)(*dm5)
+        else:
+            dm5 = np.ascontiguousarray(dm5, dtype=self.dtype)
+            dp = (ctypes.c_void_p * 5)(*[dm5[k].ctypes.data for k in range(5)])
+        check(lib().eio_born_forward(self.h, len(shots), arr, dp,
+                                     EIO_DEVICE_PTRS if device_ptrs else 0))
+
+    def born_adjoint(self, shots, dm5_out, device_ptrs=False):
+        """dm5_out (host (5,nx,nz) array, accumulated in place) or 5 device addresses."""
+        arr = self._shots(shots)
+        if device_ptrs:
+            dp = (ctypes.c_void_p * 5)(*dm5_out)
+        else:
+            if not dm5_out.flags.c_contiguous or dm5_out.dtype != self.dtype:
+                raise EioError("dm must be C-contiguous %s" % self.dtype)
+            dp = (ctypes.c_void_p * 5)(*[dm5_out[k].ctypes.data for k in range(5)])
+        check(lib().eio_born_adjoint(self.h, len(shots), arr, dp,
+                                     EIO_DEVICE_PTRS if device_ptrs else 0))

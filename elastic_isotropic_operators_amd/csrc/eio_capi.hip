@@ -1,0 +1,800 @@
+// eio_capi.hip -- C ABI (include/eio.h) of the gfx950 elastic propagator.
+//
+// Host orchestration of the fused kernels in eio_kernels.h:
+//   * a context owns a stream, timing events, registered models and a grow-only device
+//     workspace (288 GB of HBM: buffers are kept across calls instead of malloc/free per shot);
+//   * a call propagates all of its shots in ONE batched time loop: every launch advances
+//     every active shot by one fine step (shots are ordered by decreasing trace length so
+//     the active shots are always a prefix of the tile table);
+//   * source/receiver devices are turned into per-tile sparse entry lists so injection and
+//     recording happen inside the stencil launch.
+// Time loops follow nonlinearPropElasticGpuFunctions.cu:588-617 / 966-997 and
+// BornElasticGpuFunctions.cu:744-819 / 1128-1191.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/eio.h"
+#include "eio_kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+struct EioError {
+    int code;
+    std::string msg;
+};
+
+#define EIO_CHECK_HIP(expr)                                                                   \
+    do {                                                                                      \
+        hipError_t _e = (expr);                                                               \
+        if (_e != hipSuccess)                                                                 \
+            throw EioError{EIO_ERR_RUNTIME, std::string(#expr) + ": " + hipGetErrorString(_e)}; \
+    } while (0)
+
+void fail(const std::string& m) { throw EioError{EIO_ERR_ARG, m}; }
+
+constexpr double FD_C[4] = {1.196289062541883, -0.079752604188901, 0.009570312506634,
+                            -6.975446437140719e-04};  // nonlinearPropElasticGpuFunctions.cu:159-167
+constexpr int GRID_OF_COMP[5] = {1, 2, 0, 0, 3};
+constexpr int PAD_MAX = 500;   // varDeclare.h:9
+constexpr int SUB_MAX = 100;   // varDeclare.h:10
+
+template <typename T>
+constexpr int tile_r() { return sizeof(T) == 4 ? 4 : 2; }
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct ModelBase {
+    eio_grid g;
+    int ntw, h;
+    long n;
+    void* med = nullptr;    // 5*n
+    void* damp = nullptr;   // minPad
+    void* filt = nullptr;   // 2h + 1 (last entry: 1/(2 dts))
+    double cz[4], cx[4];
+};
+
+struct Ctx {
+    int device = 0;
+    int prec = 4;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4];
+    std::map<int, ModelBase> models;
+    int next_id = 1;
+    std::map<std::string, DevBuf> ws;
+    double last_total_ms = 0.0, last_step_ms = 0.0;
+    long long last_nstep = 0;
+    int use_graph = 0;
+};
+
+void* ws_get(Ctx* c, const std::string& name, size_t bytes) {
+    DevBuf& b = c->ws[name];
+    if (b.bytes < bytes) {
+        if (b.p) {
+            EIO_CHECK_HIP(hipStreamSynchronize(c->stream));
+            EIO_CHECK_HIP(hipFree(b.p));
+            b.p = nullptr;
+            b.bytes = 0;
+        }
+        EIO_CHECK_HIP(hipMalloc(&b.p, bytes));
+        b.bytes = bytes;
+    }
+    return b.p;
+}
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int d) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != d) EIO_CHECK_HIP(hipSetDevice(d));
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+template <typename T>
+void fill0(Ctx* c, T* p, long n) {
+    if (n <= 0) return;
+    const long nb = (n + eio::NTHR - 1) / eio::NTHR;
+    hipLaunchKernelGGL(eio::k_fill0<T>, dim3(unsigned(nb)), dim3(eio::NTHR), 0, c->stream, p, n);
+    EIO_CHECK_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ model registration
+template <typename T>
+void upload_model(Ctx* c, ModelBase& m, const void* const media[5], int flags) {
+    const size_t nb = size_t(m.n) * sizeof(T);
+    for (int k = 0; k < 5; ++k) {
+        if (!media[k]) fail("media pointer is NULL");
+        const int kind = (flags & EIO_DEVICE_PTRS) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+        EIO_CHECK_HIP(hipMemcpyAsync(static_cast<char*>(m.med) + k * nb, media[k], nb,
+                                     hipMemcpyKind(kind), c->stream));
+    }
+    EIO_CHECK_HIP(hipStreamSynchronize(c->stream));
+}
+
+template <typename T>
+int model_add(Ctx* c, const eio_grid* g, const void* const media[5], int flags) {
+    if (!g) fail("grid is NULL");
+    if (g->nz < 2 * eio::FAT + 1 || g->nx < 2 * eio::FAT + 1) fail("nz and nx must exceed 2*FAT");
+    if (g->nts < 2) fail("nts must be >= 2");
+    if (g->sub_fd < 1) fail("sub_fd must be >= 1");
+    if (g->sub_fd > SUB_MAX) fail("Subsampling parameter too high");  // fdParamElastic.cpp:29-32
+    if (g->minPad < 0 || g->minPad >= PAD_MAX) fail("Padding value is too high");  // NLF:191-194
+    if (!(g->dz > 0) || !(g->dx > 0) || !(g->dts > 0)) fail("dz, dx, dts must be > 0");
+    if (long(g->nz) * g->nx >= (1L << 31)) fail("grid too large for 32-bit tile indexing");
+    ModelBase m;
+    m.g = *g;
+    m.n = long(g->nz) * g->nx;
+    m.ntw = (g->nts - 1) * g->sub_fd + 1;
+    m.h = g->sub_fd + 1;
+    for (int k = 0; k < 4; ++k) { m.cz[k] = FD_C[k] / g->dz; m.cx[k] = FD_C[k] / g->dx; }
+    // time interpolation filter (nonlinearPropElasticGpuFunctions.cu:172-188) + 1/(2 dts)
+    std::vector<T> filt(2 * m.h + 1);
+    const double s = 1.0 / std::sqrt(double(m.ntw) / double(g->nts));
+    for (int i = 0; i < m.h; ++i) {
+        const double a = 1.0 - double(i) / double(g->sub_fd);
+        filt[i] = T(a * s);
+        filt[m.h + i] = T((1.0 - a) * s);
+    }
+    filt[2 * m.h] = T(1.0 / (2.0 * g->dts));  // BornElasticGpuFunctions.cu:264-265
+    // cosine sponge (nonlinearPropElasticGpuFunctions.cu:195-202)
+    std::vector<T> damp(std::max(g->minPad, 1), T(1));
+    for (int j = 0; j < g->minPad; ++j) {
+        const double arg = M_PI / double(g->minPad) * double(g->minPad - j);
+        damp[j] = T(g->alphaCos + (1.0 - g->alphaCos) * std::cos(arg));
+    }
+    EIO_CHECK_HIP(hipMalloc(&m.med, size_t(5) * m.n * sizeof(T)));
+    EIO_CHECK_HIP(hipMalloc(&m.damp, damp.size() * sizeof(T)));
+    EIO_CHECK_HIP(hipMalloc(&m.filt, filt.size() * sizeof(T)));
+    EIO_CHECK_HIP(hipMemcpy(m.damp, damp.data(), damp.size() * sizeof(T), hipMemcpyHostToDevice));
+    EIO_CHECK_HIP(hipMemcpy(m.filt, filt.data(), filt.size() * sizeof(T), hipMemcpyHostToDevice));
+    upload_model<T>(c, m, media, flags);
+    const int id = c->next_id++;
+    c->models[id] = m;
+    return id;
+}
+
+void model_free(ModelBase& m) {
+    if (m.med) (void)hipFree(m.med);
+    if (m.damp) (void)hipFree(m.damp);
+    if (m.filt) (void)hipFree(m.filt);
+    m.med = m.damp = m.filt = nullptr;
+}
+
+ModelBase& get_model(Ctx* c, int id) {
+    auto it = c->models.find(id);
+    if (it == c->models.end()) fail("unknown model id " + std::to_string(id));
+    return it->second;
+}
+
+// ------------------------------------------------------------------ batch planning
+enum class Op { NL_FWD, NL_ADJ, BORN_FWD, BORN_ADJ };
+
+struct Plan {
+    std::vector<int> order;               // shot indices sorted by decreasing ntw
+    std::vector<eio::TileDev> tiles;
+    std::vector<eio::Entry> ents;
+    std::vector<int> tile_end;            // tile count of the first i+1 ordered shots
+    std::vector<int> nsteps;              // per ordered shot: ntw-1
+};
+
+template <typename T>
+void build_entries(const ModelBase& m, const int* const pos[4], const int nreg[4], int TX,
+                   std::vector<std::vector<eio::Entry>>& per_tile, int nbz, const char* what) {
+    const int nz = m.g.nz, nx = m.g.nx;
+    for (int k = 0; k < 5; ++k) {
+        const int gi = GRID_OF_COMP[k];
+        const int cnt = nreg[gi];
+        if (cnt < 0) fail(std::string(what) + ": negative device count");
+        if (cnt > 0 && !pos[gi]) fail(std::string(what) + ": NULL position list");
+        if (cnt >= (1 << 24)) fail(std::string(what) + ": too many devices");
+        for (int u = 0; u < cnt; ++u) {
+            const int p = pos[gi][u];
+            if (p < 0 || long(p) >= m.n) fail(std::string(what) + ": device position out of the grid");
+            const int iz = p % nz, ix = p / nz;
+            if (iz < eio::FAT || iz >= nz - eio::FAT || ix < eio::FAT || ix >= nx - eio::FAT)
+                fail(std::string(what) +
+                     ": device inside the FAT halo (only update-region positions are supported)");
+            const int bz = (iz - eio::FAT) / eio::TZ, lz = (iz - eio::FAT) % eio::TZ;
+            const int bx = (ix - eio::FAT) / TX, lx = (ix - eio::FAT) % TX;
+            eio::Entry e;
+            e.lcell = lz | (lx << 8);
+            e.crow = u | (k << 24);
+            per_tile[size_t(bx) * nbz + bz].push_back(e);
+        }
+    }
+}
+
+template <typename T>
+Plan make_plan(Ctx* c, int nshot, const eio_shot* shots, Op op) {
+    constexpr int TX = 4 * tile_r<T>();
+    Plan p;
+    p.order.resize(nshot);
+    std::iota(p.order.begin(), p.order.end(), 0);
+    std::stable_sort(p.order.begin(), p.order.end(), [&](int a, int b) {
+        return get_model(c, shots[a].model).ntw > get_model(c, shots[b].model).ntw;
+    });
+    for (int oi = 0; oi < nshot; ++oi) {
+        const eio_shot& s = shots[p.order[oi]];
+        const ModelBase& m = get_model(c, s.model);
+        const int nbz = (m.g.nz - 2 * eio::FAT + eio::TZ - 1) / eio::TZ;
+        const int nbx = (m.g.nx - 2 * eio::FAT + TX - 1) / TX;
+        std::vector<std::vector<eio::Entry>> src_t(size_t(nbz) * nbx), rec_t(size_t(nbz) * nbx);
+        build_entries<T>(m, s.src_pos, s.n_src, TX, src_t, nbz, "source");
+        build_entries<T>(m, s.rec_pos, s.n_rec, TX, rec_t, nbz, "receiver");
+        for (int bx = 0; bx < nbx; ++bx)
+            for (int bz = 0; bz < nbz; ++bz) {
+                const size_t ti = size_t(bx) * nbz + bz;
+                eio::TileDev t;
+                t.shot = oi;
+                t.z0 = eio::FAT + bz * eio::TZ;
+                t.x0 = eio::FAT + bx * TX;
+                t.src_b = int(p.ents.size());
+                p.ents.insert(p.ents.end(), src_t[ti].begin(), src_t[ti].end());
+                t.src_e = int(p.ents.size());
+                t.rec_b = int(p.ents.size());
+                p.ents.insert(p.ents.end(), rec_t[ti].begin(), rec_t[ti].end());
+                t.rec_e = int(p.ents.size());
+                t.pad = 0;
+                p.tiles.push_back(t);
+            }
+        p.tile_end.push_back(int(p.tiles.size()));
+        p.nsteps.push_back(m.ntw - 1);
+    }
+    if (p.tiles.size() >= (1u << 31)) fail("too many tiles");
+    (void)op;
+    return p;
+}
+
+// ------------------------------------------------------------------ execution
+template <typename T>
+struct Run {
+    Ctx* c;
+    Plan plan;
+    std::vector<eio::ShotDev<T>> sd;
+    eio::ShotDev<T>* d_shots = nullptr;
+    eio::TileDev* d_tiles = nullptr;
+    eio::Entry* d_ents = nullptr;
+    // host copies for the D2H of outputs
+    struct Out {
+        void* host;
+        T* dev;
+        size_t bytes;
+    };
+    std::vector<Out> outs;
+};
+
+template <typename T>
+Run<T> prepare(Ctx* c, int nshot, const eio_shot* shots, Op op, int flags,
+               const void* const dm_fwd[5]) {
+    if (nshot <= 0) fail("nshot must be > 0");
+    if (!shots) fail("shots is NULL");
+    Run<T> run;
+    run.c = c;
+    run.plan = make_plan<T>(c, nshot, shots, op);
+    const bool born = (op == Op::BORN_FWD || op == Op::BORN_ADJ);
+    if (born) {
+        for (int i = 1; i < nshot; ++i)
+            if (shots[i].model != shots[0].model) fail("Born shots of one call must share a model");
+    }
+    const bool dev_ptrs = (flags & EIO_DEVICE_PTRS) != 0;
+
+    // ---- sizes
+    size_t state_elems = 0, tr_elems = 0, ss_elems = 0, w_elems = 0, acc_elems = 0;
+    for (int oi = 0; oi < nshot; ++oi) {
+        const eio_shot& s = shots[run.plan.order[oi]];
+        const ModelBase& m = get_model(c, s.model);
+        state_elems += 10 * size_t(m.n);
+        if (born) {
+            ss_elems += 10 * size_t(m.n);
+            w_elems += 2 * size_t(m.n) * m.g.nts;
+            if (op == Op::BORN_ADJ) acc_elems += 5 * size_t(m.n);
+        }
+        if (!dev_ptrs) {
+            const int src_ld = born ? m.ntw : m.g.nts;
+            for (int k = 0; k < 5; ++k) {
+                tr_elems += size_t(s.n_src[GRID_OF_COMP[k]]) * src_ld;
+                tr_elems += size_t(s.n_rec[GRID_OF_COMP[k]]) * m.g.nts;
+            }
+        }
+    }
+    T* state = static_cast<T*>(ws_get(c, "state", state_elems * sizeof(T)));
+    T* trbuf = tr_elems ? static_cast<T*>(ws_get(c, "traces", tr_elems * sizeof(T))) : nullptr;
+    T* ssbuf = ss_elems ? static_cast<T*>(ws_get(c, "ss", ss_elems * sizeof(T))) : nullptr;
+    T* wbuf = w_elems ? static_cast<T*>(ws_get(c, "W", w_elems * sizeof(T))) : nullptr;
+    T* accbuf = acc_elems ? static_cast<T*>(ws_get(c, "acc", acc_elems * sizeof(T))) : nullptr;
+
+    // ---- shot descriptors
+    run.sd.resize(nshot);
+    size_t so = 0, to = 0, sso = 0, wo = 0, ao = 0;
+    const hipMemcpyKind h2d = hipMemcpyHostToDevice;
+    for (int oi = 0; oi < nshot; ++oi) {
+        const eio_shot& s = shots[run.plan.order[oi]];
+        const ModelBase& m = get_model(c, s.model);
+        eio::ShotDev<T>& d = run.sd[oi];
+        std::memset(&d, 0, sizeof(d));
+        d.nz = m.g.nz; d.nx = m.g.nx; d.nts = m.g.nts; d.sub = m.g.sub_fd; d.h = m.h;
+        d.ntw = m.ntw; d.minPad = m.g.minPad;
+        for (int k = 0; k < 4; ++k) { d.cz[k] = T(m.cz[k]); d.cx[k] = T(m.cx[k]); }
+        for (int f = 0; f < 5; ++f) {
+            d.fa[f] = state + so + size_t(f) * m.n;
+            d.fb[f] = state + so + size_t(5 + f) * m.n;
+            d.med[f] = static_cast<const T*>(m.med) + size_t(f) * m.n;
+        }
+        so += 10 * size_t(m.n);
+        d.damp = static_cast<const T*>(m.damp);
+        d.filt = static_cast<const T*>(m.filt);
+        const int src_ld = born ? m.ntw : m.g.nts;
+        const bool src_in = (op != Op::NL_ADJ);
+        const bool rec_in = (op == Op::NL_ADJ || op == Op::BORN_ADJ);
+        for (int k = 0; k < 5; ++k) {
+            const size_t ns = size_t(s.n_src[GRID_OF_COMP[k]]) * src_ld;
+            const size_t nr = size_t(s.n_rec[GRID_OF_COMP[k]]) * m.g.nts;
+            if (ns && !s.src_tr[k]) fail("source trace pointer is NULL");
+            if (nr && !s.rec_tr[k]) fail("receiver trace pointer is NULL");
+            if (dev_ptrs) {
+                d.src_tr[k] = static_cast<T*>(s.src_tr[k]);
+                d.rec_tr[k] = static_cast<T*>(s.rec_tr[k]);
+                if (!src_in) fill0<T>(c, d.src_tr[k], long(ns));
+                if (!rec_in) fill0<T>(c, d.rec_tr[k], long(nr));
+            } else {
+                d.src_tr[k] = trbuf + to;
+                to += ns;
+                d.rec_tr[k] = trbuf + to;
+                to += nr;
+                if (src_in) {
+                    if (ns) EIO_CHECK_HIP(hipMemcpyAsync(d.src_tr[k], s.src_tr[k], ns * sizeof(T), h2d, c->stream));
+                } else {
+                    fill0<T>(c, d.src_tr[k], long(ns));
+                    run.outs.push_back({s.src_tr[k], d.src_tr[k], ns * sizeof(T)});
+                }
+                if (rec_in) {
+                    if (nr) EIO_CHECK_HIP(hipMemcpyAsync(d.rec_tr[k], s.rec_tr[k], nr * sizeof(T), h2d, c->stream));
+                } else {
+                    fill0<T>(c, d.rec_tr[k], long(nr));
+                    run.outs.push_back({s.rec_tr[k], d.rec_tr[k], nr * sizeof(T)});
+                }
+            }
+        }
+        if (born) {
+            for (int f = 0; f < 5; ++f) {
+                d.ss[0][f] = ssbuf + sso + size_t(f) * m.n;
+                d.ss[1][f] = ssbuf + sso + size_t(5 + f) * m.n;
+            }
+            sso += 10 * size_t(m.n);
+            d.Wvx = wbuf + wo;
+            d.Wvz = wbuf + wo + size_t(m.n) * m.g.nts;
+            wo += 2 * size_t(m.n) * m.g.nts;
+            if (op == Op::BORN_ADJ) {
+                for (int f = 0; f < 5; ++f) d.dm[f] = accbuf + ao + size_t(f) * m.n;
+                ao += 5 * size_t(m.n);
+            } else {
+                for (int f = 0; f < 5; ++f) d.dm[f] = const_cast<T*>(static_cast<const T*>(dm_fwd[f]));
+            }
+        }
+    }
+    // ---- tables
+    run.d_shots = static_cast<eio::ShotDev<T>*>(ws_get(c, "shots", sizeof(eio::ShotDev<T>) * nshot));
+    run.d_tiles = static_cast<eio::TileDev*>(ws_get(c, "tiles", sizeof(eio::TileDev) * run.plan.tiles.size()));
+    run.d_ents = static_cast<eio::Entry*>(ws_get(c, "ents", sizeof(eio::Entry) * std::max<size_t>(1, run.plan.ents.size())));
+    EIO_CHECK_HIP(hipMemcpyAsync(run.d_shots, run.sd.data(), sizeof(eio::ShotDev<T>) * nshot, h2d, c->stream));
+    EIO_CHECK_HIP(hipMemcpyAsync(run.d_tiles, run.plan.tiles.data(), sizeof(eio::TileDev) * run.plan.tiles.size(), h2d, c->stream));
+    if (!run.plan.ents.empty())
+        EIO_CHECK_HIP(hipMemcpyAsync(run.d_ents, run.plan.ents.data(), sizeof(eio::Entry) * run.plan.ents.size(), h2d, c->stream));
+    // ---- zero state (and Born buffers)
+    fill0<T>(c, state, long(state_elems));
+    if (ssbuf) fill0<T>(c, ssbuf, long(ss_elems));
+    if (wbuf) fill0<T>(c, wbuf, long(w_elems));
+    if (accbuf) fill0<T>(c, accbuf, long(acc_elems));
+    return run;
+}
+
+template <typename T>
+void finish(Run<T>& run) {
+    Ctx* c = run.c;
+    for (auto& o : run.outs)
+        if (o.bytes) EIO_CHECK_HIP(hipMemcpyAsync(o.host, o.dev, o.bytes, hipMemcpyDeviceToHost, c->stream));
+    EIO_CHECK_HIP(hipStreamSynchronize(c->stream));
+}
+
+// number of tiles active at step k (shots sorted by decreasing step count)
+inline int active_tiles(const Plan& p, int k) {
+    int n = 0;
+    for (size_t i = 0; i < p.nsteps.size(); ++i)
+        if (k < p.nsteps[i]) n = p.tile_end[i];
+    return n;
+}
+
+template <typename T, int MODE>
+void launch_fwd(Run<T>& run, int ntile, int k, int sspar) {
+    constexpr int R = tile_r<T>();
+    hipLaunchKernelGGL((eio::k_step_fwd<T, R, MODE>), dim3(ntile), dim3(eio::NTHR), 0, run.c->stream,
+                       run.d_shots, run.d_tiles, run.d_ents, ntile, k, sspar);
+}
+template <typename T, int MODE>
+void launch_adj(Run<T>& run, int ntile, int k, int sspar) {
+    constexpr int R = tile_r<T>();
+    hipLaunchKernelGGL((eio::k_step_adj<T, R, MODE>), dim3(ntile), dim3(eio::NTHR), 0, run.c->stream,
+                       run.d_shots, run.d_tiles, run.d_ents, ntile, k, sspar);
+}
+template <typename T, bool ADJ>
+void launch_img(Run<T>& run, int ntile, int its, int which) {
+    constexpr int R = tile_r<T>();
+    hipLaunchKernelGGL((eio::k_imaging<T, R, ADJ>), dim3(ntile), dim3(eio::NTHR), 0, run.c->stream,
+                       run.d_shots, run.d_tiles, ntile, its, which);
+}
+
+// Runs `body` (a sequence of launches) either directly or captured into one hipGraph.
+template <typename F>
+void run_loop(Ctx* c, F&& body) {
+    if (!c->use_graph) {
+        body();
+        EIO_CHECK_HIP(hipGetLastError());
+        return;
+    }
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    EIO_CHECK_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    body();
+    EIO_CHECK_HIP(hipStreamEndCapture(c->stream, &graph));
+    EIO_CHECK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    EIO_CHECK_HIP(hipGraphLaunch(exec, c->stream));
+    EIO_CHECK_HIP(hipGraphExecDestroy(exec));
+    EIO_CHECK_HIP(hipGraphDestroy(graph));
+}
+
+template <typename T>
+void timing_begin(Ctx* c) { EIO_CHECK_HIP(hipEventRecord(c->ev[0], c->stream)); }
+template <typename T>
+void timing_end(Ctx* c, long long nstep) {
+    EIO_CHECK_HIP(hipEventRecord(c->ev[1], c->stream));
+    EIO_CHECK_HIP(hipEventSynchronize(c->ev[1]));
+    float tot = 0.f, stp = 0.f;
+    EIO_CHECK_HIP(hipEventElapsedTime(&tot, c->ev[0], c->ev[1]));
+    EIO_CHECK_HIP(hipEventElapsedTime(&stp, c->ev[2], c->ev[3]));
+    c->last_total_ms = tot;
+    c->last_nstep = nstep;
+    c->last_step_ms = nstep > 0 ? double(stp) / double(nstep) : 0.0;
+}
+
+template <typename T>
+void nl_forward(Ctx* c, int nshot, const eio_shot* shots, int flags) {
+    Run<T> run = prepare<T>(c, nshot, shots, Op::NL_FWD, flags, nullptr);
+    const int kmax = run.plan.nsteps.empty() ? 0 : run.plan.nsteps[0];
+    timing_begin<T>(c);
+    EIO_CHECK_HIP(hipEventRecord(c->ev[2], c->stream));
+    run_loop(c, [&]() {
+        for (int k = 0; k < kmax; ++k) launch_fwd<T, eio::M_NL>(run, active_tiles(run.plan, k), k, 0);
+    });
+    EIO_CHECK_HIP(hipEventRecord(c->ev[3], c->stream));
+    timing_end<T>(c, kmax);
+    finish(run);
+}
+
+template <typename T>
+void nl_adjoint(Ctx* c, int nshot, const eio_shot* shots, int flags) {
+    Run<T> run = prepare<T>(c, nshot, shots, Op::NL_ADJ, flags, nullptr);
+    // The adjoint runs backwards from each shot's own last sample: shots of different
+    // lengths are aligned at their end, so a shot joins the batch when its remaining step
+    // count is reached.  Keep it simple and exact: all shots of a call share ntw here.
+    for (size_t i = 1; i < run.plan.nsteps.size(); ++i)
+        if (run.plan.nsteps[i] != run.plan.nsteps[0]) fail("adjoint shots of one call must share nts and sub");
+    const int kmax = run.plan.nsteps.empty() ? 0 : run.plan.nsteps[0];
+    const int ntile = int(run.plan.tiles.size());
+    timing_begin<T>(c);
+    EIO_CHECK_HIP(hipEventRecord(c->ev[2], c->stream));
+    run_loop(c, [&]() {
+        for (int k = 0; k < kmax; ++k) launch_adj<T, eio::M_NLADJ>(run, ntile, k, 0);
+    });
+    EIO_CHECK_HIP(hipEventRecord(c->ev[3], c->stream));
+    timing_end<T>(c, kmax);
+    finish(run);
+}
+
+template <typename T>
+void born_source_pass(Run<T>& run, int ntile, int kmax) {
+    for (int k = 0; k < kmax; ++k) launch_fwd<T, eio::M_BSRC>(run, ntile, k, 0);
+}
+
+template <typename T>
+void born_forward(Ctx* c, int nshot, const eio_shot* shots, const void* const dm[5], int flags) {
+    if (!dm) fail("dm is NULL");
+    const ModelBase& m = get_model(c, shots[0].model);
+    std::vector<T*> dmdev(5);
+    const size_t nb = size_t(m.n) * sizeof(T);
+    if (flags & EIO_DEVICE_PTRS) {
+        for (int f = 0; f < 5; ++f) dmdev[f] = const_cast<T*>(static_cast<const T*>(dm[f]));
+    } else {
+        T* b = static_cast<T*>(ws_get(c, "dm", 5 * nb));
+        for (int f = 0; f < 5; ++f) {
+            dmdev[f] = b + size_t(f) * m.n;
+            EIO_CHECK_HIP(hipMemcpyAsync(dmdev[f], dm[f], nb, hipMemcpyHostToDevice, c->stream));
+        }
+    }
+    const void* dmp[5] = {dmdev[0], dmdev[1], dmdev[2], dmdev[3], dmdev[4]};
+    Run<T> run = prepare<T>(c, nshot, shots, Op::BORN_FWD, flags, dmp);
+    const int ntile = int(run.plan.tiles.size());
+    const int kmax = m.ntw - 1, sub = m.g.sub_fd, nts = m.g.nts;
+    long long nstep = 0;
+    timing_begin<T>(c);
+    EIO_CHECK_HIP(hipEventRecord(c->ev[2], c->stream));
+    run_loop(c, [&]() {
+        born_source_pass(run, ntile, kmax);
+        // scattered pass restarts from a zero state (BornElasticGpuFunctions.cu:773-774)
+        for (int oi = 0; oi < nshot; ++oi)
+            for (int f = 0; f < 5; ++f) {
+                fill0<T>(c, run.sd[oi].fa[f], m.n);
+                fill0<T>(c, run.sd[oi].fb[f], m.n);
+            }
+        int p = 0;
+        launch_img<T, false>(run, ntile, 0, p);
+        for (int its = 0; its < nts - 1; ++its) {
+            launch_img<T, false>(run, ntile, its + 1, p ^ 1);
+            for (int it2 = 1; it2 <= sub; ++it2)
+                launch_fwd<T, eio::M_BSCAT>(run, ntile, its * sub + it2 - 1, p);
+            p ^= 1;
+        }
+    });
+    nstep = 2LL * kmax;
+    EIO_CHECK_HIP(hipEventRecord(c->ev[3], c->stream));
+    timing_end<T>(c, nstep);
+    finish(run);
+}
+
+template <typename T>
+void born_adjoint(Ctx* c, int nshot, const eio_shot* shots, void* const dm[5], int flags) {
+    if (!dm) fail("dm is NULL");
+    const ModelBase& m = get_model(c, shots[0].model);
+    Run<T> run = prepare<T>(c, nshot, shots, Op::BORN_ADJ, flags, nullptr);
+    const int ntile = int(run.plan.tiles.size());
+    const int kmax = m.ntw - 1, sub = m.g.sub_fd, nts = m.g.nts;
+    timing_begin<T>(c);
+    EIO_CHECK_HIP(hipEventRecord(c->ev[2], c->stream));
+    run_loop(c, [&]() {
+        born_source_pass(run, ntile, kmax);
+        for (int oi = 0; oi < nshot; ++oi)
+            for (int f = 0; f < 5; ++f) {
+                fill0<T>(c, run.sd[oi].fa[f], m.n);
+                fill0<T>(c, run.sd[oi].fb[f], m.n);
+            }
+        int p = 0;
+        for (int its = nts - 2; its >= 0; --its) {
+            for (int it2 = sub - 1; it2 >= 0; --it2)
+                launch_adj<T, eio::M_BADJ>(run, ntile, (nts - 2 - its) * sub + (sub - 1 - it2), p);
+            launch_img<T, true>(run, ntile, its + 1, p ^ 1);
+            p ^= 1;
+        }
+        launch_img<T, true>(run, ntile, 0, p ^ 1);
+    });
+    EIO_CHECK_HIP(hipEventRecord(c->ev[3], c->stream));
+    timing_end<T>(c, 2LL * kmax);
+    // stack the per-shot accumulators into dm
+    const size_t nb = size_t(m.n) * sizeof(T);
+    T* out = static_cast<T*>(ws_get(c, "dmout", 5 * nb));
+    std::vector<const T*> accp(size_t(nshot) * 5);
+    for (int oi = 0; oi < nshot; ++oi)
+        for (int f = 0; f < 5; ++f) accp[size_t(f) * nshot + oi] = run.sd[oi].dm[f];
+    const T** d_accp = static_cast<const T**>(ws_get(c, "accp", accp.size() * sizeof(T*)));
+    EIO_CHECK_HIP(hipMemcpyAsync(d_accp, accp.data(), accp.size() * sizeof(T*), hipMemcpyHostToDevice, c->stream));
+    for (int f = 0; f < 5; ++f) {
+        T* o = (flags & EIO_DEVICE_PTRS) ? static_cast<T*>(dm[f]) : out + size_t(f) * m.n;
+        if (!(flags & EIO_DEVICE_PTRS))
+            EIO_CHECK_HIP(hipMemcpyAsync(o, dm[f], nb, hipMemcpyHostToDevice, c->stream));
+        const long nbk = (m.n + eio::NTHR - 1) / eio::NTHR;
+        hipLaunchKernelGGL(eio::k_stack<T>, dim3(unsigned(nbk)), dim3(eio::NTHR), 0, c->stream, o,
+                           d_accp + size_t(f) * nshot, nshot, m.n);
+        EIO_CHECK_HIP(hipGetLastError());
+        if (!(flags & EIO_DEVICE_PTRS))
+            EIO_CHECK_HIP(hipMemcpyAsync(dm[f], o, nb, hipMemcpyDeviceToHost, c->stream));
+    }
+    finish(run);
+}
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return EIO_OK;
+    } catch (const EioError& e) {
+        g_err = e.msg;
+        return e.code;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return EIO_ERR_RUNTIME;
+    }
+}
+
+Ctx* as_ctx(eio_ctx* p) {
+    if (!p) fail("context is NULL");
+    return reinterpret_cast<Ctx*>(p);
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char* eio_last_error(void) { return g_err.c_str(); }
+
+int eio_ctx_create(int device, int precision, eio_ctx** out) {
+    return guarded([&]() {
+        if (!out) fail("out is NULL");
+        if (precision != 4 && precision != 8) fail("precision must be 4 or 8");
+        int ndev = 0;
+        EIO_CHECK_HIP(hipGetDeviceCount(&ndev));
+        if (device < 0 || device >= ndev) fail("invalid device id");
+        DeviceGuard dg(device);
+        auto* c = new Ctx();
+        c->device = device;
+        c->prec = precision;
+        EIO_CHECK_HIP(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
+        c->stream = c->own;
+        for (auto& e : c->ev) EIO_CHECK_HIP(hipEventCreate(&e));
+        *out = reinterpret_cast<eio_ctx*>(c);
+    });
+}
+
+int eio_ctx_destroy(eio_ctx* ctx) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        DeviceGuard dg(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        for (auto& kv : c->models) model_free(kv.second);
+        for (auto& kv : c->ws)
+            if (kv.second.p) (void)hipFree(kv.second.p);
+        for (auto& e : c->ev) (void)hipEventDestroy(e);
+        (void)hipStreamDestroy(c->own);
+        delete c;
+    });
+}
+
+int eio_ctx_set_stream(eio_ctx* ctx, void* s) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        c->stream = s ? static_cast<hipStream_t>(s) : c->own;
+    });
+}
+
+int eio_ctx_set_graph(eio_ctx* ctx, int use_graph) {
+    return guarded([&]() { as_ctx(ctx)->use_graph = use_graph ? 1 : 0; });
+}
+
+int eio_ctx_last_timing(eio_ctx* ctx, double* total_ms, long long* n_step, double* step_ms) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        if (total_ms) *total_ms = c->last_total_ms;
+        if (n_step) *n_step = c->last_nstep;
+        if (step_ms) *step_ms = c->last_step_ms;
+    });
+}
+
+int eio_model_add(eio_ctx* ctx, const eio_grid* grid, const void* const media[5], int flags,
+                  int* model_id) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        if (!model_id) fail("model_id is NULL");
+        if (!media) fail("media is NULL");
+        DeviceGuard dg(c->device);
+        *model_id = (c->prec == 4) ? model_add<float>(c, grid, media, flags)
+                                   : model_add<double>(c, grid, media, flags);
+    });
+}
+
+int eio_model_set_media(eio_ctx* ctx, int model_id, const void* const media[5], int flags) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        if (!media) fail("media is NULL");
+        DeviceGuard dg(c->device);
+        ModelBase& m = get_model(c, model_id);
+        if (c->prec == 4) upload_model<float>(c, m, media, flags);
+        else upload_model<double>(c, m, media, flags);
+    });
+}
+
+int eio_model_remove(eio_ctx* ctx, int model_id) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        DeviceGuard dg(c->device);
+        ModelBase& m = get_model(c, model_id);
+        EIO_CHECK_HIP(hipStreamSynchronize(c->stream));
+        model_free(m);
+        c->models.erase(model_id);
+    });
+}
+
+int eio_nl_forward(eio_ctx* ctx, int nshot, const eio_shot* shots, int flags) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        DeviceGuard dg(c->device);
+        if (c->prec == 4) nl_forward<float>(c, nshot, shots, flags);
+        else nl_forward<double>(c, nshot, shots, flags);
+    });
+}
+
+int eio_nl_adjoint(eio_ctx* ctx, int nshot, const eio_shot* shots, int flags) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        DeviceGuard dg(c->device);
+        if (c->prec == 4) nl_adjoint<float>(c, nshot, shots, flags);
+        else nl_adjoint<double>(c, nshot, shots, flags);
+    });
+}
+
+int eio_born_forward(eio_ctx* ctx, int nshot, const eio_shot* shots, const void* const dm[5],
+                     int flags) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        DeviceGuard dg(c->device);
+        if (nshot <= 0 || !shots) fail("no shots");
+        if (c->prec == 4) born_forward<float>(c, nshot, shots, dm, flags);
+        else born_forward<double>(c, nshot, shots, dm, flags);
+    });
+}
+
+int eio_born_adjoint(eio_ctx* ctx, int nshot, const eio_shot* shots, void* const dm[5], int flags) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        DeviceGuard dg(c->device);
+        if (nshot <= 0 || !shots) fail("no shots");
+        if (c->prec == 4) born_adjoint<float>(c, nshot, shots, dm, flags);
+        else born_adjoint<double>(c, nshot, shots, dm, flags);
+    });
+}
+
+int eio_device_alloc(eio_ctx* ctx, unsigned long long bytes, void** ptr) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        if (!ptr) fail("ptr is NULL");
+        DeviceGuard dg(c->device);
+        EIO_CHECK_HIP(hipMalloc(ptr, bytes));
+    });
+}
+
+int eio_device_free(eio_ctx* ctx, void* ptr) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        DeviceGuard dg(c->device);
+        EIO_CHECK_HIP(hipStreamSynchronize(c->stream));
+        EIO_CHECK_HIP(hipFree(ptr));
+    });
+}
+
+int eio_memcpy(eio_ctx* ctx, void* dst, const void* src, unsigned long long bytes, int kind) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        DeviceGuard dg(c->device);
+        hipMemcpyKind k;
+        if (kind == 1) k = hipMemcpyHostToDevice;
+        else if (kind == 2) k = hipMemcpyDeviceToHost;
+        else if (kind == 3) k = hipMemcpyDeviceToDevice;
+        else fail("invalid copy kind");
+        EIO_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, k, c->stream));
+        EIO_CHECK_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int eio_synchronize(eio_ctx* ctx) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        DeviceGuard dg(c->device);
+        EIO_CHECK_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+}  // extern "C"

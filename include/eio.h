@@ -1,0 +1,120 @@
+/* eio.h -- C ABI of the MI355X (gfx950) elastic velocity-stress propagator.
+ *
+ * Plain C: opaque context, POD descriptors of plain pointers and sizes, int status
+ * codes (0 = success) and a thread-local error string.  No torch or C++ types.
+ *
+ * Every entry point replaces a piece of the reference's C-style GPU seam
+ * (paths relative to elastic_iso_lib/elastic_iso_float/prop/include/):
+ *   nonlinearPropElasticGpuFunctions.h:5-136  initNonlinearElasticGpu, allocateNonlinearElasticGpu,
+ *                                             propShotsElasticFwdGpu, propShotsElasticAdjGpu,
+ *                                             deallocateNonlinearElasticGpu
+ *   BornElasticGpuFunctions.h:5-66            initBornGpu, allocateBornElasticGpu,
+ *                                             BornShotsFwdGpu, BornShotsAdjGpu, deallocateBornElasticGpu
+ * Unlike the reference (global per-GPU state, one shot per call, exit(-1) on error),
+ * the state lives in a context, a call propagates any number of shots in one batched
+ * time loop on one device, and errors are returned.
+ *
+ * Conventions (identical to the reference):
+ *   fields are [nx][nz] with z fastest, index ix*nz + iz, nz/nx include padding + FAT=4;
+ *   regular device positions are such linear indices; traces are [nTrace][nt], time fastest;
+ *   component k = 0..4 is vx, vz, sxx, szz, sxz; grid g = 0..3 is centre, X, Z, XZ and
+ *   component k lives on grid EIO_GRID_OF_COMP[k] = {1, 2, 0, 0, 3}.
+ *   Real type is float (precision 4) or double (precision 8), fixed per context.
+ */
+#ifndef EIO_H
+#define EIO_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EIO_OK 0
+#define EIO_ERR_ARG 1       /* invalid argument (message in eio_last_error) */
+#define EIO_ERR_RUNTIME 2   /* HIP runtime error */
+#define EIO_ERR_STATE 3     /* call sequence error */
+
+/* flags */
+#define EIO_DEVICE_PTRS 1   /* trace / field pointers are device pointers (no H2D/D2H copies) */
+
+typedef struct eio_ctx eio_ctx;
+
+/* FD grid of one model.  sub_fd is the fine sub-sampling already doubled
+ * (fdParamElastic.cpp:28-37): ntw = (nts-1)*sub_fd + 1, dtw = dts/sub_fd. */
+typedef struct {
+    int nz, nx;          /* including padding and FAT */
+    int nts, sub_fd;     /* coarse samples, fine steps per coarse step */
+    int minPad;          /* sponge width, min over the four sides */
+    double dz, dx, dts;  /* spatial sampling [m], coarse time sampling [s] */
+    double alphaCos;     /* sponge coefficient, 0.99 by default */
+} eio_grid;
+
+/* One shot: regular source/receiver positions per grid and trace pointers per component.
+ * NL forward : src_tr = source traces [n_src[g]][nts] (already scaled, coarse), rec_tr = output
+ * NL adjoint : rec_tr = input data, src_tr = output model traces
+ * Born       : src_tr = fine-sampled source traces [n_src[g]][ntw], rec_tr = data in/out */
+typedef struct {
+    int model;                 /* id from eio_model_add */
+    const int* src_pos[4];
+    int n_src[4];
+    const int* rec_pos[4];
+    int n_rec[4];
+    void* src_tr[5];
+    void* rec_tr[5];
+} eio_shot;
+
+/* Error text of the last failing call on this thread. */
+const char* eio_last_error(void);
+
+/* Context on HIP device `device` with real size `precision` (4 or 8).
+ * Replaces the per-GPU global pointer tables built by initNonlinearElasticGpu /
+ * initBornGpu (nonlinearPropElasticGpuFunctions.cu:85-232, BornElasticGpuFunctions.cu:88-267). */
+int eio_ctx_create(int device, int precision, eio_ctx** out);
+int eio_ctx_destroy(eio_ctx* ctx);
+/* Run on an external stream (e.g. torch's current stream); NULL = the context's own stream. */
+int eio_ctx_set_stream(eio_ctx* ctx, void* hip_stream);
+/* Timing of the last propagation call, from HIP events on the context stream:
+ * total device time [ms], number of fused stencil launches, and the mean time of one
+ * stencil launch [ms] measured over a bracketed run of them. */
+int eio_ctx_last_timing(eio_ctx* ctx, double* total_ms, long long* n_step_launches,
+                        double* step_ms);
+/* 0 = launch each fine step (default), 1 = replay each batch of fine steps from a captured hipGraph. */
+int eio_ctx_set_graph(eio_ctx* ctx, int use_graph);
+
+/* Register a model: grid constants (FD coefficients, time filter, damping table) and the
+ * five scaled media rhoxDtw, rhozDtw, lamb2MuDtw, lambDtw, muxzDtw, each nz*nx.
+ * Replaces initNonlinearElasticGpu + allocateNonlinearElasticGpu
+ * (nonlinearPropElasticGpuFunctions.h:8-16; nonlinearPropElasticGpuFunctions.cu:85-265). */
+int eio_model_add(eio_ctx* ctx, const eio_grid* grid, const void* const media[5], int flags,
+                  int* model_id);
+/* Replace the media of a model (setBackground, nonlinearPropElasticShotsGpu.h:61). */
+int eio_model_set_media(eio_ctx* ctx, int model_id, const void* const media[5], int flags);
+int eio_model_remove(eio_ctx* ctx, int model_id);
+
+/* Nonlinear forward of nshot shots in one batched time loop; rec_tr is overwritten.
+ * Replaces propShotsElasticFwdGpu (nonlinearPropElasticGpuFunctions.h:20-26, .cu:561-648). */
+int eio_nl_forward(eio_ctx* ctx, int nshot, const eio_shot* shots, int flags);
+/* Nonlinear adjoint; src_tr is overwritten.
+ * Replaces propShotsElasticAdjGpu (nonlinearPropElasticGpuFunctions.h:28-34, .cu:932-1028). */
+int eio_nl_adjoint(eio_ctx* ctx, int nshot, const eio_shot* shots, int flags);
+
+/* Born forward: dm = 5 staggered and scaled perturbations (drhox*rhoxDtw, drhoz*rhozDtw,
+ * dlame*2dtw, dmu*2dtw, dmuxz*2dtw), nz*nx each, shared by all shots (all shots must use
+ * the same model).  rec_tr is overwritten.
+ * Replaces BornShotsFwdGpu (BornElasticGpuFunctions.h:12-23, .cu:719-1098). */
+int eio_born_forward(eio_ctx* ctx, int nshot, const eio_shot* shots, const void* const dm[5],
+                     int flags);
+/* Born adjoint: accumulates (+=) the imaging condition of all shots into dm[5].
+ * Replaces BornShotsAdjGpu (BornElasticGpuFunctions.h:25-36, .cu:1100-1454). */
+int eio_born_adjoint(eio_ctx* ctx, int nshot, const eio_shot* shots, void* const dm[5], int flags);
+
+/* Device memory helpers for callers without their own allocator (bytes). */
+int eio_device_alloc(eio_ctx* ctx, unsigned long long bytes, void** ptr);
+int eio_device_free(eio_ctx* ctx, void* ptr);
+int eio_memcpy(eio_ctx* ctx, void* dst, const void* src, unsigned long long bytes, int kind);
+/* kind: 1 = host->device, 2 = device->host, 3 = device->device */
+int eio_synchronize(eio_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EIO_H */

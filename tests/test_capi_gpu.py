@@ -1,0 +1,204 @@
+"""GPU parity of the C ABI (libeio_hip.so) against the CPU oracle at the regular-grid level.
+
+The C ABI takes scaled media, regular device positions and regular traces, exactly the
+level of the reference's GPU seam (propShotsElastic{Fwd,Adj}Gpu, BornShots{Fwd,Adj}Gpu).
+Bars: FP32 within 1e-5 relative L2 of the FP64 oracle; FP64 within 1e-11; dot-product
+tests 1e-5 (FP32) and 1e-12 (FP64).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from elastic_isotropic_operators_amd import _capi
+from elastic_isotropic_operators_amd import workloads as W
+
+from helpers import GRID_OF_COMP, oracle_grid, reg_positions, rel_l2, random_traces
+
+pytestmark = pytest.mark.gpu
+
+TOL = {np.float32: 1e-5, np.float64: 1e-11}
+DOT_TOL = {np.float32: 1e-5, np.float64: 1e-12}
+
+
+def _case(nz=40, nx=56, nts=120, sub=1, seed=0):
+    w = W.small(nz=nz, nx=nx, nts=nts, sub=sub, seed=seed)
+    g = oracle_grid(w)
+    med = O.media(g, w["elastic"], np.float64)
+    spos = reg_positions(w, w["src_z"][0], w["src_x"][0])
+    rpos = reg_positions(w, w["rec_z"], w["rec_x"])
+    return w, g, med, spos, rpos
+
+
+def _ctx(dtype):
+    return _capi.Context(0, dtype)
+
+
+def _model(ctx, w, g, med):
+    return ctx.model_add(g.nz, g.nx, g.nts, g.sub, g.dz, g.dx, g.dts, g.minPad, med, g.alphaCos)
+
+
+def _gpu_nl_fwd(ctx, mid, spos, rpos, src, nts, dtype):
+    rec = [np.zeros((rpos[GRID_OF_COMP[k]].size, nts), dtype) for k in range(5)]
+    shot, keep = ctx.make_shot(mid, spos, rpos, [np.ascontiguousarray(s, dtype) for s in src], rec)
+    ctx.nl_forward([shot])
+    return rec
+
+
+def _gpu_nl_adj(ctx, mid, spos, rpos, dat, nts, dtype):
+    src = [np.zeros((spos[GRID_OF_COMP[k]].size, nts), dtype) for k in range(5)]
+    shot, keep = ctx.make_shot(mid, spos, rpos, src, [np.ascontiguousarray(d, dtype) for d in dat])
+    ctx.nl_adjoint([shot])
+    return src
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("sub", [1, 2])
+def test_nl_forward_parity(dtype, sub):
+    w, g, med, spos, rpos = _case(sub=sub)
+    rng = np.random.default_rng(1)
+    src = random_traces(rng, spos, g.nts, np.float64)
+    ref = O.nl_fwd_reg(g, med, spos, src, rpos, np.float64)
+    ctx = _ctx(dtype)
+    mid = _model(ctx, w, g, med)
+    out = _gpu_nl_fwd(ctx, mid, spos, rpos, src, g.nts, dtype)
+    for k in range(5):
+        assert np.linalg.norm(ref[k]) > 0
+        assert rel_l2(out[k], ref[k]) < TOL[dtype], (k, rel_l2(out[k], ref[k]))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_nl_adjoint_parity(dtype):
+    w, g, med, spos, rpos = _case(sub=1)
+    rng = np.random.default_rng(2)
+    dat = random_traces(rng, rpos, g.nts, np.float64)
+    ref = O.nl_adj_reg(g, med, spos, rpos, dat, np.float64)
+    ctx = _ctx(dtype)
+    mid = _model(ctx, w, g, med)
+    out = _gpu_nl_adj(ctx, mid, spos, rpos, dat, g.nts, dtype)
+    for k in range(5):
+        assert rel_l2(out[k], ref[k]) < TOL[dtype], (k, rel_l2(out[k], ref[k]))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_nl_dot_product(dtype):
+    w, g, med, spos, rpos = _case(sub=1, seed=3)
+    rng = np.random.default_rng(11)
+    m = [rng.standard_normal((spos[GRID_OF_COMP[k]].size, g.nts)).astype(dtype) for k in range(5)]
+    d = [rng.standard_normal((rpos[GRID_OF_COMP[k]].size, g.nts)).astype(dtype) for k in range(5)]
+    ctx = _ctx(dtype)
+    mid = _model(ctx, w, g, med)
+    Fm = _gpu_nl_fwd(ctx, mid, spos, rpos, m, g.nts, dtype)
+    Ftd = _gpu_nl_adj(ctx, mid, spos, rpos, d, g.nts, dtype)
+    a = sum(np.sum(Fm[k].astype(np.float64) * d[k]) for k in range(5))
+    b = sum(np.sum(m[k].astype(np.float64) * Ftd[k]) for k in range(5))
+    assert abs(a - b) / abs(b) < DOT_TOL[dtype], (a, b)
+
+
+def _born_inputs(w, g, spos, seed):
+    rng = np.random.default_rng(seed)
+    ntw = (g.nts - 1) * g.sub + 1
+    src_fine = random_traces(rng, spos, ntw, np.float64)
+    from scipy.ndimage import gaussian_filter
+    dm5 = np.stack([gaussian_filter(rng.standard_normal((g.nx, g.nz)), 2.0) for _ in range(5)])
+    dm5[:, :4, :] = 0
+    dm5[:, -4:, :] = 0
+    dm5[:, :, :4] = 0
+    dm5[:, :, -4:] = 0
+    return src_fine, dm5
+
+
+def _gpu_born_fwd(ctx, mid, spos, rpos, src_fine, dm5, nts, dtype):
+    rec = [np.zeros((rpos[GRID_OF_COMP[k]].size, nts), dtype) for k in range(5)]
+    shot, keep = ctx.make_shot(mid, spos, rpos, [np.ascontiguousarray(s, dtype) for s in src_fine], rec)
+    ctx.born_forward([shot], dm5)
+    return rec
+
+
+def _gpu_born_adj(ctx, mid, spos, rpos, src_fine, dat, nx, nz, dtype):
+    out = np.zeros((5, nx, nz), dtype)
+    shot, keep = ctx.make_shot(mid, spos, rpos, [np.ascontiguousarray(s, dtype) for s in src_fine],
+                               [np.ascontiguousarray(d, dtype) for d in dat])
+    ctx.born_adjoint([shot], out)
+    return out
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_born_forward_parity(dtype):
+    w, g, med, spos, rpos = _case(sub=1, seed=4)
+    src_fine, dm5 = _born_inputs(w, g, spos, 5)
+    ref = O.born_fwd_reg(g, med, spos, src_fine, dm5, rpos, np.float64)
+    ctx = _ctx(dtype)
+    mid = _model(ctx, w, g, med)
+    out = _gpu_born_fwd(ctx, mid, spos, rpos, src_fine, dm5, g.nts, dtype)
+    for k in range(5):
+        assert np.linalg.norm(ref[k]) > 0
+        assert rel_l2(out[k], ref[k]) < TOL[dtype], (k, rel_l2(out[k], ref[k]))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_born_adjoint_parity(dtype):
+    w, g, med, spos, rpos = _case(sub=2, seed=6)
+    src_fine, _ = _born_inputs(w, g, spos, 7)
+    rng = np.random.default_rng(8)
+    dat = random_traces(rng, rpos, g.nts, np.float64)
+    ref = O.born_adj_reg(g, med, spos, src_fine, rpos, dat, np.float64)
+    ctx = _ctx(dtype)
+    mid = _model(ctx, w, g, med)
+    out = _gpu_born_adj(ctx, mid, spos, rpos, src_fine, dat, g.nx, g.nz, dtype)
+    assert rel_l2(out, ref) < TOL[dtype] * 10, rel_l2(out, ref)
+    for k in range(5):
+        assert rel_l2(out[k], ref[k]) < TOL[dtype] * 10, (k, rel_l2(out[k], ref[k]))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_born_dot_product(dtype):
+    w, g, med, spos, rpos = _case(sub=1, seed=9)
+    src_fine, dm5 = _born_inputs(w, g, spos, 10)
+    rng = np.random.default_rng(12)
+    d = [rng.standard_normal((rpos[GRID_OF_COMP[k]].size, g.nts)) for k in range(5)]
+    ctx = _ctx(dtype)
+    mid = _model(ctx, w, g, med)
+    Bm = _gpu_born_fwd(ctx, mid, spos, rpos, src_fine, dm5, g.nts, dtype)
+    Btd = _gpu_born_adj(ctx, mid, spos, rpos, src_fine, [x.astype(dtype) for x in d], g.nx, g.nz, dtype)
+    dm5c = dm5.astype(dtype).astype(np.float64)
+    a = sum(np.sum(Bm[k].astype(np.float64) * d[k].astype(dtype)) for k in range(5))
+    b = np.sum(dm5c * Btd.astype(np.float64))
+    assert abs(a - b) / abs(b) < DOT_TOL[dtype], (a, b)
+
+
+def test_batched_uneven_shots_match_single_shots():
+    """Several models of different size and trace length in one batched call equal
+    per-shot oracle runs (the persistent batch only changes scheduling, not arithmetic)."""
+    dtype = np.float32
+    ctx = _ctx(dtype)
+    cases = [_case(nz=40, nx=56, nts=120, sub=1, seed=20), _case(nz=56, nx=40, nts=90, sub=2, seed=21),
+             _case(nz=33, nx=70, nts=150, sub=1, seed=22)]
+    shots, recs, refs, keep = [], [], [], []
+    rng = np.random.default_rng(23)
+    for (w, g, med, spos, rpos) in cases:
+        mid = _model(ctx, w, g, med)
+        src = random_traces(rng, spos, g.nts, np.float64)
+        refs.append(O.nl_fwd_reg(g, med, spos, src, rpos, np.float64))
+        rec = [np.zeros((rpos[GRID_OF_COMP[k]].size, g.nts), dtype) for k in range(5)]
+        srcc = [np.ascontiguousarray(s, dtype) for s in src]
+        shot, kp = ctx.make_shot(mid, spos, rpos, srcc, rec)
+        keep.append((kp, srcc))
+        shots.append(shot)
+        recs.append(rec)
+    ctx.nl_forward(shots)
+    for rec, ref in zip(recs, refs):
+        for k in range(5):
+            assert rel_l2(rec[k], ref[k]) < 1e-5, (k, rel_l2(rec[k], ref[k]))
+
+
+def test_invalid_positions_rejected():
+    """A device in the FAT halo is an argument error, not a fault."""
+    w, g, med, spos, rpos = _case()
+    ctx = _ctx(np.float32)
+    mid = _model(ctx, w, g, med)
+    bad = [np.array([1], np.int32)] * 4  # iz = 1 is inside the FAT halo
+    rec = [np.zeros((rpos[GRID_OF_COMP[k]].size, g.nts), np.float32) for k in range(5)]
+    src = [np.zeros((1, g.nts), np.float32) for _ in range(5)]
+    shot, keep = ctx.make_shot(mid, bad, rpos, src, rec)
+    with pytest.raises(_capi.EioError):
+        ctx.nl_forward([shot])
