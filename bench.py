@@ -1,0 +1,182 @@
+#!/usr/bin/env python3
+"""Benchmark of the batched elastic propagator on one MI355X.
+
+Workload (default "c2"): 64 layered shots of uneven size (interior nz 300-600, nx
+600-1800, nts 2000-4000 at 1 ms, sub=2, Ricker 8 Hz, 40-cell sponge), FP32, all shots
+propagated by the nonlinear forward operator in one batched time loop.  A "step" is one
+full batched propagation of the 64 shots.  Inputs (media, source traces) are resident in
+HBM before the timed region; outputs stay in HBM.
+
+Prints ONE JSON line with GCell/s (cell-update = one update-region cell for one fine step),
+the HBM roofline of the fused step kernel (80 algorithmic bytes per FP32 cell-update,
+timed with HIP events on the kernel's stream) and a CPU baseline: the FP32 OpenMP oracle
+on a bounded sample of the same workload on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from elastic_isotropic_operators_amd import _capi, geometry as G, workloads as W  # noqa: E402
+
+BYTES_PER_CELL = {"nl_fwd": 80}
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def prepare_shots(ctx, shots_w, dtype=np.float32):
+    """Upload media and source traces; allocate receiver traces.  Returns (shots, keep, cells)."""
+    shots, keep, cells = [], [], 0
+    for w in shots_w:
+        sub_fd = 2 * w["sub"]
+        dtw = G.dtw_of(w["dts"], w["sub"])
+        media = G.scaled_media(w["elastic"], dtw, dtype)
+        mid = ctx.model_add(w["nz"], w["nx"], w["nts"], sub_fd, w["dz"], w["dx"], w["dts"],
+                            W.min_pad(w), media, w["alphaCos"])
+        si = G.staggered_interps(w["nz"], w["oz"], w["dz"], w["ox"], w["dx"], w["src_z"][0],
+                                 w["src_x"][0])
+        ri = G.staggered_interps(w["nz"], w["oz"], w["dz"], w["ox"], w["dx"], w["rec_z"], w["rec_x"])
+        src = G.nl_source_traces(si, media, w["model"], dtw, w["dx"], w["dz"], dtype)
+        spos = [ip.uniq for ip in si]
+        rpos = [ip.uniq for ip in ri]
+        dsrc, drec = [], []
+        for k in range(5):
+            p = ctypes_alloc(ctx, src[k].nbytes)
+            _capi.check(_capi.lib().eio_memcpy(ctx.h, p, src[k].ctypes.data, src[k].nbytes, 1))
+            dsrc.append(p)
+            nrec = rpos[G.GRID_OF_COMP[k]].size * w["nts"] * np.dtype(dtype).itemsize
+            drec.append(ctypes_alloc(ctx, nrec))
+        shot, kp = ctx.make_shot(mid, spos, rpos, dsrc, drec)
+        keep.append((kp, spos, rpos, dsrc, drec))
+        shots.append(shot)
+        cells += W.cell_updates(w)
+    return shots, keep, cells
+
+
+def ctypes_alloc(ctx, nbytes):
+    import ctypes
+    p = ctypes.c_void_p()
+    _capi.check(_capi.lib().eio_device_alloc(ctx.h, max(int(nbytes), 4), ctypes.byref(p)))
+    return p.value
+
+
+def cpu_baseline(shots_w, target_s=12.0):
+    """FP32 OpenMP oracle on the first shot with a truncated trace length."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    O.build()
+    O.set_ftz(True)
+    w = shots_w[0]
+
+    def run(nts):
+        g = O.grid(w["nz"], w["nx"], nts, 2 * w["sub"], w["dz"], w["dx"], w["dts"], W.min_pad(w),
+                   w["alphaCos"], w["oz"], w["ox"])
+        t0 = time.perf_counter()
+        O.nl_fwd_shot(g, w["elastic"], w["src_z"][0], w["src_x"][0], w["rec_z"], w["rec_x"],
+                      w["model"][:, :, :nts], np.float32)
+        el = time.perf_counter() - t0
+        cells = (w["nz"] - 8) * (w["nx"] - 8) * (nts - 1) * 2 * w["sub"]
+        return el, cells
+
+    el, cells = run(20)
+    rate = cells / max(el, 1e-9)
+    nts = int(min(w["nts"], max(20, target_s * rate / ((w["nz"] - 8) * (w["nx"] - 8) * 2 * w["sub"]))))
+    el, cells = run(nts)
+    return dict(value=cells / el / 1e9, unit="GCell/s", cores=O.num_threads(), kind="port",
+                sample="shot 0 of the workload (%dx%d), first %d of %d coarse samples, FP32 oracle "
+                       "with flush-to-zero, %.1f s" % (w["nz"], w["nx"], nts, w["nts"], el))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c0"])
+    ap.add_argument("--scale", type=float, default=1.0, help="shrink every size (quick runs)")
+    ap.add_argument("--nshots", type=int, default=64)
+    ap.add_argument("--graph", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every box of this build has a single MI355X; "
+              "shots would be partitioned across GPUs with one all-reduce of the gradient "
+              "(see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    t_build = time.perf_counter()
+    if args.workload == "c2":
+        shots_w = W.layered_batch_c2(args.nshots, seed=2, scale=args.scale)
+        wl_name = "layered_batch_c2_%dshots" % args.nshots
+    else:
+        shots_w = [W.gaussian_c0()]
+        wl_name = shots_w[0]["name"]
+    log("workload %s built in %.1f s" % (wl_name, time.perf_counter() - t_build))
+
+    ctx = _capi.Context(0, np.float32)
+    ctx.set_graph(args.graph)
+    shots, keep, cells = prepare_shots(ctx, shots_w)
+    log("uploaded %d shots, %.3e cell-updates per step" % (len(shots), cells))
+
+    for i in range(args.warmup):
+        t0 = time.perf_counter()
+        ctx.nl_forward(shots, device_ptrs=True)
+        log("warmup %d: %.3f s" % (i, time.perf_counter() - t0))
+    _capi.check(_capi.lib().eio_synchronize(ctx.h))
+    times, step_ms, nlaunch = [], [], []
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        ctx.nl_forward(shots, device_ptrs=True)
+        _capi.check(_capi.lib().eio_synchronize(ctx.h))
+        dt = time.perf_counter() - t0
+        tot, nst, sms = ctx.last_timing()
+        times.append(dt)
+        step_ms.append(sms)
+        nlaunch.append(nst)
+        log("step %d: %.3f s, %.2f GCell/s, loop %.1f ms over %d launches" %
+            (i, dt, cells / dt / 1e9, sms * nst, nst))
+    elapsed = sum(times)
+    ms_per_step = elapsed / args.steps * 1e3
+    value = cells * args.steps / elapsed / 1e9
+    loop_s = float(np.mean([s * n for s, n in zip(step_ms, nlaunch)])) / 1e3
+    achieved = BYTES_PER_CELL["nl_fwd"] * cells / loop_s / 1e9
+    res = {
+        "metric": "GCell-updates/s (FP32 batched NL forward)",
+        "value": round(value, 3),
+        "unit": "GCell/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": wl_name, "shots": len(shots), "cell_updates_per_step": cells,
+                   "scale": args.scale, "graph": args.graph},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_step_fwd<float,4,M_NL>",
+                     "mean_launch_ms": round(float(np.mean(step_ms)), 5),
+                     "launches_per_step": int(nlaunch[-1]),
+                     "bytes_per_cell_update": BYTES_PER_CELL["nl_fwd"]},
+    }
+    if not args.no_cpu:
+        t0 = time.perf_counter()
+        res["cpu_baseline"] = cpu_baseline(shots_w)
+        log("cpu baseline %.1f s" % (time.perf_counter() - t0))
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

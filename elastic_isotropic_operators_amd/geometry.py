@@ -1,0 +1,129 @@
+"""numpy host preparation for the device-level (C ABI) path.
+
+Used by the torch FWI layer and the benchmark, which keep traces and media resident in
+HBM and call the C ABI directly.  The drop-in operator classes use the C++ host layer
+(csrc/host) that implements the same rules.
+
+  scaled_media        fdParamElastic.cpp:118-148 (stagger rho and mu, scale by 2*dtw)
+  Interp              spaceInterpGpu.cpp:78-138, 288-376 (bilinear weights, unique list)
+  staggered grids     Elastic_iso_float_prop.py:224-249 (centre, X, Z, XZ origins)
+  source scaling      nonlinearPropElasticGpu.cpp:109-144
+"""
+import numpy as np
+
+GRID_OF_COMP = (1, 2, 0, 0, 3)
+
+
+def stagger_x_adj(f):
+    """staggerX::adjoint (stagger.cpp:38-57) on (..., nx, nz)."""
+    out = np.empty_like(f)
+    out[..., 1:-1, :] = 0.5 * (f[..., 1:-1, :] + f[..., :-2, :])
+    out[..., 0, :] = 0.5 * f[..., 0, :]
+    out[..., -1, :] = f[..., -1, :] + 0.5 * f[..., -2, :]
+    return out
+
+
+def stagger_z_adj(f):
+    """staggerZ::adjoint (stagger.cpp:95-114)."""
+    out = np.empty_like(f)
+    out[..., 1:-1] = 0.5 * (f[..., 1:-1] + f[..., :-2])
+    out[..., 0] = 0.5 * f[..., 0]
+    out[..., -1] = f[..., -1] + 0.5 * f[..., -2]
+    return out
+
+
+def stagger_x_fwd(f):
+    """staggerX::forward (stagger.cpp:18-35)."""
+    out = np.empty_like(f)
+    out[..., :-1, :] = 0.5 * (f[..., :-1, :] + f[..., 1:, :])
+    out[..., -1, :] = f[..., -1, :]
+    return out
+
+
+def stagger_z_fwd(f):
+    """staggerZ::forward (stagger.cpp:75-93)."""
+    out = np.empty_like(f)
+    out[..., :-1] = 0.5 * (f[..., :-1] + f[..., 1:])
+    out[..., -1] = f[..., -1]
+    return out
+
+
+def dtw_of(dts, sub_user):
+    return np.float32(dts) / np.float32(2 * sub_user)
+
+
+def scaled_media(elastic, dtw, dtype=np.float32):
+    """(3, nx, nz) [rho, lambda, mu] -> (5, nx, nz) rhoxDtw, rhozDtw, lamb2MuDtw, lambDtw, muxzDtw."""
+    el = np.asarray(elastic, dtype=dtype)
+    rho, lam, mu = el[0], el[1], el[2]
+    two = dtype(2.0) * dtype(dtw)
+    out = np.empty((5,) + rho.shape, dtype=dtype)
+    out[0] = two / stagger_x_adj(rho)
+    out[1] = two / stagger_z_adj(rho)
+    out[2] = two * (lam + dtype(2) * mu)
+    out[3] = two * lam
+    out[4] = two * stagger_z_adj(stagger_x_adj(mu))
+    return out
+
+
+class Interp:
+    """Linear (bilinear) spatial interpolation of devices on one staggered grid."""
+
+    def __init__(self, nz, oz, dz, ox, dx, zc, xc):
+        zc = np.atleast_1d(np.asarray(zc, np.float32))
+        xc = np.atleast_1d(np.asarray(xc, np.float32))
+        wz = (zc - np.float32(oz)) / np.float32(dz)
+        wx = (xc - np.float32(ox)) / np.float32(dx)
+        zr = wz.astype(np.int64)
+        xr = wx.astype(np.int64)
+        wz = np.float32(1.0) - (wz - zr.astype(np.float32))
+        wx = np.float32(1.0) - (wx - xr.astype(np.float32))
+        b = xr * nz + zr
+        self.idx = np.stack([b, b + 1, b + nz, b + nz + 1], axis=1)
+        self.w = np.stack([wz * wx, (1 - wz) * wx, wz * (1 - wx), (1 - wz) * (1 - wx)],
+                          axis=1).astype(np.float64)
+        flat = self.idx.reshape(-1)
+        uniq, first = np.unique(flat, return_index=True)
+        order = np.argsort(first)               # first-seen order (spaceInterpGpu.cpp:288-311)
+        self.uniq = uniq[order].astype(np.int32)
+        lookup = {int(p): i for i, p in enumerate(self.uniq)}
+        self.map = np.array([lookup[int(p)] for p in flat], np.int64).reshape(self.idx.shape)
+        self.n_irreg = zc.size
+
+    def adjoint(self, irr):
+        """irregular (nIrreg, nt) -> regular (nUniq, nt)."""
+        reg = np.zeros((self.uniq.size, irr.shape[1]), dtype=irr.dtype)
+        for f in range(4):
+            np.add.at(reg, self.map[:, f], (self.w[:, f:f + 1] * irr).astype(irr.dtype))
+        return reg
+
+    def forward(self, reg):
+        """regular (nUniq, nt) -> irregular (nIrreg, nt)."""
+        out = np.zeros((self.n_irreg, reg.shape[1]), dtype=reg.dtype)
+        for f in range(4):
+            out += (self.w[:, f:f + 1] * reg[self.map[:, f]]).astype(reg.dtype)
+        return out
+
+
+def staggered_interps(nz, oz, dz, ox, dx, zc, xc):
+    """The four grids: centre, X (x-dx/2), Z (z-dz/2), XZ."""
+    origins = [(oz, ox), (oz, ox - 0.5 * dx), (oz - 0.5 * dz, ox), (oz - 0.5 * dz, ox - 0.5 * dx)]
+    return [Interp(nz, o[0], dz, o[1], dx, zc, xc) for o in origins]
+
+
+def nl_source_traces(interps, media, model, dtw, dx, dz, dtype=np.float32):
+    """Irregular model (5, nSrc, nts) -> 5 regular scaled source traces."""
+    two = dtype(2.0) * dtype(dtw)
+    area = dtype(1.0) / (dtype(dx) * dtype(dz))
+    out = []
+    for k in range(5):
+        ip = interps[GRID_OF_COMP[k]]
+        reg = ip.adjoint(np.asarray(model[k], dtype=dtype))
+        if k == 0:
+            scale = media[0].reshape(-1)[ip.uniq][:, None]
+        elif k == 1:
+            scale = media[1].reshape(-1)[ip.uniq][:, None]
+        else:
+            scale = two
+        out.append(np.ascontiguousarray((reg * scale) * area, dtype=dtype))
+    return out

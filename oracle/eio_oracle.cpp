@@ -40,6 +40,9 @@
 #include <map>
 #include <vector>
 #include <algorithm>
+#include <omp.h>
+#include <pmmintrin.h>
+#include <xmmintrin.h>
 
 namespace {
 
@@ -933,6 +936,20 @@ static Grid to_grid(const OracleGrid* g) {
 
 EIO_ORACLE_TYPED(f64, double)
 EIO_ORACLE_TYPED(f32, float)
+
+// Flush-to-zero / denormals-are-zero on every OpenMP thread (the FP32 timing twin otherwise
+// spends most of its time on denormals in the quiet part of the wavefield).
+void oracle_set_ftz(int on) {
+#pragma omp parallel
+    {
+        _MM_SET_FLUSH_ZERO_MODE(on ? _MM_FLUSH_ZERO_ON : _MM_FLUSH_ZERO_OFF);
+        _MM_SET_DENORMALS_ZERO_MODE(on ? _MM_DENORMALS_ZERO_ON : _MM_DENORMALS_ZERO_OFF);
+    }
+    _MM_SET_FLUSH_ZERO_MODE(on ? _MM_FLUSH_ZERO_ON : _MM_FLUSH_ZERO_OFF);
+    _MM_SET_DENORMALS_ZERO_MODE(on ? _MM_DENORMALS_ZERO_ON : _MM_DENORMALS_ZERO_OFF);
+}
+
+int oracle_num_threads(void) { return omp_get_max_threads(); }
 
 // Spatial interpolation of one grid: returns the number of unique regular points;
 // fills idx/w (4 per device) and uniq (capacity 4*nDev).

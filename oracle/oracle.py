@@ -236,3 +236,14 @@ def born_adj_shots(g, elastic, src_z, src_x, src_sig, rec_z, rec_x, data, dtype=
         ctypes.byref(g), _ptr(el), nshot, nsrc, _ptr(sz), _ptr(sx), _ptr(sig), rz.size, _ptr(rz),
         _ptr(rx), _ptr(dm), _ptr(da))
     return dm
+
+
+def set_ftz(on=True):
+    """Flush denormals to zero on the OpenMP threads (FP32 timing twin)."""
+    lib().oracle_set_ftz(int(bool(on)))
+
+
+def num_threads():
+    f = lib().oracle_num_threads
+    f.restype = ctypes.c_int
+    return int(f())
