@@ -50,23 +50,15 @@ def prepare_shots(ctx, shots_w, dtype=np.float32):
         rpos = [ip.uniq for ip in ri]
         dsrc, drec = [], []
         for k in range(5):
-            p = ctypes_alloc(ctx, src[k].nbytes)
-            _capi.check(_capi.lib().eio_memcpy(ctx.h, p, src[k].ctypes.data, src[k].nbytes, 1))
+            p = ctx.alloc(src[k].nbytes)
+            ctx.h2d(p, src[k])
             dsrc.append(p)
-            nrec = rpos[G.GRID_OF_COMP[k]].size * w["nts"] * np.dtype(dtype).itemsize
-            drec.append(ctypes_alloc(ctx, nrec))
+            drec.append(ctx.alloc(rpos[G.GRID_OF_COMP[k]].size * w["nts"] * np.dtype(dtype).itemsize))
         shot, kp = ctx.make_shot(mid, spos, rpos, dsrc, drec)
         keep.append((kp, spos, rpos, dsrc, drec))
         shots.append(shot)
         cells += W.cell_updates(w)
     return shots, keep, cells
-
-
-def ctypes_alloc(ctx, nbytes):
-    import ctypes
-    p = ctypes.c_void_p()
-    _capi.check(_capi.lib().eio_device_alloc(ctx.h, max(int(nbytes), 4), ctypes.byref(p)))
-    return p.value
 
 
 def cpu_baseline(shots_w, target_s=12.0):
@@ -105,7 +97,10 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="shrink every size (quick runs)")
     ap.add_argument("--nshots", type=int, default=64)
     ap.add_argument("--graph", type=int, default=0)
+    ap.add_argument("--variant", type=int, default=-1, help="FP32 step-kernel variant (-1: default)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--nts-cap", type=int, default=0, help="cap every trace length (profiling runs)")
+    ap.add_argument("--copy-probe", type=int, default=1, help="measure a plain copy kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box of this build has a single MI355X; "
@@ -117,6 +112,11 @@ def main():
     if args.workload == "c2":
         shots_w = W.layered_batch_c2(args.nshots, seed=2, scale=args.scale)
         wl_name = "layered_batch_c2_%dshots" % args.nshots
+        if args.nts_cap:
+            for w in shots_w:
+                w["nts"] = min(w["nts"], args.nts_cap)
+                w["model"] = np.ascontiguousarray(w["model"][:, :, :w["nts"]])
+            wl_name += "_nts%d" % args.nts_cap
     else:
         shots_w = [W.gaussian_c0()]
         wl_name = shots_w[0]["name"]
@@ -124,6 +124,8 @@ def main():
 
     ctx = _capi.Context(0, np.float32)
     ctx.set_graph(args.graph)
+    if args.variant >= 0:
+        ctx.set_variant(args.variant)
     shots, keep, cells = prepare_shots(ctx, shots_w)
     log("uploaded %d shots, %.3e cell-updates per step" % (len(shots), cells))
 
@@ -163,7 +165,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": wl_name, "shots": len(shots), "cell_updates_per_step": cells,
-                   "scale": args.scale, "graph": args.graph},
+                   "scale": args.scale, "graph": args.graph, "variant": args.variant},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_step_fwd<float,4,M_NL>",
@@ -171,6 +173,11 @@ def main():
                      "launches_per_step": int(nlaunch[-1]),
                      "bytes_per_cell_update": BYTES_PER_CELL["nl_fwd"]},
     }
+    if args.copy_probe:
+        nb = 2 << 30
+        cms = ctx.copy_probe(nb, 10)
+        res["roofline"]["copy_probe_GBs"] = round(2 * nb / (cms / 1e3) / 1e9, 1)
+        log("copy probe: %.3f ms for 2 x %d bytes -> %.0f GB/s" % (cms, nb, 2 * nb / cms / 1e6))
     if not args.no_cpu:
         t0 = time.perf_counter()
         res["cpu_baseline"] = cpu_baseline(shots_w)

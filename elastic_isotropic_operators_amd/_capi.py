@@ -66,6 +66,12 @@ def lib():
             ("eio_device_free", [vp, vp]),
             ("eio_memcpy", [vp, vp, vp, ctypes.c_ulonglong, ctypes.c_int]),
             ("eio_synchronize", [vp]),
+            ("eio_copy_probe", [vp, vp, vp, ctypes.c_ulonglong, ctypes.c_int,
+                                ctypes.POINTER(ctypes.c_double)]),
+            ("eio_ctx_set_variant", [vp, ctypes.c_int]),
+            ("eio_ctx_variant_info", [ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                      ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+            ("eio_ctx_num_variants", []),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -78,7 +84,8 @@ EXPORTED = ("eio_last_error", "eio_ctx_create", "eio_ctx_destroy", "eio_ctx_set_
             "eio_ctx_set_graph", "eio_ctx_last_timing", "eio_model_add", "eio_model_set_media",
             "eio_model_remove", "eio_nl_forward", "eio_nl_adjoint", "eio_born_forward",
             "eio_born_adjoint", "eio_device_alloc", "eio_device_free", "eio_memcpy",
-            "eio_synchronize")
+            "eio_synchronize", "eio_copy_probe", "eio_ctx_set_variant", "eio_ctx_variant_info",
+            "eio_ctx_num_variants")
 
 
 def check(rc):
@@ -112,8 +119,50 @@ class Context:
         except Exception:
             pass
 
+    def alloc(self, nbytes):
+        p = ctypes.c_void_p()
+        check(lib().eio_device_alloc(self.h, max(int(nbytes), 16), ctypes.byref(p)))
+        return p.value
+
+    def free(self, ptr):
+        check(lib().eio_device_free(self.h, ctypes.c_void_p(ptr)))
+
+    def h2d(self, dptr, arr):
+        arr = np.ascontiguousarray(arr)
+        check(lib().eio_memcpy(self.h, ctypes.c_void_p(dptr), ctypes.c_void_p(arr.ctypes.data),
+                               arr.nbytes, 1))
+
+    def d2h(self, arr, dptr):
+        check(lib().eio_memcpy(self.h, ctypes.c_void_p(arr.ctypes.data), ctypes.c_void_p(dptr),
+                               arr.nbytes, 2))
+
+    def copy_probe(self, nbytes, reps=10):
+        """Mean time [ms] of a plain device copy of nbytes (two buffers allocated here)."""
+        a, b = self.alloc(nbytes), self.alloc(nbytes)
+        ms = ctypes.c_double()
+        try:
+            check(lib().eio_copy_probe(self.h, ctypes.c_void_p(b), ctypes.c_void_p(a), int(nbytes),
+                                       int(reps), ctypes.byref(ms)))
+        finally:
+            self.free(a)
+            self.free(b)
+        return ms.value
+
     def set_stream(self, stream_handle):
         check(lib().eio_ctx_set_stream(self.h, ctypes.c_void_p(stream_handle)))
+
+    def set_variant(self, v):
+        check(lib().eio_ctx_set_variant(self.h, int(v)))
+
+    @staticmethod
+    def variants():
+        """[(R, WX, occ)] of the FP32 step-kernel variants."""
+        out = []
+        for v in range(lib().eio_ctx_num_variants()):
+            r, wx, oc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+            check(lib().eio_ctx_variant_info(v, ctypes.byref(r), ctypes.byref(wx), ctypes.byref(oc)))
+            out.append((r.value, wx.value, oc.value))
+        return out
 
     def set_graph(self, on):
         check(lib().eio_ctx_set_graph(self.h, int(bool(on))))

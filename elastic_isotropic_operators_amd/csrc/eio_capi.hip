@@ -49,8 +49,21 @@ constexpr int GRID_OF_COMP[5] = {1, 2, 0, 0, 3};
 constexpr int PAD_MAX = 500;   // varDeclare.h:9
 constexpr int SUB_MAX = 100;   // varDeclare.h:10
 
+// Step-kernel variants: R x-cells per lane, WX waves per workgroup (tile = 64 x WX*R),
+// OCC requested waves per SIMD.  FP32 has a table the autotuner chooses from; FP64 (dot
+// tests and oracle comparisons only) has one variant.
+struct Variant {
+    int R, WX, OCC;
+};
+constexpr Variant F32_VARIANTS[] = {{4, 4, 2}, {2, 8, 4}, {2, 4, 4}, {4, 8, 2}, {8, 4, 1}, {2, 8, 2}};
+constexpr int N_F32_VARIANTS = int(sizeof(F32_VARIANTS) / sizeof(F32_VARIANTS[0]));
+constexpr Variant F64_VARIANT = {2, 4, 1};
+
 template <typename T>
-constexpr int tile_r() { return sizeof(T) == 4 ? 4 : 2; }
+int tile_x(int variant) {
+    if (sizeof(T) == 8) return F64_VARIANT.R * F64_VARIANT.WX;
+    return F32_VARIANTS[variant].R * F32_VARIANTS[variant].WX;
+}
 
 struct DevBuf {
     void* p = nullptr;
@@ -79,6 +92,7 @@ struct Ctx {
     double last_total_ms = 0.0, last_step_ms = 0.0;
     long long last_nstep = 0;
     int use_graph = 0;
+    int variant = 1;  // best measured FP32 variant on the 64-shot layered batch
 };
 
 void* ws_get(Ctx* c, const std::string& name, size_t bytes) {
@@ -223,7 +237,7 @@ void build_entries(const ModelBase& m, const int* const pos[4], const int nreg[4
 
 template <typename T>
 Plan make_plan(Ctx* c, int nshot, const eio_shot* shots, Op op) {
-    constexpr int TX = 4 * tile_r<T>();
+    const int TX = tile_x<T>(c->variant);
     Plan p;
     p.order.resize(nshot);
     std::iota(p.order.begin(), p.order.end(), 0);
@@ -332,11 +346,9 @@ Run<T> prepare(Ctx* c, int nshot, const eio_shot* shots, Op op, int flags,
         d.nz = m.g.nz; d.nx = m.g.nx; d.nts = m.g.nts; d.sub = m.g.sub_fd; d.h = m.h;
         d.ntw = m.ntw; d.minPad = m.g.minPad;
         for (int k = 0; k < 4; ++k) { d.cz[k] = T(m.cz[k]); d.cx[k] = T(m.cx[k]); }
-        for (int f = 0; f < 5; ++f) {
-            d.fa[f] = state + so + size_t(f) * m.n;
-            d.fb[f] = state + so + size_t(5 + f) * m.n;
-            d.med[f] = static_cast<const T*>(m.med) + size_t(f) * m.n;
-        }
+        d.A = state + so;
+        d.B = state + so + 5 * size_t(m.n);
+        d.med = static_cast<const T*>(m.med);
         so += 10 * size_t(m.n);
         d.damp = static_cast<const T*>(m.damp);
         d.filt = static_cast<const T*>(m.filt);
@@ -373,19 +385,17 @@ Run<T> prepare(Ctx* c, int nshot, const eio_shot* shots, Op op, int flags,
             }
         }
         if (born) {
-            for (int f = 0; f < 5; ++f) {
-                d.ss[0][f] = ssbuf + sso + size_t(f) * m.n;
-                d.ss[1][f] = ssbuf + sso + size_t(5 + f) * m.n;
-            }
+            d.ss[0] = ssbuf + sso;
+            d.ss[1] = ssbuf + sso + 5 * size_t(m.n);
             sso += 10 * size_t(m.n);
             d.Wvx = wbuf + wo;
             d.Wvz = wbuf + wo + size_t(m.n) * m.g.nts;
             wo += 2 * size_t(m.n) * m.g.nts;
             if (op == Op::BORN_ADJ) {
-                for (int f = 0; f < 5; ++f) d.dm[f] = accbuf + ao + size_t(f) * m.n;
+                d.dm = accbuf + ao;
                 ao += 5 * size_t(m.n);
             } else {
-                for (int f = 0; f < 5; ++f) d.dm[f] = const_cast<T*>(static_cast<const T*>(dm_fwd[f]));
+                d.dm = const_cast<T*>(static_cast<const T*>(dm_fwd[0]));
             }
         }
     }
@@ -421,23 +431,54 @@ inline int active_tiles(const Plan& p, int k) {
     return n;
 }
 
+#define EIO_F32_VARIANT_SWITCH(V, CALL)                                                  \
+    switch (V) {                                                                          \
+        case 0: CALL(4, 4, 2); break;                                                     \
+        case 1: CALL(2, 8, 4); break;                                                     \
+        case 2: CALL(2, 4, 4); break;                                                     \
+        case 3: CALL(4, 8, 2); break;                                                     \
+        case 4: CALL(8, 4, 1); break;                                                     \
+        default: CALL(2, 8, 2); break;                                                    \
+    }
+
 template <typename T, int MODE>
 void launch_fwd(Run<T>& run, int ntile, int k, int sspar) {
-    constexpr int R = tile_r<T>();
-    hipLaunchKernelGGL((eio::k_step_fwd<T, R, MODE>), dim3(ntile), dim3(eio::NTHR), 0, run.c->stream,
-                       run.d_shots, run.d_tiles, run.d_ents, ntile, k, sspar);
+    hipStream_t s = run.c->stream;
+#define EIO_L(R_, WX_, OCC_)                                                                 \
+    hipLaunchKernelGGL((eio::k_step_fwd<T, R_, WX_, OCC_, MODE>), dim3(ntile), dim3(64 * WX_), 0, \
+                       s, run.d_shots, run.d_tiles, run.d_ents, ntile, k, sspar)
+    if constexpr (sizeof(T) == 8) {
+        EIO_L(2, 4, 1);
+    } else {
+        EIO_F32_VARIANT_SWITCH(run.c->variant, EIO_L)
+    }
+#undef EIO_L
 }
 template <typename T, int MODE>
 void launch_adj(Run<T>& run, int ntile, int k, int sspar) {
-    constexpr int R = tile_r<T>();
-    hipLaunchKernelGGL((eio::k_step_adj<T, R, MODE>), dim3(ntile), dim3(eio::NTHR), 0, run.c->stream,
-                       run.d_shots, run.d_tiles, run.d_ents, ntile, k, sspar);
+    hipStream_t s = run.c->stream;
+#define EIO_L(R_, WX_, OCC_)                                                                 \
+    hipLaunchKernelGGL((eio::k_step_adj<T, R_, WX_, OCC_, MODE>), dim3(ntile), dim3(64 * WX_), 0, \
+                       s, run.d_shots, run.d_tiles, run.d_ents, ntile, k, sspar)
+    if constexpr (sizeof(T) == 8) {
+        EIO_L(2, 4, 1);
+    } else {
+        EIO_F32_VARIANT_SWITCH(run.c->variant, EIO_L)
+    }
+#undef EIO_L
 }
 template <typename T, bool ADJ>
 void launch_img(Run<T>& run, int ntile, int its, int which) {
-    constexpr int R = tile_r<T>();
-    hipLaunchKernelGGL((eio::k_imaging<T, R, ADJ>), dim3(ntile), dim3(eio::NTHR), 0, run.c->stream,
-                       run.d_shots, run.d_tiles, ntile, its, which);
+    hipStream_t s = run.c->stream;
+#define EIO_L(R_, WX_, OCC_)                                                                 \
+    hipLaunchKernelGGL((eio::k_imaging<T, R_, WX_, ADJ>), dim3(ntile), dim3(64 * WX_), 0, s,   \
+                       run.d_shots, run.d_tiles, ntile, its, which)
+    if constexpr (sizeof(T) == 8) {
+        EIO_L(2, 4, 1);
+    } else {
+        EIO_F32_VARIANT_SWITCH(run.c->variant, EIO_L)
+    }
+#undef EIO_L
 }
 
 // Runs `body` (a sequence of launches) either directly or captured into one hipGraph.
@@ -516,18 +557,15 @@ template <typename T>
 void born_forward(Ctx* c, int nshot, const eio_shot* shots, const void* const dm[5], int flags) {
     if (!dm) fail("dm is NULL");
     const ModelBase& m = get_model(c, shots[0].model);
-    std::vector<T*> dmdev(5);
+    // the kernels address dm as one contiguous 5-field buffer
     const size_t nb = size_t(m.n) * sizeof(T);
-    if (flags & EIO_DEVICE_PTRS) {
-        for (int f = 0; f < 5; ++f) dmdev[f] = const_cast<T*>(static_cast<const T*>(dm[f]));
-    } else {
-        T* b = static_cast<T*>(ws_get(c, "dm", 5 * nb));
-        for (int f = 0; f < 5; ++f) {
-            dmdev[f] = b + size_t(f) * m.n;
-            EIO_CHECK_HIP(hipMemcpyAsync(dmdev[f], dm[f], nb, hipMemcpyHostToDevice, c->stream));
-        }
+    T* b = static_cast<T*>(ws_get(c, "dm", 5 * nb));
+    const hipMemcpyKind kind = (flags & EIO_DEVICE_PTRS) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    for (int f = 0; f < 5; ++f) {
+        if (!dm[f]) fail("dm pointer is NULL");
+        EIO_CHECK_HIP(hipMemcpyAsync(b + size_t(f) * m.n, dm[f], nb, kind, c->stream));
     }
-    const void* dmp[5] = {dmdev[0], dmdev[1], dmdev[2], dmdev[3], dmdev[4]};
+    const void* dmp[5] = {b, b + m.n, b + 2 * m.n, b + 3 * m.n, b + 4 * m.n};
     Run<T> run = prepare<T>(c, nshot, shots, Op::BORN_FWD, flags, dmp);
     const int ntile = int(run.plan.tiles.size());
     const int kmax = m.ntw - 1, sub = m.g.sub_fd, nts = m.g.nts;
@@ -537,11 +575,10 @@ void born_forward(Ctx* c, int nshot, const eio_shot* shots, const void* const dm
     run_loop(c, [&]() {
         born_source_pass(run, ntile, kmax);
         // scattered pass restarts from a zero state (BornElasticGpuFunctions.cu:773-774)
-        for (int oi = 0; oi < nshot; ++oi)
-            for (int f = 0; f < 5; ++f) {
-                fill0<T>(c, run.sd[oi].fa[f], m.n);
-                fill0<T>(c, run.sd[oi].fb[f], m.n);
-            }
+        for (int oi = 0; oi < nshot; ++oi) {
+            fill0<T>(c, run.sd[oi].A, 5 * m.n);
+            fill0<T>(c, run.sd[oi].B, 5 * m.n);
+        }
         int p = 0;
         launch_img<T, false>(run, ntile, 0, p);
         for (int its = 0; its < nts - 1; ++its) {
@@ -568,11 +605,10 @@ void born_adjoint(Ctx* c, int nshot, const eio_shot* shots, void* const dm[5], i
     EIO_CHECK_HIP(hipEventRecord(c->ev[2], c->stream));
     run_loop(c, [&]() {
         born_source_pass(run, ntile, kmax);
-        for (int oi = 0; oi < nshot; ++oi)
-            for (int f = 0; f < 5; ++f) {
-                fill0<T>(c, run.sd[oi].fa[f], m.n);
-                fill0<T>(c, run.sd[oi].fb[f], m.n);
-            }
+        for (int oi = 0; oi < nshot; ++oi) {
+            fill0<T>(c, run.sd[oi].A, 5 * m.n);
+            fill0<T>(c, run.sd[oi].B, 5 * m.n);
+        }
         int p = 0;
         for (int its = nts - 2; its >= 0; --its) {
             for (int it2 = sub - 1; it2 >= 0; --it2)
@@ -589,7 +625,7 @@ void born_adjoint(Ctx* c, int nshot, const eio_shot* shots, void* const dm[5], i
     T* out = static_cast<T*>(ws_get(c, "dmout", 5 * nb));
     std::vector<const T*> accp(size_t(nshot) * 5);
     for (int oi = 0; oi < nshot; ++oi)
-        for (int f = 0; f < 5; ++f) accp[size_t(f) * nshot + oi] = run.sd[oi].dm[f];
+        for (int f = 0; f < 5; ++f) accp[size_t(f) * nshot + oi] = run.sd[oi].dm + size_t(f) * m.n;
     const T** d_accp = static_cast<const T**>(ws_get(c, "accp", accp.size() * sizeof(T*)));
     EIO_CHECK_HIP(hipMemcpyAsync(d_accp, accp.data(), accp.size() * sizeof(T*), hipMemcpyHostToDevice, c->stream));
     for (int f = 0; f < 5; ++f) {
@@ -674,6 +710,24 @@ int eio_ctx_set_stream(eio_ctx* ctx, void* s) {
 int eio_ctx_set_graph(eio_ctx* ctx, int use_graph) {
     return guarded([&]() { as_ctx(ctx)->use_graph = use_graph ? 1 : 0; });
 }
+
+int eio_ctx_set_variant(eio_ctx* ctx, int variant) {
+    return guarded([&]() {
+        if (variant < 0 || variant >= N_F32_VARIANTS) fail("variant out of range");
+        as_ctx(ctx)->variant = variant;
+    });
+}
+
+int eio_ctx_variant_info(int variant, int* R, int* WX, int* occ) {
+    return guarded([&]() {
+        if (variant < 0 || variant >= N_F32_VARIANTS) fail("variant out of range");
+        if (R) *R = F32_VARIANTS[variant].R;
+        if (WX) *WX = F32_VARIANTS[variant].WX;
+        if (occ) *occ = F32_VARIANTS[variant].OCC;
+    });
+}
+
+int eio_ctx_num_variants(void) { return N_F32_VARIANTS; }
 
 int eio_ctx_last_timing(eio_ctx* ctx, double* total_ms, long long* n_step, double* step_ms) {
     return guarded([&]() {
@@ -786,6 +840,27 @@ int eio_memcpy(eio_ctx* ctx, void* dst, const void* src, unsigned long long byte
         else fail("invalid copy kind");
         EIO_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, k, c->stream));
         EIO_CHECK_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int eio_copy_probe(eio_ctx* ctx, void* dst, const void* src, unsigned long long bytes, int reps,
+                   double* ms) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        if (!dst || !src || bytes % 16 || reps < 1) fail("invalid copy probe arguments");
+        DeviceGuard dg(c->device);
+        const long n16 = long(bytes / 16);
+        const unsigned nb = unsigned(std::min<long>((n16 + eio::NTHR - 1) / eio::NTHR, 256L * 64));
+        EIO_CHECK_HIP(hipEventRecord(c->ev[0], c->stream));
+        for (int r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(eio::k_copy16, dim3(nb), dim3(eio::NTHR), 0, c->stream,
+                               static_cast<float4*>(dst), static_cast<const float4*>(src), n16);
+        EIO_CHECK_HIP(hipGetLastError());
+        EIO_CHECK_HIP(hipEventRecord(c->ev[1], c->stream));
+        EIO_CHECK_HIP(hipEventSynchronize(c->ev[1]));
+        float t = 0.f;
+        EIO_CHECK_HIP(hipEventElapsedTime(&t, c->ev[0], c->ev[1]));
+        if (ms) *ms = double(t) / reps;
     });
 }
 

@@ -10,12 +10,19 @@
 //                          extraction (:1369-1376)]
 //   imaging kernels      = Born secondary source (:1381-1459) and imaging condition (:1538-1613)
 //
-// Work decomposition: a tile is 64 (z, one wave64 lane per z) x 4R (x) update-region cells;
-// a 256-thread workgroup (4 waves) owns one tile and each lane computes R consecutive x
-// cells.  The current time level is staged through LDS with a 4-cell halo so every
-// staggered first derivative is read from LDS; x derivatives reuse a register window of
-// R+8 values.  Old level and media are streamed once from HBM, the new level is written
-// once (80 B per FP32 cell-update for the forward and adjoint steps).
+// Work decomposition: a tile is 64 (z, one wave64 lane per z) x WX*R (x) update-region
+// cells; a workgroup of WX waves owns one tile and each lane computes R consecutive x cells.
+// The current time level is staged through LDS with a 4-cell halo so every staggered first
+// derivative is read from LDS; x derivatives reuse a register window of R+8 values.  Old
+// level and media are streamed once from HBM and the new level is written once (80 B per
+// FP32 cell-update for the forward and adjoint steps).
+//
+// Memory access: each shot's state levels, media and Born slices are contiguous 5-field
+// buffers addressed through SGPR buffer descriptors.  A lane keeps one 32-bit byte offset;
+// the field and column offsets are scalar, so the per-load address cost is nil, and halo
+// reads past the end of a buffer return 0 in hardware (reads that run past the end of a
+// column land in a neighbouring column: those values only feed lanes whose results are
+// discarded).
 //
 // Sponge fusion.  The reference damps BOTH time levels after every step
 // (p0 <- d*(p0 + A p1 + s), p1 <- d*p1, then swaps).  The damped p1 is only ever used as
@@ -25,12 +32,16 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#ifndef EIO_PREFETCH
+#define EIO_PREFETCH 1
+#endif
+
 namespace eio {
 
 constexpr int FAT = 4;
 constexpr int TZ = 64;        // z extent of a tile = wave64 lanes
-constexpr int NTHR = 256;     // 4 waves per workgroup
 constexpr int ZH = TZ + 2 * FAT;
+constexpr int NTHR = 256;     // threads of the element-wise helper kernels
 
 // Modes of the fused forward kernel
 constexpr int M_NL = 0;       // NL forward: coarse source injection with time interpolation, record
@@ -44,17 +55,17 @@ template <typename T>
 struct ShotDev {
     int nz, nx, nts, sub, h, ntw, minPad, pad0;
     T cz[4], cx[4];
-    T* fa[5];              // state buffer A (older level at even steps)
-    T* fb[5];              // state buffer B
-    const T* med[5];       // rhoxDtw, rhozDtw, lamb2MuDtw, lambDtw, muxzDtw
+    T* A;                  // state level A, 5 fields of nz*nx (older level at even steps)
+    T* B;                  // state level B
+    const T* med;          // rhoxDtw, rhozDtw, lamb2MuDtw, lambDtw, muxzDtw (5 fields)
     const T* damp;         // [minPad] cosine sponge
-    const T* filt;         // [2h] time interpolation filter
+    const T* filt;         // [2h] time interpolation filter, [2h] = 1/(2 dts)
     T* src_tr[5];          // source traces (NL: [n][nts], Born: [n][ntw])
     T* rec_tr[5];          // receiver traces [n][nts]
-    T* ss[2][5];           // Born secondary sources / adjoint fields, ping-pong
+    T* ss[2];              // Born secondary sources / adjoint fields, 5 fields each, ping-pong
     T* Wvx;                // Born coarse storage [nts][nz*nx]
     T* Wvz;
-    T* dm[5];              // Born perturbation (forward) or per-shot imaging accumulator (adjoint)
+    T* dm;                 // Born perturbation (forward) or per-shot imaging accumulator (adjoint)
 };
 
 struct TileDev {
@@ -68,6 +79,30 @@ struct Entry {
     int lcell;
     int crow;
 };
+
+// ------------------------------------------------------------------ buffer access
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void* p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, int(bytes), 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T bld(rsrc_t r, unsigned voff, unsigned soff);
+template <>
+__device__ __forceinline__ float bld<float>(rsrc_t r, unsigned voff, unsigned soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, int(voff), int(soff), 0));
+}
+template <>
+__device__ __forceinline__ double bld<double>(rsrc_t r, unsigned voff, unsigned soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, int(voff), int(soff), 0));
+}
+__device__ __forceinline__ void bst(rsrc_t r, float v, unsigned voff, unsigned soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, int(voff), int(soff), 0);
+}
+__device__ __forceinline__ void bst(rsrc_t r, double v, unsigned voff, unsigned soff) {
+    using u64 = decltype(__builtin_amdgcn_raw_buffer_load_b64(rsrc_t(), 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u64, v), r, int(voff), int(soff), 0);
+}
 
 // Bijective block -> tile map that gives each of the 8 XCDs a contiguous range of tiles
 // (workgroups are dispatched round-robin over XCDs), so neighbouring tiles share an L2.
@@ -113,40 +148,75 @@ __device__ __forceinline__ StepTime adj_time(int k, int sub, int nts) {
     return t;
 }
 
+
 // ------------------------------------------------------------------ forward step
-template <typename T, int R, int MODE>
-__global__ __launch_bounds__(NTHR) void k_step_fwd(const ShotDev<T>* __restrict__ shots,
-                                                   const TileDev* __restrict__ tiles,
-                                                   const Entry* __restrict__ ents, int ntiles,
-                                                   int k, int sspar) {
-    constexpr int TX = 4 * R;
+// OCC = requested waves per SIMD (bounds the VGPR budget: 512/OCC).
+template <typename T, int R, int WX, int OCC, int MODE>
+__global__ __launch_bounds__(64 * WX) __attribute__((amdgpu_waves_per_eu(OCC)))
+void k_step_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict__ tiles,
+                const Entry* __restrict__ ents, int ntiles, int k, int sspar) {
+    constexpr int NT = 64 * WX;
+    constexpr int TX = WX * R;
     constexpr int TXH = TX + 2 * FAT;
+    constexpr unsigned SZ = sizeof(T);
     __shared__ T lds[5][TXH][ZH];
 
     const TileDev t = tiles[xcd_remap(blockIdx.x, ntiles)];
     const ShotDev<T>& S = shots[t.shot];
     const int nz = S.nz, nx = S.nx;
+    const unsigned n = unsigned(nz) * unsigned(nx);
+    const unsigned fbytes = n * SZ, colb = unsigned(nz) * SZ;
     const StepTime st = fwd_time(k, S.sub);
-    T* const* oldp = (k & 1) ? S.fb : S.fa;
-    T* const* curp = (k & 1) ? S.fa : S.fb;
-
-    // stage the current level with a 4-cell halo (cells beyond the array read as 0)
-    for (int e = threadIdx.x; e < TXH * ZH; e += NTHR) {
-        const int c = e / ZH, zz = e - c * ZH;
-        const int gx = t.x0 - FAT + c, gz = t.z0 - FAT + zz;
-        const bool ok = (gx < nx) && (gz < nz);
-        const long gi = long(gx) * nz + gz;
-#pragma unroll
-        for (int f = 0; f < 5; ++f) lds[f][c][zz] = ok ? curp[f][gi] : T(0);
-    }
-    __syncthreads();
+    T* oldp = (k & 1) ? S.B : S.A;
+    const T* curp = (k & 1) ? S.A : S.B;
+    const rsrc_t rold = make_rsrc(oldp, 5 * fbytes);
+    const rsrc_t rcur = make_rsrc(curp, 5 * fbytes);
+    const rsrc_t rmed = make_rsrc(S.med, 5 * fbytes);
 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int iz = t.z0 + lane, zz = lane + FAT;
     const int c0 = FAT + w * R, ix0 = t.x0 + w * R;
+    const unsigned voff = (unsigned(ix0) * unsigned(nz) + unsigned(iz)) * SZ;
+
+    // older level (and media, EIO_PREFETCH=2) of the own cells: issued before the halo stage
+    T o[5][R], m[5][R];
+#if EIO_PREFETCH >= 1
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int f = 0; f < 5; ++f) o[f][r] = bld<T>(rold, voff, unsigned(r) * colb + unsigned(f) * fbytes);
+#endif
+#if EIO_PREFETCH >= 2
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int f = 0; f < 5; ++f) m[f][r] = bld<T>(rmed, voff, unsigned(r) * colb + unsigned(f) * fbytes);
+#endif
+
+    // stage the current level with a 4-cell halo
+    const unsigned hbase = (unsigned(t.x0 - FAT) * unsigned(nz) + unsigned(t.z0 - FAT)) * SZ;
+    for (int e = threadIdx.x; e < TXH * ZH; e += NT) {
+        const int c = e / ZH, z = e - c * ZH;
+        const unsigned hv = hbase + unsigned(c) * colb + unsigned(z) * SZ;
+#pragma unroll
+        for (int f = 0; f < 5; ++f) lds[f][c][z] = bld<T>(rcur, hv, unsigned(f) * fbytes);
+    }
+    __syncthreads();
+
+#if EIO_PREFETCH < 1
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int f = 0; f < 5; ++f) o[f][r] = bld<T>(rold, voff, unsigned(r) * colb + unsigned(f) * fbytes);
+#endif
+#if EIO_PREFETCH < 2
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int f = 0; f < 5; ++f) m[f][r] = bld<T>(rmed, voff, unsigned(r) * colb + unsigned(f) * fbytes);
+#endif
     const T* cz = S.cz;
     const T* cx = S.cx;
-
     T dx_sxx[R], dx_sxz[R], dx_vx[R], dx_vz[R];
     {
         T q[R + 8];
@@ -168,45 +238,33 @@ __global__ __launch_bounds__(NTHR) void k_step_fwd(const ShotDev<T>* __restrict_
         for (int r = 0; r < R; ++r) dx_vz[r] = dm_q(q, r + 4, cx);
     }
 
-    T val[5][R];
     const bool zok = iz < nz - FAT;
+    T val[5][R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int ix = ix0 + r;
         const bool ok = zok && (ix < nx - FAT);
-        const long i = long(ix) * nz + iz;
         const int c = c0 + r;
-        const T* lsxz = &lds[4][c][zz];
-        const T* lszz = &lds[3][c][zz];
-        const T* lvz = &lds[1][c][zz];
-        const T* lvx = &lds[0][c][zz];
-        const T dz_sxz = dp_q(lsxz, 0, cz);
-        const T dz_szz = dm_q(lszz, 0, cz);
-        const T dz_vz = dp_q(lvz, 0, cz);
-        const T dz_vx = dm_q(lvx, 0, cz);
-        T o[5], m[5];
-        if (ok) {
-#pragma unroll
-            for (int f = 0; f < 5; ++f) { o[f] = oldp[f][i]; m[f] = S.med[f][i]; }
-        } else {
-#pragma unroll
-            for (int f = 0; f < 5; ++f) { o[f] = T(0); m[f] = T(0); }
-        }
+        const T dz_sxz = dp_q(&lds[4][c][zz], 0, cz);
+        const T dz_szz = dm_q(&lds[3][c][zz], 0, cz);
+        const T dz_vz = dp_q(&lds[1][c][zz], 0, cz);
+        const T dz_vx = dm_q(&lds[0][c][zz], 0, cz);
         const T d = ok ? damp_at(S, iz, ix) : T(1);
-        T a0 = d * o[0] + m[0] * (dx_sxx[r] + dz_sxz);
-        T a1 = d * o[1] + m[1] * (dz_szz + dx_sxz[r]);
-        T a2 = d * o[2] + (m[2] * dx_vx[r] + m[3] * dz_vz);
-        T a3 = d * o[3] + (m[3] * dx_vx[r] + m[2] * dz_vz);
-        T a4 = d * o[4] + m[4] * (dz_vx + dx_vz[r]);
-        if (MODE == M_BSCAT && ok) {  // secondary source sampled at it2-1 (kernelsGpuElastic.cu:1363-1368)
+        T a0 = d * o[0][r] + m[0][r] * (dx_sxx[r] + dz_sxz);
+        T a1 = d * o[1][r] + m[1][r] * (dz_szz + dx_sxz[r]);
+        T a2 = d * o[2][r] + (m[2][r] * dx_vx[r] + m[3][r] * dz_vz);
+        T a3 = d * o[3][r] + (m[3][r] * dx_vx[r] + m[2][r] * dz_vz);
+        T a4 = d * o[4][r] + m[4][r] * (dz_vx + dx_vz[r]);
+        if (MODE == M_BSCAT) {  // secondary source sampled at it2-1 (kernelsGpuElastic.cu:1363-1368)
             const T fl = S.filt[st.it2 - 1], fr = S.filt[S.h + st.it2 - 1];
-            T* const* sl = S.ss[sspar];
-            T* const* sr = S.ss[sspar ^ 1];
-            a0 += sl[0][i] * fl + sr[0][i] * fr;
-            a1 += sl[1][i] * fl + sr[1][i] * fr;
-            a2 += sl[2][i] * fl + sr[2][i] * fr;
-            a3 += sl[3][i] * fl + sr[3][i] * fr;
-            a4 += sl[4][i] * fl + sr[4][i] * fr;
+            const rsrc_t rl = make_rsrc(S.ss[sspar], 5 * fbytes);
+            const rsrc_t rr = make_rsrc(S.ss[sspar ^ 1], 5 * fbytes);
+            const unsigned so = unsigned(r) * colb;
+            a0 += bld<T>(rl, voff, so) * fl + bld<T>(rr, voff, so) * fr;
+            a1 += bld<T>(rl, voff, so + fbytes) * fl + bld<T>(rr, voff, so + fbytes) * fr;
+            a2 += bld<T>(rl, voff, so + 2 * fbytes) * fl + bld<T>(rr, voff, so + 2 * fbytes) * fr;
+            a3 += bld<T>(rl, voff, so + 3 * fbytes) * fl + bld<T>(rr, voff, so + 3 * fbytes) * fr;
+            a4 += bld<T>(rl, voff, so + 4 * fbytes) * fl + bld<T>(rr, voff, so + 4 * fbytes) * fr;
         }
         val[0][r] = d * a0;
         val[1][r] = d * a1;
@@ -225,7 +283,7 @@ __global__ __launch_bounds__(NTHR) void k_step_fwd(const ShotDev<T>* __restrict_
             for (int f = 0; f < 5; ++f) lds[f][c0 + r][zz] = val[f][r];
         __syncthreads();
         if (has_src) {
-            for (int e = t.src_b + threadIdx.x; e < t.src_e; e += NTHR) {
+            for (int e = t.src_b + threadIdx.x; e < t.src_e; e += NT) {
                 const Entry en = ents[e];
                 const int lz = en.lcell & 255, lx = en.lcell >> 8;
                 const int comp = en.crow >> 24, row = en.crow & 0xffffff;
@@ -247,7 +305,7 @@ __global__ __launch_bounds__(NTHR) void k_step_fwd(const ShotDev<T>* __restrict_
         }
         if (has_rec) {  // kernelsGpuElastic.cu:7-60
             const T fa = S.filt[st.it2], fb = S.filt[S.h + st.it2];
-            for (int e = t.rec_b + threadIdx.x; e < t.rec_e; e += NTHR) {
+            for (int e = t.rec_b + threadIdx.x; e < t.rec_e; e += NT) {
                 const Entry en = ents[e];
                 const int lz = en.lcell & 255, lx = en.lcell >> 8;
                 const int comp = en.crow >> 24, row = en.crow & 0xffffff;
@@ -259,22 +317,25 @@ __global__ __launch_bounds__(NTHR) void k_step_fwd(const ShotDev<T>* __restrict_
         }
     }
 
+    rsrc_t rwx, rwz;
+    T wfa = T(0), wfb = T(0);
+    if (MODE == M_BSRC) {  // coarse slices its and its+1 (kernelsGpuElastic.cu:1337-1346)
+        rwx = make_rsrc(S.Wvx + size_t(st.its) * n, 2 * fbytes);
+        rwz = make_rsrc(S.Wvz + size_t(st.its) * n, 2 * fbytes);
+        wfa = S.filt[st.it2];
+        wfb = S.filt[S.h + st.it2];
+    }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int ix = ix0 + r;
-        if (zok && ix < nx - FAT) {
-            const long i = long(ix) * nz + iz;
+        if (zok && ix0 + r < nx - FAT) {
+            const unsigned so = unsigned(r) * colb;
 #pragma unroll
-            for (int f = 0; f < 5; ++f) oldp[f][i] = val[f][r];
-            if (MODE == M_BSRC) {  // kernelsGpuElastic.cu:1337-1346
-                const long n = long(nz) * nx;
-                const T fa = S.filt[st.it2], fb = S.filt[S.h + st.it2];
-                T* wx = S.Wvx + long(st.its) * n + i;
-                T* wz = S.Wvz + long(st.its) * n + i;
-                wx[0] += val[0][r] * fa;
-                wx[n] += val[0][r] * fb;
-                wz[0] += val[1][r] * fa;
-                wz[n] += val[1][r] * fb;
+            for (int f = 0; f < 5; ++f) bst(rold, val[f][r], voff, so + unsigned(f) * fbytes);
+            if (MODE == M_BSRC) {
+                bst(rwx, bld<T>(rwx, voff, so) + val[0][r] * wfa, voff, so);
+                bst(rwx, bld<T>(rwx, voff, so + fbytes) + val[0][r] * wfb, voff, so + fbytes);
+                bst(rwz, bld<T>(rwz, voff, so) + val[1][r] * wfa, voff, so);
+                bst(rwz, bld<T>(rwz, voff, so + fbytes) + val[1][r] * wfb, voff, so + fbytes);
             }
         }
     }
@@ -285,51 +346,65 @@ __global__ __launch_bounds__(NTHR) void k_step_fwd(const ShotDev<T>* __restrict_
 // a = l2m*sxx + lam*szz, b = lam*sxx + l2m*szz, s6 = muxzDtw*sxz.  The reference applies
 // the x derivative to l2m*sxx and lam*szz separately and adds; by linearity one derivative
 // of their sum is the same operator.
-template <typename T, int R, int MODE>
-__global__ __launch_bounds__(NTHR) void k_step_adj(const ShotDev<T>* __restrict__ shots,
-                                                   const TileDev* __restrict__ tiles,
-                                                   const Entry* __restrict__ ents, int ntiles,
-                                                   int k, int sspar) {
-    constexpr int TX = 4 * R;
+template <typename T, int R, int WX, int OCC, int MODE>
+__global__ __launch_bounds__(64 * WX) __attribute__((amdgpu_waves_per_eu(OCC)))
+void k_step_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict__ tiles,
+                const Entry* __restrict__ ents, int ntiles, int k, int sspar) {
+    constexpr int NT = 64 * WX;
+    constexpr int TX = WX * R;
     constexpr int TXH = TX + 2 * FAT;
+    constexpr unsigned SZ = sizeof(T);
     __shared__ T lds[5][TXH][ZH];
 
     const TileDev t = tiles[xcd_remap(blockIdx.x, ntiles)];
     const ShotDev<T>& S = shots[t.shot];
     const int nz = S.nz, nx = S.nx;
+    const unsigned n = unsigned(nz) * unsigned(nx);
+    const unsigned fbytes = n * SZ, colb = unsigned(nz) * SZ;
     const StepTime st = adj_time(k, S.sub, S.nts);
-    T* const* oldp = (k & 1) ? S.fb : S.fa;
-    T* const* curp = (k & 1) ? S.fa : S.fb;
-
-    for (int e = threadIdx.x; e < TXH * ZH; e += NTHR) {
-        const int c = e / ZH, zz = e - c * ZH;
-        const int gx = t.x0 - FAT + c, gz = t.z0 - FAT + zz;
-        T s0 = T(0), s1 = T(0), sa = T(0), sb = T(0), s6 = T(0);
-        if ((gx < nx) && (gz < nz)) {
-            const long gi = long(gx) * nz + gz;
-            const T vx = curp[0][gi], vz = curp[1][gi], sxx = curp[2][gi], szz = curp[3][gi],
-                    sxz = curp[4][gi];
-            const T l2m = S.med[2][gi], lam = S.med[3][gi];
-            s0 = S.med[0][gi] * vx;
-            s1 = S.med[1][gi] * vz;
-            sa = l2m * sxx + lam * szz;
-            sb = lam * sxx + l2m * szz;
-            s6 = S.med[4][gi] * sxz;
-        }
-        lds[0][c][zz] = s0;
-        lds[1][c][zz] = s1;
-        lds[2][c][zz] = sa;
-        lds[3][c][zz] = sb;
-        lds[4][c][zz] = s6;
-    }
-    __syncthreads();
+    T* oldp = (k & 1) ? S.B : S.A;
+    const T* curp = (k & 1) ? S.A : S.B;
+    const rsrc_t rold = make_rsrc(oldp, 5 * fbytes);
+    const rsrc_t rcur = make_rsrc(curp, 5 * fbytes);
+    const rsrc_t rmed = make_rsrc(S.med, 5 * fbytes);
 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int iz = t.z0 + lane, zz = lane + FAT;
     const int c0 = FAT + w * R, ix0 = t.x0 + w * R;
+    const unsigned voff = (unsigned(ix0) * unsigned(nz) + unsigned(iz)) * SZ;
+
+    T o[5][R];
+#if EIO_PREFETCH >= 1
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int f = 0; f < 5; ++f) o[f][r] = bld<T>(rold, voff, unsigned(r) * colb + unsigned(f) * fbytes);
+#endif
+
+    const unsigned hbase = (unsigned(t.x0 - FAT) * unsigned(nz) + unsigned(t.z0 - FAT)) * SZ;
+    for (int e = threadIdx.x; e < TXH * ZH; e += NT) {
+        const int c = e / ZH, z = e - c * ZH;
+        const unsigned hv = hbase + unsigned(c) * colb + unsigned(z) * SZ;
+        const T vx = bld<T>(rcur, hv, 0), vz = bld<T>(rcur, hv, fbytes);
+        const T sxx = bld<T>(rcur, hv, 2 * fbytes), szz = bld<T>(rcur, hv, 3 * fbytes);
+        const T sxz = bld<T>(rcur, hv, 4 * fbytes);
+        const T l2m = bld<T>(rmed, hv, 2 * fbytes), lam = bld<T>(rmed, hv, 3 * fbytes);
+        lds[0][c][z] = bld<T>(rmed, hv, 0) * vx;
+        lds[1][c][z] = bld<T>(rmed, hv, fbytes) * vz;
+        lds[2][c][z] = l2m * sxx + lam * szz;
+        lds[3][c][z] = lam * sxx + l2m * szz;
+        lds[4][c][z] = bld<T>(rmed, hv, 4 * fbytes) * sxz;
+    }
+    __syncthreads();
+
+#if EIO_PREFETCH < 1
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int f = 0; f < 5; ++f) o[f][r] = bld<T>(rold, voff, unsigned(r) * colb + unsigned(f) * fbytes);
+#endif
     const T* cz = S.cz;
     const T* cx = S.cx;
-
     T dx_a[R], dx_s6[R], dx_s0[R], dx_s1[R];
     {
         T q[R + 8];
@@ -351,32 +426,23 @@ __global__ __launch_bounds__(NTHR) void k_step_adj(const ShotDev<T>* __restrict_
         for (int r = 0; r < R; ++r) dx_s1[r] = dm_q(q, r + 4, cx);
     }
 
-    T val[5][R];
     const bool zok = iz < nz - FAT;
+    T val[5][R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int ix = ix0 + r;
         const bool ok = zok && (ix < nx - FAT);
-        const long i = long(ix) * nz + iz;
         const int c = c0 + r;
         const T dz_s6 = dp_q(&lds[4][c][zz], 0, cz);
         const T dz_b = dm_q(&lds[3][c][zz], 0, cz);
         const T dz_s1 = dp_q(&lds[1][c][zz], 0, cz);
         const T dz_s0 = dm_q(&lds[0][c][zz], 0, cz);
-        T o[5];
-        if (ok) {
-#pragma unroll
-            for (int f = 0; f < 5; ++f) o[f] = oldp[f][i];
-        } else {
-#pragma unroll
-            for (int f = 0; f < 5; ++f) o[f] = T(0);
-        }
         const T d = ok ? damp_at(S, iz, ix) : T(1);
-        val[0][r] = d * (d * o[0] - (dx_a[r] + dz_s6));
-        val[1][r] = d * (d * o[1] - (dz_b + dx_s6[r]));
-        val[2][r] = d * (d * o[2] - dx_s0[r]);
-        val[3][r] = d * (d * o[3] - dz_s1);
-        val[4][r] = d * (d * o[4] - (dz_s0 + dx_s1[r]));
+        val[0][r] = d * (d * o[0][r] - (dx_a[r] + dz_s6));
+        val[1][r] = d * (d * o[1][r] - (dz_b + dx_s6[r]));
+        val[2][r] = d * (d * o[2][r] - dx_s0[r]);
+        val[3][r] = d * (d * o[3][r] - dz_s1);
+        val[4][r] = d * (d * o[4][r] - (dz_s0 + dx_s1[r]));
     }
 
     // adjoint: inject = receiver (data) entries, record = source entries
@@ -391,7 +457,7 @@ __global__ __launch_bounds__(NTHR) void k_step_adj(const ShotDev<T>* __restrict_
         __syncthreads();
         if (has_inj) {  // kernelsGpuElastic.cu:191-229 with it2+1
             const T fa = S.filt[st.it2 + 1], fb = S.filt[S.h + st.it2 + 1];
-            for (int e = t.rec_b + threadIdx.x; e < t.rec_e; e += NTHR) {
+            for (int e = t.rec_b + threadIdx.x; e < t.rec_e; e += NT) {
                 const Entry en = ents[e];
                 const int lz = en.lcell & 255, lx = en.lcell >> 8;
                 const int comp = en.crow >> 24, row = en.crow & 0xffffff;
@@ -407,7 +473,7 @@ __global__ __launch_bounds__(NTHR) void k_step_adj(const ShotDev<T>* __restrict_
         }
         if (has_ext) {  // kernelsGpuElastic.cu:100-116
             const T fa = S.filt[st.it2], fb = S.filt[S.h + st.it2];
-            for (int e = t.src_b + threadIdx.x; e < t.src_e; e += NTHR) {
+            for (int e = t.src_b + threadIdx.x; e < t.src_e; e += NT) {
                 const Entry en = ents[e];
                 const int lz = en.lcell & 255, lx = en.lcell >> 8;
                 const int comp = en.crow >> 24, row = en.crow & 0xffffff;
@@ -419,21 +485,25 @@ __global__ __launch_bounds__(NTHR) void k_step_adj(const ShotDev<T>* __restrict_
         }
     }
 
+    rsrc_t rsl, rsr;
+    T efa = T(0), efb = T(0);
+    if (MODE == M_BADJ) {  // kernelsGpuElastic.cu:1369-1376
+        rsl = make_rsrc(S.ss[sspar], 5 * fbytes);
+        rsr = make_rsrc(S.ss[sspar ^ 1], 5 * fbytes);
+        efa = S.filt[st.it2];
+        efb = S.filt[S.h + st.it2];
+    }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int ix = ix0 + r;
-        if (zok && ix < nx - FAT) {
-            const long i = long(ix) * nz + iz;
+        if (zok && ix0 + r < nx - FAT) {
+            const unsigned so = unsigned(r) * colb;
 #pragma unroll
-            for (int f = 0; f < 5; ++f) oldp[f][i] = val[f][r];
-            if (MODE == M_BADJ) {  // kernelsGpuElastic.cu:1369-1376
-                const T fa = S.filt[st.it2], fb = S.filt[S.h + st.it2];
-                T* const* sl = S.ss[sspar];
-                T* const* sr = S.ss[sspar ^ 1];
-#pragma unroll
-                for (int f = 0; f < 5; ++f) {
-                    sl[f][i] += val[f][r] * fa;
-                    sr[f][i] += val[f][r] * fb;
+            for (int f = 0; f < 5; ++f) {
+                const unsigned sf = so + unsigned(f) * fbytes;
+                bst(rold, val[f][r], voff, sf);
+                if (MODE == M_BADJ) {
+                    bst(rsl, bld<T>(rsl, voff, sf) + val[f][r] * efa, voff, sf);
+                    bst(rsr, bld<T>(rsr, voff, sf) + val[f][r] * efb, voff, sf);
                 }
             }
         }
@@ -444,36 +514,46 @@ __global__ __launch_bounds__(NTHR) void k_step_adj(const ShotDev<T>* __restrict_
 // FWD (kernelsGpuElastic.cu:1381-1459): ss[which] <- secondary source at coarse time its.
 // ADJ (kernelsGpuElastic.cu:1538-1613): dm += imaging condition of ss[which] at its, then
 //      ss[which] <- 0 (the buffer becomes the next, zeroed, accumulation slice).
-template <typename T, int R, bool ADJ>
-__global__ __launch_bounds__(NTHR) void k_imaging(const ShotDev<T>* __restrict__ shots,
-                                                  const TileDev* __restrict__ tiles, int ntiles,
-                                                  int its, int which) {
-    constexpr int TX = 4 * R;
+template <typename T, int R, int WX, bool ADJ>
+__global__ __launch_bounds__(64 * WX) void k_imaging(const ShotDev<T>* __restrict__ shots,
+                                                     const TileDev* __restrict__ tiles, int ntiles,
+                                                     int its, int which) {
+    constexpr int NT = 64 * WX;
+    constexpr int TX = WX * R;
     constexpr int TXH = TX + 2 * FAT;
+    constexpr unsigned SZ = sizeof(T);
     __shared__ T lds[2][TXH][ZH];
 
     const TileDev t = tiles[xcd_remap(blockIdx.x, ntiles)];
     const ShotDev<T>& S = shots[t.shot];
     const int nz = S.nz, nx = S.nx;
-    const long n = long(nz) * nx;
-    const T* wx = S.Wvx + long(its) * n;
-    const T* wz = S.Wvz + long(its) * n;
+    const unsigned n = unsigned(nz) * unsigned(nx);
+    const unsigned fbytes = n * SZ, colb = unsigned(nz) * SZ;
+    // slices its-1, its, its+1 of the coarse storage (clamped at the ends)
+    const int lo = its > 0 ? its - 1 : its;
+    const int nsl = (its < S.nts - 1 ? its + 1 : its) - lo + 1;
+    const rsrc_t rwx = make_rsrc(S.Wvx + size_t(lo) * n, unsigned(nsl) * fbytes);
+    const rsrc_t rwz = make_rsrc(S.Wvz + size_t(lo) * n, unsigned(nsl) * fbytes);
+    const unsigned scur = unsigned(its - lo) * fbytes;
 
-    for (int e = threadIdx.x; e < TXH * ZH; e += NTHR) {
-        const int c = e / ZH, zz = e - c * ZH;
-        const int gx = t.x0 - FAT + c, gz = t.z0 - FAT + zz;
-        const bool ok = (gx < nx) && (gz < nz);
-        const long gi = long(gx) * nz + gz;
-        lds[0][c][zz] = ok ? wx[gi] : T(0);
-        lds[1][c][zz] = ok ? wz[gi] : T(0);
+    const unsigned hbase = (unsigned(t.x0 - FAT) * unsigned(nz) + unsigned(t.z0 - FAT)) * SZ;
+    for (int e = threadIdx.x; e < TXH * ZH; e += NT) {
+        const int c = e / ZH, z = e - c * ZH;
+        const unsigned hv = hbase + unsigned(c) * colb + unsigned(z) * SZ;
+        lds[0][c][z] = bld<T>(rwx, hv, scur);
+        lds[1][c][z] = bld<T>(rwz, hv, scur);
     }
     __syncthreads();
 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int iz = t.z0 + lane, zz = lane + FAT;
     const int c0 = FAT + w * R, ix0 = t.x0 + w * R;
+    const unsigned voff = (unsigned(ix0) * unsigned(nz) + unsigned(iz)) * SZ;
     const T* cz = S.cz;
     const T* cx = S.cx;
+    const rsrc_t rss = make_rsrc(S.ss[which], 5 * fbytes);
+    const rsrc_t rdm = make_rsrc(S.dm, 5 * fbytes);
+    const T dinv = S.filt[2 * S.h];  // 1/(2 dts)
 
     T dvx_dx[R], dvz_dxm[R];
     {
@@ -492,45 +572,50 @@ __global__ __launch_bounds__(NTHR) void k_imaging(const ShotDev<T>* __restrict__
     for (int r = 0; r < R; ++r) {
         const int ix = ix0 + r;
         if (!(zok && ix < nx - FAT)) continue;
-        const long i = long(ix) * nz + iz;
         const int c = c0 + r;
+        const unsigned so = unsigned(r) * colb;
         const T dvz_dz = dp_q(&lds[1][c][zz], 0, cz);
         const T dvx_dzm = dm_q(&lds[0][c][zz], 0, cz);
         const T sxz_op = dvx_dzm + dvz_dxm[r];
         T tx, tz;
         if (its == 0) {
-            tx = -wx[i + n];
-            tz = -wz[i + n];
+            tx = -bld<T>(rwx, voff, so + fbytes);
+            tz = -bld<T>(rwz, voff, so + fbytes);
         } else if (its == S.nts - 1) {
-            tx = wx[i - n];
-            tz = wz[i - n];
+            tx = bld<T>(rwx, voff, so);
+            tz = bld<T>(rwz, voff, so);
         } else {
-            tx = wx[i - n] - wx[i + n];
-            tz = wz[i - n] - wz[i + n];
+            tx = bld<T>(rwx, voff, so) - bld<T>(rwx, voff, so + 2 * fbytes);
+            tz = bld<T>(rwz, voff, so) - bld<T>(rwz, voff, so + 2 * fbytes);
         }
-        const T dinv = S.filt[2 * S.h];  // 1/(2 dts), stored after the filter
-        T* const* ss = S.ss[which];
         if (!ADJ) {
-            ss[0][i] = S.dm[0][i] * tx * dinv;
-            ss[1][i] = S.dm[1][i] * tz * dinv;
-            const T dl = S.dm[2][i] * (dvx_dx[r] + dvz_dz);
-            ss[2][i] = dl + T(2) * S.dm[3][i] * dvx_dx[r];
-            ss[3][i] = dl + T(2) * S.dm[3][i] * dvz_dz;
-            ss[4][i] = S.dm[4][i] * sxz_op;
+            const T d0 = bld<T>(rdm, voff, so), d1 = bld<T>(rdm, voff, so + fbytes);
+            const T d2 = bld<T>(rdm, voff, so + 2 * fbytes), d3 = bld<T>(rdm, voff, so + 3 * fbytes);
+            const T d4 = bld<T>(rdm, voff, so + 4 * fbytes);
+            const T dl = d2 * (dvx_dx[r] + dvz_dz);
+            bst(rss, d0 * tx * dinv, voff, so);
+            bst(rss, d1 * tz * dinv, voff, so + fbytes);
+            bst(rss, dl + T(2) * d3 * dvx_dx[r], voff, so + 2 * fbytes);
+            bst(rss, dl + T(2) * d3 * dvz_dz, voff, so + 3 * fbytes);
+            bst(rss, d4 * sxz_op, voff, so + 4 * fbytes);
         } else {
-            const T a0 = ss[0][i], a1 = ss[1][i], a2 = ss[2][i], a3 = ss[3][i], a4 = ss[4][i];
-            S.dm[0][i] += a0 * tx * dinv;
-            S.dm[1][i] += a1 * tz * dinv;
-            S.dm[2][i] += (a2 + a3) * (dvx_dx[r] + dvz_dz);
-            S.dm[3][i] += T(2) * (dvx_dx[r] * a2 + dvz_dz * a3);
-            S.dm[4][i] += a4 * sxz_op;
+            const T a0 = bld<T>(rss, voff, so), a1 = bld<T>(rss, voff, so + fbytes);
+            const T a2 = bld<T>(rss, voff, so + 2 * fbytes), a3 = bld<T>(rss, voff, so + 3 * fbytes);
+            const T a4 = bld<T>(rss, voff, so + 4 * fbytes);
+            bst(rdm, bld<T>(rdm, voff, so) + a0 * tx * dinv, voff, so);
+            bst(rdm, bld<T>(rdm, voff, so + fbytes) + a1 * tz * dinv, voff, so + fbytes);
+            bst(rdm, bld<T>(rdm, voff, so + 2 * fbytes) + (a2 + a3) * (dvx_dx[r] + dvz_dz), voff,
+                so + 2 * fbytes);
+            bst(rdm, bld<T>(rdm, voff, so + 3 * fbytes) + T(2) * (dvx_dx[r] * a2 + dvz_dz * a3), voff,
+                so + 3 * fbytes);
+            bst(rdm, bld<T>(rdm, voff, so + 4 * fbytes) + a4 * sxz_op, voff, so + 4 * fbytes);
 #pragma unroll
-            for (int f = 0; f < 5; ++f) ss[f][i] = T(0);
+            for (int f = 0; f < 5; ++f) bst(rss, T(0), voff, so + unsigned(f) * fbytes);
         }
     }
 }
 
-// out[f][i] += sum_s acc_s[f][i] over the update region (Born adjoint shot stacking)
+// out[i] += sum_s acc_s[i] (Born adjoint shot stacking, one field)
 template <typename T>
 __global__ __launch_bounds__(NTHR) void k_stack(T* __restrict__ out, const T* const* __restrict__ acc,
                                                 int nacc, long n) {
@@ -539,6 +624,13 @@ __global__ __launch_bounds__(NTHR) void k_stack(T* __restrict__ out, const T* co
     T s = out[i];
     for (int a = 0; a < nacc; ++a) s += acc[a][i];
     out[i] = s;
+}
+
+// plain copy used as the bandwidth / counter calibration reference
+__global__ __launch_bounds__(NTHR) void k_copy16(float4* __restrict__ dst,
+                                                 const float4* __restrict__ src, long n16) {
+    const long stride = long(gridDim.x) * NTHR;
+    for (long i = long(blockIdx.x) * NTHR + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
 }
 
 // zero-fill (graph-capturable replacement for hipMemsetAsync)

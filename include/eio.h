@@ -80,6 +80,14 @@ int eio_ctx_last_timing(eio_ctx* ctx, double* total_ms, long long* n_step_launch
 /* 0 = launch each fine step (default), 1 = replay each batch of fine steps from a captured hipGraph. */
 int eio_ctx_set_graph(eio_ctx* ctx, int use_graph);
 
+/* FP32 step-kernel variant (tile of 64 x WX*R cells, WX waves per workgroup, R x-cells per
+ * lane, requested occupancy) used by the next calls; chosen by the autotuner
+ * (elastic_isotropic_operators_amd/autotune.py).  FP64 contexts ignore it.  No counterpart in
+ * the reference (fixed 16x16 blocks, varDeclare.h:5). */
+int eio_ctx_num_variants(void);
+int eio_ctx_variant_info(int variant, int* R, int* WX, int* occ);
+int eio_ctx_set_variant(eio_ctx* ctx, int variant);
+
 /* Register a model: grid constants (FD coefficients, time filter, damping table) and the
  * five scaled media rhoxDtw, rhozDtw, lamb2MuDtw, lambDtw, muxzDtw, each nz*nx.
  * Replaces initNonlinearElasticGpu + allocateNonlinearElasticGpu
@@ -106,6 +114,13 @@ int eio_born_forward(eio_ctx* ctx, int nshot, const eio_shot* shots, const void*
 /* Born adjoint: accumulates (+=) the imaging condition of all shots into dm[5].
  * Replaces BornShotsAdjGpu (BornElasticGpuFunctions.h:25-36, .cu:1100-1454). */
 int eio_born_adjoint(eio_ctx* ctx, int nshot, const eio_shot* shots, void* const dm[5], int flags);
+
+/* Diagnostics: copy `bytes` (multiple of 16) from src to dst (device pointers) `reps` times with
+ * a plain 16-byte-vector copy kernel on the context stream; *ms = mean kernel time.  Used to
+ * measure the achievable HBM copy bandwidth and to calibrate HBM byte counters.  No counterpart
+ * in the reference. */
+int eio_copy_probe(eio_ctx* ctx, void* dst, const void* src, unsigned long long bytes, int reps,
+                   double* ms);
 
 /* Device memory helpers for callers without their own allocator (bytes). */
 int eio_device_alloc(eio_ctx* ctx, unsigned long long bytes, void** ptr);

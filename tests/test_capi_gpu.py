@@ -29,8 +29,13 @@ def _case(nz=40, nx=56, nts=120, sub=1, seed=0):
     return w, g, med, spos, rpos
 
 
+VARIANT = [0]
+
+
 def _ctx(dtype):
-    return _capi.Context(0, dtype)
+    c = _capi.Context(0, dtype)
+    c.set_variant(VARIANT[0])
+    return c
 
 
 def _model(ctx, w, g, med):
@@ -202,3 +207,25 @@ def test_invalid_positions_rejected():
     shot, keep = ctx.make_shot(mid, bad, rpos, src, rec)
     with pytest.raises(_capi.EioError):
         ctx.nl_forward([shot])
+
+
+@pytest.mark.parametrize("v", range(6))
+def test_all_fp32_variants_forward_adjoint(v):
+    """Every FP32 step-kernel variant (tile shape / occupancy) gives the same NL forward and
+    adjoint to FP32 accuracy."""
+    if v >= len(_capi.Context.variants()):
+        pytest.skip("variant not compiled")
+    w, g, med, spos, rpos = _case(nz=70, nx=90, nts=80, sub=1, seed=30)
+    rng = np.random.default_rng(31)
+    src = random_traces(rng, spos, g.nts, np.float64)
+    dat = random_traces(rng, rpos, g.nts, np.float64)
+    ref_f = O.nl_fwd_reg(g, med, spos, src, rpos, np.float64)
+    ref_a = O.nl_adj_reg(g, med, spos, rpos, dat, np.float64)
+    ctx = _capi.Context(0, np.float32)
+    ctx.set_variant(v)
+    mid = _model(ctx, w, g, med)
+    out_f = _gpu_nl_fwd(ctx, mid, spos, rpos, src, g.nts, np.float32)
+    out_a = _gpu_nl_adj(ctx, mid, spos, rpos, dat, g.nts, np.float32)
+    for k in range(5):
+        assert rel_l2(out_f[k], ref_f[k]) < 1e-5, (v, k, rel_l2(out_f[k], ref_f[k]))
+        assert rel_l2(out_a[k], ref_a[k]) < 1e-5, (v, k, rel_l2(out_a[k], ref_a[k]))
