@@ -39,14 +39,14 @@ def _pybind_flags():
     import pybind11
     inc = [pybind11.get_include(), sysconfig.get_paths()["include"], os.path.join(ROOT, "include"),
            os.path.join(CSRC, "host")]
-    return ["-O3", "-std=c++17", "-fPIC", "-shared", "-fopenmp"] + ["-I" + i for i in inc]
+    return ["-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden"] + ["-I" + i for i in inc]
 
 
 def build_pybind():
     """One extension module per reference module name, each linking libeio_hip.so."""
     suffix = sysconfig.get_config_var("EXT_SUFFIX")
-    built = []
-    srcs = [os.path.join("host", f) for f in ("host_ops.cpp",)]
+    built, jobs = [], []
+    srcs = []
     for prec, ctype in (("float", "float"), ("double", "double")):
         for kind in ("nl", "born"):
             name = "pyElastic_iso_%s_%s" % (prec, kind)
@@ -54,10 +54,17 @@ def build_pybind():
             cmd = (["g++"] + _pybind_flags() +
                    ["-DEIO_REAL=%s" % ctype, "-DEIO_MODULE=%s" % name,
                     "-DEIO_KIND_%s=1" % kind.upper(), "host/pymodule.cpp"] + srcs +
+                   (["-DEIO_DEFINE_INTERP=1"] if (prec, kind) == ("float", "nl") else []) +
                    ["-o", out + ".tmp", "-L" + HERE, "-leio_hip", "-Wl,-rpath,$ORIGIN"])
-            _run(cmd)
-            os.replace(out + ".tmp", out)
-            built.append(out)
+            jobs.append((cmd, out))
+    procs = [(subprocess.Popen(cmd, cwd=CSRC, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True), out)
+             for cmd, out in jobs]
+    for (p, out), (cmd, _) in zip(procs, jobs):
+        txt = p.communicate()[0]
+        if p.returncode != 0:
+            raise RuntimeError("build failed: %s\n%s" % (" ".join(cmd), txt))
+        os.replace(out + ".tmp", out)
+        built.append(out)
     return built
 
 

@@ -1,0 +1,3 @@
+"""Import name used by the reference scripts."""
+from elastic_isotropic_operators_amd import Elastic_iso_float_prop as _m
+globals().update({k: v for k, v in vars(_m).items() if not k.startswith("__")})

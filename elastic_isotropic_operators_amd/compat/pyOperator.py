@@ -1,0 +1,4 @@
+"""Import name used by the reference scripts."""
+from elastic_isotropic_operators_amd.sep.pyOperator import *  # noqa: F401,F403
+from elastic_isotropic_operators_amd.sep import pyOperator as _m
+globals().update({k: v for k, v in vars(_m).items() if not k.startswith("__")})

@@ -1,0 +1,727 @@
+// host_ops.h -- C++ host layer of the drop-in operators (templated on the real type).
+//
+// Mirrors the reference's host classes on the path of the nonlinear and Born shot operators
+// (paths relative to elastic_iso_lib/elastic_iso_float/prop):
+//   FdParam           src/fdParamElastic.cpp:12-157, checks :256-324
+//   stagger helpers   src/stagger.cpp:18-114
+//   SpaceInterp       src/spaceInterpGpu.cpp:11-138 (linear + dipole), :288-376, checks :378-403
+//   time_interp_fwd   src/interpTimeLinTbb.cpp:30-58
+//   NonlinearShots    src/nonlinearPropElasticShotsGpu.cpp:80-422 + src/nonlinearPropElasticGpu.cpp:43-473
+//   BornShots         src/BornElasticShotsGpu.cpp:85-378 + src/BornElasticGpu.cpp:47-194 +
+//                     include/seismicElasticOperator2D.cpp:27-119
+// Propagation goes through the C ABI of include/eio.h (all shots of a call in one batched
+// device time loop on one GPU).  Errors are thrown as std::runtime_error.
+#pragma once
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <iostream>
+#include <map>
+#include <memory>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "eio.h"
+
+namespace eioh {
+
+constexpr int FAT = 4;
+constexpr int SUB_MAX = 100;     // varDeclare.h:10
+constexpr int GRID_OF_COMP[5] = {1, 2, 0, 0, 3};
+
+struct Axis {
+    int n = 1;
+    double o = 0.0, d = 1.0;
+};
+
+// Parameter source (parfile / command-line key=value), with the reference defaults semantics:
+// a missing key without default is an error.
+struct Params {
+    std::map<std::string, std::string> kv;
+    bool has(const std::string& k) const { return kv.count(k) > 0; }
+    int getInt(const std::string& k) const {
+        if (!has(k)) throw std::runtime_error("missing parameter " + k);
+        return std::stoi(kv.at(k));
+    }
+    int getInt(const std::string& k, int def) const { return has(k) ? std::stoi(kv.at(k)) : def; }
+    double getFloat(const std::string& k) const {
+        if (!has(k)) throw std::runtime_error("missing parameter " + k);
+        return std::stod(kv.at(k));
+    }
+    double getFloat(const std::string& k, double def) const { return has(k) ? std::stod(kv.at(k)) : def; }
+    std::string getString(const std::string& k, const std::string& def) const {
+        return has(k) ? kv.at(k) : def;
+    }
+    std::vector<int> getInts(const std::string& k, const std::vector<int>& def) const {
+        if (!has(k)) return def;
+        std::vector<int> out;
+        std::string s = kv.at(k);
+        size_t p = 0;
+        while (p <= s.size()) {
+            size_t q = s.find(',', p);
+            if (q == std::string::npos) q = s.size();
+            if (q > p) out.push_back(std::stoi(s.substr(p, q - p)));
+            p = q + 1;
+        }
+        return out;
+    }
+};
+
+inline void check_eio(int rc) {
+    if (rc != EIO_OK) throw std::runtime_error(std::string("eio: ") + eio_last_error());
+}
+
+// ---------------------------------------------------------------- staggering (stagger.cpp)
+template <typename T>
+void stagger_x_adj(int nz, int nx, const T* d, T* m) {  // stagger.cpp:38-57
+    for (int iz = 0; iz < nz; ++iz) {
+        for (int ix = 1; ix < nx - 1; ++ix)
+            m[size_t(ix) * nz + iz] = T(0.5) * (d[size_t(ix) * nz + iz] + d[size_t(ix - 1) * nz + iz]);
+        m[iz] = T(0.5) * d[iz];
+        m[size_t(nx - 1) * nz + iz] = d[size_t(nx - 1) * nz + iz] + T(0.5) * d[size_t(nx - 2) * nz + iz];
+    }
+}
+template <typename T>
+void stagger_z_adj(int nz, int nx, const T* d, T* m) {  // stagger.cpp:95-114
+    for (int ix = 0; ix < nx; ++ix) {
+        const T* a = d + size_t(ix) * nz;
+        T* b = m + size_t(ix) * nz;
+        for (int iz = 1; iz < nz - 1; ++iz) b[iz] = T(0.5) * (a[iz] + a[iz - 1]);
+        b[0] = T(0.5) * a[0];
+        b[nz - 1] = a[nz - 1] + T(0.5) * a[nz - 2];
+    }
+}
+template <typename T>
+void stagger_x_fwd_add(int nz, int nx, const T* m, T* d) {  // stagger.cpp:18-35
+    for (int iz = 0; iz < nz; ++iz) {
+        for (int ix = 0; ix < nx - 1; ++ix)
+            d[size_t(ix) * nz + iz] += T(0.5) * (m[size_t(ix) * nz + iz] + m[size_t(ix + 1) * nz + iz]);
+        d[size_t(nx - 1) * nz + iz] += m[size_t(nx - 1) * nz + iz];
+    }
+}
+template <typename T>
+void stagger_z_fwd_add(int nz, int nx, const T* m, T* d) {  // stagger.cpp:75-93
+    for (int ix = 0; ix < nx; ++ix) {
+        const T* a = m + size_t(ix) * nz;
+        T* b = d + size_t(ix) * nz;
+        for (int iz = 0; iz < nz - 1; ++iz) b[iz] += T(0.5) * (a[iz] + a[iz + 1]);
+        b[nz - 1] += a[nz - 1];
+    }
+}
+
+// ---------------------------------------------------------------- fdParamElastic
+template <typename T>
+struct FdParam {
+    int nts, sub, ntw, nz, nx, zPadPlus, zPadMinus, xPadPlus, xPadMinus, zPad, xPad, minPad;
+    int surfaceCondition, mod_par, blockSize, fat, saveWavefield;
+    double dts, ots, dtw, otw, dz, dx, oz, ox, fMax, alphaCos, errorTolerance;
+    float minVpVs, maxVpVs, Courant, dispersionRatio;
+    std::vector<T> media;  // 5*nz*nx: rhoxDtw, rhozDtw, lamb2MuDtw, lambDtw, muxzDtw
+
+    // elastic: [3][nx][nz] (rho, lambda, mu); axes of the elastic-parameter hypercube
+    FdParam(const T* elastic, const std::vector<Axis>& ax, const Params& par) {
+        surfaceCondition = par.getInt("surfaceCondition", 0);
+        nts = par.getInt("nts");
+        dts = par.getFloat("dts", 0.0);
+        ots = par.getFloat("ots", 0.0);
+        sub = par.getInt("sub");
+        sub = sub * 2;  // fdParamElastic.cpp:28
+        if (sub > SUB_MAX)
+            throw std::runtime_error("**** ERROR: 2*_sub is greater than the allowed SUB_MAX value. " +
+                                     std::to_string(sub) + " > " + std::to_string(SUB_MAX) + " ****");
+        ntw = (nts - 1) * sub + 1;
+        dtw = double(float(dts) / float(sub));
+        otw = ots;
+        nz = par.getInt("nz");
+        zPadPlus = par.getInt("zPadPlus");
+        zPadMinus = par.getInt("zPadMinus");
+        if (surfaceCondition == 0) zPad = std::min(zPadMinus, zPadPlus);
+        else if (surfaceCondition == 1) zPad = zPadPlus;
+        else throw std::runtime_error("ERROR NO IMPROPER FREE SURFACE PARAM PROVIDED");
+        mod_par = par.getInt("mod_par", 1);
+        dz = par.getFloat("dz", -1.0);
+        oz = ax.size() > 0 ? ax[0].o : 0.0;
+        nx = par.getInt("nx");
+        xPadPlus = par.getInt("xPadPlus");
+        xPadMinus = par.getInt("xPadMinus");
+        xPad = std::min(xPadMinus, xPadPlus);
+        dx = par.getFloat("dx", -1.0);
+        ox = ax.size() > 1 ? ax[1].o : 0.0;
+        if (mod_par == 2) { dz *= 1000.; ox *= 1000.; dx *= 1000.; oz *= 1000.; }
+        fMax = par.getFloat("fMax", 1000.0);
+        blockSize = par.getInt("blockSize");
+        fat = par.getInt("fat", 4);
+        minPad = std::min(zPad, xPad);
+        saveWavefield = par.getInt("saveWavefield", 0);
+        alphaCos = par.getFloat("alphaCos", 0.99);
+        errorTolerance = par.getFloat("errorTolerance", 0.000001);
+        if (surfaceCondition != 0)
+            throw std::runtime_error("surfaceCondition=1 (free surface) is not available in this build");
+        if (fat != FAT) throw std::runtime_error("fat must be 4");
+        // consistency with the elastic parameter hypercube (fdParamElastic.cpp:296-324)
+        const double scale = (mod_par == 2) ? 1000.0 : 1.0;
+        if (ax.size() < 3 || ax[0].n != nz) throw std::runtime_error("**** ERROR: nz not consistent with parfile ****");
+        if (std::abs(dz - ax[0].d * scale) > errorTolerance) throw std::runtime_error("**** ERROR: dz not consistent with parfile ****");
+        if (ax[1].n != nx) throw std::runtime_error("**** ERROR nx not consistent with parfile ****");
+        if (std::abs(dx - ax[1].d * scale) > errorTolerance) throw std::runtime_error("**** ERROR: dx not consistent with parfile ****");
+        set_model(elastic);
+    }
+
+    // vmin/vmax scan (fdParamElastic.cpp:81-98, same loop bounds), checks, media scaling
+    void set_model(const T* elastic) {
+        const size_t n = size_t(nz) * nx;
+        const T* rho = elastic;
+        const T* lam = elastic + n;
+        const T* mu = elastic + 2 * n;
+        minVpVs = 10000.f;
+        maxVpVs = -1.f;
+        for (int ix = fat; ix < nx - 2 * fat; ix++)
+            for (int iz = fat; iz < nz - 2 * fat; iz++) {
+                const size_t i = size_t(ix) * nz + iz;
+                const float vp = std::sqrt(float((lam[i] + 2 * mu[i]) / rho[i]));
+                const float vs = std::sqrt(float(mu[i] / rho[i]));
+                if (vp < minVpVs) minVpVs = vp;
+                if (vp > maxVpVs) maxVpVs = vp;
+                if (vs < minVpVs && vs != 0) minVpVs = vs;
+                if (vs > maxVpVs) maxVpVs = vs;
+            }
+        // checkFdStability (fdParamElastic.cpp:256-267)
+        Courant = float(maxVpVs * dtw * 2.0 / std::min(dz, dx));
+        if (Courant > 0.45f)
+            throw std::runtime_error("**** ERROR: Courant is too big: " + std::to_string(Courant) + " ****");
+        // checkFdDispersion (:269-281)
+        dispersionRatio = float(minVpVs / (fMax * std::max(dz, dx)));
+        if (dispersionRatio < 3.0f)
+            throw std::runtime_error("**** ERROR: Dispersion is too small: " + std::to_string(dispersionRatio) + " > 3 ****");
+        // checkModelSize (:283-294)
+        if ((nz - 2 * fat) % blockSize != 0) throw std::runtime_error("**** ERROR: nz-2*_fat not a multiple of block size ****");
+        if ((nx - 2 * fat) % blockSize != 0) throw std::runtime_error("**** ERROR: nx-2*_fat not a multiple of block size ****");
+        if (blockSize != 16) throw std::runtime_error("**** ERROR: Block size for time stepper is not consistent with parfile ****");
+        if (minPad >= 500) throw std::runtime_error("**** ERROR: Padding value is too high ****");
+        // staggered and scaled media (fdParamElastic.cpp:118-148)
+        media.assign(5 * n, T(0));
+        std::vector<T> tmp(n), muxz(n);
+        stagger_x_adj(nz, nx, rho, media.data());
+        stagger_z_adj(nz, nx, rho, media.data() + n);
+        stagger_x_adj(nz, nx, mu, tmp.data());
+        stagger_z_adj(nz, nx, tmp.data(), muxz.data());
+        const T two = T(2.0 * dtw);
+        for (size_t i = 0; i < n; ++i) {
+            media[i] = two / media[i];
+            media[n + i] = two / media[n + i];
+            media[2 * n + i] = two * (lam[i] + T(2) * mu[i]);
+            media[3 * n + i] = two * lam[i];
+            media[4 * n + i] = two * muxz[i];
+        }
+    }
+
+    size_t ncell() const { return size_t(nz) * nx; }
+    void getInfo() const {  // fdParamElastic.cpp:159-254 (abridged)
+        std::cerr << "nts = " << nts << ", dts = " << dts << ", sub (fine) = " << sub << ", ntw = " << ntw
+                  << ", dtw = " << dtw << "\nnz = " << nz << ", nx = " << nx << ", dz = " << dz << ", dx = " << dx
+                  << "\nCourant = " << Courant << ", dispersion = " << dispersionRatio << ", vmin = " << minVpVs
+                  << ", vmax = " << maxVpVs << ", minPad = " << minPad << std::endl;
+    }
+};
+
+// ---------------------------------------------------------------- spaceInterpGpu (linear)
+struct SpaceInterp {
+    std::vector<float> zCoord, xCoord;
+    std::vector<Axis> hyper;
+    int nt = 0, nFilt = 1, nFiltTotal = 4, dipole = 0, nDeviceIrreg = 0, nz = 0;
+    float zDipoleShift = 0.f, xDipoleShift = 0.f;
+    std::string method;
+    std::vector<int> gridPointIndex;       // nFiltTotal per device
+    std::vector<float> weight;
+    std::vector<int> uniq;                 // unique regular points, first-seen order
+    std::vector<int> map;                  // per (device, filter) -> unique index
+
+    SpaceInterp(std::vector<float> z, std::vector<float> x, std::vector<Axis> h, int nt_, std::string m,
+                int nf, int dip, float zs, float xs)
+        : zCoord(std::move(z)), xCoord(std::move(x)), hyper(std::move(h)), nt(nt_), method(std::move(m)) {
+        if (zCoord.size() != xCoord.size()) throw std::runtime_error("zCoord and xCoord sizes differ");
+        if (hyper.size() < 2) throw std::runtime_error("spaceInterpGpu needs a 2-D (or 3-D) hypercube");
+        if (method != "linear")
+            throw std::runtime_error("**** ERROR: Space interp method " + method +
+                                     " is not available in this build (linear only) ****");
+        nFilt = 1;
+        (void)nf;
+        dipole = dip;
+        zDipoleShift = float(zs * hyper[0].d);
+        xDipoleShift = float(xs * hyper[1].d);
+        nDeviceIrreg = int(zCoord.size());
+        nz = hyper[0].n;
+        check_out_of_bounds();
+        nFiltTotal = dipole == 1 ? 8 : 4;
+        calc_linear_weights();
+        convert_irreg_to_reg();
+    }
+
+    void check_out_of_bounds() const {  // spaceInterpGpu.cpp:378-403
+        const float zMin = float(hyper[0].o), xMin = float(hyper[1].o);
+        const float zMax = zMin + (hyper[0].n - 1) * float(hyper[0].d);
+        const float xMax = xMin + (hyper[1].n - 1) * float(hyper[1].d);
+        const float zb = float(hyper[0].d) * nFilt, xb = float(hyper[1].d) * nFilt;
+        for (int i = 0; i < nDeviceIrreg; ++i) {
+            if (zCoord[i] >= zMax - zb || zCoord[i] <= zMin + zb)
+                throw std::runtime_error("**** ERROR: One of the device is out of bounds in the z direction ****");
+            if (xCoord[i] >= xMax - xb || xCoord[i] <= xMin + xb)
+                throw std::runtime_error("**** ERROR: One of the device is out of bounds in the x direction ****");
+        }
+    }
+
+    void calc_linear_weights() {  // spaceInterpGpu.cpp:78-138
+        gridPointIndex.assign(size_t(nFiltTotal) * nDeviceIrreg, 0);
+        weight.assign(size_t(nFiltTotal) * nDeviceIrreg, 0.f);
+        const float oz = float(hyper[0].o), dz = float(hyper[0].d), ox = float(hyper[1].o), dx = float(hyper[1].d);
+        for (int d = 0; d < nDeviceIrreg; ++d) {
+            const int i1 = d * nFiltTotal;
+            for (int pole = 0; pole < (dipole == 1 ? 2 : 1); ++pole) {
+                float wz = (zCoord[d] + (pole ? zDipoleShift : 0.f) - oz) / dz;
+                float wx = (xCoord[d] + (pole ? xDipoleShift : 0.f) - ox) / dx;
+                const int zReg = int(wz), xReg = int(wx);
+                wz = 1.0f - (wz - zReg);
+                wx = 1.0f - (wx - xReg);
+                const float sgn = pole ? -1.0f : 1.0f;
+                const int b = i1 + 4 * pole;
+                gridPointIndex[b] = xReg * nz + zReg;
+                weight[b] = sgn * wz * wx;
+                gridPointIndex[b + 1] = gridPointIndex[b] + 1;
+                weight[b + 1] = sgn * (1.0f - wz) * wx;
+                gridPointIndex[b + 2] = gridPointIndex[b] + nz;
+                weight[b + 2] = sgn * wz * (1.0f - wx);
+                gridPointIndex[b + 3] = gridPointIndex[b] + nz + 1;
+                weight[b + 3] = sgn * (1.0f - wz) * (1.0f - wx);
+            }
+        }
+    }
+
+    void convert_irreg_to_reg() {  // spaceInterpGpu.cpp:288-311
+        std::map<int, int> seen;
+        uniq.clear();
+        map.assign(gridPointIndex.size(), 0);
+        for (size_t k = 0; k < gridPointIndex.size(); ++k) {
+            auto it = seen.find(gridPointIndex[k]);
+            if (it == seen.end()) {
+                seen[gridPointIndex[k]] = int(uniq.size());
+                map[k] = int(uniq.size());
+                uniq.push_back(gridPointIndex[k]);
+            } else {
+                map[k] = it->second;
+            }
+        }
+    }
+
+    int nReg() const { return int(uniq.size()); }
+
+    // regular [nReg][nt] -> irregular [nIrreg][nt] (accumulates), spaceInterpGpu.cpp:313-340
+    template <typename T>
+    void forward_add(const T* reg, T* irr) const {
+        for (int d = 0; d < nDeviceIrreg; ++d)
+            for (int f = 0; f < nFiltTotal; ++f) {
+                const size_t k = size_t(d) * nFiltTotal + f;
+                const T w = T(weight[k]);
+                const T* r = reg + size_t(map[k]) * nt;
+                T* o = irr + size_t(d) * nt;
+                for (int it = 0; it < nt; ++it) o[it] += w * r[it];
+            }
+    }
+    // irregular -> regular (overwrites), spaceInterpGpu.cpp:342-376
+    template <typename T>
+    void adjoint(T* reg, const T* irr) const {
+        std::fill(reg, reg + uniq.size() * size_t(nt), T(0));
+        for (int d = 0; d < nDeviceIrreg; ++d)
+            for (int f = 0; f < nFiltTotal; ++f) {
+                const size_t k = size_t(d) * nFiltTotal + f;
+                const T w = T(weight[k]);
+                T* r = reg + size_t(map[k]) * nt;
+                const T* in = irr + size_t(d) * nt;
+                for (int it = 0; it < nt; ++it) r[it] += w * in[it];
+            }
+    }
+    void getInfo() const {
+        std::cerr << "---------- Space Interp Info ----------\n_nDeviceIrreg= " << nDeviceIrreg
+                  << "\n_nDeviceRreg= " << uniq.size() << "\n";
+    }
+};
+
+// ---------------------------------------------------------------- interpTimeLinTbb::forward
+template <typename T>
+void time_interp_fwd(int nts, double dts, double ots, int sub, int nDev, const T* coarse, T* fine) {
+    const int ntw = (nts - 1) * sub + 1;
+    const float dtw = float(dts) / float(sub);
+    float scale = float(ntw) / float(nts);
+    scale = float(1.0 / std::sqrt(double(scale)));
+    for (int d = 0; d < nDev; ++d) {
+        const T* m = coarse + size_t(d) * nts;
+        T* o = fine + size_t(d) * ntw;
+        for (int itw = 0; itw < ntw - 1; ++itw) {
+            const float tw = float(ots) + itw * dtw;
+            float w = (tw - float(ots)) / float(dts);
+            const int lo = int(w);
+            w = w - lo;
+            o[itw] = (m[lo] * T(1.0 - w) + m[lo + 1] * T(w)) * T(scale);
+        }
+        o[ntw - 1] = T(scale) * m[nts - 1];
+    }
+}
+
+// ---------------------------------------------------------------- device context
+struct Device {
+    eio_ctx* ctx = nullptr;
+    int id = 0;
+    Device(int device, int prec) {
+        id = device;
+        check_eio(eio_ctx_create(device, prec, &ctx));
+    }
+    ~Device() {
+        if (ctx) eio_ctx_destroy(ctx);
+    }
+    Device(const Device&) = delete;
+    Device& operator=(const Device&) = delete;
+};
+
+inline int pick_device(const Params& par) {  // createGpuIdList (nonlinearPropElasticShotsGpu.cpp:45-77)
+    int nGpu = par.getInt("nGpu", -1);
+    std::vector<int> list = par.getInts("iGpu", {-1});
+    if (nGpu <= 0 && list[0] < 0)
+        throw std::runtime_error("**** ERROR: Please provide a list of GPUs to be used ****");
+    if (list[0] >= 0) {
+        std::vector<int> s = list;
+        std::sort(s.begin(), s.end());
+        if (std::unique(s.begin(), s.end()) != s.end())
+            throw std::runtime_error("**** ERROR: Please make sure there are no duplicates in the GPU Id list ****");
+        return list[0];
+    }
+    return 0;  // all shots run on one device; see the batched C ABI
+}
+
+using InterpVec = std::vector<std::shared_ptr<SpaceInterp>>;
+
+template <typename T>
+void scale_sources(const FdParam<T>& fd, const SpaceInterp& ip, int k, T* reg) {
+    // nonlinearPropElasticGpu.cpp:116-144: stresses *2dtw, vx *rhoxDtw, vz *rhozDtw, all /(dx dz)
+    const size_t n = fd.ncell();
+    const T two = T(2.0 * fd.dtw);
+    const T area = T(1.0 / (fd.dx * fd.dz));
+    for (int u = 0; u < ip.nReg(); ++u) {
+        T s;
+        if (k == 0) s = fd.media[size_t(ip.uniq[u])];
+        else if (k == 1) s = fd.media[n + size_t(ip.uniq[u])];
+        else s = two;
+        T* r = reg + size_t(u) * ip.nt;
+        for (int it = 0; it < ip.nt; ++it) r[it] = (r[it] * s) * area;
+    }
+}
+
+inline void fill_positions(eio_shot& s, const InterpVec& src4, const InterpVec& rec4) {
+    // InterpVec order: centre, X, Z, XZ
+    for (int g = 0; g < 4; ++g) {
+        s.src_pos[g] = src4[g]->uniq.data();
+        s.n_src[g] = src4[g]->nReg();
+        s.rec_pos[g] = rec4[g]->uniq.data();
+        s.n_rec[g] = rec4[g]->nReg();
+    }
+}
+
+// ---------------------------------------------------------------- nonlinear shot operator
+template <typename T>
+class NonlinearShots {
+   public:
+    NonlinearShots(const T* elastic, std::vector<Axis> ax, Params par, std::vector<InterpVec> sources,
+                   std::vector<InterpVec> receivers)
+        : ax_(std::move(ax)), par_(std::move(par)), src_(std::move(sources)), rec_(std::move(receivers)) {
+        nExp_ = par_.getInt("nExp");
+        fd_ = std::make_unique<FdParam<T>>(elastic, ax_, par_);
+        if (int(src_.size()) < nExp_) throw std::runtime_error("fewer source geometries than nExp");
+        if (rec_.size() != 1 && int(rec_.size()) < nExp_)
+            throw std::runtime_error("receiver geometries must be 1 or nExp");
+        const int device = pick_device(par_);
+        if (par_.getInt("info", 0) == 1) fd_->getInfo();
+        dev_ = std::make_unique<Device>(device, int(sizeof(T)));
+        register_model();
+    }
+    ~NonlinearShots() {
+        if (model_ >= 0 && dev_) eio_model_remove(dev_->ctx, model_);
+    }
+
+    const FdParam<T>& fd() const { return *fd_; }
+
+    // setBackground (nonlinearPropElasticShotsGpu.h:61): new fdParamElastic, same geometry
+    void setBackground(const T* elastic) {
+        fd_ = std::make_unique<FdParam<T>>(elastic, ax_, par_);
+        const size_t n = fd_->ncell();
+        const void* media[5] = {fd_->media.data(), fd_->media.data() + n, fd_->media.data() + 2 * n,
+                                fd_->media.data() + 3 * n, fd_->media.data() + 4 * n};
+        check_eio(eio_model_set_media(dev_->ctx, model_, media, 0));
+    }
+
+    // model [nSrcSig][5][nSrc][nts] with nSrcSig = 1 or nExp; data [nExp][5][nRec][nts]
+    void forward(bool add, const T* model, int nModelShots, int nSrc, T* data, int nRec) {
+        const int nts = fd_->nts;
+        if (nModelShots != 1 && nModelShots != nExp_) throw std::runtime_error("model shot axis must be 1 or nExp");
+        std::vector<eio_shot> shots(nExp_);
+        std::vector<std::vector<T>> srcReg(size_t(nExp_) * 5), recReg(size_t(nExp_) * 5);
+        for (int e = 0; e < nExp_; ++e) {
+            const InterpVec& S = src_[e];
+            const InterpVec& Rv = rec_.size() == 1 ? rec_[0] : rec_[e];
+            check_counts(S, nSrc, Rv, nRec);
+            const T* m = model + size_t(nModelShots == 1 ? 0 : e) * 5 * nSrc * nts;
+            eio_shot& s = shots[e];
+            std::memset(&s, 0, sizeof(s));
+            s.model = model_;
+            fill_positions(s, S, Rv);
+            for (int k = 0; k < 5; ++k) {
+                const SpaceInterp& ip = *S[GRID_OF_COMP[k]];
+                std::vector<T>& r = srcReg[size_t(e) * 5 + k];
+                r.assign(size_t(ip.nReg()) * nts, T(0));
+                ip.adjoint(r.data(), m + size_t(k) * nSrc * nts);
+                scale_sources(*fd_, ip, k, r.data());
+                std::vector<T>& d = recReg[size_t(e) * 5 + k];
+                d.assign(size_t(Rv[GRID_OF_COMP[k]]->nReg()) * nts, T(0));
+                s.src_tr[k] = r.data();
+                s.rec_tr[k] = d.data();
+            }
+        }
+        check_eio(eio_nl_forward(dev_->ctx, nExp_, shots.data(), 0));
+        if (!add) std::fill(data, data + size_t(nExp_) * 5 * nRec * nts, T(0));
+        for (int e = 0; e < nExp_; ++e) {
+            const InterpVec& Rv = rec_.size() == 1 ? rec_[0] : rec_[e];
+            for (int k = 0; k < 5; ++k)
+                Rv[GRID_OF_COMP[k]]->forward_add(recReg[size_t(e) * 5 + k].data(),
+                                                 data + (size_t(e) * 5 + k) * nRec * nts);
+        }
+    }
+
+    void adjoint(bool add, T* model, int nModelShots, int nSrc, const T* data, int nRec) {
+        const int nts = fd_->nts;
+        if (nModelShots != 1 && nModelShots != nExp_) throw std::runtime_error("model shot axis must be 1 or nExp");
+        std::vector<eio_shot> shots(nExp_);
+        std::vector<std::vector<T>> srcReg(size_t(nExp_) * 5), recReg(size_t(nExp_) * 5);
+        for (int e = 0; e < nExp_; ++e) {
+            const InterpVec& S = src_[e];
+            const InterpVec& Rv = rec_.size() == 1 ? rec_[0] : rec_[e];
+            check_counts(S, nSrc, Rv, nRec);
+            eio_shot& s = shots[e];
+            std::memset(&s, 0, sizeof(s));
+            s.model = model_;
+            fill_positions(s, S, Rv);
+            for (int k = 0; k < 5; ++k) {
+                const SpaceInterp& rp = *Rv[GRID_OF_COMP[k]];
+                std::vector<T>& d = recReg[size_t(e) * 5 + k];
+                d.assign(size_t(rp.nReg()) * nts, T(0));
+                rp.adjoint(d.data(), data + (size_t(e) * 5 + k) * nRec * nts);
+                std::vector<T>& r = srcReg[size_t(e) * 5 + k];
+                r.assign(size_t(S[GRID_OF_COMP[k]]->nReg()) * nts, T(0));
+                s.src_tr[k] = r.data();
+                s.rec_tr[k] = d.data();
+            }
+        }
+        check_eio(eio_nl_adjoint(dev_->ctx, nExp_, shots.data(), 0));
+        if (!add) std::fill(model, model + size_t(nModelShots) * 5 * nSrc * nts, T(0));
+        for (int e = 0; e < nExp_; ++e) {
+            const InterpVec& S = src_[e];
+            T* m = model + size_t(nModelShots == 1 ? 0 : e) * 5 * nSrc * nts;
+            for (int k = 0; k < 5; ++k) {
+                const SpaceInterp& ip = *S[GRID_OF_COMP[k]];
+                std::vector<T>& r = srcReg[size_t(e) * 5 + k];
+                scale_sources(*fd_, ip, k, r.data());  // same diagonal scaling (nonlinearPropElasticGpu.cpp:406-430)
+                ip.forward_add(r.data(), m + size_t(k) * nSrc * nts);
+            }
+        }
+    }
+
+    int nExp() const { return nExp_; }
+    int nSrcIrreg() const { return src_[0][0]->nDeviceIrreg; }
+    int nRecIrreg() const { return rec_[0][0]->nDeviceIrreg; }
+    double last_ms() const {
+        double t = 0;
+        eio_ctx_last_timing(dev_->ctx, &t, nullptr, nullptr);
+        return t;
+    }
+
+   private:
+    void register_model() {
+        eio_grid g;
+        g.nz = fd_->nz; g.nx = fd_->nx; g.nts = fd_->nts; g.sub_fd = fd_->sub; g.minPad = fd_->minPad;
+        g.dz = fd_->dz; g.dx = fd_->dx; g.dts = fd_->dts; g.alphaCos = fd_->alphaCos;
+        const size_t n = fd_->ncell();
+        const void* media[5] = {fd_->media.data(), fd_->media.data() + n, fd_->media.data() + 2 * n,
+                                fd_->media.data() + 3 * n, fd_->media.data() + 4 * n};
+        check_eio(eio_model_add(dev_->ctx, &g, media, 0, &model_));
+    }
+    void check_counts(const InterpVec& S, int nSrc, const InterpVec& Rv, int nRec) const {
+        if (S.size() != 4 || Rv.size() != 4) throw std::runtime_error("geometry must have 4 grids");
+        for (int g = 0; g < 4; ++g) {
+            if (S[g]->nDeviceIrreg != nSrc) throw std::runtime_error("source count not consistent with the model vector");
+            if (Rv[g]->nDeviceIrreg != nRec) throw std::runtime_error("receiver count not consistent with the data vector");
+            if (S[g]->nt != fd_->nts || Rv[g]->nt != fd_->nts) throw std::runtime_error("nts not consistent with parfile");
+        }
+    }
+
+    std::vector<Axis> ax_;
+    Params par_;
+    std::vector<InterpVec> src_, rec_;
+    int nExp_ = 1;
+    std::unique_ptr<Device> dev_;
+    std::unique_ptr<FdParam<T>> fd_;
+    int model_ = -1;
+};
+
+// ---------------------------------------------------------------- Born shot operator
+template <typename T>
+class BornShots {
+   public:
+    // sourcesSignals: list of [5][nSrc][nts] (1 = same for all shots, or nExp)
+    BornShots(const T* elastic, std::vector<Axis> ax, Params par, std::vector<std::vector<T>> sourcesSignals,
+              std::vector<InterpVec> sources, std::vector<InterpVec> receivers)
+        : ax_(std::move(ax)), par_(std::move(par)), sig_(std::move(sourcesSignals)), src_(std::move(sources)),
+          rec_(std::move(receivers)) {
+        nExp_ = par_.getInt("nExp");
+        fd_ = std::make_unique<FdParam<T>>(elastic, ax_, par_);
+        if (int(src_.size()) < nExp_) throw std::runtime_error("fewer source geometries than nExp");
+        if (rec_.size() != 1 && int(rec_.size()) < nExp_) throw std::runtime_error("receiver geometries must be 1 or nExp");
+        if (sig_.empty() || (sig_.size() != 1 && int(sig_.size()) < nExp_))
+            throw std::runtime_error("source signals must be 1 or nExp");
+        const int device = pick_device(par_);
+        if (par_.getInt("info", 0) == 1) fd_->getInfo();
+        dev_ = std::make_unique<Device>(device, int(sizeof(T)));
+        register_model();
+    }
+    ~BornShots() {
+        if (model_ >= 0 && dev_) eio_model_remove(dev_->ctx, model_);
+    }
+    const FdParam<T>& fd() const { return *fd_; }
+
+    void setBackground(const T* elastic) {
+        fd_ = std::make_unique<FdParam<T>>(elastic, ax_, par_);
+        const size_t n = fd_->ncell();
+        const void* media[5] = {fd_->media.data(), fd_->media.data() + n, fd_->media.data() + 2 * n,
+                                fd_->media.data() + 3 * n, fd_->media.data() + 4 * n};
+        check_eio(eio_model_set_media(dev_->ctx, model_, media, 0));
+    }
+
+    // model [3][nx][nz] (drho, dlambda, dmu); data [nExp][5][nRec][nts]
+    void forward(bool add, const T* model, T* data, int nRec) {
+        const int nts = fd_->nts;
+        const size_t n = fd_->ncell();
+        // BornElasticShotsGpu.cpp:107-150
+        std::vector<T> sl(5 * n, T(0)), tmp(n);
+        stagger_x_adj(fd_->nz, fd_->nx, model, sl.data());
+        stagger_z_adj(fd_->nz, fd_->nx, model, sl.data() + n);
+        std::copy(model + n, model + 2 * n, sl.data() + 2 * n);
+        std::copy(model + 2 * n, model + 3 * n, sl.data() + 3 * n);
+        stagger_x_adj(fd_->nz, fd_->nx, model + 2 * n, tmp.data());
+        stagger_z_adj(fd_->nz, fd_->nx, tmp.data(), sl.data() + 4 * n);
+        scale_slices(sl);
+        std::vector<std::vector<T>> fine, recReg;
+        std::vector<eio_shot> shots = build_shots(fine, recReg, nRec, nullptr);
+        const void* dm[5] = {sl.data(), sl.data() + n, sl.data() + 2 * n, sl.data() + 3 * n, sl.data() + 4 * n};
+        check_eio(eio_born_forward(dev_->ctx, nExp_, shots.data(), dm, 0));
+        if (!add) std::fill(data, data + size_t(nExp_) * 5 * nRec * nts, T(0));
+        for (int e = 0; e < nExp_; ++e) {
+            const InterpVec& Rv = rec_.size() == 1 ? rec_[0] : rec_[e];
+            for (int k = 0; k < 5; ++k)
+                Rv[GRID_OF_COMP[k]]->forward_add(recReg[size_t(e) * 5 + k].data(),
+                                                 data + (size_t(e) * 5 + k) * nRec * nts);
+        }
+    }
+
+    void adjoint(bool add, T* model, const T* data, int nRec) {
+        const size_t n = fd_->ncell();
+        std::vector<std::vector<T>> fine, recReg;
+        std::vector<eio_shot> shots = build_shots(fine, recReg, nRec, data);
+        std::vector<T> acc(5 * n, T(0));
+        void* dm[5] = {acc.data(), acc.data() + n, acc.data() + 2 * n, acc.data() + 3 * n, acc.data() + 4 * n};
+        check_eio(eio_born_adjoint(dev_->ctx, nExp_, shots.data(), dm, 0));
+        // BornElasticShotsGpu.cpp:325-371
+        scale_slices(acc);
+        if (!add) std::fill(model, model + 3 * n, T(0));
+        stagger_x_fwd_add(fd_->nz, fd_->nx, acc.data(), model);
+        stagger_z_fwd_add(fd_->nz, fd_->nx, acc.data() + n, model);
+        std::vector<T> t1(n, T(0)), t2(n, T(0));
+        stagger_z_fwd_add(fd_->nz, fd_->nx, acc.data() + 4 * n, t1.data());
+        stagger_x_fwd_add(fd_->nz, fd_->nx, t1.data(), t2.data());
+        for (size_t i = 0; i < n; ++i) {
+            model[n + i] += acc[2 * n + i];
+            model[2 * n + i] += acc[3 * n + i] + t2[i];
+        }
+    }
+
+    int nExp() const { return nExp_; }
+
+   private:
+    void scale_slices(std::vector<T>& sl) const {
+        const size_t n = fd_->ncell();
+        const T two = T(2.0 * fd_->dtw);
+        for (size_t i = 0; i < n; ++i) {
+            sl[i] *= fd_->media[i];
+            sl[n + i] *= fd_->media[n + i];
+            sl[2 * n + i] *= two;
+            sl[3 * n + i] *= two;
+            sl[4 * n + i] *= two;
+        }
+    }
+
+    // Born sources: interpolate, scale and fine-sample (seismicElasticOperator2D.cpp:27-119)
+    std::vector<eio_shot> build_shots(std::vector<std::vector<T>>& fine, std::vector<std::vector<T>>& recReg,
+                                      int nRec, const T* data) {
+        const int nts = fd_->nts, ntw = fd_->ntw;
+        std::vector<eio_shot> shots(nExp_);
+        fine.assign(size_t(nExp_) * 5, {});
+        recReg.assign(size_t(nExp_) * 5, {});
+        for (int e = 0; e < nExp_; ++e) {
+            const InterpVec& S = src_[e];
+            const InterpVec& Rv = rec_.size() == 1 ? rec_[0] : rec_[e];
+            const std::vector<T>& sig = sig_.size() == 1 ? sig_[0] : sig_[e];
+            const int nSrc = S[0]->nDeviceIrreg;
+            if (sig.size() != size_t(5) * nSrc * nts) throw std::runtime_error("source signal size not consistent");
+            for (int g = 0; g < 4; ++g)
+                if (Rv[g]->nDeviceIrreg != nRec) throw std::runtime_error("receiver count not consistent with the data vector");
+            eio_shot& s = shots[e];
+            std::memset(&s, 0, sizeof(s));
+            s.model = model_;
+            fill_positions(s, S, Rv);
+            for (int k = 0; k < 5; ++k) {
+                const SpaceInterp& ip = *S[GRID_OF_COMP[k]];
+                std::vector<T> reg(size_t(ip.nReg()) * nts, T(0));
+                ip.adjoint(reg.data(), sig.data() + size_t(k) * nSrc * nts);
+                scale_sources(*fd_, ip, k, reg.data());
+                std::vector<T>& f = fine[size_t(e) * 5 + k];
+                f.assign(size_t(ip.nReg()) * ntw, T(0));
+                time_interp_fwd(nts, fd_->dts, fd_->ots, fd_->sub, ip.nReg(), reg.data(), f.data());
+                s.src_tr[k] = f.data();
+                const SpaceInterp& rp = *Rv[GRID_OF_COMP[k]];
+                std::vector<T>& d = recReg[size_t(e) * 5 + k];
+                d.assign(size_t(rp.nReg()) * nts, T(0));
+                if (data) rp.adjoint(d.data(), data + (size_t(e) * 5 + k) * nRec * nts);
+                s.rec_tr[k] = d.data();
+            }
+        }
+        return shots;
+    }
+
+    void register_model() {
+        eio_grid g;
+        g.nz = fd_->nz; g.nx = fd_->nx; g.nts = fd_->nts; g.sub_fd = fd_->sub; g.minPad = fd_->minPad;
+        g.dz = fd_->dz; g.dx = fd_->dx; g.dts = fd_->dts; g.alphaCos = fd_->alphaCos;
+        const size_t n = fd_->ncell();
+        const void* media[5] = {fd_->media.data(), fd_->media.data() + n, fd_->media.data() + 2 * n,
+                                fd_->media.data() + 3 * n, fd_->media.data() + 4 * n};
+        check_eio(eio_model_add(dev_->ctx, &g, media, 0, &model_));
+    }
+
+    std::vector<Axis> ax_;
+    Params par_;
+    std::vector<std::vector<T>> sig_;
+    std::vector<InterpVec> src_, rec_;
+    int nExp_ = 1;
+    std::unique_ptr<Device> dev_;
+    std::unique_ptr<FdParam<T>> fd_;
+    int model_ = -1;
+};
+
+}  // namespace eioh

@@ -35,3 +35,37 @@ def random_traces(rng, spos, nt, dtype, smooth=True):
             t -= t.mean(axis=1, keepdims=True)
         out.append(np.ascontiguousarray(t, dtype=dtype))
     return out
+
+
+def operator_case(tmp_path, w, nExp=1, zSource=4, xSource=None, spacing=None, nRec=None, depthRec=2,
+                  oRec=2, dRec=2, extra=None):
+    """Write the padded VpVsRho model and a Ricker source file and return (args, coords) for
+    the parfile-driven init functions, plus the oracle-side coordinates of the same geometry."""
+    from elastic_isotropic_operators_amd.sep import Hypercube, SepVector, genericIO
+    nx_int = w["nx"] - 8 - w["xPadMinus"] - w["xPadPlus"]
+    xSource = xSource if xSource is not None else nx_int // 3
+    spacing = spacing if spacing is not None else max(1, nx_int // (2 * nExp))
+    nRec = nRec if nRec is not None else nx_int // 2 - 2
+    hyp = Hypercube.hypercube(axes=[Hypercube.axis(n=w["nz"], o=w["oz"], d=w["dz"]),
+                                    Hypercube.axis(n=w["nx"], o=w["ox"], d=w["dx"]),
+                                    Hypercube.axis(n=3)])
+    mod = SepVector.getSepVector(hyp)
+    mod.getNdArray()[...] = w["vvr"]
+    mfile = str(tmp_path / "model.H")
+    genericIO.defaultIO.writeVector(mfile, mod)
+    src = SepVector.getSepVector(Hypercube.hypercube(
+        axes=[Hypercube.axis(n=w["nts"], d=w["dts"]), Hypercube.axis(n=1), Hypercube.axis(n=5)]))
+    src.getNdArray()[...] = w["model"][:, :, :]
+    sfile = str(tmp_path / "sources.H")
+    genericIO.defaultIO.writeVector(sfile, src)
+    par = dict(nts=w["nts"], dts=w["dts"], sub=w["sub"], nz=w["nz"], nx=w["nx"], dz=w["dz"], dx=w["dx"],
+               fMax=w["fMax"], mod_par=1, zPadMinus=w["zPadMinus"], zPadPlus=w["zPadPlus"],
+               xPadMinus=w["xPadMinus"], xPadPlus=w["xPadPlus"], nExp=nExp, zSource=zSource,
+               xSource=xSource, spacingShots=spacing, nReceiver=nRec, depthReceiver=depthRec,
+               oReceiver=oRec, dReceiver=dRec, iGpu=0, blockSize=16, elasticParam=mfile, sources=sfile)
+    if extra:
+        par.update(extra)
+    args = ["%s=%s" % (k, v) for k, v in par.items()]
+    sz, sx = W.source_coords(w, zSource, xSource, nExp=nExp, spacingShots=spacing)
+    rz, rx = W.receiver_coords(w, nRec, depthRec, oRec, dRec)
+    return args, dict(sz=sz, sx=sx, rz=rz, rx=rx)

@@ -1,0 +1,134 @@
+"""CPU tests of the product's host side: C-ABI library exports, C++ host layer (parameter
+validation, media, spatial interpolation) against the oracle, numpy host prep, and the
+error behaviour of the drop-in operators that fires before any GPU work."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+from elastic_isotropic_operators_amd import _capi, geometry as G, workloads as W
+from elastic_isotropic_operators_amd.sep import genericIO
+
+from helpers import operator_case, oracle_grid, reg_positions, rel_l2
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_c_abi_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "eio.h")).read()
+    names = set(re.findall(r"^\s*(?:int|const char\*|void)\s+(eio_\w+)\s*\(", hdr, re.M))
+    assert len(names) >= 17
+    lib = ctypes.CDLL(_capi.LIB_PATH)
+    for n in sorted(names):
+        assert hasattr(lib, n), n
+    assert set(_capi.EXPORTED) <= names
+
+
+def _conv_elastic(args):
+    par = genericIO.io(params=args)
+    el = genericIO.defaultIO.getVector(par.getString("elasticParam"), ndims=3)
+    from elastic_isotropic_operators_amd import elasticParamConvertModule as C
+    c = el.clone()
+    C.ElasticConv(el, 1).forward(False, el, c)
+    return par, c
+
+
+def test_fdparam_media_and_checks(tmp_path):
+    from elastic_isotropic_operators_amd import pyElastic_iso_float_nl as nl
+    w = W.small(nz=40, nx=56, nts=100, sub=2)
+    args, _ = operator_case(tmp_path, w)
+    par, c = _conv_elastic(args)
+    fd = nl.fdParamElastic(c, par)
+    assert (fd.nts, fd.sub, fd.ntw) == (100, 4, 397)
+    med = O.media(oracle_grid(w), w["elastic"])
+    for k in range(5):
+        assert rel_l2(fd.getMedia()[k], med[k]) < 1e-6
+    # Courant > 0.45 (fdParamElastic.cpp:256-267)
+    bad = dict(par.param, sub="1", dts="0.004")
+    with pytest.raises(RuntimeError, match="Courant"):
+        nl.fdParamElastic(c, bad)
+    # dispersion < 3 (:269-281)
+    with pytest.raises(RuntimeError, match="Dispersion"):
+        nl.fdParamElastic(c, dict(par.param, fMax="200"))
+    # block size (:283-294, NLF:204-208)
+    with pytest.raises(RuntimeError, match="(?i)block size"):
+        nl.fdParamElastic(c, dict(par.param, blockSize="8"))
+    # 2*sub > SUB_MAX (:29-32)
+    with pytest.raises(RuntimeError, match="SUB_MAX"):
+        nl.fdParamElastic(c, dict(par.param, sub="51"))
+    # parfile / model consistency (:296-324)
+    with pytest.raises(RuntimeError, match="nz not consistent"):
+        nl.fdParamElastic(c, dict(par.param, nz=str(w["nz"] + 16)))
+
+
+def test_fdparam_time_axes_known_answer(tmp_path):
+    """sub=10 -> fine sub-sampling 20, ntw = 39981 for nts = 2000 (NB/Nonlinear.ipynb cell 23)."""
+    from elastic_isotropic_operators_amd import pyElastic_iso_float_nl as nl
+    w = W.small(nz=40, nx=56, nts=2000, sub=10)
+    w["dts"] = 0.004
+    args, _ = operator_case(tmp_path, w, extra=dict(dts=0.004, fMax=5))
+    par, c = _conv_elastic(args)
+    fd = nl.fdParamElastic(c, par)
+    assert (fd.sub, fd.ntw) == (20, 39981) and abs(fd.dtw - 0.0002) < 1e-9
+
+
+def test_space_interp_matches_oracle(tmp_path):
+    from elastic_isotropic_operators_amd import Elastic_iso_float_prop as PF
+    w = W.small(nz=40, nx=56, nts=100, sub=2)
+    args, geo = operator_case(tmp_path, w, nExp=3)
+    par, c = _conv_elastic(args)
+    sc, sx, sz, sxz, _ = PF.buildSourceGeometry(par, c)
+    rc, rx, rz, rxz, _ = PF.buildReceiversGeometry(par, c)
+    for e in range(3):
+        ref = reg_positions(w, geo["sz"][e], geo["sx"][e])
+        for g, lst in enumerate((sc, sx, sz, sxz)):
+            assert list(lst[e].getRegPosUnique()) == list(ref[g])
+    ref = reg_positions(w, geo["rz"], geo["rx"])
+    for g, lst in enumerate((rc, rx, rz, rxz)):
+        assert list(lst[0].getRegPosUnique()) == list(ref[g])
+        wts = np.array(lst[0].getWeights()).reshape(-1, 4)
+        assert np.allclose(wts.sum(axis=1), 1.0)
+
+
+def test_space_interp_out_of_bounds():
+    from elastic_isotropic_operators_amd import pyElastic_iso_float_nl as nl
+    from elastic_isotropic_operators_amd.sep import Hypercube
+    h = Hypercube.hypercube(axes=[Hypercube.axis(n=50, d=10.0), Hypercube.axis(n=60, d=10.0)])
+    with pytest.raises(RuntimeError, match="out of bounds in the z"):
+        nl.spaceInterpGpu(np.array([5.0], np.float32), np.array([300.0], np.float32), h, 10, "linear", 1, 0, 0, 0)
+    with pytest.raises(RuntimeError, match="not available"):
+        nl.spaceInterpGpu(np.array([200.0], np.float32), np.array([300.0], np.float32), h, 10, "sinc", 4, 0, 0, 0)
+
+
+def test_duplicate_gpu_ids_rejected_before_device_use(tmp_path):
+    from elastic_isotropic_operators_amd import Elastic_iso_float_prop as PF
+    w = W.small(nz=40, nx=56, nts=100, sub=2)
+    args, _ = operator_case(tmp_path, w, extra=dict(iGpu="0,0"))
+    (model, data, elastic, par, sc, sx, sz, sxz, rc, rx, rz, rxz, _) = PF.nonlinearOpInitFloat(args)
+    with pytest.raises(RuntimeError, match="duplicates"):
+        PF.nonlinearPropElasticShotsGpu(model, data, elastic, par, sc, sx, sz, sxz, rc, rx, rz, rxz)
+
+
+def test_numpy_host_prep_matches_oracle():
+    for w in (W.small(), W.gaussian_c0()):
+        g = oracle_grid(w)
+        med64 = O.media(g, w["elastic"])
+        med32 = G.scaled_media(w["elastic"], G.dtw_of(w["dts"], w["sub"]))
+        for k in range(5):
+            assert rel_l2(med32[k], med64[k]) < 1e-6
+        for zc, xc in ((w["src_z"][0], w["src_x"][0]), (w["rec_z"], w["rec_x"])):
+            a = reg_positions(w, zc, xc)
+            b = [ip.uniq for ip in G.staggered_interps(w["nz"], w["oz"], w["dz"], w["ox"], w["dx"], zc, xc)]
+            assert all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
+def test_workload_sizes():
+    c0 = W.gaussian_c0()
+    assert (c0["nz"], c0["nx"], W.min_pad(c0)) == (280, 472, 30)
+    assert W.cell_updates(c0) == 272 * 464 * 1998
+    shots = W.layered_batch_c2(64)
+    tot = sum(W.cell_updates(w) for w in shots)
+    assert 4e11 < tot < 6e11

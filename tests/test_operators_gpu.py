@@ -1,0 +1,135 @@
+"""The drop-in operator layer (parfile -> *OpInit* -> compiled pybind operators -> C ABI ->
+HIP) against the CPU oracle's shot-level pipeline (FP64), and its adjoint relations.
+
+Mirrors how the reference scripts use the operators (nonlinearPythonElasticFloatMain.py,
+BornPythonElasticFloatMain.py): forward, adjoint, add semantics, setBackground, dot tests.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from elastic_isotropic_operators_amd import Elastic_iso_double_prop as PD
+from elastic_isotropic_operators_amd import Elastic_iso_float_prop as PF
+from elastic_isotropic_operators_amd import workloads as W
+
+from helpers import operator_case, oracle_grid, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _small(nts=150, seed=0):
+    return W.small(nz=40, nx=56, nts=nts, sub=2, seed=seed)
+
+
+def _nl_op(P, args, init):
+    (model, data, elastic, par, sc, sx, sz, sxz, rc, rx, rz, rxz, _) = init(args)
+    op = P.nonlinearPropElasticShotsGpu(model, data, elastic, par, sc, sx, sz, sxz, rc, rx, rz, rxz)
+    return op, model, data, elastic
+
+
+def test_nl_forward_and_adjoint_vs_oracle(tmp_path):
+    w = _small()
+    args, geo = operator_case(tmp_path, w, nExp=3)
+    op, model, data, elastic = _nl_op(PF, args, PF.nonlinearOpInitFloat)
+    model.getNdArray()[0] = w["model"].astype(np.float32)
+    op.forward(False, model, data)
+    g = oracle_grid(w)
+    d = data.getNdArray()
+    for e in range(3):
+        ref = O.nl_fwd_shot(g, w["elastic"], geo["sz"][e], geo["sx"][e], geo["rz"], geo["rx"], w["model"])
+        assert np.linalg.norm(ref) > 0
+        assert rel_l2(d[e], ref) < 1e-5, (e, rel_l2(d[e], ref))
+    # adjoint of random data, stacked over shots into the single source signal
+    rng = np.random.default_rng(3)
+    dd = data.clone()
+    dd.getNdArray()[...] = rng.standard_normal(dd.getNdArray().shape)
+    mm = model.clone()
+    op.adjoint(False, mm, dd)
+    ref = sum(O.nl_adj_shot(g, w["elastic"], geo["sz"][e], geo["sx"][e], geo["rz"], geo["rx"],
+                            dd.getNdArray()[e].astype(np.float64)) for e in range(3))
+    assert rel_l2(mm.getNdArray()[0], ref) < 1e-5
+
+
+def test_nl_add_semantics(tmp_path):
+    w = _small(nts=80)
+    args, geo = operator_case(tmp_path, w, nExp=2)
+    op, model, data, elastic = _nl_op(PF, args, PF.nonlinearOpInitFloat)
+    model.getNdArray()[0] = w["model"]
+    op.forward(False, model, data)
+    d1 = data.getNdArray().copy()
+    op.forward(True, model, data)
+    assert rel_l2(data.getNdArray(), 2 * d1) < 1e-6
+
+
+def test_nl_dot_tests_fp32_and_fp64(tmp_path):
+    w = _small(nts=100, seed=1)
+    args, _ = operator_case(tmp_path, w, nExp=2)
+    op32, *_ = _nl_op(PF, args, PF.nonlinearOpInitFloat)
+    assert op32.dotTestCpp(False, 1e-5)
+    op64, *_ = _nl_op(PD, args, PD.nonlinearOpInitDouble)
+    assert op64.dotTestCpp(False, 1e-12)
+    e1, e2 = op64.dotTest(False, 1e-12)
+    assert e1 < 1e-12 and e2 < 1e-12
+
+
+def test_set_background_matches_new_operator(tmp_path):
+    w = _small(nts=80, seed=2)
+    args, geo = operator_case(tmp_path, w, nExp=1)
+    op, model, data, elastic = _nl_op(PF, args, PF.nonlinearOpInitFloat)
+    model.getNdArray()[0] = w["model"]
+    el2 = elastic.clone()
+    el2.getNdArray()[...] *= np.float32(1.02)
+    op.setBackground(el2)
+    op.forward(False, model, data)
+    g = oracle_grid(w)
+    ref = O.nl_fwd_shot(g, w["elastic"] * 1.02, geo["sz"][0], geo["sx"][0], geo["rz"], geo["rx"], w["model"])
+    assert rel_l2(data.getNdArray()[0], ref) < 1e-5
+
+
+def _born(P, args, init):
+    (model, data, elastic, par, sig, sc, sx, sz, sxz, rc, rx, rz, rxz, _) = init(args)
+    op = P.BornElasticShotsGpu(model, data, elastic, par, sig, sc, sx, sz, sxz, rc, rx, rz, rxz)
+    return op, model, data, elastic, sig
+
+
+def test_born_forward_adjoint_vs_oracle(tmp_path):
+    w = _small(nts=120, seed=4)
+    args, geo = operator_case(tmp_path, w, nExp=2)
+    op, model, data, elastic, sig = _born(PF, args, PF.BornOpInitFloat)
+    dm = W.born_perturbation(w, seed=5)
+    model.getNdArray()[...] = dm
+    op.forward(False, model, data)
+    g = oracle_grid(w)
+    for e in range(2):
+        ref = O.born_fwd_shot(g, w["elastic"], geo["sz"][e], geo["sx"][e], w["model"], geo["rz"], geo["rx"], dm)
+        assert np.linalg.norm(ref) > 0
+        assert rel_l2(data.getNdArray()[e], ref) < 1e-5, (e, rel_l2(data.getNdArray()[e], ref))
+    rng = np.random.default_rng(6)
+    dd = data.clone()
+    dd.getNdArray()[...] = rng.standard_normal(dd.getNdArray().shape)
+    mm = model.clone()
+    op.adjoint(False, mm, dd)
+    ref = O.born_adj_shots(g, w["elastic"], geo["sz"], geo["sx"], w["model"], geo["rz"], geo["rx"],
+                           dd.getNdArray().astype(np.float64))
+    assert rel_l2(mm.getNdArray(), ref) < 1e-5, rel_l2(mm.getNdArray(), ref)
+
+
+def test_born_dot_test_fp64(tmp_path):
+    w = _small(nts=100, seed=7)
+    args, _ = operator_case(tmp_path, w, nExp=2)
+    op, *_ = _born(PD, args, PD.BornOpInitDouble)
+    e1, e2 = op.dotTest(False, 1e-12)
+    assert e1 < 1e-12 and e2 < 1e-12
+
+
+def test_fwi_operator_equals_nl_forward(tmp_path):
+    w = _small(nts=80, seed=8)
+    args, _ = operator_case(tmp_path, w, nExp=2)
+    (elastic, elasticConv, data, sources, par, sc, sx, sz, sxz, rc, rx, rz, rxz, _) = PF.nonlinearFwiOpInitFloat(args)
+    fwi = PF.nonlinearFwiPropElasticShotsGpu(elasticConv, data, sources, par, sc, sx, sz, sxz, rc, rx, rz, rxz)
+    d_fwi = data.clone()
+    fwi.forward(False, elasticConv, d_fwi)
+    op, model, data2, el2 = _nl_op(PF, args, PF.nonlinearOpInitFloat)
+    model.getNdArray()[...] = sources.getNdArray()
+    op.forward(False, model, data2)
+    assert rel_l2(d_fwi.getNdArray(), data2.getNdArray()) < 1e-6

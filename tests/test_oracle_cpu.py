@@ -1,0 +1,259 @@
+"""CPU tests that pin the oracle before it is trusted as the checker.
+
+Each check is chosen so that a plausible mistake (dropped term, wrong sign or index,
+transposed operand, wrong scaling) fails it:
+  * FD coefficients vs their closed forms and polynomial exactness of D+ / D-;
+  * time-interpolation filter, damping table, Ricker: closed forms of the cited lines;
+  * staggering, spatial and time interpolation: transposition (dot) tests and exact cases;
+  * propagators: FP64 dot-product tests (NL and Born) at 1e-12;
+  * physics: P-wave moveout in a homogeneous medium (speed of the discrete wave equation),
+    Born operator vs. finite differences of the NL operator (ties Born to NL), sponge decay;
+  * FP32 twin vs FP64 within 1e-5 (the FP32 parity bar is reachable).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from elastic_isotropic_operators_amd import seis_utils as su
+from elastic_isotropic_operators_amd import workloads as W
+
+from helpers import oracle_grid, rel_l2
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_fd_coefficients_closed_form():
+    g = O.grid(30, 30, 10, 2, 1.0, 1.0, 0.001, 5)
+    coef, _, _ = O.consts(g)
+    closed = np.array([1225 / 1024, -245 / 3072, 49 / 5120, -5 / 7168])
+    assert np.allclose(coef[:4], closed, rtol=1e-10)
+    assert np.allclose(coef[4:], closed, rtol=1e-10)
+
+
+def test_staggered_derivative_polynomial_exactness():
+    """D+ f(i) approximates f'(i+1/2) exactly for polynomials up to degree 7 (8th order)."""
+    g = O.grid(30, 30, 10, 2, 1.0, 1.0, 0.001, 5)
+    c = O.consts(g)[0][:4]
+    i = 10.0
+    for deg in range(8):
+        f = lambda x: x ** deg
+        dplus = sum(c[k] * (f(i + 1 + k) - f(i - k)) for k in range(4))
+        exact = deg * (i + 0.5) ** (deg - 1) if deg > 0 else 0.0
+        assert abs(dplus - exact) <= 1e-7 * max(1.0, abs(exact)), (deg, dplus, exact)
+
+
+@pytest.mark.parametrize("sub", [2, 4, 20])
+def test_time_filter_closed_form(sub):
+    nts = 50
+    g = O.grid(30, 30, nts, sub, 10.0, 10.0, 0.004, 5)
+    _, F, _ = O.consts(g)
+    h = sub + 1
+    ntw = (nts - 1) * sub + 1
+    s = 1.0 / np.sqrt(ntw / nts)
+    i = np.arange(h)
+    assert np.allclose(F[:h], (1 - i / sub) * s)
+    assert np.allclose(F[h:], (i / sub) * s)
+    assert np.allclose(F[:h] + F[h:], s)
+
+
+def test_damping_table_closed_form():
+    g = O.grid(100, 100, 10, 2, 10.0, 10.0, 0.001, 30, alphaCos=0.99)
+    _, _, d = O.consts(g)
+    j = np.arange(30)
+    assert np.allclose(d, 0.99 + 0.01 * np.cos(np.pi * (30 - j) / 30))
+    assert abs(d[0] - 0.98) < 1e-12 and np.all(np.diff(d) > 0) and d[-1] < 1
+
+
+def test_ricker_closed_form():
+    r = su.ricker(200, 0.001, 15.0, 0.1)
+    assert abs(r[100] - 1.0) < 1e-6
+    t = 0.02
+    a = (np.pi * 15.0) ** 2
+    assert abs(r[120] - (1 - 2 * a * t * t) * np.exp(-a * t * t)) < 1e-6
+
+
+@pytest.mark.parametrize("which", ["x", "z"])
+def test_stagger_transposition(which):
+    rng = np.random.default_rng(0)
+    m = rng.standard_normal((13, 17))
+    d = rng.standard_normal((13, 17))
+    a = np.sum(O.stagger(which + "_fwd", m) * d)
+    b = np.sum(m * O.stagger(which + "_adj", d))
+    assert abs(a - b) < 1e-12 * abs(a)
+    # interior two-point average
+    if which == "x":
+        assert np.allclose(O.stagger("x_adj", d)[5, 3], 0.5 * (d[5, 3] + d[4, 3]))
+    else:
+        assert np.allclose(O.stagger("z_adj", d)[5, 3], 0.5 * (d[5, 3] + d[5, 2]))
+
+
+def test_interp_weights():
+    nz = 50
+    idx, w, uniq = O.interp(nz, 0.0, 10.0, 0.0, 10.0, [100.0, 123.0], [200.0, 257.5])
+    assert np.allclose(w.sum(axis=1), 1.0)
+    assert idx[0, 0] == 20 * nz + 10 and abs(w[0, 0] - 1.0) < 1e-12
+    # second device: wz = 12.3 -> 0.7 on iz=12, wx = 25.75 -> 0.25 on ix=25
+    assert idx[1, 0] == 25 * nz + 12
+    assert np.allclose(w[1], [0.7 * 0.25, 0.3 * 0.25, 0.7 * 0.75, 0.3 * 0.75])
+    assert len(uniq) == 8
+
+
+def test_time_interp_linear_exact():
+    nts, sub = 20, 4
+    coarse = np.stack([np.arange(nts) * 2.0 + 1.0])
+    fine = O.time_interp(coarse, 0.004, sub)
+    ntw = (nts - 1) * sub + 1
+    s = 1 / np.sqrt(ntw / nts)
+    assert np.allclose(fine[0], (np.arange(ntw) / sub * 2.0 + 1.0) * s)
+
+
+def test_media_homogeneous():
+    w = W.small(nz=20, nx=24, nts=10, sub=2)
+    el = np.zeros_like(w["elastic"])
+    el[0], el[1], el[2] = 2000.0, 1e9, 2e9
+    g = oracle_grid(w)
+    med = O.media(g, el)
+    dtw = w["dts"] / 4
+    i = (10, 10)
+    assert np.isclose(med[0][i], 2 * dtw / 2000) and np.isclose(med[1][i], 2 * dtw / 2000)
+    assert np.isclose(med[2][i], 2 * dtw * 5e9) and np.isclose(med[3][i], 2 * dtw * 1e9)
+    assert np.isclose(med[4][i], 2 * dtw * 2e9)
+    assert np.isclose(med[0][0, 5], 2 * dtw / 1000)  # half-weight boundary of staggerX^T
+
+
+def _small(nts=120, seed=0):
+    return W.small(nz=40, nx=56, nts=nts, sub=2, seed=seed)
+
+
+def test_nl_dot_product_fp64():
+    w = _small(seed=1)
+    g = oracle_grid(w)
+    rng = np.random.default_rng(2)
+    m = rng.standard_normal((5, 1, w["nts"]))
+    d = rng.standard_normal((5, len(w["rec_z"]), w["nts"]))
+    Fm = O.nl_fwd_shot(g, w["elastic"], w["src_z"][0], w["src_x"][0], w["rec_z"], w["rec_x"], m)
+    Ftd = O.nl_adj_shot(g, w["elastic"], w["src_z"][0], w["src_x"][0], w["rec_z"], w["rec_x"], d)
+    a, b = np.sum(Fm * d), np.sum(m * Ftd)
+    assert abs(a - b) / abs(b) < 1e-12
+
+
+def test_born_dot_product_fp64():
+    w = _small(seed=2)
+    g = oracle_grid(w)
+    rng = np.random.default_rng(3)
+    dm = rng.standard_normal((3, w["nx"], w["nz"])) * np.array([100.0, 1e8, 1e8])[:, None, None]
+    d = rng.standard_normal((1, 5, len(w["rec_z"]), w["nts"]))
+    B = O.born_fwd_shot(g, w["elastic"], w["src_z"][0], w["src_x"][0], w["model"], w["rec_z"], w["rec_x"], dm)
+    Bt = O.born_adj_shots(g, w["elastic"], w["src_z"][:1], w["src_x"][:1], w["model"], w["rec_z"], w["rec_x"], d)
+    a, b = np.sum(B * d[0]), np.sum(dm * Bt)
+    assert abs(a - b) / abs(b) < 1e-12
+
+
+def test_regular_level_dot_product_fp64():
+    """The unfused reference loop order is exactly adjoint at the GPU-seam level too."""
+    w = _small(seed=3)
+    g = oracle_grid(w)
+    from helpers import reg_positions
+    med = O.media(g, w["elastic"])
+    sp = reg_positions(w, w["src_z"][0], w["src_x"][0])
+    rp = reg_positions(w, w["rec_z"], w["rec_x"])
+    rng = np.random.default_rng(4)
+    m = [rng.standard_normal((sp[k].size, g.nts)) for k in O.GRID_OF_COMP]
+    d = [rng.standard_normal((rp[k].size, g.nts)) for k in O.GRID_OF_COMP]
+    Fm = O.nl_fwd_reg(g, med, sp, m, rp)
+    Ftd = O.nl_adj_reg(g, med, sp, rp, d)
+    a = sum(np.sum(x * y) for x, y in zip(Fm, d))
+    b = sum(np.sum(x * y) for x, y in zip(m, Ftd))
+    assert abs(a - b) / abs(b) < 1e-12
+
+
+def test_p_wave_moveout_homogeneous():
+    """Cross-correlation lag between receivers 400 m and 1000 m from an explosive source in a
+    homogeneous medium equals 600 m / vp: pins the stencil, media and time-step scaling."""
+    nz, nx, vp = 120, 400, 3000.0
+    vvr = np.zeros((3, nx, nz), np.float32)
+    vvr[0], vvr[1], vvr[2] = vp, vp / 1.73, 2000.0
+    w = W._grid_dict(vvr, 10.0, 10.0, 30, 30, 700, 0.001, 1, 30.0)
+    w["src_z"], w["src_x"] = W.source_coords(w, 60, 50)
+    w["rec_z"], w["rec_x"] = W.receiver_coords(w, 300, 60, 51, 1)
+    w["model"] = W.explosive_model(700, 0.001, 15.0, 0.1)
+    d = O.nl_fwd_shot(oracle_grid(w), w["elastic"], w["src_z"][0], w["src_x"][0], w["rec_z"], w["rec_x"], w["model"])
+    p = 0.5 * (d[2] + d[3])
+    xs = w["src_x"][0][0]
+    i1 = int(np.argmin(abs(w["rec_x"] - xs - 400)))
+    i2 = int(np.argmin(abs(w["rec_x"] - xs - 1000)))
+    c = np.correlate(p[i2], p[i1], "full")
+    k = int(np.argmax(c))
+    frac = 0.5 * (c[k - 1] - c[k + 1]) / (c[k - 1] - 2 * c[k] + c[k + 1])
+    lag = (k + frac - (p.shape[1] - 1)) * 0.001
+    assert abs(lag - 600.0 / vp) < 1e-3, lag
+
+
+def test_born_matches_nl_linearisation():
+    """B dm ~ [F(m + e dm) - F(m - e dm)] / 2e: the Born operator (coarse-time secondary
+    sources) agrees with the NL operator's linearisation to a few percent."""
+    w = W.small(nz=40, nx=56, nts=200, sub=2, seed=0)
+    g = oracle_grid(w)
+    dm = W.born_perturbation(w, seed=5)
+    sz, sx = w["src_z"][0], w["src_x"][0]
+    B = O.born_fwd_shot(g, w["elastic"], sz, sx, w["model"], w["rec_z"], w["rec_x"], dm)
+    eps = 1e-2
+    Fp = O.nl_fwd_shot(g, w["elastic"] + eps * dm, sz, sx, w["rec_z"], w["rec_x"], w["model"])
+    Fm = O.nl_fwd_shot(g, w["elastic"] - eps * dm, sz, sx, w["rec_z"], w["rec_x"], w["model"])
+    FD = (Fp - Fm) / (2 * eps)
+    assert rel_l2(B, FD) < 0.05
+    assert np.sum(B * FD) / np.linalg.norm(B) / np.linalg.norm(FD) > 0.999
+
+
+def test_fp32_twin_within_1e5():
+    w = _small(nts=200, seed=4)
+    g = oracle_grid(w)
+    d64 = O.nl_fwd_shot(g, w["elastic"], w["src_z"][0], w["src_x"][0], w["rec_z"], w["rec_x"], w["model"])
+    d32 = O.nl_fwd_shot(g, w["elastic"], w["src_z"][0], w["src_x"][0], w["rec_z"], w["rec_x"], w["model"],
+                        np.float32)
+    assert rel_l2(d32, d64) < 1e-5
+
+
+def test_padding_known_answers():
+    kat = json.load(open(os.path.join(GOLD, "kat.json")))
+    for key in ("pad_marmousi", "pad_layers"):
+        k = kat[key]
+        s = su.pad_sizes(k["nz"], k["nx"], k["zPad"], k["xPad"])
+        assert (s["zPadPlus"], s["xPadPlus"], s["nz"], s["nx"]) == (k["zPadPlus"], k["xPadPlus"], k["nzNew"], k["nxNew"])
+
+
+def test_padding_edge_replication():
+    m = np.arange(2 * 5 * 3, dtype=np.float32).reshape(2, 5, 3)
+    p, s = su.pad_model(m, 2, 3)
+    x0, z0 = 4 + 3, 4 + 2
+    assert np.array_equal(p[:, x0:x0 + 5, z0:z0 + 3], m)
+    assert np.all(p[:, x0, :z0] == m[:, 0, :1])
+    assert np.all(p[:, :x0, :] == p[:, x0:x0 + 1, :])
+    assert np.all(p[:, -1, :] == p[:, x0 + 4, :])
+
+
+def test_conversion_jacobian_and_datcomp_adjoints():
+    rng = np.random.default_rng(7)
+    bg = np.abs(rng.standard_normal((3, 6, 5))) + 1
+    x, y = rng.standard_normal((3, 6, 5)), rng.standard_normal((3, 6, 5))
+    assert np.isclose(np.sum(su.conv_jacobian_fwd(bg, x) * y), np.sum(x * su.conv_jacobian_adj(bg, y)))
+    d = rng.standard_normal((2, 5, 4, 7))
+    s = rng.standard_normal((2, 3, 4, 7))
+    comps = "vx,p,sxz"
+    assert np.isclose(np.sum(su.datcomp_fwd(comps, d) * s), np.sum(d * su.datcomp_adj(comps, s, d.shape)))
+    # Jacobian matches finite differences of the conversion
+    e = 1e-6
+    fd = (su.vpvsrho_to_rholamemu(bg + e * x) - su.vpvsrho_to_rholamemu(bg - e * x)) / (2 * e)
+    assert np.allclose(fd, su.conv_jacobian_fwd(bg, x), rtol=1e-6, atol=1e-6)
+
+
+def test_reference_fixture_model():
+    from elastic_isotropic_operators_amd.sep import genericIO
+    v = genericIO.defaultIO.getVector(os.path.join(GOLD, "true_layers.H"), ndims=3)
+    a = v.getNdArray()
+    assert a.shape == (3, 201, 101)
+    assert a[0, 0, 0] == 1800.0 and a[1, 0, 0] == 1300.0 and a[2, 0, 0] == 2000.0
+    assert a.min() == 1300.0 and a.max() == 2400.0
