@@ -72,6 +72,13 @@ def lib():
             ("eio_ctx_variant_info", [ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                       ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
             ("eio_ctx_num_variants", []),
+            ("eio_fwi_forward", [vp, ctypes.c_int, ctypes.POINTER(Shot), ctypes.c_int, ctypes.c_int,
+                                 ctypes.POINTER(ctypes.c_int)]),
+            ("eio_fwi_gradient", [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Shot),
+                                  ctypes.POINTER(vp), ctypes.c_int]),
+            ("eio_fwi_info", [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                              ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_ulonglong)]),
+            ("eio_fwi_release", [vp, ctypes.c_int]),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -85,7 +92,8 @@ EXPORTED = ("eio_last_error", "eio_ctx_create", "eio_ctx_destroy", "eio_ctx_set_
             "eio_model_remove", "eio_nl_forward", "eio_nl_adjoint", "eio_born_forward",
             "eio_born_adjoint", "eio_device_alloc", "eio_device_free", "eio_memcpy",
             "eio_synchronize", "eio_copy_probe", "eio_ctx_set_variant", "eio_ctx_variant_info",
-            "eio_ctx_num_variants")
+            "eio_ctx_num_variants", "eio_fwi_forward", "eio_fwi_gradient", "eio_fwi_info",
+            "eio_fwi_release")
 
 
 def check(rc):
@@ -259,3 +267,32 @@ class Context:
             dp = (ctypes.c_void_p * 5)(*[dm5_out[k].ctypes.data for k in range(5)])
         check(lib().eio_born_adjoint(self.h, len(shots), arr, dp,
                                      EIO_DEVICE_PTRS if device_ptrs else 0))
+
+    # -------------------------------------------------------------- checkpointed FWI gradient
+    def fwi_forward(self, shots, segment=0, device_ptrs=False):
+        """NL forward with checkpoints; returns the store handle."""
+        arr = self._shots(shots)
+        h = ctypes.c_int()
+        check(lib().eio_fwi_forward(self.h, len(shots), arr, EIO_DEVICE_PTRS if device_ptrs else 0,
+                                    int(segment), ctypes.byref(h)))
+        return h.value
+
+    def fwi_gradient(self, handle, shots, dm5_out, device_ptrs=False):
+        arr = self._shots(shots)
+        if device_ptrs:
+            dp = (ctypes.c_void_p * 5)(*dm5_out)
+        else:
+            if not dm5_out.flags.c_contiguous or dm5_out.dtype != self.dtype:
+                raise EioError("dm must be C-contiguous %s" % self.dtype)
+            dp = (ctypes.c_void_p * 5)(*[dm5_out[k].ctypes.data for k in range(5)])
+        check(lib().eio_fwi_gradient(self.h, int(handle), len(shots), arr, dp,
+                                     EIO_DEVICE_PTRS if device_ptrs else 0))
+
+    def fwi_info(self, handle):
+        """(segment length, number of checkpoints, device bytes held)."""
+        L, n, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_ulonglong()
+        check(lib().eio_fwi_info(self.h, int(handle), ctypes.byref(L), ctypes.byref(n), ctypes.byref(b)))
+        return L.value, n.value, b.value
+
+    def fwi_release(self, handle):
+        check(lib().eio_fwi_release(self.h, int(handle)))

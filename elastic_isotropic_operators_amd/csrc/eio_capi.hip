@@ -80,6 +80,19 @@ struct ModelBase {
     double cz[4], cx[4];
 };
 
+// Forward-wavefield store of the checkpointed FWI gradient: the batch state at the start of
+// every segment of L coarse intervals but the first (which starts from rest), plus the coarse
+// Vx/Vz ring (L+3 slices per shot) that already holds the last segment.
+struct FwiStore {
+    int prec = 4, nshot = 0, model = 0, nts = 0, sub = 0, L = 0, nseg = 0, wcap = 0;
+    unsigned long long sig = 0;  // hash of the shot geometry the store was made for
+    size_t state_elems = 0;      // elements of one batch state (10 fields per shot)
+    void* ck = nullptr;          // (nseg-1) batch states
+    void* ring = nullptr;        // per ordered shot: Vx then Vz, wcap slices each
+    bool ring_fresh = false;     // the ring holds the last segment (true right after the forward)
+    size_t bytes = 0;
+};
+
 struct Ctx {
     int device = 0;
     int prec = 4;
@@ -93,6 +106,8 @@ struct Ctx {
     long long last_nstep = 0;
     int use_graph = 0;
     int variant = 1;  // best measured FP32 variant on the 64-shot layered batch
+    std::map<int, FwiStore> fwi;
+    int next_fwi = 1;
 };
 
 void* ws_get(Ctx* c, const std::string& name, size_t bytes) {
@@ -198,7 +213,7 @@ ModelBase& get_model(Ctx* c, int id) {
 }
 
 // ------------------------------------------------------------------ batch planning
-enum class Op { NL_FWD, NL_ADJ, BORN_FWD, BORN_ADJ };
+enum class Op { NL_FWD, NL_ADJ, BORN_FWD, BORN_ADJ, FWI_FWD, FWI_GRAD };
 
 struct Plan {
     std::vector<int> order;               // shot indices sorted by decreasing ntw
@@ -294,19 +309,26 @@ struct Run {
     std::vector<Out> outs;
 };
 
+// Builds the plan, the per-shot descriptors and the device tables of one call, uploads
+// input traces and zeroes the state and the outputs.  FWI calls bring their own coarse
+// storage ring (`ring`, `wcap` slices per component and shot).
 template <typename T>
 Run<T> prepare(Ctx* c, int nshot, const eio_shot* shots, Op op, int flags,
-               const void* const dm_fwd[5]) {
+               const void* const dm_fwd[5], T* ring = nullptr, int wcap = 0) {
     if (nshot <= 0) fail("nshot must be > 0");
     if (!shots) fail("shots is NULL");
     Run<T> run;
     run.c = c;
     run.plan = make_plan<T>(c, nshot, shots, op);
     const bool born = (op == Op::BORN_FWD || op == Op::BORN_ADJ);
-    if (born) {
+    const bool fwi = (op == Op::FWI_FWD || op == Op::FWI_GRAD);
+    const bool has_ss = born || op == Op::FWI_GRAD;
+    const bool has_acc = op == Op::BORN_ADJ || op == Op::FWI_GRAD;
+    if (born || fwi) {
         for (int i = 1; i < nshot; ++i)
-            if (shots[i].model != shots[0].model) fail("Born shots of one call must share a model");
+            if (shots[i].model != shots[0].model) fail("Born and FWI shots of one call must share a model");
     }
+    if (fwi && !ring) fail("FWI call without a coarse storage ring");
     const bool dev_ptrs = (flags & EIO_DEVICE_PTRS) != 0;
 
     // ---- sizes
@@ -315,11 +337,9 @@ Run<T> prepare(Ctx* c, int nshot, const eio_shot* shots, Op op, int flags,
         const eio_shot& s = shots[run.plan.order[oi]];
         const ModelBase& m = get_model(c, s.model);
         state_elems += 10 * size_t(m.n);
-        if (born) {
-            ss_elems += 10 * size_t(m.n);
-            w_elems += 2 * size_t(m.n) * m.g.nts;
-            if (op == Op::BORN_ADJ) acc_elems += 5 * size_t(m.n);
-        }
+        if (has_ss) ss_elems += 10 * size_t(m.n);
+        if (born) w_elems += 2 * size_t(m.n) * m.g.nts;
+        if (has_acc) acc_elems += 5 * size_t(m.n);
         if (!dev_ptrs) {
             const int src_ld = born ? m.ntw : m.g.nts;
             for (int k = 0; k < 5; ++k) {
@@ -354,7 +374,7 @@ Run<T> prepare(Ctx* c, int nshot, const eio_shot* shots, Op op, int flags,
         d.filt = static_cast<const T*>(m.filt);
         const int src_ld = born ? m.ntw : m.g.nts;
         const bool src_in = (op != Op::NL_ADJ);
-        const bool rec_in = (op == Op::NL_ADJ || op == Op::BORN_ADJ);
+        const bool rec_in = (op == Op::NL_ADJ || op == Op::BORN_ADJ || op == Op::FWI_GRAD);
         for (int k = 0; k < 5; ++k) {
             const size_t ns = size_t(s.n_src[GRID_OF_COMP[k]]) * src_ld;
             const size_t nr = size_t(s.n_rec[GRID_OF_COMP[k]]) * m.g.nts;
@@ -384,19 +404,27 @@ Run<T> prepare(Ctx* c, int nshot, const eio_shot* shots, Op op, int flags,
                 }
             }
         }
-        if (born) {
+        if (has_ss) {
             d.ss[0] = ssbuf + sso;
             d.ss[1] = ssbuf + sso + 5 * size_t(m.n);
             sso += 10 * size_t(m.n);
+        }
+        if (born) {
             d.Wvx = wbuf + wo;
             d.Wvz = wbuf + wo + size_t(m.n) * m.g.nts;
+            d.wcap = m.g.nts;
             wo += 2 * size_t(m.n) * m.g.nts;
-            if (op == Op::BORN_ADJ) {
-                d.dm = accbuf + ao;
-                ao += 5 * size_t(m.n);
-            } else {
-                d.dm = const_cast<T*>(static_cast<const T*>(dm_fwd[0]));
-            }
+        } else if (fwi) {
+            d.Wvx = ring + wo;
+            d.Wvz = ring + wo + size_t(m.n) * wcap;
+            d.wcap = wcap;
+            wo += 2 * size_t(m.n) * wcap;
+        }
+        if (has_acc) {
+            d.dm = accbuf + ao;
+            ao += 5 * size_t(m.n);
+        } else if (op == Op::BORN_FWD) {
+            d.dm = const_cast<T*>(static_cast<const T*>(dm_fwd[0]));
         }
     }
     // ---- tables
@@ -407,10 +435,10 @@ Run<T> prepare(Ctx* c, int nshot, const eio_shot* shots, Op op, int flags,
     EIO_CHECK_HIP(hipMemcpyAsync(run.d_tiles, run.plan.tiles.data(), sizeof(eio::TileDev) * run.plan.tiles.size(), h2d, c->stream));
     if (!run.plan.ents.empty())
         EIO_CHECK_HIP(hipMemcpyAsync(run.d_ents, run.plan.ents.data(), sizeof(eio::Entry) * run.plan.ents.size(), h2d, c->stream));
-    // ---- zero state (and Born buffers)
+    // ---- zero state (and Born buffers; the coarse storage needs no clearing: the
+    // decimation overwrites each slot with its first contribution)
     fill0<T>(c, state, long(state_elems));
     if (ssbuf) fill0<T>(c, ssbuf, long(ss_elems));
-    if (wbuf) fill0<T>(c, wbuf, long(w_elems));
     if (accbuf) fill0<T>(c, accbuf, long(acc_elems));
     return run;
 }
@@ -441,12 +469,16 @@ inline int active_tiles(const Plan& p, int k) {
         default: CALL(2, 8, 2); break;                                                    \
     }
 
+// dlo/dhi: decimation segment bounds (coarse index of the first interval, coarse index of
+// the slot shared with the next segment, or -1); `dsh` replaces the run's shot table.
 template <typename T, int MODE>
-void launch_fwd(Run<T>& run, int ntile, int k, int sspar) {
+void launch_fwd(Run<T>& run, int ntile, int k, int sspar, int dlo = 0, int dhi = -1,
+                const eio::ShotDev<T>* dsh = nullptr) {
     hipStream_t s = run.c->stream;
+    const eio::ShotDev<T>* tab = dsh ? dsh : run.d_shots;
 #define EIO_L(R_, WX_, OCC_)                                                                 \
     hipLaunchKernelGGL((eio::k_step_fwd<T, R_, WX_, OCC_, MODE>), dim3(ntile), dim3(64 * WX_), 0, \
-                       s, run.d_shots, run.d_tiles, run.d_ents, ntile, k, sspar)
+                       s, tab, run.d_tiles, run.d_ents, ntile, k, sspar, dlo, dhi)
     if constexpr (sizeof(T) == 8) {
         EIO_L(2, 4, 1);
     } else {
@@ -550,7 +582,7 @@ void nl_adjoint(Ctx* c, int nshot, const eio_shot* shots, int flags) {
 
 template <typename T>
 void born_source_pass(Run<T>& run, int ntile, int kmax) {
-    for (int k = 0; k < kmax; ++k) launch_fwd<T, eio::M_BSRC>(run, ntile, k, 0);
+    for (int k = 0; k < kmax; ++k) launch_fwd<T, eio::M_BSRC>(run, ntile, k, 0, 0, -1);
 }
 
 template <typename T>
@@ -595,6 +627,9 @@ void born_forward(Ctx* c, int nshot, const eio_shot* shots, const void* const dm
 }
 
 template <typename T>
+void stack_into(Run<T>& run, const ModelBase& m, int nshot, void* const dm[5], int flags);
+
+template <typename T>
 void born_adjoint(Ctx* c, int nshot, const eio_shot* shots, void* const dm[5], int flags) {
     if (!dm) fail("dm is NULL");
     const ModelBase& m = get_model(c, shots[0].model);
@@ -620,7 +655,14 @@ void born_adjoint(Ctx* c, int nshot, const eio_shot* shots, void* const dm[5], i
     });
     EIO_CHECK_HIP(hipEventRecord(c->ev[3], c->stream));
     timing_end<T>(c, 2LL * kmax);
-    // stack the per-shot accumulators into dm
+    stack_into(run, m, nshot, dm, flags);
+    finish(run);
+}
+
+// dm[f] += sum over shots of the per-shot imaging accumulators (BornElasticShotsGpu.cpp:313-323)
+template <typename T>
+void stack_into(Run<T>& run, const ModelBase& m, int nshot, void* const dm[5], int flags) {
+    Ctx* c = run.c;
     const size_t nb = size_t(m.n) * sizeof(T);
     T* out = static_cast<T*>(ws_get(c, "dmout", 5 * nb));
     std::vector<const T*> accp(size_t(nshot) * 5);
@@ -639,6 +681,174 @@ void born_adjoint(Ctx* c, int nshot, const eio_shot* shots, void* const dm[5], i
         if (!(flags & EIO_DEVICE_PTRS))
             EIO_CHECK_HIP(hipMemcpyAsync(dm[f], o, nb, hipMemcpyDeviceToHost, c->stream));
     }
+}
+
+// ------------------------------------------------------------------ checkpointed FWI gradient
+// The reference's FWI gradient is the Born adjoint of the data residual at the current model
+// (efwiMain.py:229-476 -> BornElasticShotsGpu.cpp:228-378), whose source pass stores Vx and
+// Vz at every coarse sample (nz*nx*nts per component and shot).  Here the NL forward that
+// produces the predicted data also saves the batch state every L coarse intervals; the
+// gradient call sweeps backwards and, just before the imaging condition first needs a
+// segment, recomputes that segment from its checkpoint into a ring of L+3 coarse slices.
+// The source injection of the NL forward (coarse traces, time-interpolated in the step) is
+// the same arithmetic as the Born source pass on pre-fine-sampled traces.
+
+unsigned long long geometry_hash(int nshot, const eio_shot* shots) {
+    unsigned long long h = 1469598103934665603ULL;
+    auto mix = [&](long long v) {
+        h ^= (unsigned long long)v;
+        h *= 1099511628211ULL;
+    };
+    mix(nshot);
+    for (int i = 0; i < nshot; ++i) {
+        mix(shots[i].model);
+        for (int g = 0; g < 4; ++g) {
+            mix(shots[i].n_src[g]);
+            mix(shots[i].n_rec[g]);
+            for (int u = 0; u < shots[i].n_src[g] && shots[i].src_pos[g]; ++u) mix(shots[i].src_pos[g][u]);
+            for (int u = 0; u < shots[i].n_rec[g] && shots[i].rec_pos[g]; ++u) mix(shots[i].rec_pos[g][u]);
+        }
+    }
+    return h;
+}
+
+void fwi_free(FwiStore& f) {
+    if (f.ck) (void)hipFree(f.ck);
+    if (f.ring) (void)hipFree(f.ring);
+    f.ck = f.ring = nullptr;
+}
+
+template <typename T>
+void copy_dev(Ctx* c, T* dst, const T* src, size_t n) {
+    if (!n) return;
+    const unsigned nb = unsigned(std::min<size_t>((n + eio::NTHR - 1) / eio::NTHR, 256 * 64));
+    hipLaunchKernelGGL(eio::k_copy<T>, dim3(nb), dim3(eio::NTHR), 0, c->stream, dst, src, long(n));
+    EIO_CHECK_HIP(hipGetLastError());
+}
+
+inline int seg_begin(const FwiStore& f, int j) { return j * f.L; }
+inline int seg_end(const FwiStore& f, int j) { return std::min((j + 1) * f.L, f.nts - 1); }
+
+template <typename T>
+int fwi_forward(Ctx* c, int nshot, const eio_shot* shots, int flags, int segment) {
+    if (nshot <= 0 || !shots) fail("no shots");
+    const ModelBase& m = get_model(c, shots[0].model);
+    const int nts = m.g.nts, sub = m.g.sub_fd;
+    FwiStore f;
+    f.prec = int(sizeof(T));
+    f.nshot = nshot;
+    f.model = shots[0].model;
+    f.nts = nts;
+    f.sub = sub;
+    f.L = segment > 0 ? segment : int(std::ceil(std::sqrt(double(nts - 1))));
+    f.L = std::max(1, std::min(f.L, nts - 1));
+    f.nseg = (nts - 1 + f.L - 1) / f.L;
+    f.wcap = std::min(f.L + 3, nts);
+    f.sig = geometry_hash(nshot, shots);
+    f.state_elems = size_t(nshot) * 10 * size_t(m.n);
+    const size_t ring_elems = size_t(nshot) * 2 * size_t(m.n) * f.wcap;
+    const size_t ck_elems = size_t(f.nseg - 1) * f.state_elems;
+    f.bytes = (ring_elems + ck_elems) * sizeof(T);
+    EIO_CHECK_HIP(hipMalloc(&f.ring, ring_elems * sizeof(T)));
+    if (ck_elems && hipMalloc(&f.ck, ck_elems * sizeof(T)) != hipSuccess) {
+        (void)hipGetLastError();
+        fwi_free(f);
+        throw EioError{EIO_ERR_RUNTIME, "out of device memory for " + std::to_string(f.nseg - 1) +
+                                            " checkpoints; pass a longer segment"};
+    }
+    try {
+        Run<T> run = prepare<T>(c, nshot, shots, Op::FWI_FWD, flags, nullptr, static_cast<T*>(f.ring), f.wcap);
+        const int ntile = int(run.plan.tiles.size());
+        T* state = run.sd[0].A;  // the batch state is one contiguous buffer in plan order
+        timing_begin<T>(c);
+        EIO_CHECK_HIP(hipEventRecord(c->ev[2], c->stream));
+        run_loop(c, [&]() {
+            for (int j = 0; j < f.nseg; ++j) {
+                if (j > 0) copy_dev<T>(c, static_cast<T*>(f.ck) + size_t(j - 1) * f.state_elems, state, f.state_elems);
+                const int k0 = seg_begin(f, j) * sub, k1 = seg_end(f, j) * sub;
+                if (j + 1 < f.nseg) {
+                    for (int k = k0; k < k1; ++k) launch_fwd<T, eio::M_NL>(run, ntile, k, 0);
+                } else {  // the last segment also fills the ring
+                    for (int k = k0; k < k1; ++k)
+                        launch_fwd<T, eio::M_NL | eio::F_DEC>(run, ntile, k, 0, seg_begin(f, j), -1);
+                }
+            }
+        });
+        EIO_CHECK_HIP(hipEventRecord(c->ev[3], c->stream));
+        timing_end<T>(c, (long long)(nts - 1) * sub);
+        finish(run);
+    } catch (...) {
+        (void)hipStreamSynchronize(c->stream);
+        fwi_free(f);
+        throw;
+    }
+    f.ring_fresh = true;
+    const int id = c->next_fwi++;
+    c->fwi[id] = f;
+    return id;
+}
+
+template <typename T>
+void fwi_gradient(Ctx* c, int handle, int nshot, const eio_shot* shots, void* const dm[5], int flags) {
+    if (!dm) fail("dm is NULL");
+    auto it = c->fwi.find(handle);
+    if (it == c->fwi.end()) fail("unknown FWI store handle " + std::to_string(handle));
+    FwiStore& f = it->second;
+    if (nshot != f.nshot || !shots || geometry_hash(nshot, shots) != f.sig)
+        fail("shots differ from those of the FWI forward call of this store");
+    const ModelBase& m = get_model(c, f.model);
+    if (m.g.nts != f.nts || m.g.sub_fd != f.sub) fail("model time axis changed since the FWI forward call");
+    Run<T> run = prepare<T>(c, nshot, shots, Op::FWI_GRAD, flags, nullptr, static_cast<T*>(f.ring), f.wcap);
+    const int ntile = int(run.plan.tiles.size());
+    const int nts = f.nts, sub = f.sub;
+    // forward-recompute state and its shot table
+    T* fstate = static_cast<T*>(ws_get(c, "fstate", f.state_elems * sizeof(T)));
+    std::vector<eio::ShotDev<T>> sdf = run.sd;
+    size_t off = 0;
+    for (int oi = 0; oi < nshot; ++oi) {
+        const size_t n = size_t(sdf[oi].nz) * sdf[oi].nx;
+        sdf[oi].A = fstate + off;
+        sdf[oi].B = fstate + off + 5 * n;
+        off += 10 * n;
+    }
+    auto* d_sdf = static_cast<eio::ShotDev<T>*>(ws_get(c, "shots_f", sizeof(eio::ShotDev<T>) * nshot));
+    EIO_CHECK_HIP(hipMemcpyAsync(d_sdf, sdf.data(), sizeof(eio::ShotDev<T>) * nshot, hipMemcpyHostToDevice,
+                                 c->stream));
+    long long nrecomp = 0;
+    for (int j = 0; j + 1 < f.nseg; ++j) nrecomp += (long long)(seg_end(f, j) - seg_begin(f, j)) * sub;
+    timing_begin<T>(c);
+    EIO_CHECK_HIP(hipEventRecord(c->ev[2], c->stream));
+    auto recompute = [&](int j) {
+        if (j == 0) fill0<T>(c, fstate, long(f.state_elems));
+        else copy_dev<T>(c, fstate, static_cast<const T*>(f.ck) + size_t(j - 1) * f.state_elems, f.state_elems);
+        const int dhi = (j + 1 < f.nseg) ? seg_end(f, j) : -1;
+        for (int k = seg_begin(f, j) * sub; k < seg_end(f, j) * sub; ++k)
+            launch_fwd<T, eio::M_RECOMP>(run, ntile, k, 0, seg_begin(f, j), dhi, d_sdf);
+    };
+    const bool fresh = f.ring_fresh;
+    f.ring_fresh = false;
+    if (!fresh) nrecomp += (long long)(seg_end(f, f.nseg - 1) - seg_begin(f, f.nseg - 1)) * sub;
+    run_loop(c, [&]() {
+        // a store used before no longer holds the last segment in its ring
+        if (!fresh) recompute(f.nseg - 1);
+        int p = 0;
+        int jn = f.nseg - 1;  // next segment start to cross going backwards
+        for (int its = nts - 2; its >= 0; --its) {
+            if (jn >= 1 && its == seg_begin(f, jn)) {
+                // segment jn-1 is recomputed before the imaging at its+1 first needs slot its
+                recompute(jn - 1);
+                --jn;
+            }
+            for (int it2 = sub - 1; it2 >= 0; --it2)
+                launch_adj<T, eio::M_BADJ>(run, ntile, (nts - 2 - its) * sub + (sub - 1 - it2), p);
+            launch_img<T, true>(run, ntile, its + 1, p ^ 1);
+            p ^= 1;
+        }
+        launch_img<T, true>(run, ntile, 0, p ^ 1);
+    });
+    EIO_CHECK_HIP(hipEventRecord(c->ev[3], c->stream));
+    timing_end<T>(c, (long long)(nts - 1) * sub + nrecomp);
+    stack_into(run, m, nshot, dm, flags);
     finish(run);
 }
 
@@ -692,6 +902,7 @@ int eio_ctx_destroy(eio_ctx* ctx) {
         DeviceGuard dg(c->device);
         (void)hipStreamSynchronize(c->stream);
         for (auto& kv : c->models) model_free(kv.second);
+        for (auto& kv : c->fwi) fwi_free(kv.second);
         for (auto& kv : c->ws)
             if (kv.second.p) (void)hipFree(kv.second.p);
         for (auto& e : c->ev) (void)hipEventDestroy(e);
@@ -808,6 +1019,49 @@ int eio_born_adjoint(eio_ctx* ctx, int nshot, const eio_shot* shots, void* const
         if (nshot <= 0 || !shots) fail("no shots");
         if (c->prec == 4) born_adjoint<float>(c, nshot, shots, dm, flags);
         else born_adjoint<double>(c, nshot, shots, dm, flags);
+    });
+}
+
+int eio_fwi_forward(eio_ctx* ctx, int nshot, const eio_shot* shots, int flags, int segment, int* handle) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        if (!handle) fail("handle is NULL");
+        if (segment < 0) fail("segment must be >= 0");
+        DeviceGuard dg(c->device);
+        *handle = (c->prec == 4) ? fwi_forward<float>(c, nshot, shots, flags, segment)
+                                 : fwi_forward<double>(c, nshot, shots, flags, segment);
+    });
+}
+
+int eio_fwi_gradient(eio_ctx* ctx, int handle, int nshot, const eio_shot* shots, void* const dm[5], int flags) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        DeviceGuard dg(c->device);
+        if (c->prec == 4) fwi_gradient<float>(c, handle, nshot, shots, dm, flags);
+        else fwi_gradient<double>(c, handle, nshot, shots, dm, flags);
+    });
+}
+
+int eio_fwi_info(eio_ctx* ctx, int handle, int* segment, int* nckpt, unsigned long long* bytes) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        auto it = c->fwi.find(handle);
+        if (it == c->fwi.end()) fail("unknown FWI store handle " + std::to_string(handle));
+        if (segment) *segment = it->second.L;
+        if (nckpt) *nckpt = it->second.nseg - 1;
+        if (bytes) *bytes = it->second.bytes;
+    });
+}
+
+int eio_fwi_release(eio_ctx* ctx, int handle) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        auto it = c->fwi.find(handle);
+        if (it == c->fwi.end()) fail("unknown FWI store handle " + std::to_string(handle));
+        DeviceGuard dg(c->device);
+        EIO_CHECK_HIP(hipStreamSynchronize(c->stream));
+        fwi_free(it->second);
+        c->fwi.erase(it);
     });
 }
 

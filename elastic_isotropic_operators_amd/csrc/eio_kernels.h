@@ -43,10 +43,16 @@ constexpr int TZ = 64;        // z extent of a tile = wave64 lanes
 constexpr int ZH = TZ + 2 * FAT;
 constexpr int NTHR = 256;     // threads of the element-wise helper kernels
 
-// Modes of the fused forward kernel
-constexpr int M_NL = 0;       // NL forward: coarse source injection with time interpolation, record
-constexpr int M_BSRC = 1;     // Born source pass: fine source injection, coarse Vx/Vz decimation
-constexpr int M_BSCAT = 2;    // Born scattered pass: secondary-source injection, record
+// Modes of the fused forward kernel: bit flags
+constexpr int F_INJ_COARSE = 1;  // source injection from coarse traces, time-interpolated on the fly
+constexpr int F_INJ_FINE = 2;    // source injection from pre-fine-sampled traces
+constexpr int F_REC = 4;         // receiver recording
+constexpr int F_DEC = 8;         // coarse decimation of Vx, Vz into the storage ring
+constexpr int F_SS = 16;         // Born secondary-source injection
+constexpr int M_NL = F_INJ_COARSE | F_REC;      // NL forward
+constexpr int M_BSRC = F_INJ_FINE | F_DEC;      // Born source pass
+constexpr int M_BSCAT = F_SS | F_REC;           // Born scattered pass
+constexpr int M_RECOMP = F_INJ_COARSE | F_DEC;  // forward recompute of a checkpointed segment
 // Modes of the fused adjoint kernel
 constexpr int M_NLADJ = 0;    // NL adjoint: data injection, source-trace extraction
 constexpr int M_BADJ = 1;     // Born receiver pass: data injection, adjoint-field extraction
@@ -63,8 +69,10 @@ struct ShotDev {
     T* src_tr[5];          // source traces (NL: [n][nts], Born: [n][ntw])
     T* rec_tr[5];          // receiver traces [n][nts]
     T* ss[2];              // Born secondary sources / adjoint fields, 5 fields each, ping-pong
-    T* Wvx;                // Born coarse storage [nts][nz*nx]
+    T* Wvx;                // Born coarse storage: wcap slices of nz*nx, coarse index its in slot its % wcap
     T* Wvz;
+    int wcap;              // nts for full storage, segment length + 3 for the checkpointed gradient
+    int pad1;
     T* dm;                 // Born perturbation (forward) or per-shot imaging accumulator (adjoint)
 };
 
@@ -154,7 +162,7 @@ __device__ __forceinline__ StepTime adj_time(int k, int sub, int nts) {
 template <typename T, int R, int WX, int OCC, int MODE>
 __global__ __launch_bounds__(64 * WX) __attribute__((amdgpu_waves_per_eu(OCC)))
 void k_step_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict__ tiles,
-                const Entry* __restrict__ ents, int ntiles, int k, int sspar) {
+                const Entry* __restrict__ ents, int ntiles, int k, int sspar, int dlo, int dhi) {
     constexpr int NT = 64 * WX;
     constexpr int TX = WX * R;
     constexpr int TXH = TX + 2 * FAT;
@@ -255,7 +263,7 @@ void k_step_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict_
         T a2 = d * o[2][r] + (m[2][r] * dx_vx[r] + m[3][r] * dz_vz);
         T a3 = d * o[3][r] + (m[3][r] * dx_vx[r] + m[2][r] * dz_vz);
         T a4 = d * o[4][r] + m[4][r] * (dz_vx + dx_vz[r]);
-        if (MODE == M_BSCAT) {  // secondary source sampled at it2-1 (kernelsGpuElastic.cu:1363-1368)
+        if (MODE & F_SS) {  // secondary source sampled at it2-1 (kernelsGpuElastic.cu:1363-1368)
             const T fl = S.filt[st.it2 - 1], fr = S.filt[S.h + st.it2 - 1];
             const rsrc_t rl = make_rsrc(S.ss[sspar], 5 * fbytes);
             const rsrc_t rr = make_rsrc(S.ss[sspar ^ 1], 5 * fbytes);
@@ -273,8 +281,8 @@ void k_step_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict_
         val[4][r] = d * a4;
     }
 
-    const bool has_src = (MODE != M_BSCAT) && (t.src_e > t.src_b);
-    const bool has_rec = (MODE != M_BSRC) && (t.rec_e > t.rec_b);
+    const bool has_src = (MODE & (F_INJ_COARSE | F_INJ_FINE)) && (t.src_e > t.src_b);
+    const bool has_rec = (MODE & F_REC) && (t.rec_e > t.rec_b);
     if (has_src || has_rec) {  // uniform per tile
         __syncthreads();
 #pragma unroll
@@ -289,7 +297,7 @@ void k_step_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict_
                 const int comp = en.crow >> 24, row = en.crow & 0xffffff;
                 const T* tr = S.src_tr[comp];
                 T s;
-                if (MODE == M_NL) {  // kernelsGpuElastic.cu:172-189 with it2-1
+                if (MODE & F_INJ_COARSE) {  // kernelsGpuElastic.cu:172-189 with it2-1
                     const long b = long(row) * S.nts + st.its;
                     s = tr[b] * S.filt[st.it2 - 1] + tr[b + 1] * S.filt[S.h + st.it2 - 1];
                 } else {             // pre-fine-sampled source at itw-1 = k (kernelsGpuElastic.cu:121-168)
@@ -317,13 +325,23 @@ void k_step_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict_
         }
     }
 
-    rsrc_t rwx, rwz;
+    // Decimation.  A slot's first contribution in the pass overwrites it (no zero fill of
+    // the storage): slot its at the first fine step of interval dlo, slot its+1 at the first
+    // fine step of every interval except the one ending on dhi (that slot already holds the
+    // contributions of the later segment, recomputed earlier).
+    rsrc_t rwx0, rwx1, rwz0, rwz1;
     T wfa = T(0), wfb = T(0);
-    if (MODE == M_BSRC) {  // coarse slices its and its+1 (kernelsGpuElastic.cu:1337-1346)
-        rwx = make_rsrc(S.Wvx + size_t(st.its) * n, 2 * fbytes);
-        rwz = make_rsrc(S.Wvz + size_t(st.its) * n, 2 * fbytes);
+    bool sta = false, stb = false;
+    if (MODE & F_DEC) {  // coarse slices its and its+1 (kernelsGpuElastic.cu:1337-1346)
+        const size_t s0 = size_t(st.its % S.wcap) * n, s1 = size_t((st.its + 1) % S.wcap) * n;
+        rwx0 = make_rsrc(S.Wvx + s0, fbytes);
+        rwx1 = make_rsrc(S.Wvx + s1, fbytes);
+        rwz0 = make_rsrc(S.Wvz + s0, fbytes);
+        rwz1 = make_rsrc(S.Wvz + s1, fbytes);
         wfa = S.filt[st.it2];
         wfb = S.filt[S.h + st.it2];
+        sta = st.it2 == 1 && st.its == dlo;
+        stb = st.it2 == 1 && st.its + 1 != dhi;
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -331,11 +349,21 @@ void k_step_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict_
             const unsigned so = unsigned(r) * colb;
 #pragma unroll
             for (int f = 0; f < 5; ++f) bst(rold, val[f][r], voff, so + unsigned(f) * fbytes);
-            if (MODE == M_BSRC) {
-                bst(rwx, bld<T>(rwx, voff, so) + val[0][r] * wfa, voff, so);
-                bst(rwx, bld<T>(rwx, voff, so + fbytes) + val[0][r] * wfb, voff, so + fbytes);
-                bst(rwz, bld<T>(rwz, voff, so) + val[1][r] * wfa, voff, so);
-                bst(rwz, bld<T>(rwz, voff, so + fbytes) + val[1][r] * wfb, voff, so + fbytes);
+            if (MODE & F_DEC) {
+                T xa = val[0][r] * wfa, za = val[1][r] * wfa;
+                T xb = val[0][r] * wfb, zb = val[1][r] * wfb;
+                if (!sta) {
+                    xa = bld<T>(rwx0, voff, so) + xa;
+                    za = bld<T>(rwz0, voff, so) + za;
+                }
+                if (!stb) {
+                    xb = bld<T>(rwx1, voff, so) + xb;
+                    zb = bld<T>(rwz1, voff, so) + zb;
+                }
+                bst(rwx0, xa, voff, so);
+                bst(rwz0, za, voff, so);
+                bst(rwx1, xb, voff, so);
+                bst(rwz1, zb, voff, so);
             }
         }
     }
@@ -529,19 +557,26 @@ __global__ __launch_bounds__(64 * WX) void k_imaging(const ShotDev<T>* __restric
     const int nz = S.nz, nx = S.nx;
     const unsigned n = unsigned(nz) * unsigned(nx);
     const unsigned fbytes = n * SZ, colb = unsigned(nz) * SZ;
-    // slices its-1, its, its+1 of the coarse storage (clamped at the ends)
-    const int lo = its > 0 ? its - 1 : its;
-    const int nsl = (its < S.nts - 1 ? its + 1 : its) - lo + 1;
-    const rsrc_t rwx = make_rsrc(S.Wvx + size_t(lo) * n, unsigned(nsl) * fbytes);
-    const rsrc_t rwz = make_rsrc(S.Wvz + size_t(lo) * n, unsigned(nsl) * fbytes);
-    const unsigned scur = unsigned(its - lo) * fbytes;
+    // slices its-1, its, its+1 of the coarse storage ring (ends are clamped; unused there)
+    const size_t sc = size_t(its % S.wcap) * n;
+    const size_t sm = size_t((its > 0 ? its - 1 : its) % S.wcap) * n;
+    const size_t sp = size_t((its < S.nts - 1 ? its + 1 : its) % S.wcap) * n;
+    const rsrc_t rwx = make_rsrc(S.Wvx + sc, fbytes), rwz = make_rsrc(S.Wvz + sc, fbytes);
+    const rsrc_t rwxm = make_rsrc(S.Wvx + sm, fbytes), rwzm = make_rsrc(S.Wvz + sm, fbytes);
+    const rsrc_t rwxp = make_rsrc(S.Wvx + sp, fbytes), rwzp = make_rsrc(S.Wvz + sp, fbytes);
 
+    // Halo fill.  Only update-region cells of a slot are ever written (the decimation
+    // overwrites or accumulates them), so FAT cells are masked to the zero the wavefield
+    // holds there instead of being read.
     const unsigned hbase = (unsigned(t.x0 - FAT) * unsigned(nz) + unsigned(t.z0 - FAT)) * SZ;
     for (int e = threadIdx.x; e < TXH * ZH; e += NT) {
         const int c = e / ZH, z = e - c * ZH;
+        const int hz = t.z0 - FAT + z, hx = t.x0 - FAT + c;
+        const bool upd = hz >= FAT && hz < nz - FAT && hx >= FAT && hx < nx - FAT;
         const unsigned hv = hbase + unsigned(c) * colb + unsigned(z) * SZ;
-        lds[0][c][z] = bld<T>(rwx, hv, scur);
-        lds[1][c][z] = bld<T>(rwz, hv, scur);
+        const T a = bld<T>(rwx, hv, 0u), b = bld<T>(rwz, hv, 0u);
+        lds[0][c][z] = upd ? a : T(0);
+        lds[1][c][z] = upd ? b : T(0);
     }
     __syncthreads();
 
@@ -579,14 +614,14 @@ __global__ __launch_bounds__(64 * WX) void k_imaging(const ShotDev<T>* __restric
         const T sxz_op = dvx_dzm + dvz_dxm[r];
         T tx, tz;
         if (its == 0) {
-            tx = -bld<T>(rwx, voff, so + fbytes);
-            tz = -bld<T>(rwz, voff, so + fbytes);
+            tx = -bld<T>(rwxp, voff, so);
+            tz = -bld<T>(rwzp, voff, so);
         } else if (its == S.nts - 1) {
-            tx = bld<T>(rwx, voff, so);
-            tz = bld<T>(rwz, voff, so);
+            tx = bld<T>(rwxm, voff, so);
+            tz = bld<T>(rwzm, voff, so);
         } else {
-            tx = bld<T>(rwx, voff, so) - bld<T>(rwx, voff, so + 2 * fbytes);
-            tz = bld<T>(rwz, voff, so) - bld<T>(rwz, voff, so + 2 * fbytes);
+            tx = bld<T>(rwxm, voff, so) - bld<T>(rwxp, voff, so);
+            tz = bld<T>(rwzm, voff, so) - bld<T>(rwzp, voff, so);
         }
         if (!ADJ) {
             const T d0 = bld<T>(rdm, voff, so), d1 = bld<T>(rdm, voff, so + fbytes);
@@ -631,6 +666,13 @@ __global__ __launch_bounds__(NTHR) void k_copy16(float4* __restrict__ dst,
                                                  const float4* __restrict__ src, long n16) {
     const long stride = long(gridDim.x) * NTHR;
     for (long i = long(blockIdx.x) * NTHR + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
+// element copy (checkpoint save/restore; any size and alignment)
+template <typename T>
+__global__ __launch_bounds__(NTHR) void k_copy(T* __restrict__ dst, const T* __restrict__ src, long n) {
+    const long stride = long(gridDim.x) * NTHR;
+    for (long i = long(blockIdx.x) * NTHR + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
 // zero-fill (graph-capturable replacement for hipMemsetAsync)

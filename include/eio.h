@@ -115,6 +115,30 @@ int eio_born_forward(eio_ctx* ctx, int nshot, const eio_shot* shots, const void*
  * Replaces BornShotsAdjGpu (BornElasticGpuFunctions.h:25-36, .cu:1100-1454). */
 int eio_born_adjoint(eio_ctx* ctx, int nshot, const eio_shot* shots, void* const dm[5], int flags);
 
+/* Checkpointed FWI gradient, in two calls around the residual.  The reference computes the
+ * gradient as the Born adjoint of the data residual at the current model (efwiMain.py:229-476
+ * calling BornElasticShotsGpu::adjoint, BornElasticShotsGpu.cpp:228-378, whose source pass keeps
+ * Vx, Vz at every coarse sample, BornElasticGpuFunctions.cu:1128-1191); these entry points replace
+ * the NL forward + Born adjoint pair of one gradient.
+ *
+ * eio_fwi_forward: NL forward of all shots (same model; src_tr coarse [nReg][nts] as for
+ * eio_nl_forward; rec_tr is overwritten with the predicted data), saving the batch state every
+ * `segment` coarse intervals (0: ceil(sqrt(nts-1))) into a store owned by the context.
+ * *handle names the store. */
+int eio_fwi_forward(eio_ctx* ctx, int nshot, const eio_shot* shots, int flags, int segment,
+                    int* handle);
+/* eio_fwi_gradient: Born adjoint at the model of the store, with rec_tr = the adjoint source
+ * (data residual, regular traces) and src_tr = the same sources as the forward call; the
+ * forward wavefield is recomputed segment by segment from the checkpoints.  Accumulates (+=)
+ * the stacked imaging condition into dm[5] (same slices as eio_born_adjoint).  The shots must
+ * have the geometry and model of the forward call, and the model's media must not change in
+ * between.  The store stays valid for further gradient calls until released. */
+int eio_fwi_gradient(eio_ctx* ctx, int handle, int nshot, const eio_shot* shots, void* const dm[5],
+                     int flags);
+/* Store facts: segment length (coarse intervals), number of checkpoints, device bytes held. */
+int eio_fwi_info(eio_ctx* ctx, int handle, int* segment, int* nckpt, unsigned long long* bytes);
+int eio_fwi_release(eio_ctx* ctx, int handle);
+
 /* Diagnostics: copy `bytes` (multiple of 16) from src to dst (device pointers) `reps` times with
  * a plain 16-byte-vector copy kernel on the context stream; *ms = mean kernel time.  Used to
  * measure the achievable HBM copy bandwidth and to calibrate HBM byte counters.  No counterpart

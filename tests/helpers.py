@@ -69,3 +69,27 @@ def operator_case(tmp_path, w, nExp=1, zSource=4, xSource=None, spacing=None, nR
     sz, sx = W.source_coords(w, zSource, xSource, nExp=nExp, spacingShots=spacing)
     rz, rx = W.receiver_coords(w, nRec, depthRec, oRec, dRec)
     return args, dict(sz=sz, sx=sx, rz=rz, rx=rx)
+
+
+def smooth_field(rng, shape, sigma=4.0):
+    from scipy.ndimage import gaussian_filter
+    f = gaussian_filter(rng.standard_normal(shape), sigma)
+    return f / np.sqrt(np.mean(f ** 2))
+
+
+def fwi_case(nts=800, dts=0.0005, fdom=12.0, seed=0, nz=48, nx=64, nExp=1):
+    """Small FWI problem: smooth random background m0 = (rho, lambda, mu), a 3 % smooth
+    perturbation dtrue for the observed data, explosive Ricker sources."""
+    w = W.small(nz=nz, nx=nx, nts=nts, sub=1, fDom=fdom, nExp=nExp, seed=seed)
+    w["dts"] = dts
+    w["model"] = W.explosive_model(nts, dts, fdom, 0.05)
+    m0 = w["elastic"].astype(np.float64)
+    rng = np.random.default_rng(seed + 100)
+    dtrue = np.stack([0.03 * m0[k].mean() * smooth_field(rng, m0.shape[1:]) for k in range(3)])
+    return w, oracle_grid(w), m0, dtrue
+
+
+def fwi_directions(m0, n, seed=33, frac=0.01):
+    rng = np.random.default_rng(seed)
+    return [np.stack([frac * m0[k].mean() * smooth_field(rng, m0.shape[1:]) for k in range(3)])
+            for _ in range(n)]

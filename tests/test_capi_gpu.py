@@ -229,3 +229,23 @@ def test_all_fp32_variants_forward_adjoint(v):
     for k in range(5):
         assert rel_l2(out_f[k], ref_f[k]) < 1e-5, (v, k, rel_l2(out_f[k], ref_f[k]))
         assert rel_l2(out_a[k], ref_a[k]) < 1e-5, (v, k, rel_l2(out_a[k], ref_a[k]))
+
+
+def test_born_adjoint_with_reused_storage_of_another_shape():
+    """The coarse Vx/Vz storage is a context workspace that is never cleared: a Born call on a
+    larger grid leaves nonzero values where the next, smaller grid has its FAT halo.  The
+    imaging condition must not read them (regression test)."""
+    dtype = np.float32
+    ctx = _ctx(dtype)
+    w1, g1, med1, sp1, rp1 = _case(nz=64, nx=80, nts=100, sub=1, seed=50)
+    src1, _ = _born_inputs(w1, g1, sp1, 51)
+    d1 = random_traces(np.random.default_rng(52), rp1, g1.nts, np.float64)
+    m1 = _model(ctx, w1, g1, med1)
+    _gpu_born_adj(ctx, m1, sp1, rp1, src1, d1, g1.nx, g1.nz, dtype)
+    w, g, med, spos, rpos = _case(nz=40, nx=56, nts=100, sub=1, seed=53)
+    src_fine, _ = _born_inputs(w, g, spos, 54)
+    dat = random_traces(np.random.default_rng(55), rpos, g.nts, np.float64)
+    ref = O.born_adj_reg(g, med, spos, src_fine, rpos, dat, np.float64)
+    mid = _model(ctx, w, g, med)
+    out = _gpu_born_adj(ctx, mid, spos, rpos, src_fine, dat, g.nx, g.nz, dtype)
+    assert rel_l2(out, ref) < 1e-4, rel_l2(out, ref)

@@ -1,0 +1,192 @@
+"""GPU tests of the checkpointed FWI gradient.
+
+C-ABI level: eio_fwi_forward records the same data as the NL forward, and eio_fwi_gradient
+(Born adjoint with segment recomputation from checkpoints into a ring of coarse slices) equals
+the oracle's full-storage Born adjoint, for segment lengths that exercise every ring case.
+torch level: ElasticFwi's autograd gradient against the FP64 oracle gradient (<= 1e-4 relative
+L2 in FP32) and against finite differences of the objective on 10 directions.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from elastic_isotropic_operators_amd import _capi
+
+from helpers import GRID_OF_COMP, fwi_case, fwi_directions, oracle_grid, random_traces, reg_positions, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = {np.float32: 1e-5, np.float64: 1e-11}
+
+
+def _reg_case(nz=40, nx=56, nts=120, sub=2, seed=40, nshot=1):
+    from elastic_isotropic_operators_amd import workloads as W
+    w = W.small(nz=nz, nx=nx, nts=nts, sub=sub, seed=seed, nExp=nshot)
+    g = oracle_grid(w)
+    med = O.media(g, w["elastic"], np.float64)
+    spos = [reg_positions(w, w["src_z"][e], w["src_x"][e]) for e in range(nshot)]
+    rpos = reg_positions(w, w["rec_z"], w["rec_x"])
+    return w, g, med, spos, rpos
+
+
+def _zeros_like_traces(pos, nts, dtype):
+    return [np.zeros((pos[GRID_OF_COMP[k]].size, nts), dtype) for k in range(5)]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("segment", [0, 1, 2, 5, 118, 119, 500])
+def test_checkpointed_gradient_matches_full_storage_oracle(dtype, segment):
+    """nts = 120 (119 coarse intervals): segment 0 -> 11 (10 checkpoints, a short last
+    segment), 1 and 2 (ring of 4/5 slices), 118 (one interval in the last segment), 119 and
+    500 (one segment, no checkpoint)."""
+    w, g, med, spos, rpos = _reg_case()
+    spos = spos[0]
+    rng = np.random.default_rng(41)
+    src = random_traces(rng, spos, g.nts, np.float64)
+    res = random_traces(rng, rpos, g.nts, np.float64)
+    ref_d = O.nl_fwd_reg(g, med, spos, src, rpos, np.float64)
+    src_fine = [O.time_interp(s, g.dts, g.sub) for s in src]
+    ref_g = O.born_adj_reg(g, med, spos, src_fine, rpos, res, np.float64)
+
+    ctx = _capi.Context(0, dtype)
+    mid = ctx.model_add(g.nz, g.nx, g.nts, g.sub, g.dz, g.dx, g.dts, g.minPad, med, g.alphaCos)
+    srcc = [np.ascontiguousarray(s, dtype) for s in src]
+    rec = _zeros_like_traces(rpos, g.nts, dtype)
+    shot, keep = ctx.make_shot(mid, spos, rpos, srcc, rec)
+    h = ctx.fwi_forward([shot], segment)
+    L, nck, nbytes = ctx.fwi_info(h)
+    Lexp = min(segment if segment > 0 else 11, g.nts - 1)
+    assert L == Lexp and nck == -(-(g.nts - 1) // Lexp) - 1 and nbytes > 0
+    for k in range(5):
+        assert rel_l2(rec[k], ref_d[k]) < TOL[dtype], (k, rel_l2(rec[k], ref_d[k]))
+
+    resc = [np.ascontiguousarray(r, dtype) for r in res]
+    shot2, keep2 = ctx.make_shot(mid, spos, rpos, srcc, resc)
+    out = np.zeros((5, g.nx, g.nz), dtype)
+    ctx.fwi_gradient(h, [shot2], out)
+    assert rel_l2(out, ref_g) < 10 * TOL[dtype], rel_l2(out, ref_g)
+    # the store can be used again and gives the same gradient
+    out2 = np.zeros((5, g.nx, g.nz), dtype)
+    ctx.fwi_gradient(h, [shot2], out2)
+    assert np.array_equal(out, out2)
+    ctx.fwi_release(h)
+    with pytest.raises(_capi.EioError, match="unknown FWI store"):
+        ctx.fwi_gradient(h, [shot2], out2)
+
+
+def test_checkpointed_gradient_several_shots():
+    """Three shots in one batch: stacked gradient equals the sum of per-shot oracle adjoints."""
+    dtype = np.float32
+    w, g, med, spos, rpos = _reg_case(nz=48, nx=72, nts=90, sub=1, seed=42, nshot=3)
+    rng = np.random.default_rng(43)
+    ctx = _capi.Context(0, dtype)
+    mid = ctx.model_add(g.nz, g.nx, g.nts, g.sub, g.dz, g.dx, g.dts, g.minPad, med, g.alphaCos)
+    shots, keep, resid, ref = [], [], [], np.zeros((5, g.nx, g.nz))
+    srcs = []
+    for e in range(3):
+        src = random_traces(rng, spos[e], g.nts, np.float64)
+        res = random_traces(rng, rpos, g.nts, np.float64)
+        ref += O.born_adj_reg(g, med, spos[e], [O.time_interp(s, g.dts, g.sub) for s in src], rpos, res)
+        srcc = [np.ascontiguousarray(s, dtype) for s in src]
+        rec = _zeros_like_traces(rpos, g.nts, dtype)
+        s, kp = ctx.make_shot(mid, spos[e], rpos, srcc, rec)
+        shots.append(s)
+        keep.append((kp, srcc, rec))
+        resid.append([np.ascontiguousarray(r, dtype) for r in res])
+        srcs.append(srcc)
+    h = ctx.fwi_forward(shots, 4)
+    shots2 = []
+    for e in range(3):
+        s, kp = ctx.make_shot(mid, spos[e], rpos, srcs[e], resid[e])
+        shots2.append(s)
+        keep.append(kp)
+    out = np.zeros((5, g.nx, g.nz), dtype)
+    ctx.fwi_gradient(h, shots2, out)
+    assert rel_l2(out, ref) < 1e-4, rel_l2(out, ref)
+    # a gradient call with other shots than the store's is an argument error
+    with pytest.raises(_capi.EioError, match="differ"):
+        ctx.fwi_gradient(h, shots2[:2], out)
+    ctx.fwi_release(h)
+
+
+# ------------------------------------------------------------------ torch layer
+def _engine(w, g, dtype, segment=0):
+    import torch
+    from elastic_isotropic_operators_amd.fwi import ElasticFwi
+    grid = dict(nz=w["nz"], nx=w["nx"], dz=w["dz"], dx=w["dx"], oz=w["oz"], ox=w["ox"], nts=w["nts"],
+                dts=w["dts"], sub=w["sub"], minPad=g.minPad, alphaCos=w["alphaCos"])
+    return ElasticFwi(grid, w["src_z"], w["src_x"], w["model"], w["rec_z"], w["rec_x"], dtype=dtype,
+                      segment=segment)
+
+
+def _oracle_gradient(w, g, m0, dobs):
+    sz, sx, rz, rx, src = w["src_z"], w["src_x"], w["rec_z"], w["rec_x"], w["model"]
+    res = np.stack([O.nl_fwd_shot(g, m0, sz[e], sx[e], rz, rx, src) for e in range(sz.shape[0])]) - dobs
+    return O.born_adj_shots(g, m0, sz, sx, src, rz, rx, res), res
+
+
+@pytest.mark.parametrize("tdtype", ["float32", "float64"])
+def test_autograd_gradient_vs_fp64_oracle(tdtype):
+    import torch
+    dtype = getattr(torch, tdtype)
+    w, g, m0, dtrue = fwi_case(nts=400, dts=0.001, nExp=2, nz=48, nx=72)
+    sz, sx, rz, rx, src = w["src_z"], w["src_x"], w["rec_z"], w["rec_x"], w["model"]
+    dobs = np.stack([O.nl_fwd_shot(g, m0 + dtrue, sz[e], sx[e], rz, rx, src) for e in range(2)])
+    ref, res_ref = _oracle_gradient(w, g, m0, dobs)
+    eng = _engine(w, g, dtype)
+    m = torch.tensor(m0, dtype=dtype, device="cuda", requires_grad=True)
+    d_obs = torch.tensor(dobs, dtype=dtype, device="cuda")
+    loss = eng.misfit(m, d_obs)
+    loss.backward()
+    grad = m.grad.double().cpu().numpy()
+    tol = 1e-4 if dtype == torch.float32 else 1e-10
+    assert rel_l2(grad, ref) < tol, rel_l2(grad, ref)
+    assert abs(loss.item() - 0.5 * np.sum(res_ref ** 2)) / (0.5 * np.sum(res_ref ** 2)) < tol * 10
+
+
+def test_autograd_gradient_finite_differences():
+    """FP64 GPU: central finite differences of the objective on 10 smooth random directions
+    against <gradient, direction>; the bar is the O(dts^2) coarse-time gap of the Born
+    operator that the oracle test pins (< 2 % at dts = 0.5 ms)."""
+    import torch
+    w, g, m0, dtrue = fwi_case(nts=800, dts=0.0005)
+    eng = _engine(w, g, torch.float64)
+    mt = torch.tensor(m0 + dtrue, device="cuda")
+    with torch.no_grad():
+        d_obs, _, _ = eng.forward_data(mt)
+    m = torch.tensor(m0, device="cuda", requires_grad=True)
+    eng.misfit(m, d_obs).backward()
+    grad = m.grad.cpu().numpy()
+
+    def J(x):
+        with torch.no_grad():
+            return float(eng.misfit(torch.tensor(x, device="cuda"), d_obs))
+
+    fd, ad = [], []
+    for v in fwi_directions(m0, 10):
+        eps = 1e-2
+        fd.append((J(m0 + eps * v) - J(m0 - eps * v)) / (2 * eps))
+        ad.append(np.sum(grad * v))
+    fd, ad = np.array(fd), np.array(ad)
+    gap = np.linalg.norm(fd - ad) / np.linalg.norm(fd)
+    corr = fd @ ad / np.linalg.norm(fd) / np.linalg.norm(ad)
+    assert gap < 0.02 and corr > 0.9999, (gap, corr)
+
+
+def test_autograd_chain_rule_through_conversion_and_datcomp():
+    """Gradient with respect to (vp, vs, rho) through the torch conversion and the vx,vz,p
+    component selection equals ElasticConvJab.adjoint of the (rho, lambda, mu) gradient of the
+    same objective (elasticParamConvertModule.py:91-108, dataCompModule.py:79-107)."""
+    import torch
+    from elastic_isotropic_operators_amd import seis_utils as su
+    from elastic_isotropic_operators_amd.fwi import datcomp, vpvsrho_to_rholamemu
+    w, g, m0, dtrue = fwi_case(nts=300, dts=0.001)
+    eng = _engine(w, g, torch.float64, segment=7)
+    vvr = torch.tensor(w["vvr"].astype(np.float64), device="cuda", requires_grad=True)
+    with torch.no_grad():
+        d_obs = datcomp(eng.forward_data(vpvsrho_to_rholamemu(vvr) * 1.01)[0], "vx,vz,p")
+    eng.misfit(vpvsrho_to_rholamemu(vvr), d_obs, components="vx,vz,p").backward()
+    m = vpvsrho_to_rholamemu(vvr.detach()).clone().requires_grad_()
+    eng.misfit(m, d_obs, components="vx,vz,p").backward()
+    ref = su.conv_jacobian_adj(w["vvr"].astype(np.float64), m.grad.cpu().numpy())
+    assert rel_l2(vvr.grad.cpu().numpy(), ref) < 1e-10
