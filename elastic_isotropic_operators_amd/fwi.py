@@ -19,6 +19,10 @@ any parameter conversion written in torch (``vpvsrho_to_rholamemu``).
 
 The gradient ignores the model dependence of the vx/vz source scaling (rhoxDtw, rhozDtw at
 the source positions), as the reference's Born-based gradient does.
+
+Import order: torch's ROCm wheel carries its own HIP runtime with the system runtime's soname;
+torch must be imported before libeio_hip.so is loaded, so the library binds to that runtime
+and torch's device pointers and streams are valid for it.
 """
 import weakref
 
@@ -139,6 +143,9 @@ class ElasticFwi:
     def __init__(self, grid, src_z, src_x, sources, rec_z, rec_x, device=0, dtype=torch.float32,
                  segment=0, variant=None):
         if not torch.cuda.is_available():
+            if _capi._lib is not None:
+                raise RuntimeError("torch sees no GPU because libeio_hip.so was loaded before torch: "
+                                   "import torch first so both share torch's HIP runtime")
             raise RuntimeError("ElasticFwi needs a GPU (the propagator is HIP-only)")
         self.g = dict(grid)
         self.nz, self.nx, self.nts = int(grid["nz"]), int(grid["nx"]), int(grid["nts"])

@@ -11,6 +11,7 @@ Workloads (sizes are interior sizes, before padding):
   gaussian_c0             200 x 400 Gaussian-anomaly model, nts=1000, 1 shot (NL forward)
   gaussian_c1             same grid, 8 shots, smoothed random perturbations (Born)
   layered_batch_c2        64 layered shots of uneven grid and trace length (batched NL)
+  fwi_c3                  32-shot elastic FWI gradient on a faulted layered model with salt
 """
 import numpy as np
 
@@ -156,6 +157,54 @@ def layered_batch_c2(nshots=64, seed=2, scale=1.0):
         w["name"] = "c2_shot%02d" % i
         shots.append(w)
     return shots
+
+
+def _faulted_salt_vvr(rng, nz, nx, nfault=3):
+    """Layered model (layered_vvr ranges) cut by `nfault` dipping normal faults and one
+    salt-like body (vp 4500, vs 2600, Gardner density)."""
+    vvr = layered_vvr(rng, nz, nx).astype(np.float64)
+    iz = np.arange(nz)[None, :].astype(np.float64)
+    ix = np.arange(nx)[:, None].astype(np.float64)
+    for _ in range(nfault):
+        x0 = rng.uniform(0.15, 0.85) * nx
+        dip = np.deg2rad(rng.uniform(45.0, 75.0)) * (1 if rng.random() < 0.5 else -1)
+        throw = int(rng.integers(max(2, nz // 60), max(3, nz // 20) + 1))
+        hanging = ix > x0 + iz / np.tan(dip)      # cells beyond the fault plane
+        shifted = np.concatenate([np.repeat(vvr[:, :, :1], throw, axis=2), vvr[:, :, :-throw]], axis=2)
+        vvr = np.where(hanging[None], shifted, vvr)
+    zc, xc = rng.uniform(0.35, 0.6) * nz, rng.uniform(0.3, 0.7) * nx
+    a, b = rng.uniform(0.08, 0.15) * nx, rng.uniform(0.12, 0.2) * nz
+    ang = np.arctan2(iz - zc, ix - xc)
+    r = 1.0 + 0.15 * np.sin(3.0 * ang + rng.uniform(0, 2 * np.pi))
+    salt = ((ix - xc) / a) ** 2 + ((iz - zc) / b) ** 2 < r ** 2
+    vvr[0][salt] = 4500.0
+    vvr[1][salt] = 2600.0
+    vvr[2][salt] = 310.0 * 4500.0 ** 0.25
+    return vvr.astype(np.float32)
+
+
+def fwi_c3(nshots=32, seed=3, scale=1.0, smooth_cells=25.0):
+    """Elastic FWI gradient workload: 500x1500 interior at 10 m (40-cell sponge -> 600x1592
+    padded), nts=3000 at 1 ms, sub=2, Ricker 8 Hz, `nshots` evenly spaced surface shots,
+    receivers every 2 cells at the surface.  True model: seeded layered model with 3 dipping
+    faults and a salt body; starting model: the true model smoothed with a 25-cell Gaussian.
+    Returns (true workload, starting workload); both share geometry and sources."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    nz, nx, nts = int(500 * scale), int(1500 * scale), int(3000 * scale)
+    vvr = _faulted_salt_vvr(rng, nz, nx)
+    vvr0 = np.stack([gaussian_filter(vvr[k].astype(np.float64), smooth_cells * scale, mode="nearest")
+                     for k in range(3)]).astype(np.float32)
+    out = []
+    for v, tag in ((vvr, "true"), (vvr0, "start")):
+        w = _grid_dict(v, 10.0, 10.0, 40, 40, nts, 0.001, 2, 20.0)
+        sp = max(1, nx // nshots)
+        w["src_z"], w["src_x"] = source_coords(w, 3, sp // 2 + 1, nExp=nshots, spacingShots=sp)
+        w["rec_z"], w["rec_x"] = receiver_coords(w, nx // 2, 1, 1, 2)
+        w["model"] = explosive_model(nts, w["dts"], 8.0, 0.15)
+        w["name"] = "fwi_c3_%s_%dx%d_nts%d_%dshots" % (tag, w["nz"], w["nx"], nts, nshots)
+        out.append(w)
+    return out[0], out[1]
 
 
 def cell_updates(w):
