@@ -1,16 +1,23 @@
 #!/usr/bin/env python3
-"""Benchmark of the batched elastic propagator on one MI355X.
+"""Benchmark of the elastic propagator on one MI355X.
 
-Workload (default "c2"): 64 layered shots of uneven size (interior nz 300-600, nx
-600-1800, nts 2000-4000 at 1 ms, sub=2, Ricker 8 Hz, 40-cell sponge), FP32, all shots
-propagated by the nonlinear forward operator in one batched time loop.  A "step" is one
-full batched propagation of the 64 shots.  Inputs (media, source traces) are resident in
-HBM before the timed region; outputs stay in HBM.
+Headline (value, GCell/s): workload "c2", 64 layered shots of uneven size (interior nz
+300-600, nx 600-1800, nts 2000-4000 at 1 ms, sub=2, Ricker 8 Hz, 40-cell sponge), FP32, all
+shots propagated by the nonlinear forward operator in one batched time loop.  A "step" is
+one full batched propagation of the 64 shots.  Inputs (media, source traces) are resident
+in HBM before the timed region; outputs stay in HBM.
+
+FWI leg (fwi_gradient, s/shot): the 32-shot elastic FWI gradient (600x1592 padded grid,
+nts=3000, sub=2) through the torch autograd layer with checkpointed forward wavefields,
+plus the relative L2 distance of the FP32 gradient from the FP64 gradient on two of the
+shots.  It runs in a child process because torch must be imported before the propagator
+library is loaded (both then share torch's HIP runtime), while the headline leg and its
+profiles use the system runtime without torch.
 
 Prints ONE JSON line with GCell/s (cell-update = one update-region cell for one fine step),
 the HBM roofline of the fused step kernel (80 algorithmic bytes per FP32 cell-update,
-timed with HIP events on the kernel's stream) and a CPU baseline: the FP32 OpenMP oracle
-on a bounded sample of the same workload on this host's cores.
+timed with HIP events on the kernel's stream), the FWI leg and a CPU baseline: the FP32
+OpenMP oracle on a bounded sample of the same workload on this host's cores.
 """
 import argparse
 import json
@@ -27,6 +34,8 @@ from elastic_isotropic_operators_amd import _capi, geometry as G, workloads as W
 
 BYTES_PER_CELL = {"nl_fwd": 80}
 HBM_PEAK_GBS = 8000.0
+METRIC = "GCell-updates/s + FWI-gradient s/shot at \u22641e-4 rel-L2 vs FP64, 1 MI355X"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_step_fwd_layered_batch_c2_traffic.json")
 
 
 def log(msg):
@@ -88,6 +97,82 @@ def cpu_baseline(shots_w, target_s=12.0):
                        "with flush-to-zero, %.1f s" % (w["nz"], w["nx"], nts, w["nts"], el))
 
 
+def fwi_child(args):
+    """FWI gradient leg (child process): torch first, then the propagator library."""
+    import torch
+    from elastic_isotropic_operators_amd.fwi import ElasticFwi, datcomp
+
+    t0 = time.perf_counter()
+    wt, w0 = W.fwi_c3(nshots=args.fwi_shots, scale=args.fwi_scale)
+    grid = dict(nz=w0["nz"], nx=w0["nx"], dz=w0["dz"], dx=w0["dx"], oz=w0["oz"], ox=w0["ox"],
+                nts=w0["nts"], dts=w0["dts"], sub=w0["sub"], minPad=W.min_pad(w0), alphaCos=w0["alphaCos"])
+    comps = "vx,vz,p"
+
+    def engine(dtype, sel):
+        return ElasticFwi(grid, w0["src_z"][sel], w0["src_x"][sel], w0["model"], w0["rec_z"], w0["rec_x"],
+                          dtype=dtype)
+
+    eng = engine(torch.float32, slice(None))
+    with torch.no_grad():
+        d_obs = datcomp(eng.forward_data(torch.tensor(wt["elastic"], device="cuda"))[0], comps)
+    torch.cuda.synchronize()
+    log("fwi: workload %s, observed data in %.1f s" % (w0["name"], time.perf_counter() - t0))
+    m = torch.tensor(w0["elastic"], device="cuda", requires_grad=True)
+
+    def gradient():
+        m.grad = None
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        loss = eng.misfit(m, d_obs, comps)
+        loss.backward()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t
+
+    for i in range(args.fwi_warmup):
+        log("fwi warmup %d: %.2f s" % (i, gradient()))
+    times = []
+    for i in range(args.fwi_steps):
+        times.append(gradient())
+        log("fwi gradient %d: %.2f s (%.3f s/shot)" % (i, times[-1], times[-1] / eng.nshot))
+    L, nck, nbytes = eng.last_store_info
+    # per shot: forward (nts-1 coarse intervals) + recompute of every segment but the last
+    # (L intervals each) + adjoint, at cell_updates / (nts-1) cell-updates per interval
+    per_interval = W.cell_updates(w0) / (w0["nts"] - 1)
+    grad_cells = eng.nshot * per_interval * (2 * (w0["nts"] - 1) + L * nck)
+    g32 = m.grad.detach().double()
+    # FP32 vs FP64 on two of the shots at full size
+    sel = [0, eng.nshot // 2]
+    gs = {}
+    for dt in (torch.float64, torch.float32):
+        e2 = engine(dt, sel)
+        m2 = torch.tensor(w0["elastic"], dtype=dt, device="cuda", requires_grad=True)
+        e2.misfit(m2, d_obs[sel].to(dt), comps).backward()
+        gs[dt] = m2.grad.detach().double()
+        del e2, m2
+    rel = float(torch.linalg.norm(gs[torch.float32] - gs[torch.float64]) / torch.linalg.norm(gs[torch.float64]))
+    log("fwi: FP32 vs FP64 gradient on shots %s: rel L2 %.2e" % (sel, rel))
+    t = float(np.mean(times))
+    out = {"s_per_shot": round(t / eng.nshot, 4), "s_per_gradient": round(t, 3), "shots": eng.nshot,
+           "workload": w0["name"], "dtype": "f32", "steps": args.fwi_steps, "warmup": args.fwi_warmup,
+           "gcell_per_s": round(grad_cells / t / 1e9, 2), "rel_l2_vs_fp64": rel, "fp64_check_shots": sel,
+           "checkpoint_interval": L, "checkpoints": nck, "store_bytes": nbytes,
+           "full_storage_bytes": 2 * 4 * w0["nz"] * w0["nx"] * w0["nts"] * eng.nshot,
+           "grad_norm": float(torch.linalg.norm(g32))}
+    print("FWI_JSON " + json.dumps(out), flush=True)
+
+
+def run_fwi_leg(args):
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--fwi-child", "--fwi-shots", str(args.fwi_shots),
+           "--fwi-steps", str(args.fwi_steps), "--fwi-warmup", str(args.fwi_warmup),
+           "--fwi-scale", str(args.fwi_scale)]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=900)
+    for line in r.stdout.splitlines():
+        if line.startswith("FWI_JSON "):
+            return json.loads(line[len("FWI_JSON "):])
+    raise RuntimeError("FWI leg failed (exit %d)" % r.returncode)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,7 +186,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nts-cap", type=int, default=0, help="cap every trace length (profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=1, help="measure a plain copy kernel")
+    ap.add_argument("--fwi", type=int, default=1, help="run the FWI gradient leg")
+    ap.add_argument("--fwi-shots", type=int, default=32)
+    ap.add_argument("--fwi-steps", type=int, default=1)
+    ap.add_argument("--fwi-warmup", type=int, default=1)
+    ap.add_argument("--fwi-scale", type=float, default=1.0)
+    ap.add_argument("--fwi-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.fwi_child:
+        fwi_child(args)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box of this build has a single MI355X; "
               "shots would be partitioned across GPUs with one all-reduce of the gradient "
@@ -151,8 +245,9 @@ def main():
     value = cells * args.steps / elapsed / 1e9
     loop_s = float(np.mean([s * n for s, n in zip(step_ms, nlaunch)])) / 1e3
     achieved = BYTES_PER_CELL["nl_fwd"] * cells / loop_s / 1e9
+    R, WX, OCC = _capi.Context.variants()[args.variant if args.variant >= 0 else 1]
     res = {
-        "metric": "GCell-updates/s (FP32 batched NL forward)",
+        "metric": METRIC,
         "value": round(value, 3),
         "unit": "GCell/s",
         "n_gpus": 1,
@@ -168,7 +263,7 @@ def main():
                    "scale": args.scale, "graph": args.graph, "variant": args.variant},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_step_fwd<float,4,M_NL>",
+                     "kernel": "k_step_fwd<float,%d,%d,%d,M_NL>" % (R, WX, OCC),
                      "mean_launch_ms": round(float(np.mean(step_ms)), 5),
                      "launches_per_step": int(nlaunch[-1]),
                      "bytes_per_cell_update": BYTES_PER_CELL["nl_fwd"]},
@@ -178,6 +273,14 @@ def main():
         cms = ctx.copy_probe(nb, 10)
         res["roofline"]["copy_probe_GBs"] = round(2 * nb / (cms / 1e3) / 1e9, 1)
         log("copy probe: %.3f ms for 2 x %d bytes -> %.0f GB/s" % (cms, nb, 2 * nb / cms / 1e6))
+    if os.path.exists(TRAFFIC_FILE) and args.variant < 0 and args.workload == "c2" and args.scale == 1.0:
+        tr = json.load(open(TRAFFIC_FILE))
+        res["roofline"]["traffic"] = tr["bytes_per_launch"]
+        res["roofline"]["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
+    if args.fwi:
+        t0 = time.perf_counter()
+        res["fwi_gradient"] = run_fwi_leg(args)
+        log("fwi leg %.1f s" % (time.perf_counter() - t0))
     if not args.no_cpu:
         t0 = time.perf_counter()
         res["cpu_baseline"] = cpu_baseline(shots_w)

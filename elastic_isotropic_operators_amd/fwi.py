@@ -158,6 +158,7 @@ class ElasticFwi:
         if variant is not None:
             self.ctx.set_variant(variant)
         self.mid = None
+        self.last_store_info = None  # (interval, checkpoints, device bytes) of the last store
         src_z = np.atleast_2d(np.asarray(src_z, np.float64))
         src_x = np.atleast_2d(np.asarray(src_x, np.float64))
         self.nshot = src_z.shape[0]
@@ -266,6 +267,7 @@ class ElasticFwi:
         if checkpoint:
             h = self.ctx.fwi_forward(shots, self.segment, device_ptrs=True)
             store = _Store(self, h, srcs)
+            self.last_store_info = self.ctx.fwi_info(h)
         else:
             self.ctx.nl_forward(shots, device_ptrs=True)
         return self._reg_to_irreg(recs), store, media
