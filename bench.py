@@ -30,7 +30,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from elastic_isotropic_operators_amd import _capi, geometry as G, workloads as W  # noqa: E402
+from elastic_isotropic_operators_amd import _capi, autotune, geometry as G, workloads as W  # noqa: E402
 
 BYTES_PER_CELL = {"nl_fwd": 80}
 HBM_PEAK_GBS = 8000.0
@@ -43,8 +43,9 @@ def log(msg):
 
 
 def prepare_shots(ctx, shots_w, dtype=np.float32):
-    """Upload media and source traces; allocate receiver traces.  Returns (shots, keep, cells)."""
-    shots, keep, cells = [], [], 0
+    """Upload media and source traces; allocate receiver traces.
+    Returns (shots, keep, cells, grids) with grids = per shot (nz, nx, ntw-1, n_src, n_rec)."""
+    shots, keep, cells, grids = [], [], 0, []
     for w in shots_w:
         sub_fd = 2 * w["sub"]
         dtw = G.dtw_of(w["dts"], w["sub"])
@@ -67,7 +68,9 @@ def prepare_shots(ctx, shots_w, dtype=np.float32):
         keep.append((kp, spos, rpos, dsrc, drec))
         shots.append(shot)
         cells += W.cell_updates(w)
-    return shots, keep, cells
+        grids.append((w["nz"], w["nx"], (w["nts"] - 1) * sub_fd, sum(p.size for p in spos),
+                      sum(p.size for p in rpos)))
+    return shots, keep, cells, grids
 
 
 def cpu_baseline(shots_w, target_s=12.0):
@@ -183,6 +186,10 @@ def main():
     ap.add_argument("--nshots", type=int, default=64)
     ap.add_argument("--graph", type=int, default=0)
     ap.add_argument("--variant", type=int, default=-1, help="FP32 step-kernel variant (-1: default)")
+    ap.add_argument("--time-block", type=int, default=1, help="fine steps per launch (1 or 2)")
+    ap.add_argument("--variant2", type=int, default=0, help="two-step kernel variant")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="pick time block and variant by timing every candidate (overrides the flags)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nts-cap", type=int, default=0, help="cap every trace length (profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=1, help="measure a plain copy kernel")
@@ -220,8 +227,19 @@ def main():
     ctx.set_graph(args.graph)
     if args.variant >= 0:
         ctx.set_variant(args.variant)
-    shots, keep, cells = prepare_shots(ctx, shots_w)
+    ctx.set_time_block(args.time_block, args.variant2)
+    shots, keep, cells, grids = prepare_shots(ctx, shots_w)
     log("uploaded %d shots, %.3e cell-updates per step" % (len(shots), cells))
+    tuned = None
+    if args.autotune:
+        tuned = autotune.tune_nl_forward(ctx, shots, grids, log=log)
+        args.time_block, v = tuned["choice"]
+        if args.time_block == 2:
+            args.variant2 = v
+        else:
+            args.variant = v
+        log("autotune choice: time block %d, variant %d%s" % (args.time_block, v,
+                                                             " (cached)" if tuned["cached"] else ""))
 
     for i in range(args.warmup):
         t0 = time.perf_counter()
@@ -245,7 +263,12 @@ def main():
     value = cells * args.steps / elapsed / 1e9
     loop_s = float(np.mean([s * n for s, n in zip(step_ms, nlaunch)])) / 1e3
     achieved = BYTES_PER_CELL["nl_fwd"] * cells / loop_s / 1e9
-    R, WX, OCC = _capi.Context.variants()[args.variant if args.variant >= 0 else 1]
+    if args.time_block == 2:
+        R2, WX2, OCC2 = _capi.Context.time_block_variants()[args.variant2]
+        kname = "k_step2_fwd<float,%d,%d,%d> (2 fine steps per launch)" % (R2, WX2, OCC2)
+    else:
+        R, WX, OCC = _capi.Context.variants()[args.variant if args.variant >= 0 else 1]
+        kname = "k_step_fwd<float,%d,%d,%d,M_NL>" % (R, WX, OCC)
     res = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -260,10 +283,14 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": wl_name, "shots": len(shots), "cell_updates_per_step": cells,
-                   "scale": args.scale, "graph": args.graph, "variant": args.variant},
+                   "scale": args.scale, "graph": args.graph, "variant": args.variant,
+                   "time_block": args.time_block, "variant2": args.variant2,
+                   "autotune": None if tuned is None else {"choice": list(tuned["choice"]),
+                                                           "ms_per_fine_step": tuned["table"],
+                                                           "cached": tuned["cached"]}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_step_fwd<float,%d,%d,%d,M_NL>" % (R, WX, OCC),
+                     "kernel": kname,
                      "mean_launch_ms": round(float(np.mean(step_ms)), 5),
                      "launches_per_step": int(nlaunch[-1]),
                      "bytes_per_cell_update": BYTES_PER_CELL["nl_fwd"]},

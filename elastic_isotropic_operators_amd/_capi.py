@@ -79,6 +79,11 @@ def lib():
             ("eio_fwi_info", [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                               ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_ulonglong)]),
             ("eio_fwi_release", [vp, ctypes.c_int]),
+            ("eio_ctx_num_time_block_variants", []),
+            ("eio_ctx_time_block_info", [ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                         ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+            ("eio_ctx_set_time_block", [vp, ctypes.c_int, ctypes.c_int]),
+            ("eio_ctx_set_step_limit", [vp, ctypes.c_int]),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -93,7 +98,8 @@ EXPORTED = ("eio_last_error", "eio_ctx_create", "eio_ctx_destroy", "eio_ctx_set_
             "eio_born_adjoint", "eio_device_alloc", "eio_device_free", "eio_memcpy",
             "eio_synchronize", "eio_copy_probe", "eio_ctx_set_variant", "eio_ctx_variant_info",
             "eio_ctx_num_variants", "eio_fwi_forward", "eio_fwi_gradient", "eio_fwi_info",
-            "eio_fwi_release")
+            "eio_fwi_release", "eio_ctx_num_time_block_variants", "eio_ctx_time_block_info",
+            "eio_ctx_set_time_block", "eio_ctx_set_step_limit")
 
 
 def check(rc):
@@ -169,6 +175,24 @@ class Context:
         for v in range(lib().eio_ctx_num_variants()):
             r, wx, oc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
             check(lib().eio_ctx_variant_info(v, ctypes.byref(r), ctypes.byref(wx), ctypes.byref(oc)))
+            out.append((r.value, wx.value, oc.value))
+        return out
+
+    def set_time_block(self, steps, variant=0):
+        """Fine steps per launch of the FP32 NL forward (1 or 2) and the two-step variant."""
+        check(lib().eio_ctx_set_time_block(self.h, int(steps), int(variant)))
+
+    def set_step_limit(self, max_steps):
+        """Stop NL loops after max_steps fine steps (0: no limit); for tuning runs."""
+        check(lib().eio_ctx_set_step_limit(self.h, int(max_steps)))
+
+    @staticmethod
+    def time_block_variants():
+        """[(R2, WX, occ)] of the two-step NL forward kernel variants."""
+        out = []
+        for v in range(lib().eio_ctx_num_time_block_variants()):
+            r, wx, oc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+            check(lib().eio_ctx_time_block_info(v, ctypes.byref(r), ctypes.byref(wx), ctypes.byref(oc)))
             out.append((r.value, wx.value, oc.value))
         return out
 

@@ -24,6 +24,7 @@
 
 #include "../../include/eio.h"
 #include "eio_kernels.h"
+#include "eio_step2.h"
 
 namespace {
 
@@ -58,6 +59,14 @@ struct Variant {
 constexpr Variant F32_VARIANTS[] = {{4, 4, 2}, {2, 8, 4}, {2, 4, 4}, {4, 8, 2}, {8, 4, 1}, {2, 8, 2}};
 constexpr int N_F32_VARIANTS = int(sizeof(F32_VARIANTS) / sizeof(F32_VARIANTS[0]));
 constexpr Variant F64_VARIANT = {2, 4, 1};
+
+// Two-step (temporally blocked) NL forward variants: R2 owned x-cells per lane, WX waves,
+// OCC waves per SIMD; owned tile = 56 x WX*R2.  FP32 only.
+struct Variant2 {
+    int R2, WX, OCC;
+};
+constexpr Variant2 F32_VARIANTS2[] = {{2, 8, 2}, {2, 8, 4}, {4, 4, 2}, {1, 8, 4}, {4, 8, 1}};
+constexpr int N_F32_VARIANTS2 = int(sizeof(F32_VARIANTS2) / sizeof(F32_VARIANTS2[0]));
 
 template <typename T>
 int tile_x(int variant) {
@@ -106,6 +115,9 @@ struct Ctx {
     long long last_nstep = 0;
     int use_graph = 0;
     int variant = 1;  // best measured FP32 variant on the 64-shot layered batch
+    int time_block = 1;  // fine steps per launch of the FP32 NL forward (1 or 2)
+    int variant2 = 0;    // two-step kernel variant
+    int step_limit = 0;  // > 0: NL loops stop after this many fine steps (tuning runs)
     std::map<int, FwiStore> fwi;
     int next_fwi = 1;
 };
@@ -250,9 +262,51 @@ void build_entries(const ModelBase& m, const int* const pos[4], const int nreg[4
     }
 }
 
+// Entries of a two-step tile: sources of the step-1 region [x0-4, x0+TX+4) x [z0-4, z0+60)
+// (every tile whose region holds the source gets it), receivers of the owned cells only;
+// lcell relative to (z0-4, x0-4).
+void build_entries2(const ModelBase& m, const int* const pos[4], const int nreg[4], int TX, bool halo,
+                    std::vector<std::vector<eio::Entry>>& per_tile, int nbz, int nbx, const char* what) {
+    const int nz = m.g.nz, nx = m.g.nx;
+    for (int k = 0; k < 5; ++k) {
+        const int gi = GRID_OF_COMP[k];
+        const int cnt = nreg[gi];
+        if (cnt < 0) fail(std::string(what) + ": negative device count");
+        if (cnt > 0 && !pos[gi]) fail(std::string(what) + ": NULL position list");
+        if (cnt >= (1 << 24)) fail(std::string(what) + ": too many devices");
+        for (int u = 0; u < cnt; ++u) {
+            const int p = pos[gi][u];
+            if (p < 0 || long(p) >= m.n) fail(std::string(what) + ": device position out of the grid");
+            const int iz = p % nz, ix = p / nz;
+            if (iz < eio::FAT || iz >= nz - eio::FAT || ix < eio::FAT || ix >= nx - eio::FAT)
+                fail(std::string(what) +
+                     ": device inside the FAT halo (only update-region positions are supported)");
+            const int h = halo ? eio::FAT : 0;
+            const int bz_lo = std::max(0, (iz - eio::FAT - eio::TZ2 - h + 1 + eio::TZ2 - 1) / eio::TZ2);
+            const int bz_hi = std::min(nbz - 1, (iz - eio::FAT + h) / eio::TZ2);
+            const int bx_lo = std::max(0, (ix - eio::FAT - TX - h + 1 + TX - 1) / TX);
+            const int bx_hi = std::min(nbx - 1, (ix - eio::FAT + h) / TX);
+            for (int bx = bx_lo; bx <= bx_hi; ++bx)
+                for (int bz = bz_lo; bz <= bz_hi; ++bz) {
+                    const int z0 = eio::FAT + bz * eio::TZ2, x0 = eio::FAT + bx * TX;
+                    const int lz = iz - (z0 - eio::FAT), lx = ix - (x0 - eio::FAT);
+                    if (lz < eio::FAT - h || lz >= eio::FAT + eio::TZ2 + h || lx < eio::FAT - h ||
+                        lx >= eio::FAT + TX + h)
+                        continue;
+                    eio::Entry e;
+                    e.lcell = lz | (lx << 8);
+                    e.crow = u | (k << 24);
+                    per_tile[size_t(bx) * nbz + bz].push_back(e);
+                }
+        }
+    }
+}
+
 template <typename T>
-Plan make_plan(Ctx* c, int nshot, const eio_shot* shots, Op op) {
-    const int TX = tile_x<T>(c->variant);
+Plan make_plan(Ctx* c, int nshot, const eio_shot* shots, Op op, bool two_step = false) {
+    const int TX = two_step ? F32_VARIANTS2[c->variant2].R2 * F32_VARIANTS2[c->variant2].WX
+                            : tile_x<T>(c->variant);
+    const int TZ = two_step ? eio::TZ2 : eio::TZ;
     Plan p;
     p.order.resize(nshot);
     std::iota(p.order.begin(), p.order.end(), 0);
@@ -262,17 +316,22 @@ Plan make_plan(Ctx* c, int nshot, const eio_shot* shots, Op op) {
     for (int oi = 0; oi < nshot; ++oi) {
         const eio_shot& s = shots[p.order[oi]];
         const ModelBase& m = get_model(c, s.model);
-        const int nbz = (m.g.nz - 2 * eio::FAT + eio::TZ - 1) / eio::TZ;
+        const int nbz = (m.g.nz - 2 * eio::FAT + TZ - 1) / TZ;
         const int nbx = (m.g.nx - 2 * eio::FAT + TX - 1) / TX;
         std::vector<std::vector<eio::Entry>> src_t(size_t(nbz) * nbx), rec_t(size_t(nbz) * nbx);
-        build_entries<T>(m, s.src_pos, s.n_src, TX, src_t, nbz, "source");
-        build_entries<T>(m, s.rec_pos, s.n_rec, TX, rec_t, nbz, "receiver");
+        if (two_step) {
+            build_entries2(m, s.src_pos, s.n_src, TX, true, src_t, nbz, nbx, "source");
+            build_entries2(m, s.rec_pos, s.n_rec, TX, false, rec_t, nbz, nbx, "receiver");
+        } else {
+            build_entries<T>(m, s.src_pos, s.n_src, TX, src_t, nbz, "source");
+            build_entries<T>(m, s.rec_pos, s.n_rec, TX, rec_t, nbz, "receiver");
+        }
         for (int bx = 0; bx < nbx; ++bx)
             for (int bz = 0; bz < nbz; ++bz) {
                 const size_t ti = size_t(bx) * nbz + bz;
                 eio::TileDev t;
                 t.shot = oi;
-                t.z0 = eio::FAT + bz * eio::TZ;
+                t.z0 = eio::FAT + bz * TZ;
                 t.x0 = eio::FAT + bx * TX;
                 t.src_b = int(p.ents.size());
                 p.ents.insert(p.ents.end(), src_t[ti].begin(), src_t[ti].end());
@@ -314,12 +373,13 @@ struct Run {
 // storage ring (`ring`, `wcap` slices per component and shot).
 template <typename T>
 Run<T> prepare(Ctx* c, int nshot, const eio_shot* shots, Op op, int flags,
-               const void* const dm_fwd[5], T* ring = nullptr, int wcap = 0) {
+               const void* const dm_fwd[5], T* ring = nullptr, int wcap = 0, bool two_step = false) {
     if (nshot <= 0) fail("nshot must be > 0");
     if (!shots) fail("shots is NULL");
     Run<T> run;
     run.c = c;
-    run.plan = make_plan<T>(c, nshot, shots, op);
+    run.plan = make_plan<T>(c, nshot, shots, op, two_step);
+    const int nlev = two_step ? 4 : 2;  // state levels per shot
     const bool born = (op == Op::BORN_FWD || op == Op::BORN_ADJ);
     const bool fwi = (op == Op::FWI_FWD || op == Op::FWI_GRAD);
     const bool has_ss = born || op == Op::FWI_GRAD;
@@ -336,7 +396,7 @@ Run<T> prepare(Ctx* c, int nshot, const eio_shot* shots, Op op, int flags,
     for (int oi = 0; oi < nshot; ++oi) {
         const eio_shot& s = shots[run.plan.order[oi]];
         const ModelBase& m = get_model(c, s.model);
-        state_elems += 10 * size_t(m.n);
+        state_elems += 5 * nlev * size_t(m.n);
         if (has_ss) ss_elems += 10 * size_t(m.n);
         if (born) w_elems += 2 * size_t(m.n) * m.g.nts;
         if (has_acc) acc_elems += 5 * size_t(m.n);
@@ -368,8 +428,12 @@ Run<T> prepare(Ctx* c, int nshot, const eio_shot* shots, Op op, int flags,
         for (int k = 0; k < 4; ++k) { d.cz[k] = T(m.cz[k]); d.cx[k] = T(m.cx[k]); }
         d.A = state + so;
         d.B = state + so + 5 * size_t(m.n);
+        if (two_step) {
+            d.C = state + so + 10 * size_t(m.n);
+            d.D = state + so + 15 * size_t(m.n);
+        }
         d.med = static_cast<const T*>(m.med);
-        so += 10 * size_t(m.n);
+        so += 5 * nlev * size_t(m.n);
         d.damp = static_cast<const T*>(m.damp);
         d.filt = static_cast<const T*>(m.filt);
         const int src_ld = born ? m.ntw : m.g.nts;
@@ -499,6 +563,21 @@ void launch_adj(Run<T>& run, int ntile, int k, int sspar) {
     }
 #undef EIO_L
 }
+void launch_fwd2(Run<float>& run, int ntile, int k, int par) {
+    hipStream_t s = run.c->stream;
+#define EIO_L2(R_, WX_, OCC_)                                                                  \
+    hipLaunchKernelGGL((eio::k_step2_fwd<float, R_, WX_, OCC_>), dim3(ntile), dim3(64 * WX_), 0, s, \
+                       run.d_shots, run.d_tiles, run.d_ents, ntile, k, par)
+    switch (run.c->variant2) {
+        case 0: EIO_L2(2, 8, 2); break;
+        case 1: EIO_L2(2, 8, 4); break;
+        case 2: EIO_L2(4, 4, 2); break;
+        case 3: EIO_L2(1, 8, 4); break;
+        default: EIO_L2(4, 8, 1); break;
+    }
+#undef EIO_L2
+}
+
 template <typename T, bool ADJ>
 void launch_img(Run<T>& run, int ntile, int its, int which) {
     hipStream_t s = run.c->stream;
@@ -548,15 +627,26 @@ void timing_end(Ctx* c, long long nstep) {
 
 template <typename T>
 void nl_forward(Ctx* c, int nshot, const eio_shot* shots, int flags) {
-    Run<T> run = prepare<T>(c, nshot, shots, Op::NL_FWD, flags, nullptr);
-    const int kmax = run.plan.nsteps.empty() ? 0 : run.plan.nsteps[0];
+    const bool two = sizeof(T) == 4 && c->time_block == 2;
+    Run<T> run = prepare<T>(c, nshot, shots, Op::NL_FWD, flags, nullptr, nullptr, 0, two);
+    int kmax = run.plan.nsteps.empty() ? 0 : run.plan.nsteps[0];
+    if (c->step_limit > 0) kmax = std::min(kmax, c->step_limit);
+    long long nlaunch = 0;
     timing_begin<T>(c);
     EIO_CHECK_HIP(hipEventRecord(c->ev[2], c->stream));
     run_loop(c, [&]() {
-        for (int k = 0; k < kmax; ++k) launch_fwd<T, eio::M_NL>(run, active_tiles(run.plan, k), k, 0);
+        if constexpr (sizeof(T) == 4) {
+            if (two) {
+                // pairs (k, k+1); a shot whose last pair has one step does only step k
+                for (int k = 0; k < kmax; k += 2, ++nlaunch)
+                    launch_fwd2(run, active_tiles(run.plan, k), k, (k >> 1) & 1);
+                return;
+            }
+        }
+        for (int k = 0; k < kmax; ++k, ++nlaunch) launch_fwd<T, eio::M_NL>(run, active_tiles(run.plan, k), k, 0);
     });
     EIO_CHECK_HIP(hipEventRecord(c->ev[3], c->stream));
-    timing_end<T>(c, kmax);
+    timing_end<T>(c, nlaunch);
     finish(run);
 }
 
@@ -568,7 +658,8 @@ void nl_adjoint(Ctx* c, int nshot, const eio_shot* shots, int flags) {
     // count is reached.  Keep it simple and exact: all shots of a call share ntw here.
     for (size_t i = 1; i < run.plan.nsteps.size(); ++i)
         if (run.plan.nsteps[i] != run.plan.nsteps[0]) fail("adjoint shots of one call must share nts and sub");
-    const int kmax = run.plan.nsteps.empty() ? 0 : run.plan.nsteps[0];
+    int kmax = run.plan.nsteps.empty() ? 0 : run.plan.nsteps[0];
+    if (c->step_limit > 0) kmax = std::min(kmax, c->step_limit);
     const int ntile = int(run.plan.tiles.size());
     timing_begin<T>(c);
     EIO_CHECK_HIP(hipEventRecord(c->ev[2], c->stream));
@@ -939,6 +1030,34 @@ int eio_ctx_variant_info(int variant, int* R, int* WX, int* occ) {
 }
 
 int eio_ctx_num_variants(void) { return N_F32_VARIANTS; }
+
+int eio_ctx_set_time_block(eio_ctx* ctx, int steps, int variant) {
+    return guarded([&]() {
+        if (steps != 1 && steps != 2) fail("time block must be 1 or 2 fine steps");
+        if (variant < 0 || variant >= N_F32_VARIANTS2) fail("two-step variant out of range");
+        Ctx* c = as_ctx(ctx);
+        c->time_block = steps;
+        c->variant2 = variant;
+    });
+}
+
+int eio_ctx_time_block_info(int variant, int* R2, int* WX, int* occ) {
+    return guarded([&]() {
+        if (variant < 0 || variant >= N_F32_VARIANTS2) fail("two-step variant out of range");
+        if (R2) *R2 = F32_VARIANTS2[variant].R2;
+        if (WX) *WX = F32_VARIANTS2[variant].WX;
+        if (occ) *occ = F32_VARIANTS2[variant].OCC;
+    });
+}
+
+int eio_ctx_num_time_block_variants(void) { return N_F32_VARIANTS2; }
+
+int eio_ctx_set_step_limit(eio_ctx* ctx, int max_steps) {
+    return guarded([&]() {
+        if (max_steps < 0) fail("step limit must be >= 0");
+        as_ctx(ctx)->step_limit = max_steps;
+    });
+}
 
 int eio_ctx_last_timing(eio_ctx* ctx, double* total_ms, long long* n_step, double* step_ms) {
     return guarded([&]() {

@@ -63,6 +63,8 @@ struct ShotDev {
     T cz[4], cx[4];
     T* A;                  // state level A, 5 fields of nz*nx (older level at even steps)
     T* B;                  // state level B
+    T* C;                  // levels C, D: second pair of the two-step NL forward (else NULL)
+    T* D;
     const T* med;          // rhoxDtw, rhozDtw, lamb2MuDtw, lambDtw, muxzDtw (5 fields)
     const T* damp;         // [minPad] cosine sponge
     const T* filt;         // [2h] time interpolation filter, [2h] = 1/(2 dts)

@@ -1,0 +1,292 @@
+// eio_step2.h -- two fine steps per launch for the NL forward (temporal blocking).
+//
+// The single-step kernel moves 80 B per FP32 cell-update (old, current, media read; new
+// written).  Two leapfrog steps u1 = d(d u_old + A u_cur + s_k), u2 = d(d u_cur + A u1 + s_k+1)
+// fused in one launch read u_old, u_cur and the media once and write u1 and u2: 100 B per two
+// cell-updates.  u1 is needed by step 2 on a 4-cell halo around the tile, so step 1 is
+// computed redundantly on that halo (from u_cur on an 8-cell halo, staged in LDS); the
+// redundant work costs L2 and VALU, not HBM.
+//
+// Because neighbouring tiles read u_cur's halo while this tile writes, the two results go
+// to a second pair of state levels (A,B -> C,D, then C,D -> A,B): 4 levels per shot.
+//
+// Tile: 56 owned z rows [z0, z0+56) x TX owned columns [x0, x0+TX).  Lane l of every wave
+// holds row z0-4+l (64 rows: owned rows plus the 4-row halo of u1).  Step 1 runs on the
+// TX+8 columns [x0-4, x0+TX+4), R1 per wave; step 2 on the TX owned columns, R2 per wave.
+// LDS: u_cur on (TX+16) x 72 cells from (x0-8, z0-8), u1 on (TX+8) x 64 from (x0-4, z0-4).
+//
+// Sources: each tile's list holds every source of the step-1 region (lcell relative to
+// (z0-4, x0-4)), injected into its own copy of u1; only owned cells are stored.  Receivers
+// are owned-only.  Arithmetic per cell is that of k_step_fwd in mode M_NL.
+#pragma once
+#include "eio_kernels.h"
+
+namespace eio {
+
+constexpr int TZ2 = 56;       // owned z rows of a two-step tile
+constexpr int Z1 = 64;        // rows of u1 (= lanes)
+constexpr int ZC = 72;        // rows of u_cur staged in LDS
+
+template <typename T, int R2, int WX, int OCC>
+__global__ __launch_bounds__(64 * WX) __attribute__((amdgpu_waves_per_eu(OCC)))
+void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict__ tiles,
+                 const Entry* __restrict__ ents, int ntiles, int k, int par) {
+    static_assert(8 % WX == 0, "WX must divide 8");
+    constexpr int NT = 64 * WX;
+    constexpr int TX = WX * R2;        // owned columns
+    constexpr int R1 = R2 + 8 / WX;    // step-1 columns per wave
+    constexpr int TX1 = TX + 8;        // step-1 columns
+    constexpr int TXC = TX + 16;       // staged u_cur columns
+    constexpr unsigned SZ = sizeof(T);
+    __shared__ T lc[5][TXC][ZC];
+    __shared__ T l1[5][TX1][Z1];
+
+    const TileDev t = tiles[xcd_remap(blockIdx.x, ntiles)];
+    const ShotDev<T>& S = shots[t.shot];
+    const int nz = S.nz, nx = S.nx;
+    const unsigned n = unsigned(nz) * unsigned(nx);
+    const unsigned fbytes = n * SZ, colb = unsigned(nz) * SZ;
+    const bool two = k + 1 < S.ntw - 1;   // the shot's last step may be a single one
+    const StepTime st1 = fwd_time(k, S.sub);
+    const StepTime st2 = fwd_time(k + 1, S.sub);
+    const T* oldp = par ? S.C : S.A;
+    const T* curp = par ? S.D : S.B;
+    T* out1 = par ? S.A : S.C;
+    T* out2 = par ? S.B : S.D;
+    const rsrc_t rold = make_rsrc(oldp, 5 * fbytes);
+    const rsrc_t rcur = make_rsrc(curp, 5 * fbytes);
+    const rsrc_t rmed = make_rsrc(S.med, 5 * fbytes);
+    const rsrc_t rw1 = make_rsrc(out1, 5 * fbytes);
+    const rsrc_t rw2 = make_rsrc(out2, 5 * fbytes);
+
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int iz = t.z0 - FAT + lane;                       // row of this lane (both steps)
+    const bool zin = iz >= FAT && iz < nz - FAT;            // update-region row
+    const bool zown = lane >= FAT && lane < FAT + TZ2 && zin;
+    const int j1 = w * R1;                                   // first step-1 column (rel x0-4)
+    const int ix1 = t.x0 - FAT + j1;
+    // The lane offset of a row above the grid would wrap; such lanes only produce values
+    // that are masked, so their loads are pointed at row FAT instead.
+    const int izl = iz < 0 ? 0 : iz;
+
+    // ---- older level and media of the step-1 cells (columns ix1 .. ix1+R1-1)
+    T o[5][R1], m[5][R1];
+#pragma unroll
+    for (int r = 0; r < R1; ++r) {
+        const int ix = ix1 + r;
+        const unsigned vo = (unsigned(ix < 0 ? 0 : ix) * unsigned(nz) + unsigned(izl)) * SZ;
+#pragma unroll
+        for (int f = 0; f < 5; ++f) o[f][r] = bld<T>(rold, vo, unsigned(f) * fbytes);
+#pragma unroll
+        for (int f = 0; f < 5; ++f) m[f][r] = bld<T>(rmed, vo, unsigned(f) * fbytes);
+    }
+
+    // ---- stage u_cur on (TX+16) x 72 cells from (x0-8, z0-8); cells off the grid are 0
+    for (int e = threadIdx.x; e < TXC * ZC; e += NT) {
+        const int c = e / ZC, z = e - c * ZC;
+        const int gx = t.x0 - 2 * FAT + c, gz = t.z0 - 2 * FAT + z;
+        const bool in = gx >= 0 && gx < nx && gz >= 0 && gz < nz;
+        const unsigned hv = in ? (unsigned(gx) * unsigned(nz) + unsigned(gz)) * SZ : 0u;
+#pragma unroll
+        for (int f = 0; f < 5; ++f) {
+            const T v = bld<T>(rcur, hv, unsigned(f) * fbytes);
+            lc[f][c][z] = in ? v : T(0);
+        }
+    }
+    __syncthreads();
+
+    const T* cz = S.cz;
+    const T* cx = S.cx;
+    // ---- step 1 on columns j1 .. j1+R1-1 (rel x0-4) = lc columns j1+4 .. ; lane row = lc row lane+4
+    {
+        T dx_sxx[R1], dx_sxz[R1], dx_vx[R1], dx_vz[R1];
+        {
+            T q[R1 + 8];
+#pragma unroll
+            for (int j = 0; j < R1 + 8; ++j) q[j] = lc[2][j1 + j][lane + FAT];
+#pragma unroll
+            for (int r = 0; r < R1; ++r) dx_sxx[r] = dm_q(q, r + 4, cx);
+#pragma unroll
+            for (int j = 0; j < R1 + 8; ++j) q[j] = lc[4][j1 + j][lane + FAT];
+#pragma unroll
+            for (int r = 0; r < R1; ++r) dx_sxz[r] = dp_q(q, r + 4, cx);
+#pragma unroll
+            for (int j = 0; j < R1 + 8; ++j) q[j] = lc[0][j1 + j][lane + FAT];
+#pragma unroll
+            for (int r = 0; r < R1; ++r) dx_vx[r] = dp_q(q, r + 4, cx);
+#pragma unroll
+            for (int j = 0; j < R1 + 8; ++j) q[j] = lc[1][j1 + j][lane + FAT];
+#pragma unroll
+            for (int r = 0; r < R1; ++r) dx_vz[r] = dm_q(q, r + 4, cx);
+        }
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {
+            const int ix = ix1 + r;
+            const bool ok = zin && ix >= FAT && ix < nx - FAT;
+            const int c = j1 + r + FAT;
+            const T dz_sxz = dp_q(&lc[4][c][lane + FAT], 0, cz);
+            const T dz_szz = dm_q(&lc[3][c][lane + FAT], 0, cz);
+            const T dz_vz = dp_q(&lc[1][c][lane + FAT], 0, cz);
+            const T dz_vx = dm_q(&lc[0][c][lane + FAT], 0, cz);
+            const T d = ok ? damp_at(S, iz, ix) : T(1);
+            const T a0 = d * o[0][r] + m[0][r] * (dx_sxx[r] + dz_sxz);
+            const T a1 = d * o[1][r] + m[1][r] * (dz_szz + dx_sxz[r]);
+            const T a2 = d * o[2][r] + (m[2][r] * dx_vx[r] + m[3][r] * dz_vz);
+            const T a3 = d * o[3][r] + (m[3][r] * dx_vx[r] + m[2][r] * dz_vz);
+            const T a4 = d * o[4][r] + m[4][r] * (dz_vx + dx_vz[r]);
+            l1[0][j1 + r][lane] = ok ? d * a0 : T(0);
+            l1[1][j1 + r][lane] = ok ? d * a1 : T(0);
+            l1[2][j1 + r][lane] = ok ? d * a2 : T(0);
+            l1[3][j1 + r][lane] = ok ? d * a3 : T(0);
+            l1[4][j1 + r][lane] = ok ? d * a4 : T(0);
+        }
+    }
+    __syncthreads();
+
+    const bool has_src = t.src_e > t.src_b;
+    const bool has_rec = t.rec_e > t.rec_b;
+    // ---- step-1 sources on the whole step-1 region (kernelsGpuElastic.cu:172-189)
+    if (has_src) {
+        for (int e = t.src_b + threadIdx.x; e < t.src_e; e += NT) {
+            const Entry en = ents[e];
+            const int lz = en.lcell & 255, lx = en.lcell >> 8;
+            const int comp = en.crow >> 24, row = en.crow & 0xffffff;
+            const T* tr = S.src_tr[comp];
+            const long b = long(row) * S.nts + st1.its;
+            const T s = tr[b] * S.filt[st1.it2 - 1] + tr[b + 1] * S.filt[S.h + st1.it2 - 1];
+            l1[comp][lx][lz] += damp_at(S, t.z0 - FAT + lz, t.x0 - FAT + lx) * s;
+        }
+        __syncthreads();
+    }
+    // ---- step-1 receivers (kernelsGpuElastic.cu:7-60)
+    if (has_rec) {
+        const T fa = S.filt[st1.it2], fb = S.filt[S.h + st1.it2];
+        for (int e = t.rec_b + threadIdx.x; e < t.rec_e; e += NT) {
+            const Entry en = ents[e];
+            const int lz = en.lcell & 255, lx = en.lcell >> 8;
+            const int comp = en.crow >> 24, row = en.crow & 0xffffff;
+            const T v = l1[comp][lx][lz];
+            T* tr = S.rec_tr[comp] + long(row) * S.nts + st1.its;
+            tr[0] += v * fa;
+            tr[1] += v * fb;
+        }
+    }
+    // ---- store u1 on owned cells: step-1 columns inside [x0, x0+TX)
+#pragma unroll
+    for (int r = 0; r < R1; ++r) {
+        const int j = j1 + r, ix = ix1 + r;
+        if (zown && j >= FAT && j < FAT + TX && ix < nx - FAT) {
+            const unsigned vo = (unsigned(ix) * unsigned(nz) + unsigned(iz)) * SZ;
+#pragma unroll
+            for (int f = 0; f < 5; ++f) bst(rw1, l1[f][j][lane], vo, unsigned(f) * fbytes);
+        }
+    }
+    if (!two) return;
+
+    // ---- step 2 on owned columns j2 .. j2+R2-1 (rel x0) = l1 columns j2+4 ..
+    const int j2 = w * R2;
+    const int ix2 = t.x0 + j2;
+    const int lz2 = lane < FAT ? FAT : (lane > FAT + TZ2 - 1 ? FAT + TZ2 - 1 : lane);  // in-range LDS rows
+    T v2[5][R2];
+    {
+        T dx_sxx[R2], dx_sxz[R2], dx_vx[R2], dx_vz[R2];
+        {
+            T q[R2 + 8];
+#pragma unroll
+            for (int j = 0; j < R2 + 8; ++j) q[j] = l1[2][j2 + j][lz2];
+#pragma unroll
+            for (int r = 0; r < R2; ++r) dx_sxx[r] = dm_q(q, r + 4, cx);
+#pragma unroll
+            for (int j = 0; j < R2 + 8; ++j) q[j] = l1[4][j2 + j][lz2];
+#pragma unroll
+            for (int r = 0; r < R2; ++r) dx_sxz[r] = dp_q(q, r + 4, cx);
+#pragma unroll
+            for (int j = 0; j < R2 + 8; ++j) q[j] = l1[0][j2 + j][lz2];
+#pragma unroll
+            for (int r = 0; r < R2; ++r) dx_vx[r] = dp_q(q, r + 4, cx);
+#pragma unroll
+            for (int j = 0; j < R2 + 8; ++j) q[j] = l1[1][j2 + j][lz2];
+#pragma unroll
+            for (int r = 0; r < R2; ++r) dx_vz[r] = dm_q(q, r + 4, cx);
+        }
+#pragma unroll
+        for (int r = 0; r < R2; ++r) {
+            const int ix = ix2 + r;
+            const bool ok = zown && ix < nx - FAT;
+            const int c = j2 + r + FAT;
+            const unsigned vo = (unsigned(ix) * unsigned(nz) + unsigned(izl)) * SZ;
+            T mm[5];
+#pragma unroll
+            for (int f = 0; f < 5; ++f) mm[f] = bld<T>(rmed, vo, unsigned(f) * fbytes);
+            // older level of step 2 = u_cur at this cell (lc column j2+r+8, row lane+4)
+            T oo[5];
+#pragma unroll
+            for (int f = 0; f < 5; ++f) oo[f] = lc[f][c + FAT][lz2 + FAT];
+            const T dz_sxz = dp_q(&l1[4][c][lz2], 0, cz);
+            const T dz_szz = dm_q(&l1[3][c][lz2], 0, cz);
+            const T dz_vz = dp_q(&l1[1][c][lz2], 0, cz);
+            const T dz_vx = dm_q(&l1[0][c][lz2], 0, cz);
+            const T d = ok ? damp_at(S, iz, ix) : T(1);
+            const T a0 = d * oo[0] + mm[0] * (dx_sxx[r] + dz_sxz);
+            const T a1 = d * oo[1] + mm[1] * (dz_szz + dx_sxz[r]);
+            const T a2 = d * oo[2] + (mm[2] * dx_vx[r] + mm[3] * dz_vz);
+            const T a3 = d * oo[3] + (mm[3] * dx_vx[r] + mm[2] * dz_vz);
+            const T a4 = d * oo[4] + mm[4] * (dz_vx + dx_vz[r]);
+            v2[0][r] = d * a0;
+            v2[1][r] = d * a1;
+            v2[2][r] = d * a2;
+            v2[3][r] = d * a3;
+            v2[4][r] = d * a4;
+        }
+    }
+    if (has_src || has_rec) {  // stage u2 in the l1 area (uniform per tile)
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R2; ++r)
+#pragma unroll
+            for (int f = 0; f < 5; ++f)
+                if (zown) l1[f][j2 + r + FAT][lane] = v2[f][r];
+        __syncthreads();
+        if (has_src) {  // owned sources only
+            for (int e = t.src_b + threadIdx.x; e < t.src_e; e += NT) {
+                const Entry en = ents[e];
+                const int lz = en.lcell & 255, lx = en.lcell >> 8;
+                if (lz < FAT || lz >= FAT + TZ2 || lx < FAT || lx >= FAT + TX) continue;
+                const int comp = en.crow >> 24, row = en.crow & 0xffffff;
+                const T* tr = S.src_tr[comp];
+                const long b = long(row) * S.nts + st2.its;
+                const T s = tr[b] * S.filt[st2.it2 - 1] + tr[b + 1] * S.filt[S.h + st2.it2 - 1];
+                l1[comp][lx][lz] += damp_at(S, t.z0 - FAT + lz, t.x0 - FAT + lx) * s;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < R2; ++r)
+#pragma unroll
+                for (int f = 0; f < 5; ++f)
+                    if (zown) v2[f][r] = l1[f][j2 + r + FAT][lane];
+        }
+        if (has_rec) {
+            const T fa = S.filt[st2.it2], fb = S.filt[S.h + st2.it2];
+            for (int e = t.rec_b + threadIdx.x; e < t.rec_e; e += NT) {
+                const Entry en = ents[e];
+                const int lz = en.lcell & 255, lx = en.lcell >> 8;
+                const int comp = en.crow >> 24, row = en.crow & 0xffffff;
+                const T v = l1[comp][lx][lz];
+                T* tr = S.rec_tr[comp] + long(row) * S.nts + st2.its;
+                tr[0] += v * fa;
+                tr[1] += v * fb;
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R2; ++r) {
+        const int ix = ix2 + r;
+        if (zown && ix < nx - FAT) {
+            const unsigned vo = (unsigned(ix) * unsigned(nz) + unsigned(iz)) * SZ;
+#pragma unroll
+            for (int f = 0; f < 5; ++f) bst(rw2, v2[f][r], vo, unsigned(f) * fbytes);
+        }
+    }
+}
+
+}  // namespace eio

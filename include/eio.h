@@ -81,12 +81,22 @@ int eio_ctx_last_timing(eio_ctx* ctx, double* total_ms, long long* n_step_launch
 int eio_ctx_set_graph(eio_ctx* ctx, int use_graph);
 
 /* FP32 step-kernel variant (tile of 64 x WX*R cells, WX waves per workgroup, R x-cells per
- * lane, requested occupancy) used by the next calls; chosen by the autotuner
- * (elastic_isotropic_operators_amd/autotune.py).  FP64 contexts ignore it.  No counterpart in
- * the reference (fixed 16x16 blocks, varDeclare.h:5). */
+ * lane, requested occupancy) used by the next calls.  FP64 contexts ignore it.  No counterpart
+ * in the reference (fixed 16x16 blocks, varDeclare.h:5). */
 int eio_ctx_num_variants(void);
 int eio_ctx_variant_info(int variant, int* R, int* WX, int* occ);
 int eio_ctx_set_variant(eio_ctx* ctx, int variant);
+/* Fine steps per launch of the FP32 NL forward: 1 (one fused step per launch) or 2 (two
+ * steps per launch, temporally blocked: 50 instead of 80 bytes of HBM traffic per
+ * cell-update, four state levels per shot), and the variant of the two-step kernel
+ * (owned tile 56 x WX*R2).  Other operators always take one step per launch.  No
+ * counterpart in the reference. */
+int eio_ctx_num_time_block_variants(void);
+int eio_ctx_time_block_info(int variant, int* R2, int* WX, int* occ);
+int eio_ctx_set_time_block(eio_ctx* ctx, int steps, int variant);
+/* Tuning runs: NL forward and adjoint loops stop after `max_steps` fine steps (0: no limit).
+ * Outputs of a limited call are partial.  No counterpart in the reference. */
+int eio_ctx_set_step_limit(eio_ctx* ctx, int max_steps);
 
 /* Register a model: grid constants (FD coefficients, time filter, damping table) and the
  * five scaled media rhoxDtw, rhozDtw, lamb2MuDtw, lambDtw, muxzDtw, each nz*nx.

@@ -249,3 +249,91 @@ def test_born_adjoint_with_reused_storage_of_another_shape():
     mid = _model(ctx, w, g, med)
     out = _gpu_born_adj(ctx, mid, spos, rpos, src_fine, dat, g.nx, g.nz, dtype)
     assert rel_l2(out, ref) < 1e-4, rel_l2(out, ref)
+
+
+def _scattered_positions(rng, nz, nx, count):
+    """`count` distinct update-region positions (ix*nz+iz) on each of the 4 grids, spread over
+    the whole grid so that many fall on or next to tile edges."""
+    out = []
+    for _ in range(4):
+        iz = rng.integers(4, nz - 4, size=4 * count)
+        ix = rng.integers(4, nx - 4, size=4 * count)
+        p = np.unique(ix * nz + iz)[:count]
+        out.append(rng.permutation(p).astype(np.int32))
+    return out
+
+
+@pytest.mark.parametrize("v", range(5))
+@pytest.mark.parametrize("sub_fd", [2, 3])
+def test_two_step_nl_forward_matches_oracle(v, sub_fd):
+    """The temporally blocked NL forward (two fine steps per launch, step 1 recomputed on the
+    tile halo) equals the oracle for every variant, with sources and receivers scattered over
+    the grid (many on tile edges, so halo injection is exercised) and with an odd number of
+    fine steps (sub_fd = 3, nts = 60: 177 steps, the last launch does one step)."""
+    if v >= len(_capi.Context.time_block_variants()):
+        pytest.skip("variant not compiled")
+    w = W.small(nz=150, nx=130, nts=60, sub=1, seed=60)
+    g = O.grid(w["nz"], w["nx"], w["nts"], sub_fd, w["dz"], w["dx"], w["dts"], W.min_pad(w),
+               w["alphaCos"], w["oz"], w["ox"])
+    med = O.media(g, w["elastic"], np.float64)
+    rng = np.random.default_rng(61 + v)
+    spos = _scattered_positions(rng, g.nz, g.nx, 40)
+    rpos = _scattered_positions(rng, g.nz, g.nx, 60)
+    src = random_traces(rng, spos, g.nts, np.float64)
+    ref = O.nl_fwd_reg(g, med, spos, src, rpos, np.float64)
+    ctx = _capi.Context(0, np.float32)
+    ctx.set_time_block(2, v)
+    mid = _model(ctx, w, g, med)
+    out = _gpu_nl_fwd(ctx, mid, spos, rpos, src, g.nts, np.float32)
+    for k in range(5):
+        assert np.linalg.norm(ref[k]) > 0
+        assert rel_l2(out[k], ref[k]) < 1e-5, (v, k, rel_l2(out[k], ref[k]))
+
+
+def test_two_step_uneven_batch_matches_single_step():
+    """An uneven batch (different grids, trace lengths and fine sub-samplings, even and odd
+    step counts) gives the same data with two steps per launch as with one."""
+    rng = np.random.default_rng(70)
+    cases = [_case(nz=70, nx=96, nts=90, sub=1, seed=71), _case(nz=96, nx=70, nts=61, sub=2, seed=72),
+             _case(nz=45, nx=120, nts=120, sub=1, seed=73)]
+    outs = {}
+    for tb in (1, 2):
+        ctx = _capi.Context(0, np.float32)
+        ctx.set_time_block(tb, 0)
+        shots, keep, recs = [], [], []
+        for i, (w, g, med, spos, rpos) in enumerate(cases):
+            if i == 1:  # odd fine sub-sampling through the C ABI: odd step count
+                g = O.grid(g.nz, g.nx, g.nts, 3, g.dz, g.dx, g.dts, g.minPad, g.alphaCos)
+            mid = _model(ctx, w, g, med)
+            src = random_traces(np.random.default_rng(80 + i), spos, g.nts, np.float32)
+            rec = [np.zeros((rpos[GRID_OF_COMP[k]].size, g.nts), np.float32) for k in range(5)]
+            s, kp = ctx.make_shot(mid, spos, rpos, src, rec)
+            shots.append(s)
+            keep.append((kp, src))
+            recs.append(rec)
+        ctx.nl_forward(shots)
+        outs[tb] = recs
+    for a, b in zip(outs[1], outs[2]):
+        for k in range(5):
+            assert rel_l2(b[k], a[k]) < 2e-6, (k, rel_l2(b[k], a[k]))
+
+
+def test_autotuner_choice_keeps_parity():
+    """The autotuner times every candidate kernel on the caller's batch with a step limit,
+    leaves the fastest set on the context and lifts the limit: the next full call is exact."""
+    from elastic_isotropic_operators_amd import autotune
+    w, g, med, spos, rpos = _case(nz=70, nx=90, nts=80, sub=1, seed=90)
+    rng = np.random.default_rng(91)
+    src = random_traces(rng, spos, g.nts, np.float32)
+    ref = O.nl_fwd_reg(g, med, spos, src, rpos, np.float64)
+    ctx = _capi.Context(0, np.float32)
+    mid = _model(ctx, w, g, med)
+    rec = [np.zeros((rpos[GRID_OF_COMP[k]].size, g.nts), np.float32) for k in range(5)]
+    shot, keep = ctx.make_shot(mid, spos, rpos, src, rec)
+    grids = [(g.nz, g.nx, (g.nts - 1) * g.sub, sum(p.size for p in spos), sum(p.size for p in rpos))]
+    res = autotune.tune_nl_forward(ctx, [shot], grids, steps=32, device_ptrs=False, use_cache=False)
+    assert len(res["table"]) == len(autotune.candidates()) and not res["cached"]
+    assert all(v > 0 for v in res["table"].values())
+    ctx.nl_forward([shot])
+    for k in range(5):
+        assert rel_l2(rec[k], ref[k]) < 1e-5, (res["choice"], k, rel_l2(rec[k], ref[k]))

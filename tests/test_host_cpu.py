@@ -132,3 +132,13 @@ def test_workload_sizes():
     shots = W.layered_batch_c2(64)
     tot = sum(W.cell_updates(w) for w in shots)
     assert 4e11 < tot < 6e11
+
+
+def test_autotune_signature_and_candidates():
+    from elastic_isotropic_operators_amd import autotune
+    a = autotune.signature([(300, 600, 7996, 4, 600), (400, 800, 5000, 4, 800)])
+    b = autotune.signature([(400, 800, 5000, 4, 800), (300, 600, 7996, 4, 600)])
+    assert a == b and len(a) == 16
+    assert a != autotune.signature([(300, 600, 7996, 4, 600)])
+    c = autotune.candidates()
+    assert (1, 0) in c and (2, 0) in c and len(set(c)) == len(c)
