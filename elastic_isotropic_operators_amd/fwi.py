@@ -14,15 +14,16 @@ Here the forward and the gradient are one hand-written HIP pair behind the C ABI
 ~sqrt(nts) coarse samples instead of the reference's full nz*nx*nts storage of Vx and Vz,
 and the gradient recomputes one segment at a time.  Everything stays resident in HBM:
 media, traces and the gradient are torch tensors whose device addresses go straight to the
-C ABI on torch's current stream.  torch autograd supplies C^T and the chain rule through
-any parameter conversion written in torch (``vpvsrho_to_rholamemu``).
+C ABI (torch's stream is synchronized before each call; the call synchronizes its own).
+torch autograd supplies C^T and the chain rule through any parameter conversion written in
+torch (``vpvsrho_to_rholamemu``).
 
 The gradient ignores the model dependence of the vx/vz source scaling (rhoxDtw, rhozDtw at
 the source positions), as the reference's Born-based gradient does.
 
 Import order: torch's ROCm wheel carries its own HIP runtime with the system runtime's soname;
 torch must be imported before libeio_hip.so is loaded, so the library binds to that runtime
-and torch's device pointers and streams are valid for it.
+and torch's device pointers are valid for it.
 """
 import weakref
 
@@ -187,8 +188,13 @@ class ElasticFwi:
         self.rec_w = [torch.as_tensor(ip.w, dtype=dtype, device=self.device) for ip in rips]
 
     # -------------------------------------------------------------- C-ABI plumbing
-    def _bind_stream(self):
-        self.ctx.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
+    def _torch_ready(self):
+        """The propagator runs on its context's own stream; torch's work on the inputs must be
+        complete first.  (torch's default stream is the legacy null stream, handle 0, which
+        eio_ctx_set_stream would read as 'own stream' and which does not order work on a
+        non-blocking stream.)  The C-ABI calls synchronize their stream before returning, so
+        their outputs are ready for torch."""
+        torch.cuda.current_stream(self.device).synchronize()
 
     def _set_media(self, media):
         n = self.nz * self.nx
@@ -258,11 +264,11 @@ class ElasticFwi:
         store the gradient needs.  Returns (data, store or None, media)."""
         model = model.detach().to(device=self.device, dtype=self.dtype)
         media = scaled_media(model, self.dtw)
-        self._bind_stream()
-        self._set_media(media)
         srcs = self._scaled_sources(media)
         recs = self._empty_recs()
         shots, keep = self._shots(srcs, recs)
+        self._torch_ready()
+        self._set_media(media)
         store = None
         if checkpoint:
             h = self.ctx.fwi_forward(shots, self.segment, device_ptrs=True)
@@ -277,11 +283,11 @@ class ElasticFwi:
         (drho, dlambda, dmu) (3, nx, nz)."""
         if store is None or store.handle is None:
             raise RuntimeError("no checkpoint store: run the forward with gradients enabled")
-        self._bind_stream()
-        self._set_media(media)
         recs = self._irreg_to_reg(grad_data.to(self.dtype).contiguous())
         shots, keep = self._shots(store.srcs, recs)
         acc = torch.zeros((5, self.nx, self.nz), dtype=self.dtype, device=self.device)
+        self._torch_ready()
+        self._set_media(media)
         n = self.nz * self.nx
         ptrs = [acc.data_ptr() + k * n * acc.element_size() for k in range(5)]
         self.ctx.fwi_gradient(store.handle, shots, ptrs, device_ptrs=True)

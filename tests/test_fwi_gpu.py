@@ -217,3 +217,19 @@ def test_full_size_gradient_fp32_within_1e4_of_fp64():
         torch.cuda.empty_cache()
     assert np.linalg.norm(grads[torch.float64]) > 0
     assert rel_l2(grads[torch.float32], grads[torch.float64]) < 1e-4, rel_l2(grads[torch.float32], grads[torch.float64])
+
+
+def test_autograd_gradient_is_deterministic():
+    """Three gradients of the same objective are bitwise equal (guards the ordering between
+    torch's stream, which produces media and residuals, and the propagator's stream)."""
+    import torch
+    w, g, m0, dtrue = fwi_case(nts=400, dts=0.001, nExp=2, nz=48, nx=72)
+    eng = _engine(w, g, torch.float32, segment=9)
+    with torch.no_grad():
+        d_obs = eng.forward_data(torch.tensor(m0 + dtrue, dtype=torch.float32, device="cuda"))[0]
+    grads = []
+    for _ in range(3):
+        m = torch.tensor(m0, dtype=torch.float32, device="cuda", requires_grad=True)
+        eng.misfit(m, d_obs).backward()
+        grads.append(m.grad.cpu().numpy())
+    assert np.array_equal(grads[0], grads[1]) and np.array_equal(grads[0], grads[2])
