@@ -266,9 +266,9 @@ class ElasticFwi:
         media = scaled_media(model, self.dtw)
         srcs = self._scaled_sources(media)
         recs = self._empty_recs()
-        shots, keep = self._shots(srcs, recs)
         self._torch_ready()
         self._set_media(media)
+        shots, keep = self._shots(srcs, recs)
         store = None
         if checkpoint:
             h = self.ctx.fwi_forward(shots, self.segment, device_ptrs=True)
@@ -284,10 +284,10 @@ class ElasticFwi:
         if store is None or store.handle is None:
             raise RuntimeError("no checkpoint store: run the forward with gradients enabled")
         recs = self._irreg_to_reg(grad_data.to(self.dtype).contiguous())
-        shots, keep = self._shots(store.srcs, recs)
         acc = torch.zeros((5, self.nx, self.nz), dtype=self.dtype, device=self.device)
         self._torch_ready()
         self._set_media(media)
+        shots, keep = self._shots(store.srcs, recs)
         n = self.nz * self.nx
         ptrs = [acc.data_ptr() + k * n * acc.element_size() for k in range(5)]
         self.ctx.fwi_gradient(store.handle, shots, ptrs, device_ptrs=True)
