@@ -63,9 +63,10 @@ constexpr Variant F64_VARIANT = {2, 4, 1};
 // Two-step (temporally blocked) NL forward variants: R2 owned x-cells per lane, WX waves,
 // OCC waves per SIMD; owned tile = 56 x WX*R2.  FP32 only.
 struct Variant2 {
-    int R2, WX, OCC;
+    int R2, WX, OCC, NTS;
 };
-constexpr Variant2 F32_VARIANTS2[] = {{2, 8, 2}, {2, 8, 4}, {4, 4, 2}, {1, 8, 4}, {4, 8, 1}};
+constexpr Variant2 F32_VARIANTS2[] = {{2, 8, 2, 0}, {2, 8, 4, 0}, {4, 4, 2, 0}, {1, 8, 4, 0},
+                                      {4, 8, 1, 0}, {2, 8, 4, 1}, {2, 8, 2, 1}, {4, 8, 1, 1}};
 constexpr int N_F32_VARIANTS2 = int(sizeof(F32_VARIANTS2) / sizeof(F32_VARIANTS2[0]));
 
 template <typename T>
@@ -565,15 +566,18 @@ void launch_adj(Run<T>& run, int ntile, int k, int sspar) {
 }
 void launch_fwd2(Run<float>& run, int ntile, int k, int par) {
     hipStream_t s = run.c->stream;
-#define EIO_L2(R_, WX_, OCC_)                                                                  \
-    hipLaunchKernelGGL((eio::k_step2_fwd<float, R_, WX_, OCC_>), dim3(ntile), dim3(64 * WX_), 0, s, \
+#define EIO_L2(R_, WX_, OCC_, NT_)                                                                   \
+    hipLaunchKernelGGL((eio::k_step2_fwd<float, R_, WX_, OCC_, NT_>), dim3(ntile), dim3(64 * WX_), 0, s, \
                        run.d_shots, run.d_tiles, run.d_ents, ntile, k, par)
     switch (run.c->variant2) {
-        case 0: EIO_L2(2, 8, 2); break;
-        case 1: EIO_L2(2, 8, 4); break;
-        case 2: EIO_L2(4, 4, 2); break;
-        case 3: EIO_L2(1, 8, 4); break;
-        default: EIO_L2(4, 8, 1); break;
+        case 0: EIO_L2(2, 8, 2, 0); break;
+        case 1: EIO_L2(2, 8, 4, 0); break;
+        case 2: EIO_L2(4, 4, 2, 0); break;
+        case 3: EIO_L2(1, 8, 4, 0); break;
+        case 4: EIO_L2(4, 8, 1, 0); break;
+        case 5: EIO_L2(2, 8, 4, 1); break;
+        case 6: EIO_L2(2, 8, 2, 1); break;
+        default: EIO_L2(4, 8, 1, 1); break;
     }
 #undef EIO_L2
 }

@@ -109,6 +109,10 @@ __device__ __forceinline__ double bld<double>(rsrc_t r, unsigned voff, unsigned 
 __device__ __forceinline__ void bst(rsrc_t r, float v, unsigned voff, unsigned soff) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, int(voff), int(soff), 0);
 }
+// nontemporal store (cache policy nt): streamed results that no later access of the launch reads
+__device__ __forceinline__ void bst_nt(rsrc_t r, float v, unsigned voff, unsigned soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, int(voff), int(soff), 2);
+}
 __device__ __forceinline__ void bst(rsrc_t r, double v, unsigned voff, unsigned soff) {
     using u64 = decltype(__builtin_amdgcn_raw_buffer_load_b64(rsrc_t(), 0, 0, 0));
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u64, v), r, int(voff), int(soff), 0);

@@ -27,7 +27,14 @@ constexpr int TZ2 = 56;       // owned z rows of a two-step tile
 constexpr int Z1 = 64;        // rows of u1 (= lanes)
 constexpr int ZC = 72;        // rows of u_cur staged in LDS
 
-template <typename T, int R2, int WX, int OCC>
+template <typename T>
+__device__ __forceinline__ void bst_pol(rsrc_t r, T v, unsigned voff, unsigned soff, bool nt) {
+    if (nt) bst_nt(r, v, voff, soff);
+    else bst(r, v, voff, soff);
+}
+
+// NTS: store u1 and u2 with the nontemporal cache policy
+template <typename T, int R2, int WX, int OCC, int NTS>
 __global__ __launch_bounds__(64 * WX) __attribute__((amdgpu_waves_per_eu(OCC)))
 void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict__ tiles,
                  const Entry* __restrict__ ents, int ntiles, int k, int par) {
@@ -69,7 +76,24 @@ void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
     // that are masked, so their loads are pointed at row FAT instead.
     const int izl = iz < 0 ? 0 : iz;
 
-    // ---- older level and media of the step-1 cells (columns ix1 .. ix1+R1-1)
+    // ---- u_cur on (TX+16) x 72 cells from (x0-8, z0-8) (cells off the grid are 0): all
+    // loads are issued first, then the older level and media of the step-1 cells, and only
+    // then are the staged values written to LDS, so every load of the tile is in flight at once
+    constexpr int NFILL = (TXC * ZC + NT - 1) / NT;
+    T hf[NFILL][5];
+#pragma unroll
+    for (int it = 0; it < NFILL; ++it) {
+        const int e = threadIdx.x + it * NT;
+        const int c = e / ZC, z = e - c * ZC;
+        const int gx = t.x0 - 2 * FAT + c, gz = t.z0 - 2 * FAT + z;
+        const bool in = e < TXC * ZC && gx >= 0 && gx < nx && gz >= 0 && gz < nz;
+        const unsigned hv = in ? (unsigned(gx) * unsigned(nz) + unsigned(gz)) * SZ : 0u;
+#pragma unroll
+        for (int f = 0; f < 5; ++f) {
+            const T v = bld<T>(rcur, hv, unsigned(f) * fbytes);
+            hf[it][f] = in ? v : T(0);
+        }
+    }
     T o[5][R1], m[5][R1];
 #pragma unroll
     for (int r = 0; r < R1; ++r) {
@@ -80,17 +104,13 @@ void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
 #pragma unroll
         for (int f = 0; f < 5; ++f) m[f][r] = bld<T>(rmed, vo, unsigned(f) * fbytes);
     }
-
-    // ---- stage u_cur on (TX+16) x 72 cells from (x0-8, z0-8); cells off the grid are 0
-    for (int e = threadIdx.x; e < TXC * ZC; e += NT) {
-        const int c = e / ZC, z = e - c * ZC;
-        const int gx = t.x0 - 2 * FAT + c, gz = t.z0 - 2 * FAT + z;
-        const bool in = gx >= 0 && gx < nx && gz >= 0 && gz < nz;
-        const unsigned hv = in ? (unsigned(gx) * unsigned(nz) + unsigned(gz)) * SZ : 0u;
 #pragma unroll
-        for (int f = 0; f < 5; ++f) {
-            const T v = bld<T>(rcur, hv, unsigned(f) * fbytes);
-            lc[f][c][z] = in ? v : T(0);
+    for (int it = 0; it < NFILL; ++it) {
+        const int e = threadIdx.x + it * NT;
+        if (e < TXC * ZC) {
+            const int c = e / ZC, z = e - c * ZC;
+#pragma unroll
+            for (int f = 0; f < 5; ++f) lc[f][c][z] = hf[it][f];
         }
     }
     __syncthreads();
@@ -178,7 +198,7 @@ void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
         if (zown && j >= FAT && j < FAT + TX && ix < nx - FAT) {
             const unsigned vo = (unsigned(ix) * unsigned(nz) + unsigned(iz)) * SZ;
 #pragma unroll
-            for (int f = 0; f < 5; ++f) bst(rw1, l1[f][j][lane], vo, unsigned(f) * fbytes);
+            for (int f = 0; f < 5; ++f) bst_pol<T>(rw1, l1[f][j][lane], vo, unsigned(f) * fbytes, NTS);
         }
     }
     if (!two) return;
@@ -284,7 +304,7 @@ void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
         if (zown && ix < nx - FAT) {
             const unsigned vo = (unsigned(ix) * unsigned(nz) + unsigned(iz)) * SZ;
 #pragma unroll
-            for (int f = 0; f < 5; ++f) bst(rw2, v2[f][r], vo, unsigned(f) * fbytes);
+            for (int f = 0; f < 5; ++f) bst_pol<T>(rw2, v2[f][r], vo, unsigned(f) * fbytes, NTS);
         }
     }
 }
