@@ -207,13 +207,29 @@ void k_step_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict_
         for (int f = 0; f < 5; ++f) m[f][r] = bld<T>(rmed, voff, unsigned(r) * colb + unsigned(f) * fbytes);
 #endif
 
-    // stage the current level with a 4-cell halo
+    // stage the current level with a 4-cell halo: every load of the stage is issued before
+    // the first LDS write (register staging), so they are all in flight together
     const unsigned hbase = (unsigned(t.x0 - FAT) * unsigned(nz) + unsigned(t.z0 - FAT)) * SZ;
-    for (int e = threadIdx.x; e < TXH * ZH; e += NT) {
-        const int c = e / ZH, z = e - c * ZH;
-        const unsigned hv = hbase + unsigned(c) * colb + unsigned(z) * SZ;
+    {
+        constexpr int NFILL = (TXH * ZH + NT - 1) / NT;
+        T hf[NFILL][5];
 #pragma unroll
-        for (int f = 0; f < 5; ++f) lds[f][c][z] = bld<T>(rcur, hv, unsigned(f) * fbytes);
+        for (int it = 0; it < NFILL; ++it) {
+            const int e = threadIdx.x + it * NT;
+            const int c = e / ZH, z = e - c * ZH;
+            const unsigned hv = e < TXH * ZH ? hbase + unsigned(c) * colb + unsigned(z) * SZ : 0u;
+#pragma unroll
+            for (int f = 0; f < 5; ++f) hf[it][f] = bld<T>(rcur, hv, unsigned(f) * fbytes);
+        }
+#pragma unroll
+        for (int it = 0; it < NFILL; ++it) {
+            const int e = threadIdx.x + it * NT;
+            if (e < TXH * ZH) {
+                const int c = e / ZH, z = e - c * ZH;
+#pragma unroll
+                for (int f = 0; f < 5; ++f) lds[f][c][z] = hf[it][f];
+            }
+        }
     }
     __syncthreads();
 
@@ -416,18 +432,32 @@ void k_step_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict_
 #endif
 
     const unsigned hbase = (unsigned(t.x0 - FAT) * unsigned(nz) + unsigned(t.z0 - FAT)) * SZ;
-    for (int e = threadIdx.x; e < TXH * ZH; e += NT) {
-        const int c = e / ZH, z = e - c * ZH;
-        const unsigned hv = hbase + unsigned(c) * colb + unsigned(z) * SZ;
-        const T vx = bld<T>(rcur, hv, 0), vz = bld<T>(rcur, hv, fbytes);
-        const T sxx = bld<T>(rcur, hv, 2 * fbytes), szz = bld<T>(rcur, hv, 3 * fbytes);
-        const T sxz = bld<T>(rcur, hv, 4 * fbytes);
-        const T l2m = bld<T>(rmed, hv, 2 * fbytes), lam = bld<T>(rmed, hv, 3 * fbytes);
-        lds[0][c][z] = bld<T>(rmed, hv, 0) * vx;
-        lds[1][c][z] = bld<T>(rmed, hv, fbytes) * vz;
-        lds[2][c][z] = l2m * sxx + lam * szz;
-        lds[3][c][z] = lam * sxx + l2m * szz;
-        lds[4][c][z] = bld<T>(rmed, hv, 4 * fbytes) * sxz;
+    {
+        // all stage loads in flight before the first LDS write (register staging)
+        constexpr int NFILL = (TXH * ZH + NT - 1) / NT;
+        T hv_[NFILL][5], hm[NFILL][5];
+#pragma unroll
+        for (int it = 0; it < NFILL; ++it) {
+            const int e = threadIdx.x + it * NT;
+            const int c = e / ZH, z = e - c * ZH;
+            const unsigned hv = e < TXH * ZH ? hbase + unsigned(c) * colb + unsigned(z) * SZ : 0u;
+#pragma unroll
+            for (int f = 0; f < 5; ++f) hv_[it][f] = bld<T>(rcur, hv, unsigned(f) * fbytes);
+#pragma unroll
+            for (int f = 0; f < 5; ++f) hm[it][f] = bld<T>(rmed, hv, unsigned(f) * fbytes);
+        }
+#pragma unroll
+        for (int it = 0; it < NFILL; ++it) {
+            const int e = threadIdx.x + it * NT;
+            if (e < TXH * ZH) {
+                const int c = e / ZH, z = e - c * ZH;
+                lds[0][c][z] = hm[it][0] * hv_[it][0];
+                lds[1][c][z] = hm[it][1] * hv_[it][1];
+                lds[2][c][z] = hm[it][2] * hv_[it][2] + hm[it][3] * hv_[it][3];
+                lds[3][c][z] = hm[it][3] * hv_[it][2] + hm[it][2] * hv_[it][3];
+                lds[4][c][z] = hm[it][4] * hv_[it][4];
+            }
+        }
     }
     __syncthreads();
 
