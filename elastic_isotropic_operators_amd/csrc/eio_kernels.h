@@ -551,6 +551,11 @@ void k_step_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict_
 
     rsrc_t rsl, rsr;
     T efa = T(0), efb = T(0);
+    // Adjoint fields of interval its go to ss[sspar] (slot its, weight F[it2]) and
+    // ss[sspar^1] (slot its+1, weight F[h+it2]).  Slots are never cleared: the imaging
+    // condition reads slot its+1 after the interval, and slot its starts fresh with the
+    // interval's first step.  (The slot of its = nts-1 is zero from the call's set-up.)
+    const bool first = st.it2 == S.sub - 1, last = st.it2 == 0;
     if (MODE == M_BADJ) {  // kernelsGpuElastic.cu:1369-1376
         rsl = make_rsrc(S.ss[sspar], 5 * fbytes);
         rsr = make_rsrc(S.ss[sspar ^ 1], 5 * fbytes);
@@ -566,8 +571,11 @@ void k_step_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict_
                 const unsigned sf = so + unsigned(f) * fbytes;
                 bst(rold, val[f][r], voff, sf);
                 if (MODE == M_BADJ) {
-                    bst(rsl, bld<T>(rsl, voff, sf) + val[f][r] * efa, voff, sf);
-                    bst(rsr, bld<T>(rsr, voff, sf) + val[f][r] * efb, voff, sf);
+                    // slot its: the interval's first (reverse) step overwrites it; slot its+1:
+                    // its weight F[h+0] is 0 at it2 = 0 (nothing to add)
+                    const T l = val[f][r] * efa;
+                    bst(rsl, first ? l : bld<T>(rsl, voff, sf) + l, voff, sf);
+                    if (!last) bst(rsr, bld<T>(rsr, voff, sf) + val[f][r] * efb, voff, sf);
                 }
             }
         }
@@ -576,8 +584,8 @@ void k_step_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict_
 
 // ------------------------------------------------------------------ Born imaging
 // FWD (kernelsGpuElastic.cu:1381-1459): ss[which] <- secondary source at coarse time its.
-// ADJ (kernelsGpuElastic.cu:1538-1613): dm += imaging condition of ss[which] at its, then
-//      ss[which] <- 0 (the buffer becomes the next, zeroed, accumulation slice).
+// ADJ (kernelsGpuElastic.cu:1538-1613): dm += imaging condition of ss[which] at its (the
+//      reference then clears the slice; here the next interval's first step overwrites it).
 template <typename T, int R, int WX, bool ADJ>
 __global__ __launch_bounds__(64 * WX) void k_imaging(const ShotDev<T>* __restrict__ shots,
                                                      const TileDev* __restrict__ tiles, int ntiles,
@@ -680,8 +688,6 @@ __global__ __launch_bounds__(64 * WX) void k_imaging(const ShotDev<T>* __restric
             bst(rdm, bld<T>(rdm, voff, so + 3 * fbytes) + T(2) * (dvx_dx[r] * a2 + dvz_dz * a3), voff,
                 so + 3 * fbytes);
             bst(rdm, bld<T>(rdm, voff, so + 4 * fbytes) + a4 * sxz_op, voff, so + 4 * fbytes);
-#pragma unroll
-            for (int f = 0; f < 5; ++f) bst(rss, T(0), voff, so + unsigned(f) * fbytes);
         }
     }
 }
