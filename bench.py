@@ -35,7 +35,7 @@ from elastic_isotropic_operators_amd import _capi, autotune, geometry as G, work
 BYTES_PER_CELL = {"nl_fwd": 80}
 HBM_PEAK_GBS = 8000.0
 METRIC = "GCell-updates/s + FWI-gradient s/shot at \u22641e-4 rel-L2 vs FP64, 1 MI355X"
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_step_fwd_layered_batch_c2_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_step2_fwd_layered_batch_c2_traffic.json")
 
 
 def log(msg):
@@ -300,10 +300,15 @@ def main():
         cms = ctx.copy_probe(nb, 10)
         res["roofline"]["copy_probe_GBs"] = round(2 * nb / (cms / 1e3) / 1e9, 1)
         log("copy probe: %.3f ms for 2 x %d bytes -> %.0f GB/s" % (cms, nb, 2 * nb / cms / 1e6))
-    if os.path.exists(TRAFFIC_FILE) and args.variant < 0 and args.workload == "c2" and args.scale == 1.0:
+    if os.path.exists(TRAFFIC_FILE) and args.workload == "c2":
+        # measured HBM bytes per cell-update of this kernel (rocprofv3 FETCH_SIZE/WRITE_SIZE,
+        # copy-kernel calibrated) times the cell-updates of this run's average launch
         tr = json.load(open(TRAFFIC_FILE))
-        res["roofline"]["traffic"] = tr["bytes_per_launch"]
-        res["roofline"]["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
+        if tr["time_block"] == args.time_block and tr["variant2"] == args.variant2:
+            upd_per_launch = cells / float(nlaunch[-1])
+            res["roofline"]["traffic"] = round(tr["bytes_per_cell_update"] * upd_per_launch)
+            res["roofline"]["traffic_bytes_per_cell_update"] = round(tr["bytes_per_cell_update"], 2)
+            res["roofline"]["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
     if args.fwi:
         t0 = time.perf_counter()
         res["fwi_gradient"] = run_fwi_leg(args)
