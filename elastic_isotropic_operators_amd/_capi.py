@@ -84,6 +84,8 @@ def lib():
                                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
             ("eio_ctx_set_time_block", [vp, ctypes.c_int, ctypes.c_int]),
             ("eio_ctx_set_step_limit", [vp, ctypes.c_int]),
+            ("eio_nl_forward_wavefield", [vp, ctypes.c_int, ctypes.POINTER(Shot), ctypes.c_int, ctypes.c_int, vp]),
+            ("eio_nl_adjoint_wavefield", [vp, ctypes.c_int, ctypes.POINTER(Shot), ctypes.c_int, ctypes.c_int, vp]),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -99,7 +101,8 @@ EXPORTED = ("eio_last_error", "eio_ctx_create", "eio_ctx_destroy", "eio_ctx_set_
             "eio_synchronize", "eio_copy_probe", "eio_ctx_set_variant", "eio_ctx_variant_info",
             "eio_ctx_num_variants", "eio_fwi_forward", "eio_fwi_gradient", "eio_fwi_info",
             "eio_fwi_release", "eio_ctx_num_time_block_variants", "eio_ctx_time_block_info",
-            "eio_ctx_set_time_block", "eio_ctx_set_step_limit")
+            "eio_ctx_set_time_block", "eio_ctx_set_step_limit", "eio_nl_forward_wavefield",
+            "eio_nl_adjoint_wavefield")
 
 
 def check(rc):
@@ -269,6 +272,25 @@ class Context:
     def nl_adjoint(self, shots, device_ptrs=False):
         arr = self._shots(shots)
         check(lib().eio_nl_adjoint(self.h, len(shots), arr, EIO_DEVICE_PTRS if device_ptrs else 0))
+
+    def nl_forward_wavefield(self, shots, shot, wavefield, device_ptrs=False):
+        """NL forward saving shot `shot`'s wavefield into `wavefield` ((nts, 5, nx, nz) host array
+        of the context dtype, or a device address)."""
+        arr = self._shots(shots)
+        w = ctypes.c_void_p(int(wavefield)) if device_ptrs else self._wfld_ptr(wavefield)
+        check(lib().eio_nl_forward_wavefield(self.h, len(shots), arr, EIO_DEVICE_PTRS if device_ptrs else 0,
+                                             int(shot), w))
+
+    def nl_adjoint_wavefield(self, shots, shot, wavefield, device_ptrs=False):
+        arr = self._shots(shots)
+        w = ctypes.c_void_p(int(wavefield)) if device_ptrs else self._wfld_ptr(wavefield)
+        check(lib().eio_nl_adjoint_wavefield(self.h, len(shots), arr, EIO_DEVICE_PTRS if device_ptrs else 0,
+                                             int(shot), w))
+
+    def _wfld_ptr(self, a):
+        if not isinstance(a, np.ndarray) or not a.flags.c_contiguous or a.dtype != self.dtype:
+            raise EioError("wavefield must be a C-contiguous %s array" % self.dtype)
+        return ctypes.c_void_p(a.ctypes.data)
 
     def born_forward(self, shots, dm5, device_ptrs=False):
         arr = self._shots(shots)

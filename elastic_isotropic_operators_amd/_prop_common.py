@@ -180,6 +180,11 @@ def make(real):
                 elasticParam = elasticParam.getCpp()
             if "getCpp" in dir(paramP):
                 paramP = paramP.getCpp()
+            ax = elasticParam.getHyper().axes
+            self._grid_axes = (Hypercube.axis(n=ax[0].n, o=ax[0].o, d=ax[0].d),
+                               Hypercube.axis(n=ax[1].n, o=ax[1].o, d=ax[1].d))
+            self._dts = paramP.getFloat("dts")
+            self._ots = paramP.getFloat("ots", 0.0)
             self.pyOp = pynl.nonlinearPropElasticShotsGpu(
                 elasticParam, paramP.param, sourcesVectorCenterGrid, sourcesVectorXGrid, sourcesVectorZGrid,
                 sourcesVectorXZGrid, receiversVectorCenterGrid, receiversVectorXGrid, receiversVectorZGrid,
@@ -211,13 +216,34 @@ def make(real):
                 self.pyOp.setBackground(elasticParam)
 
         def forwardWavefield(self, add, model, data):
-            raise NotImplementedError("wavefield saving is not available in this build")
+            """Forward, keeping the wavefield of shot wavefieldShotNumber when saveWavefield=1
+            (Elastic_iso_float_prop.py:971-979)."""
+            if "getCpp" in dir(model):
+                model = model.getCpp()
+            if "getCpp" in dir(data):
+                data = data.getCpp()
+            with pynl.ostream_redirect():
+                self.pyOp.forwardWavefield(add, model, data)
 
         def adjointWavefield(self, add, model, data):
-            raise NotImplementedError("wavefield saving is not available in this build")
+            """Adjoint, keeping the scaled adjoint wavefield (Elastic_iso_float_prop.py:991-999)."""
+            if "getCpp" in dir(model):
+                model = model.getCpp()
+            if "getCpp" in dir(data):
+                data = data.getCpp()
+            with pynl.ostream_redirect():
+                self.pyOp.adjointWavefield(add, model, data)
 
         def getWavefield(self):
-            raise NotImplementedError("wavefield saving is not available in this build")
+            """The saved wavefield as a float4D vector with axes z, x, component (5), time
+            (Elastic_iso_float_prop.py:1001-1003; seismicElasticOperator2D.cpp:174-191)."""
+            w = self.pyOp.getWavefield()
+            hz, hx = self._grid_axes
+            hyper = Hypercube.hypercube(axes=[hz, hx, Hypercube.axis(n=5, o=0.0, d=1.0),
+                                              Hypercube.axis(n=w.shape[0], o=self._ots, d=self._dts)])
+            vec = SepVector.getSepVector(hyper, storage=_storage(real))
+            vec.getNdArray()[...] = w
+            return vec
 
         def dotTestCpp(self, verb=False, maxError=.00001):
             with pynl.ostream_redirect():

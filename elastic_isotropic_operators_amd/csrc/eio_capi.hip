@@ -629,10 +629,33 @@ void timing_end(Ctx* c, long long nstep) {
     c->last_step_ms = nstep > 0 ? double(stp) / double(nstep) : 0.0;
 }
 
+// Saved wavefield of one shot of a call: [nts][5][nz*nx] on the device, zeroed, attached to
+// the shot's descriptor (re-uploaded); copied to `out` (host, or device with EIO_DEVICE_PTRS)
+// when the call finishes.
 template <typename T>
-void nl_forward(Ctx* c, int nshot, const eio_shot* shots, int flags) {
-    const bool two = sizeof(T) == 4 && c->time_block == 2;
+void attach_wavefield(Run<T>& run, int shot, void* out, int flags) {
+    Ctx* c = run.c;
+    if (!out) fail("wavefield pointer is NULL");
+    const int ns = int(run.sd.size());
+    if (shot < 0 || shot >= ns) fail("wavefield shot index out of range");
+    int oi = 0;
+    while (run.plan.order[oi] != shot) ++oi;
+    eio::ShotDev<T>& d = run.sd[oi];
+    const size_t elems = size_t(d.nts) * 5 * size_t(d.nz) * d.nx;
+    T* w = (flags & EIO_DEVICE_PTRS) ? static_cast<T*>(out) : static_cast<T*>(ws_get(c, "wfld", elems * sizeof(T)));
+    fill0<T>(c, w, long(elems));
+    d.wfld = w;
+    if (!(flags & EIO_DEVICE_PTRS)) run.outs.push_back({out, w, elems * sizeof(T)});
+    EIO_CHECK_HIP(hipMemcpyAsync(run.d_shots + oi, &run.sd[oi], sizeof(eio::ShotDev<T>), hipMemcpyHostToDevice,
+                                 c->stream));
+}
+
+template <typename T>
+void nl_forward(Ctx* c, int nshot, const eio_shot* shots, int flags, int wshot = -1, void* wfld = nullptr) {
+    // the saved wavefield is written by the one-step kernel
+    const bool two = sizeof(T) == 4 && c->time_block == 2 && !wfld;
     Run<T> run = prepare<T>(c, nshot, shots, Op::NL_FWD, flags, nullptr, nullptr, 0, two);
+    if (wfld) attach_wavefield(run, wshot, wfld, flags);
     int kmax = run.plan.nsteps.empty() ? 0 : run.plan.nsteps[0];
     if (c->step_limit > 0) kmax = std::min(kmax, c->step_limit);
     long long nlaunch = 0;
@@ -655,8 +678,9 @@ void nl_forward(Ctx* c, int nshot, const eio_shot* shots, int flags) {
 }
 
 template <typename T>
-void nl_adjoint(Ctx* c, int nshot, const eio_shot* shots, int flags) {
+void nl_adjoint(Ctx* c, int nshot, const eio_shot* shots, int flags, int wshot = -1, void* wfld = nullptr) {
     Run<T> run = prepare<T>(c, nshot, shots, Op::NL_ADJ, flags, nullptr);
+    if (wfld) attach_wavefield(run, wshot, wfld, flags);
     // The adjoint runs backwards from each shot's own last sample: shots of different
     // lengths are aligned at their end, so a shot joins the batch when its remaining step
     // count is reached.  Keep it simple and exact: all shots of a call share ntw here.
@@ -1112,6 +1136,28 @@ int eio_nl_forward(eio_ctx* ctx, int nshot, const eio_shot* shots, int flags) {
         DeviceGuard dg(c->device);
         if (c->prec == 4) nl_forward<float>(c, nshot, shots, flags);
         else nl_forward<double>(c, nshot, shots, flags);
+    });
+}
+
+int eio_nl_forward_wavefield(eio_ctx* ctx, int nshot, const eio_shot* shots, int flags, int shot,
+                             void* wavefield) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        DeviceGuard dg(c->device);
+        if (!wavefield) fail("wavefield is NULL");
+        if (c->prec == 4) nl_forward<float>(c, nshot, shots, flags, shot, wavefield);
+        else nl_forward<double>(c, nshot, shots, flags, shot, wavefield);
+    });
+}
+
+int eio_nl_adjoint_wavefield(eio_ctx* ctx, int nshot, const eio_shot* shots, int flags, int shot,
+                             void* wavefield) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        DeviceGuard dg(c->device);
+        if (!wavefield) fail("wavefield is NULL");
+        if (c->prec == 4) nl_adjoint<float>(c, nshot, shots, flags, shot, wavefield);
+        else nl_adjoint<double>(c, nshot, shots, flags, shot, wavefield);
     });
 }
 

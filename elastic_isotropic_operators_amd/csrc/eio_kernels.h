@@ -76,7 +76,21 @@ struct ShotDev {
     int wcap;              // nts for full storage, segment length + 3 for the checkpointed gradient
     int pad1;
     T* dm;                 // Born perturbation (forward) or per-shot imaging accumulator (adjoint)
+    T* wfld;               // saved coarse-sampled wavefield [nts][5][nz*nx] of this shot, or NULL
 };
+
+// Accumulate the new values of one cell into the saved wavefield, time slices its and its+1
+// with weights wa, wb (kernelsGpuElastic.cu:1299-1335).
+template <typename T>
+__device__ __forceinline__ void save_wfld(T* wfld, unsigned n, int its, size_t cell, const T v[5], T wa, T wb) {
+    T* a = wfld + size_t(its) * 5 * n + cell;
+    T* b = a + size_t(5) * n;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+        a[size_t(f) * n] += v[f] * wa;
+        b[size_t(f) * n] += v[f] * wb;
+    }
+}
 
 struct TileDev {
     int shot, z0, x0;
@@ -371,6 +385,11 @@ void k_step_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict_
             const unsigned so = unsigned(r) * colb;
 #pragma unroll
             for (int f = 0; f < 5; ++f) bst(rold, val[f][r], voff, so + unsigned(f) * fbytes);
+            if ((MODE & F_REC) && S.wfld) {  // uniform per shot
+                const T v5[5] = {val[0][r], val[1][r], val[2][r], val[3][r], val[4][r]};
+                save_wfld<T>(S.wfld, n, st.its, size_t(ix0 + r) * nz + iz, v5, S.filt[st.it2],
+                             S.filt[S.h + st.it2]);
+            }
             if (MODE & F_DEC) {
                 T xa = val[0][r] * wfa, za = val[1][r] * wfa;
                 T xb = val[0][r] * wfb, zb = val[1][r] * wfb;
@@ -570,6 +589,11 @@ void k_step_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict_
             for (int f = 0; f < 5; ++f) {
                 const unsigned sf = so + unsigned(f) * fbytes;
                 bst(rold, val[f][r], voff, sf);
+                if (MODE == M_NLADJ && S.wfld) {  // uniform per shot
+                    T* wa = S.wfld + size_t(st.its) * 5 * n + size_t(f) * n + size_t(ix0 + r) * nz + iz;
+                    wa[0] += val[f][r] * S.filt[st.it2];
+                    wa[size_t(5) * n] += val[f][r] * S.filt[S.h + st.it2];
+                }
                 if (MODE == M_BADJ) {
                     // slot its: the interval's first (reverse) step overwrites it; slot its+1:
                     // its weight F[h+0] is 0 at it2 = 0 (nothing to add)

@@ -459,8 +459,10 @@ class NonlinearShots {
         check_eio(eio_model_set_media(dev_->ctx, model_, media, 0));
     }
 
-    // model [nSrcSig][5][nSrc][nts] with nSrcSig = 1 or nExp; data [nExp][5][nRec][nts]
-    void forward(bool add, const T* model, int nModelShots, int nSrc, T* data, int nRec) {
+    // model [nSrcSig][5][nSrc][nts] with nSrcSig = 1 or nExp; data [nExp][5][nRec][nts].
+    // save: also keep the wavefield of shot wavefieldShotNumber when saveWavefield != 0
+    // (nonlinearPropElasticShotsGpu.cpp:185-300).
+    void forward(bool add, const T* model, int nModelShots, int nSrc, T* data, int nRec, bool save = false) {
         const int nts = fd_->nts;
         if (nModelShots != 1 && nModelShots != nExp_) throw std::runtime_error("model shot axis must be 1 or nExp");
         std::vector<eio_shot> shots(nExp_);
@@ -486,7 +488,12 @@ class NonlinearShots {
                 s.rec_tr[k] = d.data();
             }
         }
-        check_eio(eio_nl_forward(dev_->ctx, nExp_, shots.data(), 0));
+        if (save_shot(save) >= 0) {
+            wavefield_.assign(size_t(nts) * 5 * fd_->ncell(), T(0));
+            check_eio(eio_nl_forward_wavefield(dev_->ctx, nExp_, shots.data(), 0, save_shot(save), wavefield_.data()));
+        } else {
+            check_eio(eio_nl_forward(dev_->ctx, nExp_, shots.data(), 0));
+        }
         if (!add) std::fill(data, data + size_t(nExp_) * 5 * nRec * nts, T(0));
         for (int e = 0; e < nExp_; ++e) {
             const InterpVec& Rv = rec_.size() == 1 ? rec_[0] : rec_[e];
@@ -496,7 +503,7 @@ class NonlinearShots {
         }
     }
 
-    void adjoint(bool add, T* model, int nModelShots, int nSrc, const T* data, int nRec) {
+    void adjoint(bool add, T* model, int nModelShots, int nSrc, const T* data, int nRec, bool save = false) {
         const int nts = fd_->nts;
         if (nModelShots != 1 && nModelShots != nExp_) throw std::runtime_error("model shot axis must be 1 or nExp");
         std::vector<eio_shot> shots(nExp_);
@@ -520,7 +527,26 @@ class NonlinearShots {
                 s.rec_tr[k] = d.data();
             }
         }
-        check_eio(eio_nl_adjoint(dev_->ctx, nExp_, shots.data(), 0));
+        if (save_shot(save) >= 0) {
+            const size_t n = fd_->ncell();
+            wavefield_.assign(size_t(nts) * 5 * n, T(0));
+            check_eio(eio_nl_adjoint_wavefield(dev_->ctx, nExp_, shots.data(), 0, save_shot(save), wavefield_.data()));
+            // nonlinearPropElasticGpu.cpp:432-448: rhoxDtw, rhozDtw, 2*dtw, all / (dx*dz)
+            const T area = T(1.0 / (fd_->dx * fd_->dz));
+            const T two = T(2.0 * fd_->dtw);
+            for (int it = 0; it < nts; ++it) {
+                T* w = wavefield_.data() + size_t(it) * 5 * n;
+                for (size_t i = 0; i < n; ++i) {
+                    w[i] *= fd_->media[i] * area;
+                    w[n + i] *= fd_->media[n + i] * area;
+                    w[2 * n + i] *= two * area;
+                    w[3 * n + i] *= two * area;
+                    w[4 * n + i] *= two * area;
+                }
+            }
+        } else {
+            check_eio(eio_nl_adjoint(dev_->ctx, nExp_, shots.data(), 0));
+        }
         if (!add) std::fill(model, model + size_t(nModelShots) * 5 * nSrc * nts, T(0));
         for (int e = 0; e < nExp_; ++e) {
             const InterpVec& S = src_[e];
@@ -535,6 +561,8 @@ class NonlinearShots {
     }
 
     int nExp() const { return nExp_; }
+    // the last saved wavefield [nts][5][nx][nz] (empty before any saving call)
+    const std::vector<T>& wavefield() const { return wavefield_; }
     int nSrcIrreg() const { return src_[0][0]->nDeviceIrreg; }
     int nRecIrreg() const { return rec_[0][0]->nDeviceIrreg; }
     double last_ms() const {
@@ -544,6 +572,13 @@ class NonlinearShots {
     }
 
    private:
+    // shot whose wavefield a saving call keeps, or -1
+    int save_shot(bool save) const {
+        if (!save || par_.getInt("saveWavefield", 0) == 0) return -1;
+        const int s = par_.getInt("wavefieldShotNumber", 0);
+        if (s < 0 || s >= nExp_) throw std::runtime_error("wavefieldShotNumber out of range");
+        return s;
+    }
     void register_model() {
         eio_grid g;
         g.nz = fd_->nz; g.nx = fd_->nx; g.nts = fd_->nts; g.sub_fd = fd_->sub; g.minPad = fd_->minPad;
@@ -569,6 +604,7 @@ class NonlinearShots {
     std::unique_ptr<Device> dev_;
     std::unique_ptr<FdParam<T>> fd_;
     int model_ = -1;
+    std::vector<T> wavefield_;
 };
 
 // ---------------------------------------------------------------- Born shot operator

@@ -234,6 +234,36 @@ PYBIND11_MODULE(EIO_MODULE, m) {
                  py::array_t<real> el = array_of(elasticParam, false);
                  op.setBackground(el.data());
              })
+        .def("forwardWavefield",
+             [](NL& op, bool add, py::object model, py::object data) {
+                 py::array_t<real> mo = array_of(model, false), da = array_of(data, true);
+                 int ms, ns, ds, nr;
+                 trace_dims(mo, op.fd().nts, ms, ns);
+                 trace_dims(da, op.fd().nts, ds, nr);
+                 if (ds != op.nExp()) throw std::runtime_error("data shot axis must equal nExp");
+                 py::gil_scoped_release rel;
+                 op.forward(add, mo.data(), ms, ns, da.mutable_data(), nr, true);
+             })
+        .def("adjointWavefield",
+             [](NL& op, bool add, py::object model, py::object data) {
+                 py::array_t<real> mo = array_of(model, true), da = array_of(data, false);
+                 int ms, ns, ds, nr;
+                 trace_dims(mo, op.fd().nts, ms, ns);
+                 trace_dims(da, op.fd().nts, ds, nr);
+                 if (ds != op.nExp()) throw std::runtime_error("data shot axis must equal nExp");
+                 py::gil_scoped_release rel;
+                 op.adjoint(add, mo.mutable_data(), ms, ns, da.data(), nr, true);
+             })
+        .def("getWavefield",
+             // [nts][5][nx][nz] (SEP axes z, x, component, time), a copy
+             [](NL& op) {
+                 const auto& w = op.wavefield();
+                 if (w.empty()) throw std::runtime_error("no wavefield saved (saveWavefield=1 and a *Wavefield call)");
+                 const auto& fd = op.fd();
+                 py::array_t<real> out({py::ssize_t(fd.nts), py::ssize_t(5), py::ssize_t(fd.nx), py::ssize_t(fd.nz)});
+                 std::memcpy(out.mutable_data(), w.data(), w.size() * sizeof(real));
+                 return out;
+             })
         .def("lastPropagationMs", &NL::last_ms)
         .def("dotTest",
              [](NL& op, bool verbose, float maxError) {

@@ -114,6 +114,17 @@ int eio_nl_forward(eio_ctx* ctx, int nshot, const eio_shot* shots, int flags);
 /* Nonlinear adjoint; src_tr is overwritten.
  * Replaces propShotsElasticAdjGpu (nonlinearPropElasticGpuFunctions.h:28-34, .cu:932-1028). */
 int eio_nl_adjoint(eio_ctx* ctx, int nshot, const eio_shot* shots, int flags);
+/* The same calls, also saving the coarse-sampled wavefield of shot `shot` (index into shots)
+ * into `wavefield` [nts][5][nx][nz] (host pointer, or device pointer with EIO_DEVICE_PTRS):
+ * every fine step's update-region values accumulated into slices its and its+1 with the time
+ * filter, as the receivers are; the adjoint wavefield is unscaled (the reference scales it on
+ * the host by rhoxDtw, rhozDtw, 2*dtw and 1/(dx*dz)).  Replace
+ * propShotsElastic{Fwd,Adj}GpuWavefield[Streams] (nonlinearPropElasticGpuFunctions.h:38-136,
+ * .cu:649-927, 1029-1263; rows and columns of the streamed variant). */
+int eio_nl_forward_wavefield(eio_ctx* ctx, int nshot, const eio_shot* shots, int flags, int shot,
+                             void* wavefield);
+int eio_nl_adjoint_wavefield(eio_ctx* ctx, int nshot, const eio_shot* shots, int flags, int shot,
+                             void* wavefield);
 
 /* Born forward: dm = 5 staggered and scaled perturbations (drhox*rhoxDtw, drhoz*rhozDtw,
  * dlame*2dtw, dmu*2dtw, dmuxz*2dtw), nz*nx each, shared by all shots (all shots must use

@@ -198,11 +198,29 @@ struct RegGeom {
 // component -> grid: vx on X, vz on Z, sxx/szz on centre, sxz on XZ
 const int COMP_GRID[5] = {1, 2, 0, 0, 3};
 
+// Coarse-sampled wavefield [nts][5][nz*nx] of the update region, accumulated like the data
+// (kernelsGpuElastic.cu:1299-1335, the row/column ranges of the streamed variant)
+template <typename T>
+void save_wavefield(const Consts<T>& c, const Fields<T>& p, int its, T wa, T wb, T* wfld) {
+    if (!wfld) return;
+    const size_t n = size_t(c.nz) * c.nx;
+    T* a = wfld + size_t(its) * 5 * n;
+    T* b = a + 5 * n;
+    for (int k = 0; k < 5; ++k)
+        for (int ix = FAT; ix < c.nx - FAT; ++ix)
+            for (int iz = FAT; iz < c.nz - FAT; ++iz) {
+                const size_t i = size_t(ix) * c.nz + iz;
+                a[k * n + i] += p.f[k][i] * wa;
+                b[k * n + i] += p.f[k][i] * wb;
+            }
+}
+
 // ---------------------------------------------------------------- NL (regular level)
 // nonlinearPropElasticGpuFunctions.cu:561-648 ; traces are [nReg][nts], per component
+// (wfld != nullptr: also the forward wavefield, nonlinearPropElasticGpuFunctions.cu:649-760)
 template <typename T>
 void nl_fwd_reg(const Consts<T>& c, const Media<T>& m, const RegGeom& src, const T* const mod[5],
-                const RegGeom& rec, T* const dat[5]) {
+                const RegGeom& rec, T* const dat[5], T* wfld = nullptr) {
     const size_t n = size_t(c.nz) * c.nx;
     Fields<T> a(n), b(n);
     Fields<T>*p0 = &a, *p1 = &b;
@@ -220,6 +238,7 @@ void nl_fwd_reg(const Consts<T>& c, const Media<T>& m, const RegGeom& src, const
                                                  mod[k][size_t(c.nts) * is + its + 1] * c.F[c.h + it2 - 1];
             }
             damp_both(c, *p0, *p1);
+            save_wavefield(c, *p0, its, c.F[it2], c.F[c.h + it2], wfld);
             for (int k = 0; k < 5; ++k) {  // kernelsGpuElastic.cu:7-60
                 const int gi = COMP_GRID[k];
                 for (int ir = 0; ir < rec.n[gi]; ++ir) {
@@ -233,10 +252,11 @@ void nl_fwd_reg(const Consts<T>& c, const Media<T>& m, const RegGeom& src, const
     }
 }
 
-// nonlinearPropElasticGpuFunctions.cu:932-1028
+// nonlinearPropElasticGpuFunctions.cu:932-1028 (wfld: the adjoint wavefield before the host
+// scaling, nonlinearPropElasticGpuFunctions.cu:1029-1090)
 template <typename T>
 void nl_adj_reg(const Consts<T>& c, const Media<T>& m, const RegGeom& src, T* const mod[5],
-                const RegGeom& rec, const T* const dat[5]) {
+                const RegGeom& rec, const T* const dat[5], T* wfld = nullptr) {
     const size_t n = size_t(c.nz) * c.nx;
     Fields<T> a(n), b(n);
     std::vector<T> scr[7];
@@ -256,6 +276,7 @@ void nl_adj_reg(const Consts<T>& c, const Media<T>& m, const RegGeom& src, T* co
                                                  dat[k][size_t(c.nts) * ir + its + 1] * c.F[c.h + it2 + 1];
             }
             damp_both(c, *p0, *p1);
+            save_wavefield(c, *p0, its, c.F[it2], c.F[c.h + it2], wfld);
             for (int k = 0; k < 5; ++k) {  // kernelsGpuElastic.cu:100-116
                 const int gi = COMP_GRID[k];
                 for (int is = 0; is < src.n[gi]; ++is) {
@@ -852,6 +873,30 @@ static Grid to_grid(const OracleGrid* g) {
         RegGeom s, r;                                                                              \
         for (int k = 0; k < 4; ++k) { s.pos[k] = spos[k]; s.n[k] = sn[k]; r.pos[k] = rpos[k]; r.n[k] = rn[k]; } \
         nl_fwd_reg(c, m, s, mod, r, dat);                                                          \
+    }                                                                                              \
+    void oracle_nl_fwd_reg_wfld_##SUF(const OracleGrid* g, const T* media, const int* const* spos, \
+                                      const int* sn, const T* const* mod, const int* const* rpos,  \
+                                      const int* rn, T* const* dat, T* wfld) {                     \
+        Grid gr = to_grid(g);                                                                      \
+        Consts<T> c = make_consts<T>(gr);                                                          \
+        const size_t n = size_t(gr.nz) * gr.nx;                                                    \
+        Media<T> m = {media, media + n, media + 2 * n, media + 3 * n, media + 4 * n};              \
+        RegGeom s, r;                                                                              \
+        for (int k = 0; k < 4; ++k) { s.pos[k] = spos[k]; s.n[k] = sn[k]; r.pos[k] = rpos[k]; r.n[k] = rn[k]; } \
+        std::fill(wfld, wfld + size_t(gr.nts) * 5 * n, T(0));                                      \
+        nl_fwd_reg(c, m, s, mod, r, dat, wfld);                                                    \
+    }                                                                                              \
+    void oracle_nl_adj_reg_wfld_##SUF(const OracleGrid* g, const T* media, const int* const* spos, \
+                                      const int* sn, T* const* mod, const int* const* rpos,        \
+                                      const int* rn, const T* const* dat, T* wfld) {               \
+        Grid gr = to_grid(g);                                                                      \
+        Consts<T> c = make_consts<T>(gr);                                                          \
+        const size_t n = size_t(gr.nz) * gr.nx;                                                    \
+        Media<T> m = {media, media + n, media + 2 * n, media + 3 * n, media + 4 * n};              \
+        RegGeom s, r;                                                                              \
+        for (int k = 0; k < 4; ++k) { s.pos[k] = spos[k]; s.n[k] = sn[k]; r.pos[k] = rpos[k]; r.n[k] = rn[k]; } \
+        std::fill(wfld, wfld + size_t(gr.nts) * 5 * n, T(0));                                      \
+        nl_adj_reg(c, m, s, mod, r, dat, wfld);                                                    \
     }                                                                                              \
     void oracle_nl_adj_reg_##SUF(const OracleGrid* g, const T* media, const int* const* spos,      \
                                  const int* sn, T* const* mod, const int* const* rpos,             \

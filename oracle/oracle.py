@@ -126,9 +126,10 @@ def _pos_arrays(pos4):
 GRID_OF_COMP = (1, 2, 0, 0, 3)  # vx->X, vz->Z, sxx/szz->centre, sxz->XZ
 
 
-def nl_fwd_reg(g, med, spos, src_traces, rpos, dtype=np.float64):
+def nl_fwd_reg(g, med, spos, src_traces, rpos, dtype=np.float64, wavefield=False):
     """Regular-level NL forward.  spos/rpos: 4 position arrays (centre, X, Z, XZ);
-    src_traces: 5 arrays (nReg, nts).  Returns 5 data arrays (nRegRec, nts)."""
+    src_traces: 5 arrays (nReg, nts).  Returns 5 data arrays (nRegRec, nts), and with
+    wavefield=True also the coarse-sampled wavefield (nts, 5, nx, nz)."""
     med = _c(med, dtype)
     sa, sp, sn = _pos_arrays(spos)
     ra, rp, rn = _pos_arrays(rpos)
@@ -136,12 +137,19 @@ def nl_fwd_reg(g, med, spos, src_traces, rpos, dtype=np.float64):
     dats = [np.zeros((ra[GRID_OF_COMP[k]].size, g.nts), dtype) for k in range(5)]
     mp = (ctypes.c_void_p * 5)(*[a.ctypes.data for a in mods])
     dp = (ctypes.c_void_p * 5)(*[a.ctypes.data for a in dats])
+    if wavefield:
+        w = np.zeros((g.nts, 5, g.nx, g.nz), dtype)
+        getattr(lib(), "oracle_nl_fwd_reg_wfld_" + _suffix(dtype))(ctypes.byref(g), _ptr(med), sp, sn,
+                                                                   mp, rp, rn, dp, _ptr(w))
+        return dats, w
     getattr(lib(), "oracle_nl_fwd_reg_" + _suffix(dtype))(ctypes.byref(g), _ptr(med), sp, sn, mp,
                                                           rp, rn, dp)
     return dats
 
 
-def nl_adj_reg(g, med, spos, rpos, dat_traces, dtype=np.float64):
+def nl_adj_reg(g, med, spos, rpos, dat_traces, dtype=np.float64, wavefield=False):
+    """Regular-level NL adjoint; with wavefield=True also the adjoint wavefield before any
+    scaling (nts, 5, nx, nz)."""
     med = _c(med, dtype)
     sa, sp, sn = _pos_arrays(spos)
     ra, rp, rn = _pos_arrays(rpos)
@@ -149,6 +157,11 @@ def nl_adj_reg(g, med, spos, rpos, dat_traces, dtype=np.float64):
     mods = [np.zeros((sa[GRID_OF_COMP[k]].size, g.nts), dtype) for k in range(5)]
     mp = (ctypes.c_void_p * 5)(*[a.ctypes.data for a in mods])
     dp = (ctypes.c_void_p * 5)(*[a.ctypes.data for a in dats])
+    if wavefield:
+        w = np.zeros((g.nts, 5, g.nx, g.nz), dtype)
+        getattr(lib(), "oracle_nl_adj_reg_wfld_" + _suffix(dtype))(ctypes.byref(g), _ptr(med), sp, sn,
+                                                                   mp, rp, rn, dp, _ptr(w))
+        return mods, w
     getattr(lib(), "oracle_nl_adj_reg_" + _suffix(dtype))(ctypes.byref(g), _ptr(med), sp, sn, mp,
                                                           rp, rn, dp)
     return mods

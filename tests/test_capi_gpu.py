@@ -337,3 +337,41 @@ def test_autotuner_choice_keeps_parity():
     ctx.nl_forward([shot])
     for k in range(5):
         assert rel_l2(rec[k], ref[k]) < 1e-5, (res["choice"], k, rel_l2(rec[k], ref[k]))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_saved_wavefields_match_oracle(dtype):
+    """eio_nl_{forward,adjoint}_wavefield: the saved coarse wavefield of one shot of a batch
+    equals the oracle's, and the data of the call equal a plain call's."""
+    w, g, med, spos, rpos = _case(nz=40, nx=56, nts=70, sub=2, seed=95)
+    rng = np.random.default_rng(96)
+    src = random_traces(rng, spos, g.nts, np.float64)
+    dat = random_traces(rng, rpos, g.nts, np.float64)
+    ref_d, ref_w = O.nl_fwd_reg(g, med, spos, src, rpos, np.float64, wavefield=True)
+    ref_m, ref_wa = O.nl_adj_reg(g, med, spos, rpos, dat, np.float64, wavefield=True)
+    ctx = _ctx(dtype)
+    mid = _model(ctx, w, g, med)
+    tol = TOL[dtype]
+    # forward: the saved shot is the second of a 2-shot batch (the first one is a decoy)
+    recs = [[np.zeros((rpos[GRID_OF_COMP[k]].size, g.nts), dtype) for k in range(5)] for _ in range(2)]
+    srcs = [[np.ascontiguousarray(s * (1 + e), dtype) for s in src] for e in range(2)]
+    shots, keep = [], []
+    for e in range(2):
+        s, kp = ctx.make_shot(mid, spos, rpos, srcs[1 - e], recs[e])
+        shots.append(s)
+        keep.append(kp)
+    wf = np.zeros((g.nts, 5, g.nx, g.nz), dtype)
+    ctx.nl_forward_wavefield(shots, 1, wf)
+    assert rel_l2(wf, ref_w) < tol, rel_l2(wf, ref_w)
+    for k in range(5):
+        assert rel_l2(recs[1][k], ref_d[k]) < tol
+    # adjoint
+    mods = [np.zeros((spos[GRID_OF_COMP[k]].size, g.nts), dtype) for k in range(5)]
+    s, kp = ctx.make_shot(mid, spos, rpos, mods, [np.ascontiguousarray(d, dtype) for d in dat])
+    wa = np.zeros((g.nts, 5, g.nx, g.nz), dtype)
+    ctx.nl_adjoint_wavefield([s], 0, wa)
+    assert rel_l2(wa, ref_wa) < tol, rel_l2(wa, ref_wa)
+    for k in range(5):
+        assert rel_l2(mods[k], ref_m[k]) < tol
+    with pytest.raises(_capi.EioError, match="out of range"):
+        ctx.nl_adjoint_wavefield([s], 1, wa)

@@ -133,3 +133,48 @@ def test_fwi_operator_equals_nl_forward(tmp_path):
     model.getNdArray()[...] = sources.getNdArray()
     op.forward(False, model, data2)
     assert rel_l2(d_fwi.getNdArray(), data2.getNdArray()) < 1e-6
+
+
+def test_forward_and_adjoint_wavefield(tmp_path):
+    """forwardWavefield / adjointWavefield / getWavefield with saveWavefield=1 and
+    wavefieldShotNumber=1: the wavefield (axes z, x, component, time) interpolated to the
+    receivers with the operator's own geometry reproduces the recorded data of that shot, and
+    the scaled adjoint wavefield interpolated to the source reproduces the adjoint model."""
+    from elastic_isotropic_operators_amd import geometry as G
+    w = _small(nts=90)
+    args, geo = operator_case(tmp_path, w, nExp=3, extra=dict(saveWavefield=1, wavefieldShotNumber=1))
+    op, model, data, elastic = _nl_op(PF, args, PF.nonlinearOpInitFloat)
+    model.getNdArray()[0] = w["model"].astype(np.float32)
+    op.forwardWavefield(False, model, data)
+    wv = op.getWavefield()
+    ax = wv.getHyper().axes
+    assert [a.n for a in ax] == [w["nz"], w["nx"], 5, w["nts"]]
+    wf = wv.getNdArray().astype(np.float64)          # (nts, 5, nx, nz)
+    d = data.getNdArray()[1]
+    d2 = data.clone()
+    op.forward(False, model, d2)
+    assert np.array_equal(d2.getNdArray(), data.getNdArray())
+    ri = G.staggered_interps(w["nz"], w["oz"], w["dz"], w["ox"], w["dx"], geo["rz"], geo["rx"])
+    for k, gi in enumerate(G.GRID_OF_COMP):
+        reg = wf[:, k].reshape(w["nts"], -1)[:, ri[gi].uniq].T
+        assert rel_l2(ri[gi].forward(reg), d[k]) < 1e-5
+    # adjoint
+    rng = np.random.default_rng(4)
+    dd = data.clone()
+    dd.getNdArray()[...] = rng.standard_normal(dd.getNdArray().shape)
+    mm = model.clone()
+    op.adjointWavefield(False, mm, dd)
+    wa = op.getWavefield().getNdArray().astype(np.float64)
+    si = G.staggered_interps(w["nz"], w["oz"], w["dz"], w["ox"], w["dx"], geo["sz"][1], geo["sx"][1])
+    mm1 = model.clone()
+    op.adjoint(False, mm1, dd)
+    assert np.array_equal(mm1.getNdArray(), mm.getNdArray())
+    # shot 1's own contribution to the (shot-stacked) adjoint model
+    dd1 = dd.clone()
+    dd1.getNdArray()[...] = 0
+    dd1.getNdArray()[1] = dd.getNdArray()[1]
+    m1 = model.clone()
+    op.adjoint(False, m1, dd1)
+    for k, gi in enumerate(G.GRID_OF_COMP):
+        reg = wa[:, k].reshape(w["nts"], -1)[:, si[gi].uniq].T
+        assert rel_l2(si[gi].forward(reg), m1.getNdArray()[0, k]) < 1e-5

@@ -289,3 +289,28 @@ def test_fwi_gradient_finite_differences():
     assert gap_fine < 0.02 and corr_fine > 0.9999, (gap_fine, corr_fine)
     gap_coarse, _ = _fwi_fd_gap(400, 0.001, 4)
     assert gap_coarse > 3.0 * gap_fine, (gap_coarse, gap_fine)
+
+
+def test_saved_wavefield_is_what_the_receivers_see():
+    """The coarse-sampled wavefield uses the receivers' time filter, so sampling it at the
+    receiver positions gives the recorded data exactly (forward), and sampling the adjoint
+    wavefield at the source positions gives the adjoint source traces (adjoint)."""
+    from helpers import random_traces, reg_positions
+    w = W.small(nz=40, nx=56, nts=60, sub=1, seed=5)
+    g = oracle_grid(w)
+    med = O.media(g, w["elastic"])
+    sp = reg_positions(w, w["src_z"][0], w["src_x"][0])
+    rp = reg_positions(w, w["rec_z"], w["rec_x"])
+    rng = np.random.default_rng(6)
+    src = random_traces(rng, sp, g.nts, np.float64)
+    dat, wf = O.nl_fwd_reg(g, med, sp, src, rp, wavefield=True)
+    assert np.array_equal(np.concatenate(dat), np.concatenate(O.nl_fwd_reg(g, med, sp, src, rp)))
+    for k, gi in enumerate(O.GRID_OF_COMP):
+        samp = wf[:, k].reshape(g.nts, -1)[:, rp[gi]].T
+        assert np.abs(samp - dat[k]).max() <= 1e-12 * np.abs(dat[k]).max()
+    assert np.count_nonzero(wf[:, :, :4]) == 0 and np.count_nonzero(wf[:, :, :, -4:]) == 0
+    d = random_traces(rng, rp, g.nts, np.float64)
+    mod, wa = O.nl_adj_reg(g, med, sp, rp, d, wavefield=True)
+    for k, gi in enumerate(O.GRID_OF_COMP):
+        samp = wa[:, k].reshape(g.nts, -1)[:, sp[gi]].T
+        assert np.abs(samp - mod[k]).max() <= 1e-12 * np.abs(mod[k]).max()
