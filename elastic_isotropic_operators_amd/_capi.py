@@ -23,7 +23,8 @@ class EioError(RuntimeError):
 class Grid(ctypes.Structure):
     _fields_ = [("nz", ctypes.c_int), ("nx", ctypes.c_int), ("nts", ctypes.c_int),
                 ("sub_fd", ctypes.c_int), ("minPad", ctypes.c_int), ("dz", ctypes.c_double),
-                ("dx", ctypes.c_double), ("dts", ctypes.c_double), ("alphaCos", ctypes.c_double)]
+                ("dx", ctypes.c_double), ("dts", ctypes.c_double), ("alphaCos", ctypes.c_double),
+                ("surface", ctypes.c_int)]
 
 
 class Shot(ctypes.Structure):
@@ -208,10 +209,11 @@ class Context:
         return t.value, n.value, s.value
 
     def model_add(self, nz, nx, nts, sub_fd, dz, dx, dts, minPad, media, alphaCos=0.99,
-                  device_ptrs=None):
-        """media: array (5, nx, nz) on the host, or 5 device addresses (device_ptrs)."""
+                  device_ptrs=None, surface=0):
+        """media: array (5, nx, nz) on the host, or 5 device addresses (device_ptrs);
+        surface=1: free surface at row 2*FAT."""
         g = Grid(int(nz), int(nx), int(nts), int(sub_fd), int(minPad), float(dz), float(dx),
-                 float(dts), float(alphaCos))
+                 float(dts), float(alphaCos), int(surface))
         mid = ctypes.c_int()
         if device_ptrs is not None:
             arr = (ctypes.c_void_p * 5)(*device_ptrs)

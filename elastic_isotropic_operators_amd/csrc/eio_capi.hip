@@ -180,6 +180,10 @@ int model_add(Ctx* c, const eio_grid* g, const void* const media[5], int flags) 
     if (g->minPad < 0 || g->minPad >= PAD_MAX) fail("Padding value is too high");  // NLF:191-194
     if (!(g->dz > 0) || !(g->dx > 0) || !(g->dts > 0)) fail("dz, dx, dts must be > 0");
     if (long(g->nz) * g->nx >= (1L << 31)) fail("grid too large for 32-bit tile indexing");
+    if (g->surface != 0 && g->surface != 1) fail("surface must be 0 or 1");
+    // the free-surface adjoint needs rows up to 2*FAT+4 damped independently of z
+    if (g->surface == 1 && g->nz - 1 - eio::FAT - (eio::SURF + eio::FAT) < g->minPad)
+        fail("free surface: nz too small for the bottom sponge");
     ModelBase m;
     m.g = *g;
     m.n = long(g->nz) * g->nx;
@@ -388,6 +392,8 @@ Run<T> prepare(Ctx* c, int nshot, const eio_shot* shots, Op op, int flags,
     if (born || fwi) {
         for (int i = 1; i < nshot; ++i)
             if (shots[i].model != shots[0].model) fail("Born and FWI shots of one call must share a model");
+        if (get_model(c, shots[0].model).g.surface)
+            fail("the free surface is available for the nonlinear operator only");
     }
     if (fwi && !ring) fail("FWI call without a coarse storage ring");
     const bool dev_ptrs = (flags & EIO_DEVICE_PTRS) != 0;
@@ -425,7 +431,7 @@ Run<T> prepare(Ctx* c, int nshot, const eio_shot* shots, Op op, int flags,
         eio::ShotDev<T>& d = run.sd[oi];
         std::memset(&d, 0, sizeof(d));
         d.nz = m.g.nz; d.nx = m.g.nx; d.nts = m.g.nts; d.sub = m.g.sub_fd; d.h = m.h;
-        d.ntw = m.ntw; d.minPad = m.g.minPad;
+        d.ntw = m.ntw; d.minPad = m.g.minPad; d.surface = m.g.surface;
         for (int k = 0; k < 4; ++k) { d.cz[k] = T(m.cz[k]); d.cx[k] = T(m.cx[k]); }
         d.A = state + so;
         d.B = state + so + 5 * size_t(m.n);
@@ -652,8 +658,10 @@ void attach_wavefield(Run<T>& run, int shot, void* out, int flags) {
 
 template <typename T>
 void nl_forward(Ctx* c, int nshot, const eio_shot* shots, int flags, int wshot = -1, void* wfld = nullptr) {
-    // the saved wavefield is written by the one-step kernel
-    const bool two = sizeof(T) == 4 && c->time_block == 2 && !wfld;
+    // the saved wavefield and the free surface are handled by the one-step kernel
+    bool any_fs = false;
+    for (int i = 0; i < nshot && shots; ++i) any_fs = any_fs || get_model(c, shots[i].model).g.surface;
+    const bool two = sizeof(T) == 4 && c->time_block == 2 && !wfld && !any_fs;
     Run<T> run = prepare<T>(c, nshot, shots, Op::NL_FWD, flags, nullptr, nullptr, 0, two);
     if (wfld) attach_wavefield(run, wshot, wfld, flags);
     int kmax = run.plan.nsteps.empty() ? 0 : run.plan.nsteps[0];

@@ -59,7 +59,8 @@ constexpr int M_BADJ = 1;     // Born receiver pass: data injection, adjoint-fie
 
 template <typename T>
 struct ShotDev {
-    int nz, nx, nts, sub, h, ntw, minPad, pad0;
+    int nz, nx, nts, sub, h, ntw, minPad;
+    int surface;           // 1: free surface at row SURF (no sponge towards the top)
     T cz[4], cx[4];
     T* A;                  // state level A, 5 fields of nz*nx (older level at even steps)
     T* B;                  // state level B
@@ -140,10 +141,48 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
     return x * q + (x < r ? x : r) + j;
 }
 
+// kernelsGpuElastic.cu:1192-1221; with a free surface 1223-1252 (no top side)
 template <typename T>
 __device__ __forceinline__ T damp_at(const ShotDev<T>& S, int iz, int ix) {
-    const int d = min(min(iz - FAT, ix - FAT), min(S.nz - iz - 1 - FAT, S.nx - ix - 1 - FAT));
+    const int top = S.surface ? S.nz : iz - FAT;
+    const int d = min(min(top, ix - FAT), min(S.nz - iz - 1 - FAT, S.nx - ix - 1 - FAT));
     return d < S.minPad ? S.damp[d] : T(1);
+}
+
+// ------------------------------------------------------------------ free surface
+// Surface row and the top-block map M of the current level (kernelsGpuElastic.cu:536-587) on
+// one column of an LDS stage whose row r is global row r: sigmazz = 0 on the surface and odd
+// above it, sigmaxz odd about the half row above the surface, vx = 0 above the surface, vz = 0
+// on and above it.  Its transpose clears the rows M overwrites and folds their negated values
+// into the mirrored rows.  Only the top tile (z0 = FAT) of a free-surface shot touches these rows.
+constexpr int SURF = 2 * FAT;
+
+template <typename T>
+__device__ __forceinline__ void fs_map_col(T* vx, T* vz, T* szz, T* sxz) {
+    szz[SURF] = T(0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) szz[SURF - 1 - k] = -szz[SURF + 1 + k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sxz[SURF - k] = -sxz[SURF + 1 + k];
+#pragma unroll
+    for (int r = SURF - 4; r < SURF; ++r) vx[r] = T(0);
+#pragma unroll
+    for (int r = SURF - 4; r <= SURF; ++r) vz[r] = T(0);
+}
+template <typename T>
+__device__ __forceinline__ void fs_map_col_transpose(T* vx, T* vz, T* szz, T* sxz) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) szz[SURF + 1 + k] -= szz[SURF - 1 - k];
+#pragma unroll
+    for (int r = SURF - 4; r <= SURF; ++r) szz[r] = T(0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sxz[SURF + 1 + k] -= sxz[SURF - k];
+#pragma unroll
+    for (int r = SURF - 3; r <= SURF; ++r) sxz[r] = T(0);
+#pragma unroll
+    for (int r = SURF - 4; r < SURF; ++r) vx[r] = T(0);
+#pragma unroll
+    for (int r = SURF - 4; r <= SURF; ++r) vz[r] = T(0);
 }
 
 // D-: sum_k C_k (q[m+k] - q[m-1-k]) ; D+: sum_k C_k (q[m+1+k] - q[m-k])
@@ -247,6 +286,29 @@ void k_step_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict_
     }
     __syncthreads();
 
+    // Free surface, top tile: the map M on every staged column (LDS row r is global row r
+    // there), and the mapped rows of the own columns written back, so that this level is
+    // M(cur) when it serves as the older level of the next step (the reference applies M to
+    // the level in memory).  Other tiles reading these rows as halo apply M themselves; M
+    // depends only on rows below the surface, which no launch writes concurrently.
+    const bool fs_top = S.surface && t.z0 == FAT;
+    if (fs_top) {  // uniform per tile
+        for (int c = threadIdx.x; c < TXH; c += NT)
+            fs_map_col<T>(&lds[0][c][0], &lds[1][c][0], &lds[3][c][0], &lds[4][c][0]);
+        __syncthreads();
+        const rsrc_t rcw = make_rsrc(curp, 5 * fbytes);
+        for (int e = threadIdx.x; e < TX * (FAT + 1); e += NT) {
+            const int cc = e / (FAT + 1), rr = SURF - FAT + (e - cc * (FAT + 1));
+            if (t.x0 + cc < nx - FAT) {
+                const unsigned vo = (unsigned(t.x0 + cc) * unsigned(nz) + unsigned(rr)) * SZ;
+                bst(rcw, lds[0][FAT + cc][rr], vo, 0u);
+                bst(rcw, lds[1][FAT + cc][rr], vo, fbytes);
+                bst(rcw, lds[3][FAT + cc][rr], vo, 3 * fbytes);
+                bst(rcw, lds[4][FAT + cc][rr], vo, 4 * fbytes);
+            }
+        }
+    }
+
 #if EIO_PREFETCH < 1
 #pragma unroll
     for (int r = 0; r < R; ++r)
@@ -294,11 +356,22 @@ void k_step_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict_
         const T dz_vz = dp_q(&lds[1][c][zz], 0, cz);
         const T dz_vx = dm_q(&lds[0][c][zz], 0, cz);
         const T d = ok ? damp_at(S, iz, ix) : T(1);
-        T a0 = d * o[0][r] + m[0][r] * (dx_sxx[r] + dz_sxz);
-        T a1 = d * o[1][r] + m[1][r] * (dz_szz + dx_sxz[r]);
-        T a2 = d * o[2][r] + (m[2][r] * dx_vx[r] + m[3][r] * dz_vz);
-        T a3 = d * o[3][r] + (m[3][r] * dx_vx[r] + m[2][r] * dz_vz);
-        T a4 = d * o[4][r] + m[4][r] * (dz_vx + dx_vz[r]);
+        T a0, a1, a2, a3, a4;
+        if (!fs_top) {
+            a0 = d * o[0][r] + m[0][r] * (dx_sxx[r] + dz_sxz);
+            a1 = d * o[1][r] + m[1][r] * (dz_szz + dx_sxz[r]);
+            a2 = d * o[2][r] + (m[2][r] * dx_vx[r] + m[3][r] * dz_vz);
+            a3 = d * o[3][r] + (m[3][r] * dx_vx[r] + m[2][r] * dz_vz);
+            a4 = d * o[4][r] + m[4][r] * (dz_vx + dx_vz[r]);
+        } else {  // kernelsGpuElastic.cu:589-776: updated rows per component, surface modulus
+            const bool on = iz == SURF, below = iz > SURF;
+            a0 = d * o[0][r] + ((on || below) ? m[0][r] * (dx_sxx[r] + dz_sxz) : T(0));
+            a1 = d * o[1][r] + (below ? m[1][r] * (dz_szz + dx_sxz[r]) : T(0));
+            a2 = d * o[2][r] + (below ? (m[2][r] * dx_vx[r] + m[3][r] * dz_vz)
+                                      : (on ? (m[2][r] - m[3][r] * m[3][r] / m[2][r]) * dx_vx[r] : T(0)));
+            a3 = d * o[3][r] + (below ? (m[3][r] * dx_vx[r] + m[2][r] * dz_vz) : T(0));
+            a4 = d * o[4][r] + (below ? m[4][r] * (dz_vx + dx_vz[r]) : T(0));
+        }
         if (MODE & F_SS) {  // secondary source sampled at it2-1 (kernelsGpuElastic.cu:1363-1368)
             const T fl = S.filt[st.it2 - 1], fr = S.filt[S.h + st.it2 - 1];
             const rsrc_t rl = make_rsrc(S.ss[sspar], 5 * fbytes);
@@ -465,16 +538,32 @@ void k_step_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict_
 #pragma unroll
             for (int f = 0; f < 5; ++f) hm[it][f] = bld<T>(rmed, hv, unsigned(f) * fbytes);
         }
+        const bool fsm = S.surface && t.z0 == FAT;  // top tile of a free-surface shot
 #pragma unroll
         for (int it = 0; it < NFILL; ++it) {
             const int e = threadIdx.x + it * NT;
             if (e < TXH * ZH) {
                 const int c = e / ZH, z = e - c * ZH;
-                lds[0][c][z] = hm[it][0] * hv_[it][0];
-                lds[1][c][z] = hm[it][1] * hv_[it][1];
-                lds[2][c][z] = hm[it][2] * hv_[it][2] + hm[it][3] * hv_[it][3];
-                lds[3][c][z] = hm[it][3] * hv_[it][2] + hm[it][2] * hv_[it][3];
-                lds[4][c][z] = hm[it][4] * hv_[it][4];
+                if (!fsm) {
+                    lds[0][c][z] = hm[it][0] * hv_[it][0];
+                    lds[1][c][z] = hm[it][1] * hv_[it][1];
+                    lds[2][c][z] = hm[it][2] * hv_[it][2] + hm[it][3] * hv_[it][3];
+                    lds[3][c][z] = hm[it][3] * hv_[it][2] + hm[it][2] * hv_[it][3];
+                    lds[4][c][z] = hm[it][4] * hv_[it][4];
+                } else {
+                    // the transpose of the free-surface body: each product only where the
+                    // forward updates that component (LDS row z is global row z here), and the
+                    // surface row's sigmaxx with its modulus
+                    const bool on = z == SURF, below = z > SURF;
+                    lds[0][c][z] = (on || below) ? hm[it][0] * hv_[it][0] : T(0);
+                    lds[1][c][z] = below ? hm[it][1] * hv_[it][1] : T(0);
+                    lds[2][c][z] = below ? hm[it][2] * hv_[it][2] + hm[it][3] * hv_[it][3]
+                                         : ((on && hm[it][2] != T(0))
+                                                ? (hm[it][2] - hm[it][3] * hm[it][3] / hm[it][2]) * hv_[it][2]
+                                                : T(0));
+                    lds[3][c][z] = below ? hm[it][3] * hv_[it][2] + hm[it][2] * hv_[it][3] : T(0);
+                    lds[4][c][z] = below ? hm[it][4] * hv_[it][4] : T(0);
+                }
             }
         }
     }
@@ -526,6 +615,25 @@ void k_step_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict_
         val[2][r] = d * (d * o[2][r] - dx_s0[r]);
         val[3][r] = d * (d * o[3][r] - dz_s1);
         val[4][r] = d * (d * o[4][r] - (dz_s0 + dx_s1[r]));
+    }
+
+    // Free surface, top tile: M^T on the new adjoint state, before the data injection (the
+    // staged rows of the own columns are global rows here)
+    if (S.surface && t.z0 == FAT) {  // uniform per tile
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int f = 0; f < 5; ++f) lds[f][c0 + r][zz] = val[f][r];
+        __syncthreads();
+        for (int c = threadIdx.x; c < TX; c += NT)
+            fs_map_col_transpose<T>(&lds[0][FAT + c][0], &lds[1][FAT + c][0], &lds[3][FAT + c][0],
+                                    &lds[4][FAT + c][0]);
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int f = 0; f < 5; ++f) val[f][r] = lds[f][c0 + r][zz];
     }
 
     // adjoint: inject = receiver (data) entries, record = source entries

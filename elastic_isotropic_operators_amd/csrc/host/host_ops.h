@@ -158,8 +158,8 @@ struct FdParam {
         saveWavefield = par.getInt("saveWavefield", 0);
         alphaCos = par.getFloat("alphaCos", 0.99);
         errorTolerance = par.getFloat("errorTolerance", 0.000001);
-        if (surfaceCondition != 0)
-            throw std::runtime_error("surfaceCondition=1 (free surface) is not available in this build");
+        if (surfaceCondition != 0 && surfaceCondition != 1)
+            throw std::runtime_error("ERROR NO IMPROPER FREE SURFACE PARAM PROVIDED");
         if (fat != FAT) throw std::runtime_error("fat must be 4");
         // consistency with the elastic parameter hypercube (fdParamElastic.cpp:296-324)
         const double scale = (mod_par == 2) ? 1000.0 : 1.0;
@@ -583,6 +583,7 @@ class NonlinearShots {
         eio_grid g;
         g.nz = fd_->nz; g.nx = fd_->nx; g.nts = fd_->nts; g.sub_fd = fd_->sub; g.minPad = fd_->minPad;
         g.dz = fd_->dz; g.dx = fd_->dx; g.dts = fd_->dts; g.alphaCos = fd_->alphaCos;
+        g.surface = fd_->surfaceCondition;
         const size_t n = fd_->ncell();
         const void* media[5] = {fd_->media.data(), fd_->media.data() + n, fd_->media.data() + 2 * n,
                                 fd_->media.data() + 3 * n, fd_->media.data() + 4 * n};
@@ -618,6 +619,8 @@ class BornShots {
           rec_(std::move(receivers)) {
         nExp_ = par_.getInt("nExp");
         fd_ = std::make_unique<FdParam<T>>(elastic, ax_, par_);
+        if (fd_->surfaceCondition != 0)
+            throw std::runtime_error("surfaceCondition=1 (free surface) is available for the nonlinear operator only");
         if (int(src_.size()) < nExp_) throw std::runtime_error("fewer source geometries than nExp");
         if (rec_.size() != 1 && int(rec_.size()) < nExp_) throw std::runtime_error("receiver geometries must be 1 or nExp");
         if (sig_.empty() || (sig_.size() != 1 && int(sig_.size()) < nExp_))
@@ -744,6 +747,7 @@ class BornShots {
         eio_grid g;
         g.nz = fd_->nz; g.nx = fd_->nx; g.nts = fd_->nts; g.sub_fd = fd_->sub; g.minPad = fd_->minPad;
         g.dz = fd_->dz; g.dx = fd_->dx; g.dts = fd_->dts; g.alphaCos = fd_->alphaCos;
+        g.surface = fd_->surfaceCondition;
         const size_t n = fd_->ncell();
         const void* media[5] = {fd_->media.data(), fd_->media.data() + n, fd_->media.data() + 2 * n,
                                 fd_->media.data() + 3 * n, fd_->media.data() + 4 * n};

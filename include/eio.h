@@ -46,6 +46,9 @@ typedef struct {
     int minPad;          /* sponge width, min over the four sides */
     double dz, dx, dts;  /* spatial sampling [m], coarse time sampling [s] */
     double alphaCos;     /* sponge coefficient, 0.99 by default */
+    int surface;         /* 0: sponge on all four sides; 1: free surface at row 2*FAT and no
+                          * sponge towards the top (surfaceCondition, fdParamElastic.cpp:18;
+                          * NL forward and adjoint only) */
 } eio_grid;
 
 /* One shot: regular source/receiver positions per grid and trace pointers per component.

@@ -29,6 +29,10 @@
 //                                                  include/seismicElasticOperator2D.cpp:27-119
 //   shot stacking and Born model staggering      : src/BornElasticShotsGpu.cpp:85-378,
 //                                                  src/nonlinearPropElasticShotsGpu.cpp:80-422
+//   free surface (surface = 1), forward          : src/kernelsGpuElastic.cu:536-776 (top and body),
+//                                                  1223-1252 (damping), src/nonlinearPropElasticGpuFunctions.cu:603-614
+//   free surface, adjoint: the exact transpose of that forward step (derived below; checked by
+//   the FP64 dot test), in the reference's adjoint loop order
 //
 // Layout conventions (same as the reference): every 2-D field is [nx][nz] with
 // z fastest (index ix*nz + iz); traces are [nTrace][nts] with time fastest;
@@ -56,11 +60,12 @@ struct Grid {
     int minPad;
     double dz, dx, dts, alphaCos;
     double oz, ox;         // origin of the centre grid (used by the spatial interpolation)
+    int surface;           // 1: free surface at row 2*FAT (fdParamElastic.cpp:18)
 };
 
 template <typename T>
 struct Consts {
-    int nz, nx, nts, sub, ntw, h, minPad;
+    int nz, nx, nts, sub, ntw, h, minPad, surface;
     T cz[4], cx[4];
     std::vector<T> F;     // 2*(sub+1) time-interpolation filter
     std::vector<T> damp;  // minPad cosine damping coefficients
@@ -74,6 +79,7 @@ Consts<T> make_consts(const Grid& g) {
     c.ntw = (g.nts - 1) * g.sub + 1;
     c.h = g.sub + 1;
     c.minPad = g.minPad;
+    c.surface = g.surface;
     for (int k = 0; k < 4; ++k) { c.cz[k] = T(FD_C[k] / g.dz); c.cx[k] = T(FD_C[k] / g.dx); }
     // nonlinearPropElasticGpuFunctions.cu:172-188
     c.F.assign(2 * c.h, T(0));
@@ -116,6 +122,77 @@ template <typename T>
 inline T dplus(const T* f, long i, long s, const T* c) {
     return c[0] * (f[i + s] - f[i]) + c[1] * (f[i + 2 * s] - f[i - s]) +
            c[2] * (f[i + 3 * s] - f[i - 2 * s]) + c[3] * (f[i + 4 * s] - f[i - 3 * s]);
+}
+
+// ---------------------------------------------------------------- free surface
+// Surface row and the top-block map M of the current level (kernelsGpuElastic.cu:536-587),
+// column by column: sigmazz = 0 on the surface and odd above it, sigmaxz odd about the half
+// row above the surface (rows s-3..s), vx = 0 above the surface, vz = 0 on and above it.
+const int SURF = 2 * FAT;
+
+template <typename T>
+void fs_top_map(const Consts<T>& c, Fields<T>& p1) {
+    const long nz = c.nz;
+    for (int ix = FAT; ix < c.nx - FAT; ++ix) {
+        const long b = long(ix) * nz;
+        T* szz = p1.f[3].data() + b;
+        T* sxz = p1.f[4].data() + b;
+        T* vx = p1.f[0].data() + b;
+        T* vz = p1.f[1].data() + b;
+        szz[SURF] = 0;
+        for (int k = 0; k < 4; ++k) szz[SURF - 1 - k] = -szz[SURF + 1 + k];
+        for (int k = 0; k < 4; ++k) sxz[SURF - k] = -sxz[SURF + 1 + k];
+        for (int r = SURF - 4; r < SURF; ++r) vx[r] = 0;
+        for (int r = SURF - 4; r <= SURF; ++r) vz[r] = 0;
+    }
+}
+
+// Transpose of fs_top_map (rows that M overwrites receive nothing; the mirrored source rows
+// collect the negated values of their images).
+template <typename T>
+void fs_top_map_transpose(const Consts<T>& c, Fields<T>& p) {
+    const long nz = c.nz;
+    for (int ix = FAT; ix < c.nx - FAT; ++ix) {
+        const long b = long(ix) * nz;
+        T* szz = p.f[3].data() + b;
+        T* sxz = p.f[4].data() + b;
+        T* vx = p.f[0].data() + b;
+        T* vz = p.f[1].data() + b;
+        for (int k = 0; k < 4; ++k) szz[SURF + 1 + k] -= szz[SURF - 1 - k];
+        for (int r = SURF - 4; r <= SURF; ++r) szz[r] = 0;
+        for (int k = 0; k < 4; ++k) sxz[SURF + 1 + k] -= sxz[SURF - k];
+        for (int r = SURF - 3; r <= SURF; ++r) sxz[r] = 0;
+        for (int r = SURF - 4; r < SURF; ++r) vx[r] = 0;
+        for (int r = SURF - 4; r <= SURF; ++r) vz[r] = 0;
+    }
+}
+
+// Free-surface body (kernelsGpuElastic.cu:589-776): vx updated on and below the surface, vz,
+// sigmazz, sigmaxz below it, sigmaxx below it and on it with the modulus
+// lamb2MuDtw - lambDtw^2/lamb2MuDtw and no z derivative.
+template <typename T>
+void step_fwd_fs(const Consts<T>& c, const Media<T>& m, Fields<T>& p0, Fields<T>& p1) {
+    fs_top_map(c, p1);
+    const long nz = c.nz;
+    const T *vx = p1.f[0].data(), *vz = p1.f[1].data(), *sxx = p1.f[2].data(),
+            *szz = p1.f[3].data(), *sxz = p1.f[4].data();
+#pragma omp parallel for schedule(static)
+    for (int ix = FAT; ix < c.nx - FAT; ++ix) {
+        for (int iz = SURF; iz < c.nz - FAT; ++iz) {
+            const long i = long(ix) * nz + iz;
+            const T dxp_vx = dplus(vx, i, nz, c.cx);
+            p0.f[0][i] += m.rhox[i] * (dminus(sxx, i, nz, c.cx) + dplus(sxz, i, 1, c.cz));
+            if (iz == SURF) {
+                p0.f[2][i] += (m.l2m[i] - m.lam[i] * m.lam[i] / m.l2m[i]) * dxp_vx;
+                continue;
+            }
+            const T dzp_vz = dplus(vz, i, 1, c.cz);
+            p0.f[1][i] += m.rhoz[i] * (dminus(szz, i, 1, c.cz) + dplus(sxz, i, nz, c.cx));
+            p0.f[2][i] += m.l2m[i] * dxp_vx + m.lam[i] * dzp_vz;
+            p0.f[3][i] += m.lam[i] * dxp_vx + m.l2m[i] * dzp_vz;
+            p0.f[4][i] += m.muxz[i] * (dminus(vx, i, 1, c.cz) + dminus(vz, i, nz, c.cx));
+        }
+    }
 }
 
 // kernelsGpuElastic.cu:298-376: p0 += A(p1), in place, update region only.
@@ -172,14 +249,58 @@ void step_adj(const Consts<T>& c, const Media<T>& m, Fields<T>& p0, const Fields
     }
 }
 
-// kernelsGpuElastic.cu:1192-1221: both time levels, update region only.
+// Transpose of the free-surface step: p0 += B^T p1 (the body operator B restricted to the
+// rows the forward updates, with the surface row's sigmaxx modulus), then p0 <- M^T p0.
+// With L the damped adjoint state, L_k = D M^T (B^T L_k+1 + D L_k+2) + D R^T d_k, which in
+// the reference's adjoint loop (step, inject, damp both, extract, swap) is this step
+// followed by the usual injection and damping; M^T commutes with D on the top rows (their
+// damping depends on x only).
+template <typename T>
+void step_adj_fs(const Consts<T>& c, const Media<T>& m, Fields<T>& p0, const Fields<T>& p1,
+                 std::vector<T> (&scr)[7]) {
+    const long nz = c.nz, n = long(c.nz) * c.nx;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) {
+        const int iz = int(i % nz);
+        const bool below = iz > SURF, on = iz == SURF;
+        scr[0][i] = (below || on) ? m.rhox[i] * p1.f[0][i] : T(0);
+        scr[1][i] = below ? m.rhoz[i] * p1.f[1][i] : T(0);
+        scr[2][i] = below ? m.l2m[i] * p1.f[2][i]
+                          : (on ? (m.l2m[i] - m.lam[i] * m.lam[i] / m.l2m[i]) * p1.f[2][i] : T(0));
+        scr[3][i] = below ? m.lam[i] * p1.f[2][i] : T(0);
+        scr[4][i] = below ? m.l2m[i] * p1.f[3][i] : T(0);
+        scr[5][i] = below ? m.lam[i] * p1.f[3][i] : T(0);
+        scr[6][i] = below ? m.muxz[i] * p1.f[4][i] : T(0);
+    }
+    const T *vxr = scr[0].data(), *vzr = scr[1].data(), *sxx_l2m = scr[2].data(),
+            *sxx_lam = scr[3].data(), *szz_l2m = scr[4].data(), *szz_lam = scr[5].data(),
+            *sxz_mu = scr[6].data();
+#pragma omp parallel for schedule(static)
+    for (int ix = FAT; ix < c.nx - FAT; ++ix) {
+        for (int iz = FAT; iz < c.nz - FAT; ++iz) {
+            const long i = long(ix) * nz + iz;
+            p0.f[0][i] -= dminus(sxx_l2m, i, nz, c.cx) + dminus(szz_lam, i, nz, c.cx) +
+                          dplus(sxz_mu, i, 1, c.cz);
+            p0.f[1][i] -= dminus(sxx_lam, i, 1, c.cz) + dminus(szz_l2m, i, 1, c.cz) +
+                          dplus(sxz_mu, i, nz, c.cx);
+            p0.f[2][i] -= dplus(vxr, i, nz, c.cx);
+            p0.f[3][i] -= dplus(vzr, i, 1, c.cz);
+            p0.f[4][i] -= dminus(vxr, i, 1, c.cz) + dminus(vzr, i, nz, c.cx);
+        }
+    }
+    fs_top_map_transpose(c, p0);
+}
+
+// kernelsGpuElastic.cu:1192-1221 (1223-1252 with a free surface: no damping towards the top):
+// both time levels, update region only.
 template <typename T>
 void damp_both(const Consts<T>& c, Fields<T>& p0, Fields<T>& p1) {
     const long nz = c.nz;
 #pragma omp parallel for schedule(static)
     for (int ix = FAT; ix < c.nx - FAT; ++ix) {
         for (int iz = FAT; iz < c.nz - FAT; ++iz) {
-            const int d = std::min(std::min(iz - FAT, ix - FAT),
+            const int top = c.surface ? c.nz : iz - FAT;
+            const int d = std::min(std::min(top, ix - FAT),
                                    std::min(c.nz - iz - 1 - FAT, c.nx - ix - 1 - FAT));
             if (d < c.minPad) {
                 const T w = c.damp[d];
@@ -230,7 +351,8 @@ void nl_fwd_reg(const Consts<T>& c, const Media<T>& m, const RegGeom& src, const
     }
     for (int its = 0; its < c.nts - 1; ++its) {
         for (int it2 = 1; it2 < c.sub + 1; ++it2) {
-            step_fwd(c, m, *p0, *p1);
+            if (c.surface) step_fwd_fs(c, m, *p0, *p1);
+            else step_fwd(c, m, *p0, *p1);
             for (int k = 0; k < 5; ++k) {  // kernelsGpuElastic.cu:172-189, it2-1 lag
                 const int gi = COMP_GRID[k];
                 for (int is = 0; is < src.n[gi]; ++is)
@@ -268,7 +390,8 @@ void nl_adj_reg(const Consts<T>& c, const Media<T>& m, const RegGeom& src, T* co
     }
     for (int its = c.nts - 2; its > -1; --its) {
         for (int it2 = c.sub - 1; it2 > -1; --it2) {
-            step_adj(c, m, *p0, *p1, scr);
+            if (c.surface) step_adj_fs(c, m, *p0, *p1, scr);
+            else step_adj(c, m, *p0, *p1, scr);
             for (int k = 0; k < 5; ++k) {  // kernelsGpuElastic.cu:191-229, it2+1
                 const int gi = COMP_GRID[k];
                 for (int ir = 0; ir < rec.n[gi]; ++ir)
@@ -843,12 +966,14 @@ extern "C" {
 struct OracleGrid {
     int nz, nx, nts, sub, minPad;
     double dz, dx, dts, alphaCos, oz, ox;
+    int surface;
 };
 
 static Grid to_grid(const OracleGrid* g) {
     Grid r;
     r.nz = g->nz; r.nx = g->nx; r.nts = g->nts; r.sub = g->sub; r.minPad = g->minPad;
     r.dz = g->dz; r.dx = g->dx; r.dts = g->dts; r.alphaCos = g->alphaCos; r.oz = g->oz; r.ox = g->ox;
+    r.surface = g->surface;
     return r;
 }
 

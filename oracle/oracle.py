@@ -25,7 +25,8 @@ class OracleGrid(ctypes.Structure):
     _fields_ = [("nz", ctypes.c_int), ("nx", ctypes.c_int), ("nts", ctypes.c_int),
                 ("sub", ctypes.c_int), ("minPad", ctypes.c_int),
                 ("dz", ctypes.c_double), ("dx", ctypes.c_double), ("dts", ctypes.c_double),
-                ("alphaCos", ctypes.c_double), ("oz", ctypes.c_double), ("ox", ctypes.c_double)]
+                ("alphaCos", ctypes.c_double), ("oz", ctypes.c_double), ("ox", ctypes.c_double),
+                ("surface", ctypes.c_int)]
 
 
 def build():
@@ -45,10 +46,11 @@ def lib():
     return _lib
 
 
-def grid(nz, nx, nts, sub_fd, dz, dx, dts, minPad, alphaCos=0.99, oz=0.0, ox=0.0):
-    """Grid description; ``sub_fd`` is the fine sub-sampling (twice the user ``sub``)."""
+def grid(nz, nx, nts, sub_fd, dz, dx, dts, minPad, alphaCos=0.99, oz=0.0, ox=0.0, surface=0):
+    """Grid description; ``sub_fd`` is the fine sub-sampling (twice the user ``sub``);
+    surface=1: free surface at row 2*FAT."""
     return OracleGrid(int(nz), int(nx), int(nts), int(sub_fd), int(minPad), float(dz), float(dx),
-                      float(dts), float(alphaCos), float(oz), float(ox))
+                      float(dts), float(alphaCos), float(oz), float(ox), int(surface))
 
 
 def _suffix(dtype):

@@ -178,3 +178,31 @@ def test_forward_and_adjoint_wavefield(tmp_path):
     for k, gi in enumerate(G.GRID_OF_COMP):
         reg = wa[:, k].reshape(w["nts"], -1)[:, si[gi].uniq].T
         assert rel_l2(si[gi].forward(reg), m1.getNdArray()[0, k]) < 1e-5
+
+
+def test_free_surface_operator_vs_oracle(tmp_path):
+    """surfaceCondition=1 through the parfile: the padding rule puts the free surface on the
+    first interior row, the NL operator matches the oracle's shot-level free-surface forward,
+    and the FP64 operator passes the dot test; the Born operator refuses it."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(5)
+    nz, nx, nts = 40, 56, 120
+    vp = gaussian_filter(2200 + 400 * rng.random((nx, nz)), 3)
+    rho = 1800 + 300 * gaussian_filter(rng.random((nx, nz)), 3)
+    w = W._grid_dict(np.stack([vp, vp / 1.8, rho]).astype(np.float32), 10.0, 10.0, 10, 10, nts, 0.002, 2, 20.0,
+                     surfaceCondition=1)
+    w["model"] = W.explosive_model(nts, w["dts"], 20.0, 0.05)
+    args, geo = operator_case(tmp_path, w, nExp=2, zSource=1, depthRec=1, extra=dict(surfaceCondition=1))
+    op, model, data, elastic = _nl_op(PF, args, PF.nonlinearOpInitFloat)
+    model.getNdArray()[0] = w["model"]
+    op.forward(False, model, data)
+    g = O.grid(w["nz"], w["nx"], w["nts"], 2 * w["sub"], w["dz"], w["dx"], w["dts"], W.min_pad(w), w["alphaCos"],
+               w["oz"], w["ox"], surface=1)
+    for e in range(2):
+        ref = O.nl_fwd_shot(g, w["elastic"], geo["sz"][e], geo["sx"][e], geo["rz"], geo["rx"], w["model"])
+        assert rel_l2(data.getNdArray()[e], ref) < 1e-5, (e, rel_l2(data.getNdArray()[e], ref))
+    op64, *_ = _nl_op(PD, args, PD.nonlinearOpInitDouble)
+    assert op64.dotTestCpp(False, 1e-12)
+    (m, d, el, par, sig, sc, sx, sz, sxz, rc, rx, rz, rxz, _) = PF.BornOpInitFloat(args)
+    with pytest.raises(RuntimeError, match="nonlinear operator only"):
+        PF.BornElasticShotsGpu(m, d, el, par, sig, sc, sx, sz, sxz, rc, rx, rz, rxz)

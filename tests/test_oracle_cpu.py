@@ -314,3 +314,79 @@ def test_saved_wavefield_is_what_the_receivers_see():
     for k, gi in enumerate(O.GRID_OF_COMP):
         samp = wa[:, k].reshape(g.nts, -1)[:, sp[gi]].T
         assert np.abs(samp - mod[k]).max() <= 1e-12 * np.abs(mod[k]).max()
+
+
+def _fs_grid(nz=40, nx=56, nts=80, seed=0):
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    vp = gaussian_filter(2200 + 400 * rng.random((nx, nz)), 3)
+    rho = 1800 + 300 * gaussian_filter(rng.random((nx, nz)), 3)
+    w = W._grid_dict(np.stack([vp, vp / 1.8, rho]).astype(np.float32), 10.0, 10.0, 10, 10, nts, 0.002, 1, 20.0,
+                     surfaceCondition=1)
+    g = O.grid(w["nz"], w["nx"], w["nts"], 2, w["dz"], w["dx"], w["dts"], W.min_pad(w), w["alphaCos"], w["oz"],
+               w["ox"], surface=1)
+    return w, g
+
+
+def _scattered(rng, g, count, zmin):
+    out = []
+    for _ in range(4):
+        iz = rng.integers(zmin, g.nz - 4, size=4 * count)
+        ix = rng.integers(4, g.nx - 4, size=4 * count)
+        out.append(np.unique(ix * g.nz + iz)[:count].astype(np.int32))
+    return out
+
+
+@pytest.mark.parametrize("zmin", [8, 4])
+def test_free_surface_dot_product_fp64(zmin):
+    """The free-surface NL adjoint is the exact transpose of the free-surface forward
+    (kernelsGpuElastic.cu:536-776, 1223-1252), with devices below the surface (zmin=8) and
+    anywhere in the update region including the rows above it (zmin=4)."""
+    w, g = _fs_grid()
+    assert (w["zPadMinus"], g.surface) == (4, 1)
+    med = O.media(g, w["elastic"])
+    rng = np.random.default_rng(7)
+    sp, rp = _scattered(rng, g, 10, zmin), _scattered(rng, g, 15, zmin)
+    m = [rng.standard_normal((sp[O.GRID_OF_COMP[k]].size, g.nts)) for k in range(5)]
+    d = [rng.standard_normal((rp[O.GRID_OF_COMP[k]].size, g.nts)) for k in range(5)]
+    Fm = O.nl_fwd_reg(g, med, sp, m, rp)
+    Ftd = O.nl_adj_reg(g, med, sp, rp, d)
+    a = sum(np.sum(x * y) for x, y in zip(Fm, d))
+    b = sum(np.sum(x * y) for x, y in zip(m, Ftd))
+    assert b != 0 and abs(a - b) / abs(b) < 1e-12, (a, b)
+
+
+def test_free_surface_traction_free_and_rayleigh_velocity():
+    """A free surface is traction free (sigmazz = 0 on the surface row, vz = 0 on and above it
+    at every saved sample) and carries a Rayleigh wave: in a homogeneous Poisson solid
+    (vp = sqrt(3) vs) a vertical surface force's vz arrival travels along the surface at
+    0.9194 vs (within 3 %)."""
+    w, g = _fs_grid(nts=60)
+    med = O.media(g, w["elastic"])
+    rng = np.random.default_rng(8)
+    sp, rp = _scattered(rng, g, 5, 8), _scattered(rng, g, 5, 8)
+    src = [rng.standard_normal((sp[O.GRID_OF_COMP[k]].size, g.nts)) for k in range(5)]
+    _, wf = O.nl_fwd_reg(g, med, sp, src, rp, wavefield=True)
+    assert np.abs(wf[:, 3, 4:-4, 8]).max() == 0 and np.abs(wf[:, 1, 4:-4, 4:9]).max() == 0
+    assert np.abs(wf[:, 3, 4:-4, 9]).max() > 0
+    vs, nz, nx = 2000.0, 60, 300
+    vvr = np.stack([np.full((nx, nz), vs * np.sqrt(3.0)), np.full((nx, nz), vs), np.full((nx, nz), 2000.0)])
+    w = W._grid_dict(vvr.astype(np.float32), 10.0, 10.0, 30, 30, 700, 0.001, 1, 25.0, surfaceCondition=1)
+    g = O.grid(w["nz"], w["nx"], w["nts"], 2, w["dz"], w["dx"], w["dts"], W.min_pad(w), w["alphaCos"], w["oz"],
+               w["ox"], surface=1)
+    sz, sx = W.source_coords(w, 1, 60)
+    rz, rx = W.receiver_coords(w, 2, 1, 100, 60)   # 400 m and 1000 m from the source
+    model = np.zeros((5, 1, w["nts"]), np.float32)
+    model[1, 0] = su.ricker(w["nts"], w["dts"], 15.0, 0.08)
+    vz = O.nl_fwd_shot(g, w["elastic"], sz[0], sx[0], rz, rx, model)[1]
+
+    def window(tr):
+        i = int(np.argmax(np.abs(tr)))
+        out = np.zeros_like(tr)
+        out[max(0, i - 100):i + 100] = tr[max(0, i - 100):i + 100]
+        return out
+
+    a, b = window(vz[0]), window(vz[1])
+    lag = int(np.argmax(np.correlate(b, a, mode="full"))) - (len(a) - 1)
+    v = 600.0 / (lag * w["dts"])
+    assert abs(v / (0.9194 * vs) - 1.0) < 0.03, v
