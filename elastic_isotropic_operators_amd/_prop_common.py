@@ -13,6 +13,7 @@ import numpy as np
 
 from .sep import Hypercube, SepVector, genericIO, pyOperator as Op
 from . import elasticParamConvertModule as ElaConv
+from . import geometry as G
 
 
 def _storage(real):
@@ -61,36 +62,16 @@ def make(real):
         method = parObject.getString("sourceInterpMethod", "linear")
         nfilt = parObject.getInt("sourceInterpNumFilters", 4)
         lists = [[], [], [], []]
-        sourceGeomFile = parObject.getString("sourceGeomFile", "None")
-        if sourceGeomFile != "None":
-            geom = genericIO.defaultIO.getVector(sourceGeomFile, ndims=3).getNdArray()
-            nExpFile, nSimSou = geom.shape[2], geom.shape[1]
-            if nExp != nExpFile:
-                raise ValueError("ERROR! nExp (%d) not consistent with number of shots provided within "
-                                 "sourceGeomFile (%d)" % (nExp, nExpFile))
-            sourceAxis = Hypercube.axis(n=nExp, o=1.0, d=1.0)
-            for iExp in range(nExp):
-                zc = SepVector.getSepVector(ns=[nSimSou])
-                xc = SepVector.getSepVector(ns=[nSimSou])
-                zc.getNdArray()[:] = geom[2, :, iExp]
-                xc.getNdArray()[:] = geom[0, :, iExp]
-                for l, s in zip(lists, _interp4(zc, xc, hypers, nts, method, nfilt, dipole,
-                                               zDipoleShift, xDipoleShift)):
-                    l.append(s)
-        else:
-            ozSource = parObject.getInt("zSource") - 1 + parObject.getInt("zPadMinus") + parObject.getInt("fat", 4)
-            oxSource = parObject.getInt("xSource") - 1 + parObject.getInt("xPadMinus") + parObject.getInt("fat", 4)
-            spacingShots = parObject.getInt("spacingShots")
-            sourceAxis = Hypercube.axis(n=nExp, o=ox + oxSource * dx, d=spacingShots * dx)
-            for iExp in range(nExp):
-                zc = SepVector.getSepVector(ns=[1])
-                xc = SepVector.getSepVector(ns=[1])
-                zc.set(oz + ozSource * dz)
-                xc.set(ox + oxSource * dx)
-                for l, s in zip(lists, _interp4(zc, xc, hypers, nts, method, nfilt, dipole,
-                                               zDipoleShift, xDipoleShift)):
-                    l.append(s)
-                oxSource = oxSource + spacingShots
+        coords, (n, o, d) = G.parfile_source_coords(parObject, oz, dz, ox, dx, genericIO.defaultIO.getVector)
+        sourceAxis = Hypercube.axis(n=n, o=o, d=d)
+        for zarr, xarr in coords:
+            zc = SepVector.getSepVector(ns=[len(zarr)])
+            xc = SepVector.getSepVector(ns=[len(xarr)])
+            zc.getNdArray()[:] = zarr
+            xc.getNdArray()[:] = xarr
+            for l, s in zip(lists, _interp4(zc, xc, hypers, nts, method, nfilt, dipole,
+                                           zDipoleShift, xDipoleShift)):
+                l.append(s)
         return lists[0], lists[1], lists[2], lists[3], sourceAxis
 
     def buildReceiversGeometry(parObject, elasticParam):
@@ -104,30 +85,13 @@ def make(real):
         hypers = _grid_hypers(elasticParam)
         method = parObject.getString("recInterpMethod", "linear")
         nfilt = parObject.getInt("recInterpNumFilters", 4)
-        recGeomFile = parObject.getString("recGeomFile", "none")
-        if recGeomFile != "none":
-            geom = genericIO.defaultIO.getVector(recGeomFile, ndims=3).getNdArray()
-            xCoord, zCoord = geom[0, :, 0], geom[2, :, 0]
-            nRec = parObject.getInt("nReceiver")
-            if nRec != len(xCoord):
-                raise ValueError("ERROR [buildReceiversGeometry]: Number of receiver coordinates (%s) not "
-                                 "consistent with parameter file (nReceiver=%d)" % (len(xCoord), nRec))
-            receiverAxis = Hypercube.axis(n=nRec, o=1.0, d=1.0)
-            zc = SepVector.getSepVector(ns=[len(zCoord)])
-            xc = SepVector.getSepVector(ns=[len(xCoord)])
-            zc.getNdArray()[:] = zCoord
-            xc.getNdArray()[:] = xCoord
-        else:
-            ozR = parObject.getInt("depthReceiver") - 1 + parObject.getInt("zPadMinus") + parObject.getInt("fat", 4)
-            nxR = parObject.getInt("nReceiver")
-            oxR = parObject.getInt("oReceiver") - 1 + parObject.getInt("xPadMinus") + parObject.getInt("fat", 4)
-            dxR = parObject.getInt("dReceiver")
-            receiverAxis = Hypercube.axis(n=nxR, o=ox + oxR * dx, d=dxR * dx)
-            zc = SepVector.getSepVector(ns=[nxR])
-            xc = SepVector.getSepVector(ns=[nxR])
-            irec = np.arange(nxR)
-            zc.getNdArray()[:] = oz + ozR * dz + 0 * irec
-            xc.getNdArray()[:] = ox + oxR * dx + dxR * dx * irec
+        zarr, xarr, (n, o, d) = G.parfile_receiver_coords(parObject, oz, dz, ox, dx,
+                                                          genericIO.defaultIO.getVector)
+        receiverAxis = Hypercube.axis(n=n, o=o, d=d)
+        zc = SepVector.getSepVector(ns=[len(zarr)])
+        xc = SepVector.getSepVector(ns=[len(xarr)])
+        zc.getNdArray()[:] = zarr
+        xc.getNdArray()[:] = xarr
         grids = _interp4(zc, xc, hypers, nts, method, nfilt, dipole, zDipoleShift, xDipoleShift)
         return [grids[0]], [grids[1]], [grids[2]], [grids[3]], receiverAxis
 

@@ -361,7 +361,8 @@ def lbfgs(problem, m0, n_iter, memory=None, h0=None, init_step=None, step_frac=0
     tensor: the reference's illumination option, efwiMain.py:404-410) or the scaling
     s'y / y'y; pairs with s'y <= 0 are skipped.  The first step length is `init_step`
     (initStep) or the one moving the largest model entry by step_frac of the model's maximum;
-    later iterations try the unit step first."""
+    later iterations try the unit step first.  callback(iteration, model, phi, gradient,
+    residual) runs at the starting model (iteration 0) and after every iteration."""
     search = _line_search(stepper)
     t0 = time.time()
     res = SolverResult()
@@ -369,6 +370,8 @@ def lbfgs(problem, m0, n_iter, memory=None, h0=None, init_step=None, step_frac=0
     phi, g, _ = problem.evaluate(m)
     res.objective.append(phi)
     phi0, g0norm = phi, math.sqrt(_dot(g, g))
+    if callback is not None:
+        callback(0, m, phi, g, problem.last_residual)
     pairs = []
     it = 0
     while not _stop(res, it, n_iter, phi, phi0, math.sqrt(_dot(g, g)), g0norm, tol_obj, tol_grad):
@@ -437,6 +440,8 @@ def nlcg(problem, m0, n_iter, beta_type="FR", init_step=None, step_frac=0.01, st
     phi, g, _ = problem.evaluate(m)
     res.objective.append(phi)
     phi0, g0norm = phi, math.sqrt(_dot(g, g))
+    if callback is not None:
+        callback(0, m, phi, g, problem.last_residual)
     p = _free_direction(problem, m, -g)
     alpha_prev, gp_prev = None, None
     it = 0

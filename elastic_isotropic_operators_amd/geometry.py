@@ -111,6 +111,51 @@ def staggered_interps(nz, oz, dz, ox, dx, zc, xc):
     return [Interp(nz, o[0], dz, o[1], dx, zc, xc) for o in origins]
 
 
+def parfile_source_coords(par, oz, dz, ox, dx, read_vector=None):
+    """Per-shot source coordinates of buildSourceGeometry (Elastic_iso_float_prop.py:261-347):
+    from sourceGeomFile ([nExp][nSimSou][x, y, z] in SEP axis order), else one device per
+    shot at zSource, xSource + iExp*spacingShots (1-based samples shifted by zPadMinus,
+    xPadMinus and fat).  Returns ([(zc, xc) per shot], (n, o, d) of the source axis)."""
+    nExp = par.getInt("nExp")
+    geom_file = par.getString("sourceGeomFile", "None")
+    if geom_file != "None":
+        geom = read_vector(geom_file, ndims=3).getNdArray()
+        if nExp != geom.shape[2]:
+            raise ValueError("ERROR! nExp (%d) not consistent with number of shots provided within "
+                             "sourceGeomFile (%d)" % (nExp, geom.shape[2]))
+        coords = [(np.array(geom[2, :, e], np.float64), np.array(geom[0, :, e], np.float64))
+                  for e in range(nExp)]
+        return coords, (nExp, 1.0, 1.0)
+    fat = par.getInt("fat", 4)
+    ozS = par.getInt("zSource") - 1 + par.getInt("zPadMinus") + fat
+    oxS = par.getInt("xSource") - 1 + par.getInt("xPadMinus") + fat
+    sp = par.getInt("spacingShots")
+    coords = [(np.array([oz + ozS * dz]), np.array([ox + (oxS + e * sp) * dx])) for e in range(nExp)]
+    return coords, (nExp, ox + oxS * dx, sp * dx)
+
+
+def parfile_receiver_coords(par, oz, dz, ox, dx, read_vector=None):
+    """Receiver coordinates of buildReceiversGeometry (Elastic_iso_float_prop.py:584-641):
+    from recGeomFile, else nReceiver devices at depthReceiver from oReceiver every dReceiver
+    samples.  Returns (zc, xc, (n, o, d) of the receiver axis)."""
+    geom_file = par.getString("recGeomFile", "none")
+    if geom_file != "none":
+        geom = read_vector(geom_file, ndims=3).getNdArray()
+        xc, zc = np.array(geom[0, :, 0], np.float64), np.array(geom[2, :, 0], np.float64)
+        n = par.getInt("nReceiver")
+        if n != len(xc):
+            raise ValueError("ERROR [buildReceiversGeometry]: Number of receiver coordinates (%s) not "
+                             "consistent with parameter file (nReceiver=%d)" % (len(xc), n))
+        return zc, xc, (n, 1.0, 1.0)
+    fat = par.getInt("fat", 4)
+    ozR = par.getInt("depthReceiver") - 1 + par.getInt("zPadMinus") + fat
+    n = par.getInt("nReceiver")
+    oxR = par.getInt("oReceiver") - 1 + par.getInt("xPadMinus") + fat
+    dR = par.getInt("dReceiver")
+    irec = np.arange(n)
+    return oz + ozR * dz + 0 * irec, ox + oxR * dx + dR * dx * irec, (n, ox + oxR * dx, dR * dx)
+
+
 def nl_source_traces(interps, media, model, dtw, dx, dz, dtype=np.float32):
     """Irregular model (5, nSrc, nts) -> 5 regular scaled source traces."""
     two = dtype(2.0) * dtype(dtw)

@@ -258,7 +258,58 @@ def test_lbfgs_on_problem_layer_decreases_and_respects_bounds():
     prob = P.FwiProblem(eng, d_obs, components="vx,vz,p", mod_par=1, min_bound=lo, max_bound=hi)
     seen = []
     res = P.lbfgs(prob, m0, 8, callback=lambda it, m, phi, g, r: seen.append((it, phi)))
-    assert res.iterations == 8 and [s[0] for s in seen] == list(range(1, 9))
+    assert res.iterations == 8 and [s[0] for s in seen] == list(range(0, 9))
+    assert [s[1] for s in seen] == res.objective
     assert res.objective[-1] < 0.2 * res.objective[0]
     assert bool((res.model >= lo).all()) and bool((res.model <= hi).all())
     assert eng.n_store == prob.n_forward  # every forward kept its checkpoints for the gradient
+
+
+def test_parfile_coordinates_match_workload_geometry(tmp_path):
+    """The parfile geometry rules (Elastic_iso_float_prop.py:311-347, 615-641) and the source
+    and receiver geometry files."""
+    from elastic_isotropic_operators_amd import geometry as G, workloads as W
+    from elastic_isotropic_operators_amd.sep import Hypercube, SepVector, genericIO
+    w = W.small(nz=40, nx=56, nts=50, nExp=3)
+    par = genericIO.io(params=["nExp=3", "zSource=3", "xSource=5", "spacingShots=7", "zPadMinus=%d" % w["zPadMinus"],
+                               "xPadMinus=%d" % w["xPadMinus"], "nReceiver=9", "depthReceiver=2",
+                               "oReceiver=4", "dReceiver=3"])
+    coords, (n, o, d) = G.parfile_source_coords(par, w["oz"], w["dz"], w["ox"], w["dx"])
+    sz, sx = W.source_coords(w, 3, 5, nExp=3, spacingShots=7)
+    np.testing.assert_allclose(np.stack([c[0] for c in coords]), sz)
+    np.testing.assert_allclose(np.stack([c[1] for c in coords]), sx)
+    assert (n, d) == (3, 7 * w["dx"]) and o == pytest.approx(sx[0, 0])
+    rz, rx, (nr, orr, dr) = G.parfile_receiver_coords(par, w["oz"], w["dz"], w["ox"], w["dx"])
+    rz2, rx2 = W.receiver_coords(w, 9, 2, 4, 3)
+    np.testing.assert_allclose(rz, rz2)
+    np.testing.assert_allclose(rx, rx2)
+    assert nr == 9 and dr == 3 * w["dx"]
+    # geometry files: [x, y, z] along the slowest axis, devices, shots along the fastest
+    geom = SepVector.getSepVector(Hypercube.hypercube(axes=[Hypercube.axis(n=3), Hypercube.axis(n=2),
+                                                            Hypercube.axis(n=3)]))
+    a = geom.getNdArray()               # (3 coords, 2 devices, 3 shots)
+    a[0] = np.arange(6).reshape(2, 3) * 10.0 + 100.0
+    a[2] = np.arange(6).reshape(2, 3) * 5.0 + 30.0
+    genericIO.defaultIO.writeVector(str(tmp_path / "sg.H"), geom)
+    par = genericIO.io(params=["nExp=3", "sourceGeomFile=%s" % (tmp_path / "sg.H"), "nReceiver=2",
+                               "recGeomFile=%s" % (tmp_path / "sg.H")])
+    coords, (n, o, d) = G.parfile_source_coords(par, 0.0, 1.0, 0.0, 1.0, genericIO.defaultIO.getVector)
+    assert (n, o, d) == (3, 1.0, 1.0)
+    for e in range(3):
+        np.testing.assert_allclose(coords[e][0], a[2, :, e])
+        np.testing.assert_allclose(coords[e][1], a[0, :, e])
+    rz, rx, _ = G.parfile_receiver_coords(par, 0.0, 1.0, 0.0, 1.0, genericIO.defaultIO.getVector)
+    np.testing.assert_allclose(rz, a[2, :, 0])
+    np.testing.assert_allclose(rx, a[0, :, 0])
+    par = genericIO.io(params=["nExp=2", "sourceGeomFile=%s" % (tmp_path / "sg.H")])
+    with pytest.raises(ValueError):
+        G.parfile_source_coords(par, 0.0, 1.0, 0.0, 1.0, genericIO.defaultIO.getVector)
+
+
+def test_efwi_rejects_unavailable_options():
+    from elastic_isotropic_operators_amd import efwi
+    from elastic_isotropic_operators_amd.sep import genericIO
+    with pytest.raises(NotImplementedError):
+        efwi.build_problem(genericIO.io(params=["spline=1"]))
+    with pytest.raises(RuntimeError):
+        efwi.build_problem(genericIO.io(params=["surfaceCondition=1"]))
