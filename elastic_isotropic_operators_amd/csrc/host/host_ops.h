@@ -4,7 +4,8 @@
 // (paths relative to elastic_iso_lib/elastic_iso_float/prop):
 //   FdParam           src/fdParamElastic.cpp:12-157, checks :256-324
 //   stagger helpers   src/stagger.cpp:18-114
-//   SpaceInterp       src/spaceInterpGpu.cpp:11-138 (linear + dipole), :288-376, checks :378-403
+//   SpaceInterp       src/spaceInterpGpu.cpp:11-138 (linear + dipole), :139-207 (sinc, gauss), :288-376,
+//                     checks :378-403
 //   time_interp_fwd   src/interpTimeLinTbb.cpp:30-58
 //   NonlinearShots    src/nonlinearPropElasticShotsGpu.cpp:80-422 + src/nonlinearPropElasticGpu.cpp:43-473
 //   BornShots         src/BornElasticShotsGpu.cpp:85-378 + src/BornElasticGpu.cpp:47-194 +
@@ -227,7 +228,7 @@ struct FdParam {
     }
 };
 
-// ---------------------------------------------------------------- spaceInterpGpu (linear)
+// ---------------------------------------------------------------- spaceInterpGpu
 struct SpaceInterp {
     std::vector<float> zCoord, xCoord;
     std::vector<Axis> hyper;
@@ -244,19 +245,26 @@ struct SpaceInterp {
         : zCoord(std::move(z)), xCoord(std::move(x)), hyper(std::move(h)), nt(nt_), method(std::move(m)) {
         if (zCoord.size() != xCoord.size()) throw std::runtime_error("zCoord and xCoord sizes differ");
         if (hyper.size() < 2) throw std::runtime_error("spaceInterpGpu needs a 2-D (or 3-D) hypercube");
-        if (method != "linear")
-            throw std::runtime_error("**** ERROR: Space interp method " + method +
-                                     " is not available in this build (linear only) ****");
-        nFilt = 1;
-        (void)nf;
+        if (method != "linear" && method != "sinc" && method != "gauss")
+            throw std::runtime_error("**** ERROR: Space interp method not defined ****");
+        if (method != "linear" && nf < 1) throw std::runtime_error("**** ERROR: nFilt must be >= 1 ****");
+        nFilt = method == "linear" ? 1 : nf;
         dipole = dip;
         zDipoleShift = float(zs * hyper[0].d);
         xDipoleShift = float(xs * hyper[1].d);
         nDeviceIrreg = int(zCoord.size());
         nz = hyper[0].n;
         check_out_of_bounds();
-        nFiltTotal = dipole == 1 ? 8 : 4;
-        calc_linear_weights();
+        if (method == "linear") {
+            nFiltTotal = dipole == 1 ? 8 : 4;
+            calc_linear_weights();
+        } else {
+            if (dipole == 1)
+                throw std::runtime_error(method == "sinc"
+                    ? "**** ERROR: Spatial sinc interpolation and dipole acquisition not implemented ****"
+                    : "**** ERROR: Gaussian spatial interpolation and dipole acquisition not implemented ****");
+            calc_filter_weights(method == "sinc");
+        }
         convert_irreg_to_reg();
     }
 
@@ -296,6 +304,57 @@ struct SpaceInterp {
                 gridPointIndex[b + 3] = gridPointIndex[b] + nz + 1;
                 weight[b + 3] = sgn * (1.0f - wz) * (1.0f - wx);
             }
+        }
+    }
+
+    // sin(x)/x with a Taylor guard near 0 (the series 1 - x^2/6 + x^4/120 where sin(x)/x
+    // would lose digits), evaluated in double.
+    static double sinc_pi(double x) {
+        const double ax = std::fabs(x);
+        if (ax >= 1.2207e-4) return std::sin(x) / x;
+        const double x2 = x * x;
+        return 1.0 - x2 / 6.0 + x2 * x2 / 120.0;
+    }
+    static double normal_pdf(double x, double mean, double sd) {
+        const double e = (x - mean) * (x - mean) / (2.0 * sd * sd);
+        return std::exp(-e) / (sd * 2.5066282746310002);  // sqrt(2 pi)
+    }
+
+    // spaceInterpGpu.cpp:139-207: per device the (2 nFilt)^2 regular points whose indices run
+    // from int(position) - nFilt + 1 to int(position) + nFilt, x-major, with separable sinc
+    // weights (sinc(pi dz_cells) sinc(pi dx_cells)) or the product of two normal densities of
+    // standard deviation 2 cells, normalised to sum 1.  Coordinates are float and the
+    // kernels double, then rounded to float, as there.  The arrays hold the whole stencil:
+    // the reference's constructor resets its per-device count to 4 after choosing nFilt
+    // (spaceInterpGpu.cpp:38-41), which is consistent only for nFilt = 1.
+    void calc_filter_weights(bool sinc) {
+        const int n2 = 2 * nFilt;
+        nFiltTotal = n2 * n2;
+        gridPointIndex.assign(size_t(nFiltTotal) * nDeviceIrreg, 0);
+        weight.assign(size_t(nFiltTotal) * nDeviceIrreg, 0.f);
+        const float oz = float(hyper[0].o), dz = float(hyper[0].d), ox = float(hyper[1].o), dx = float(hyper[1].d);
+        for (int d = 0; d < nDeviceIrreg; ++d) {
+            const size_t off = size_t(d) * nFiltTotal;
+            const float zf = zCoord[d], xf = xCoord[d];
+            const int zr = int((zf - oz) / dz), xr = int((xf - ox) / dx);
+            float sum = 0.f;
+            for (int ix = 0; ix < n2; ++ix)
+                for (int iz = 0; iz < n2; ++iz) {
+                    const int px = xr + ix - nFilt + 1, pz = zr + iz - nFilt + 1;
+                    const float xcur = float(px) * dx + ox, zcur = float(pz) * dz + oz;
+                    double v;
+                    if (sinc) {
+                        const float xin = (xf - xcur) / dx, zin = (zf - zcur) / dz;
+                        v = sinc_pi(M_PI * zin) * sinc_pi(M_PI * xin);
+                    } else {
+                        v = normal_pdf(xcur, xf, double(2.f * dx)) * normal_pdf(zcur, zf, double(2.f * dz));
+                    }
+                    const size_t k = off + size_t(n2) * ix + iz;
+                    weight[k] = float(v);
+                    gridPointIndex[k] = nz * px + pz;
+                    sum += weight[k];
+                }
+            for (int i = 0; i < nFiltTotal; ++i) weight[off + i] = weight[off + i] / sum;
         }
     }
 

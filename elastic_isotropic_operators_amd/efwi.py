@@ -9,7 +9,8 @@ model (SEP [z, x, 3]), the source wavelet ([t, 5] or [t, 1, 5]), the observed da
 operator (zSource/xSource/spacingShots or sourceGeomFile, nReceiver/depthReceiver/
 oReceiver/dReceiver or recGeomFile), maskGradient, dataMaskFile, minBound_par1..3 /
 maxBound_par1..3 or minBoundVector / maxBoundVector, solver (lbfgs | nlcg | sd), stepper,
-initStep, illumination (L-BFGS initial inverse Hessian), nIter, folder, prefix, saveObj,
+initStep, illumination (L-BFGS initial inverse Hessian), sourceInterpMethod/recInterpMethod
+(linear, sinc, gauss) with their NumFilters, nIter, folder, prefix, saveObj,
 saveRes, saveGrad, saveModel, iterSampling.  Every shot runs batched on one GPU with the
 checkpointed gradient (fwi.ElasticFwi); the problem and solvers are fwi_problem's.
 
@@ -111,8 +112,14 @@ def build_problem(par, device=0):
     dm = par.getString("dataMaskFile", "noDataMask")
     data_mask = None if dm == "noDataMask" else _read(dm, 4).getNdArray().astype(np.float64)
     dtype = torch.float64 if par.getString("precision", "float") == "double" else torch.float32
+    if par.getInt("dipole", 0) != 0:
+        raise NotImplementedError("dipole=1 is available in the operator classes, not in the FWI driver")
+    src_interp = (par.getString("sourceInterpMethod", "linear"), par.getInt("sourceInterpNumFilters", 4))
+    rec_interp = (par.getString("recInterpMethod", "linear"), par.getInt("recInterpNumFilters", 4))
     eng = ElasticFwi(grid, src_z, src_x, sig, rec_z, rec_x, device=device, dtype=dtype,
-                     segment=par.getInt("checkpointInterval", 0))
+                     segment=par.getInt("checkpointInterval", 0),
+                     src_interp=(src_interp[0], 1 if src_interp[0] == "linear" else src_interp[1]),
+                     rec_interp=(rec_interp[0], 1 if rec_interp[0] == "linear" else rec_interp[1]))
     prob = FwiProblem(eng, d_obs, components=components, mod_par=par.getInt("mod_par", 0),
                       grad_mask=grad_mask, data_mask=data_mask,
                       min_bound=_bounds(par, model0.shape, "min"), max_bound=_bounds(par, model0.shape, "max"),

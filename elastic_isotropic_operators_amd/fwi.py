@@ -139,10 +139,12 @@ class ElasticFwi:
     sources: (5, nSrc, nts) shared by all shots, or (nShot, 5, nSrc, nts)
     rec_z, rec_x: (nRec,) receiver coordinates shared by all shots
     segment: checkpoint interval in coarse samples (0: ceil(sqrt(nts-1)))
+    src_interp, rec_interp: (method, nfilt) of the spatial interpolation of sources and
+             receivers: ("linear", 1), ("sinc", n) or ("gauss", n) (spaceInterpGpu)
     """
 
     def __init__(self, grid, src_z, src_x, sources, rec_z, rec_x, device=0, dtype=torch.float32,
-                 segment=0, variant=None):
+                 segment=0, variant=None, src_interp=("linear", 1), rec_interp=("linear", 1)):
         if not torch.cuda.is_available():
             if _capi._lib is not None:
                 raise RuntimeError("torch sees no GPU because libeio_hip.so was loaded before torch: "
@@ -174,14 +176,14 @@ class ElasticFwi:
         # spaceInterpGpu.cpp:313-376), plus the positions the media scaling is read at
         self.src_pos, self.src_reg = [], []
         for e in range(self.nshot):
-            ips = G.staggered_interps(self.nz, oz, dz, ox, dx, src_z[e], src_x[e])
+            ips = G.staggered_interps(self.nz, oz, dz, ox, dx, src_z[e], src_x[e], *src_interp)
             self.src_pos.append([ip.uniq for ip in ips])
             self.src_reg.append([torch.as_tensor(ips[GRID_OF_COMP[k]].adjoint(sig[e, k]), dtype=dtype,
                                                  device=self.device) for k in range(5)])
         self.src_idx = [[torch.as_tensor(self.src_pos[e][GRID_OF_COMP[k]].astype(np.int64), device=self.device)
                          for k in range(2)] for e in range(self.nshot)]
         rips = G.staggered_interps(self.nz, oz, dz, ox, dx, np.asarray(rec_z, np.float64),
-                                   np.asarray(rec_x, np.float64))
+                                   np.asarray(rec_x, np.float64), *rec_interp)
         self.rec_pos = [ip.uniq for ip in rips]
         self.nrec = rips[0].n_irreg
         self.rec_map = [torch.as_tensor(ip.map, device=self.device) for ip in rips]

@@ -5,7 +5,7 @@ HBM and call the C ABI directly.  The drop-in operator classes use the C++ host 
 (csrc/host) that implements the same rules.
 
   scaled_media        fdParamElastic.cpp:118-148 (stagger rho and mu, scale by 2*dtw)
-  Interp              spaceInterpGpu.cpp:78-138, 288-376 (bilinear weights, unique list)
+  Interp              spaceInterpGpu.cpp:78-207, 288-376 (bilinear/sinc/gauss weights, unique list)
   staggered grids     Elastic_iso_float_prop.py:224-249 (centre, X, Z, XZ origins)
   source scaling      nonlinearPropElasticGpu.cpp:109-144
 """
@@ -66,22 +66,69 @@ def scaled_media(elastic, dtw, dtype=np.float32):
     return out
 
 
-class Interp:
-    """Linear (bilinear) spatial interpolation of devices on one staggered grid."""
+def _sinc_pi(x):
+    """sin(x)/x in double with the Taylor series near 0."""
+    x = np.asarray(x, np.float64)
+    small = np.abs(x) < 1.2207e-4
+    safe = np.where(small, 1.0, x)
+    x2 = x * x
+    return np.where(small, 1.0 - x2 / 6.0 + x2 * x2 / 120.0, np.sin(safe) / safe)
 
-    def __init__(self, nz, oz, dz, ox, dx, zc, xc):
+
+def _normal_pdf(x, mean, sd):
+    return np.exp(-((x - mean) ** 2) / (2.0 * sd * sd)) / (sd * 2.5066282746310002)
+
+
+class Interp:
+    """Spatial interpolation of devices on one staggered grid (spaceInterpGpu.cpp:11-207):
+    linear (bilinear, 4 points), or sinc / gauss on the (2 nfilt)^2 points from
+    int(position) - nfilt + 1 to int(position) + nfilt, x-major, with separable weights
+    normalised to sum 1 (sinc(pi u) sinc(pi v), or normal densities of standard deviation
+    2 cells).  Coordinates in float32, kernels in double rounded to float32, sums in float32,
+    as there.  The whole stencil is kept for any nfilt (the reference keeps 4 entries per
+    device, consistent only for nfilt = 1: spaceInterpGpu.cpp:38-41)."""
+
+    def __init__(self, nz, oz, dz, ox, dx, zc, xc, method="linear", nfilt=1):
         zc = np.atleast_1d(np.asarray(zc, np.float32))
         xc = np.atleast_1d(np.asarray(xc, np.float32))
-        wz = (zc - np.float32(oz)) / np.float32(dz)
-        wx = (xc - np.float32(ox)) / np.float32(dx)
+        oz32, dz32, ox32, dx32 = np.float32(oz), np.float32(dz), np.float32(ox), np.float32(dx)
+        wz = (zc - oz32) / dz32
+        wx = (xc - ox32) / dx32
         zr = wz.astype(np.int64)
         xr = wx.astype(np.int64)
-        wz = np.float32(1.0) - (wz - zr.astype(np.float32))
-        wx = np.float32(1.0) - (wx - xr.astype(np.float32))
-        b = xr * nz + zr
-        self.idx = np.stack([b, b + 1, b + nz, b + nz + 1], axis=1)
-        self.w = np.stack([wz * wx, (1 - wz) * wx, wz * (1 - wx), (1 - wz) * (1 - wx)],
-                          axis=1).astype(np.float64)
+        if method == "linear":
+            wz = np.float32(1.0) - (wz - zr.astype(np.float32))
+            wx = np.float32(1.0) - (wx - xr.astype(np.float32))
+            b = xr * nz + zr
+            self.idx = np.stack([b, b + 1, b + nz, b + nz + 1], axis=1)
+            self.w = np.stack([wz * wx, (1 - wz) * wx, wz * (1 - wx), (1 - wz) * (1 - wx)],
+                              axis=1).astype(np.float64)
+        elif method in ("sinc", "gauss"):
+            n2 = 2 * int(nfilt)
+            idx, w = [], []
+            for ix in range(n2):
+                for iz in range(n2):
+                    px = xr + ix - nfilt + 1
+                    pz = zr + iz - nfilt + 1
+                    xcur = px.astype(np.float32) * dx32 + ox32
+                    zcur = pz.astype(np.float32) * dz32 + oz32
+                    if method == "sinc":
+                        xin = (xc - xcur) / dx32
+                        zin = (zc - zcur) / dz32
+                        v = _sinc_pi(np.pi * zin.astype(np.float64)) * _sinc_pi(np.pi * xin.astype(np.float64))
+                    else:
+                        v = (_normal_pdf(xcur.astype(np.float64), xc.astype(np.float64), float(np.float32(2) * dx32))
+                             * _normal_pdf(zcur.astype(np.float64), zc.astype(np.float64),
+                                           float(np.float32(2) * dz32)))
+                    idx.append(px * nz + pz)
+                    w.append(v.astype(np.float32))
+            total = np.zeros(zc.shape, np.float32)
+            for v in w:  # sequential float sum, as the reference's loop
+                total = (total + v).astype(np.float32)
+            self.idx = np.stack(idx, axis=1)
+            self.w = np.stack([v / total for v in w], axis=1).astype(np.float64)
+        else:
+            raise ValueError("**** ERROR: Space interp method not defined ****")
         flat = self.idx.reshape(-1)
         uniq, first = np.unique(flat, return_index=True)
         order = np.argsort(first)               # first-seen order (spaceInterpGpu.cpp:288-311)
@@ -93,22 +140,22 @@ class Interp:
     def adjoint(self, irr):
         """irregular (nIrreg, nt) -> regular (nUniq, nt)."""
         reg = np.zeros((self.uniq.size, irr.shape[1]), dtype=irr.dtype)
-        for f in range(4):
+        for f in range(self.idx.shape[1]):
             np.add.at(reg, self.map[:, f], (self.w[:, f:f + 1] * irr).astype(irr.dtype))
         return reg
 
     def forward(self, reg):
         """regular (nUniq, nt) -> irregular (nIrreg, nt)."""
         out = np.zeros((self.n_irreg, reg.shape[1]), dtype=reg.dtype)
-        for f in range(4):
+        for f in range(self.idx.shape[1]):
             out += (self.w[:, f:f + 1] * reg[self.map[:, f]]).astype(reg.dtype)
         return out
 
 
-def staggered_interps(nz, oz, dz, ox, dx, zc, xc):
+def staggered_interps(nz, oz, dz, ox, dx, zc, xc, method="linear", nfilt=1):
     """The four grids: centre, X (x-dx/2), Z (z-dz/2), XZ."""
     origins = [(oz, ox), (oz, ox - 0.5 * dx), (oz - 0.5 * dz, ox), (oz - 0.5 * dz, ox - 0.5 * dx)]
-    return [Interp(nz, o[0], dz, o[1], dx, zc, xc) for o in origins]
+    return [Interp(nz, o[0], dz, o[1], dx, zc, xc, method, nfilt) for o in origins]
 
 
 def parfile_source_coords(par, oz, dz, ox, dx, read_vector=None):

@@ -24,6 +24,8 @@
 //   Born imaging forward / adjoint               : src/kernelsGpuElastic.cu:1381-1459 / 1538-1613
 //   media scaling and staggering                 : src/fdParamElastic.cpp:118-148, src/stagger.cpp:18-114
 //   bilinear spatial interpolation               : src/spaceInterpGpu.cpp:78-138, 288-376
+//   sinc and Gaussian spatial interpolation      : src/spaceInterpGpu.cpp:139-207 (intended stencil;
+//                                                  see make_interp)
 //   coarse->fine time interpolation              : src/interpTimeLinTbb.cpp:9-104
 //   per-shot source scaling / packing            : src/nonlinearPropElasticGpu.cpp:43-473,
 //                                                  include/seismicElasticOperator2D.cpp:27-119
@@ -650,27 +652,57 @@ void media_scaled(const Grid& g, const T* elastic, T* out) {
 // spaceInterpGpu.cpp:78-138 (linear, no dipole) + :288-311 unique list
 struct Interp {
     int nIrreg;
-    std::vector<int> idx;      // 4 per device, non-unique
-    std::vector<double> w;     // 4 per device
+    int nf = 4;                // regular points per device
+    std::vector<int> idx;      // nf per device, non-unique
+    std::vector<double> w;     // nf per device
     std::vector<int> uniq;     // unique regular positions, first-seen order
-    std::vector<int> map;      // 4 per device -> index into uniq
+    std::vector<int> map;      // nf per device -> index into uniq
 };
+// method 0: bilinear (spaceInterpGpu.cpp:78-138).  Methods 1 (sinc) and 2 (Gaussian,
+// standard deviation 2 cells) follow calcSincWeights / calcGaussWeights
+// (spaceInterpGpu.cpp:139-207): the (2 nfilt)^2 regular points whose indices run from
+// int(position) - nfilt + 1 to int(position) + nfilt along z and x, stored x-major, with
+// separable weights normalised to sum 1:
+//   sinc:  sin(pi u)/(pi u) * sin(pi v)/(pi v)      u, v = device - point, in cells
+//   gauss: exp(-u^2/8) * exp(-v^2/8)                 (the pdf constants cancel in the normalisation)
+// Computed here in double from the device offsets.
 Interp make_interp(int nz, double oz, double dz, double ox, double dx, int nDev, const double* zc,
-                   const double* xc) {
+                   const double* xc, int method = 0, int nfilt = 1) {
     Interp ip;
     ip.nIrreg = nDev;
-    ip.idx.resize(4 * size_t(nDev));
-    ip.w.resize(4 * size_t(nDev));
+    const int n2 = 2 * nfilt;
+    ip.nf = method == 0 ? 4 : n2 * n2;
+    ip.idx.resize(size_t(ip.nf) * nDev);
+    ip.w.resize(size_t(ip.nf) * nDev);
     for (int d = 0; d < nDev; ++d) {
         double wz = (zc[d] - oz) / dz, wx = (xc[d] - ox) / dx;
         const int zr = int(wz), xr = int(wx);
-        wz = 1.0 - (wz - zr);
-        wx = 1.0 - (wx - xr);
-        const int b = xr * nz + zr;
-        ip.idx[4 * d + 0] = b;          ip.w[4 * d + 0] = wz * wx;
-        ip.idx[4 * d + 1] = b + 1;      ip.w[4 * d + 1] = (1.0 - wz) * wx;
-        ip.idx[4 * d + 2] = b + nz;     ip.w[4 * d + 2] = wz * (1.0 - wx);
-        ip.idx[4 * d + 3] = b + nz + 1; ip.w[4 * d + 3] = (1.0 - wz) * (1.0 - wx);
+        int* id = ip.idx.data() + size_t(ip.nf) * d;
+        double* ww = ip.w.data() + size_t(ip.nf) * d;
+        if (method == 0) {
+            wz = 1.0 - (wz - zr);
+            wx = 1.0 - (wx - xr);
+            const int b = xr * nz + zr;
+            id[0] = b;          ww[0] = wz * wx;
+            id[1] = b + 1;      ww[1] = (1.0 - wz) * wx;
+            id[2] = b + nz;     ww[2] = wz * (1.0 - wx);
+            id[3] = b + nz + 1; ww[3] = (1.0 - wz) * (1.0 - wx);
+            continue;
+        }
+        auto kern = [method](double u) {
+            if (method == 1) return u == 0.0 ? 1.0 : std::sin(M_PI * u) / (M_PI * u);
+            return std::exp(-u * u / 8.0);
+        };
+        double sum = 0.0;
+        for (int ix = 0; ix < n2; ++ix)
+            for (int iz = 0; iz < n2; ++iz) {
+                const int pz = zr + iz - nfilt + 1, px = xr + ix - nfilt + 1;
+                const double v = kern(wz - pz) * kern(wx - px);
+                id[n2 * ix + iz] = px * nz + pz;
+                ww[n2 * ix + iz] = v;
+                sum += v;
+            }
+        for (int f = 0; f < ip.nf; ++f) ww[f] /= sum;
     }
     std::map<int, int> seen;
     ip.map.resize(ip.idx.size());
@@ -691,9 +723,9 @@ Interp make_interp(int nz, double oz, double dz, double ox, double dx, int nDev,
 template <typename T>
 void interp_fwd_add(const Interp& ip, int nt, const T* reg, T* irr) {
     for (int d = 0; d < ip.nIrreg; ++d)
-        for (int f = 0; f < 4; ++f) {
-            const T w = T(ip.w[4 * d + f]);
-            const T* r = reg + size_t(ip.map[4 * d + f]) * nt;
+        for (int f = 0; f < ip.nf; ++f) {
+            const T w = T(ip.w[size_t(ip.nf) * d + f]);
+            const T* r = reg + size_t(ip.map[size_t(ip.nf) * d + f]) * nt;
             T* o = irr + size_t(d) * nt;
             for (int it = 0; it < nt; ++it) o[it] += w * r[it];
         }
@@ -703,9 +735,9 @@ template <typename T>
 void interp_adj(const Interp& ip, int nt, T* reg, const T* irr) {
     std::fill(reg, reg + ip.uniq.size() * size_t(nt), T(0));
     for (int d = 0; d < ip.nIrreg; ++d)
-        for (int f = 0; f < 4; ++f) {
-            const T w = T(ip.w[4 * d + f]);
-            T* r = reg + size_t(ip.map[4 * d + f]) * nt;
+        for (int f = 0; f < ip.nf; ++f) {
+            const T w = T(ip.w[size_t(ip.nf) * d + f]);
+            T* r = reg + size_t(ip.map[size_t(ip.nf) * d + f]) * nt;
             const T* in = irr + size_t(d) * nt;
             for (int it = 0; it < nt; ++it) r[it] += w * in[it];
         }
@@ -1126,6 +1158,16 @@ int oracle_num_threads(void) { return omp_get_max_threads(); }
 int oracle_interp(int nz, double oz, double dz, double ox, double dx, int nDev, const double* zc,
                   const double* xc, int* idx, double* w, int* uniq) {
     Interp ip = make_interp(nz, oz, dz, ox, dx, nDev, zc, xc);
+    std::copy(ip.idx.begin(), ip.idx.end(), idx);
+    std::copy(ip.w.begin(), ip.w.end(), w);
+    std::copy(ip.uniq.begin(), ip.uniq.end(), uniq);
+    return int(ip.uniq.size());
+}
+// Same for method 0 (bilinear), 1 (sinc), 2 (Gaussian) with nfilt: nf = 4 or (2 nfilt)^2
+// entries per device in idx/w (capacity nf*nDev for uniq).
+int oracle_interp_m(int nz, double oz, double dz, double ox, double dx, int nDev, const double* zc,
+                    const double* xc, int method, int nfilt, int* idx, double* w, int* uniq) {
+    Interp ip = make_interp(nz, oz, dz, ox, dx, nDev, zc, xc, method, nfilt);
     std::copy(ip.idx.begin(), ip.idx.end(), idx);
     std::copy(ip.w.begin(), ip.w.end(), w);
     std::copy(ip.uniq.begin(), ip.uniq.end(), uniq);

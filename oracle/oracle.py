@@ -102,19 +102,54 @@ def time_interp(coarse, dts, sub_fd, dtype=np.float64):
     return out
 
 
-def interp(nz, oz, dz, ox, dx, zc, xc):
-    """Bilinear weights of one staggered grid: (idx[nDev,4], w[nDev,4], unique positions)."""
+_METHODS = {"linear": 0, "sinc": 1, "gauss": 2}
+
+
+def interp(nz, oz, dz, ox, dx, zc, xc, method="linear", nfilt=1):
+    """Spatial interpolation weights of one staggered grid: (idx[nDev,nf], w[nDev,nf], unique
+    positions), nf = 4 (linear) or (2 nfilt)^2 (sinc, gauss)."""
     zc = _c(zc, np.float64)
     xc = _c(xc, np.float64)
     nd = zc.size
-    idx = np.zeros((nd, 4), np.int32)
-    w = np.zeros((nd, 4), np.float64)
-    uniq = np.zeros(4 * nd, np.int32)
-    f = lib().oracle_interp
+    m = _METHODS[method]
+    nf = 4 if m == 0 else (2 * nfilt) ** 2
+    idx = np.zeros((nd, nf), np.int32)
+    w = np.zeros((nd, nf), np.float64)
+    uniq = np.zeros(nf * nd, np.int32)
+    f = lib().oracle_interp_m
     f.restype = ctypes.c_int
     nu = f(nz, ctypes.c_double(oz), ctypes.c_double(dz), ctypes.c_double(ox), ctypes.c_double(dx),
-           nd, _ptr(zc), _ptr(xc), _ptr(idx), _ptr(w), _ptr(uniq))
+           nd, _ptr(zc), _ptr(xc), int(m), int(nfilt), _ptr(idx), _ptr(w), _ptr(uniq))
     return idx, w, uniq[:nu].copy()
+
+
+def staggered_interp(g, zc, xc, method="linear", nfilt=1):
+    """interp on the four staggered grids of g (centre, X, Z, XZ)."""
+    origins = [(g.oz, g.ox), (g.oz, g.ox - 0.5 * g.dx), (g.oz - 0.5 * g.dz, g.ox),
+               (g.oz - 0.5 * g.dz, g.ox - 0.5 * g.dx)]
+    return [interp(g.nz, o[0], g.dz, o[1], g.dx, zc, xc, method, nfilt) for o in origins]
+
+
+def irreg_to_reg(ip, irr):
+    """Adjoint of the interpolation: irregular (nDev, nt) -> regular (nUniq, nt)."""
+    idx, w, uniq = ip
+    pos = {int(p): i for i, p in enumerate(uniq)}
+    reg = np.zeros((uniq.size, irr.shape[1]), np.float64)
+    for d in range(idx.shape[0]):
+        for f in range(idx.shape[1]):
+            reg[pos[int(idx[d, f])]] += w[d, f] * irr[d]
+    return reg
+
+
+def reg_to_irreg(ip, reg):
+    """Forward of the interpolation: regular (nUniq, nt) -> irregular (nDev, nt)."""
+    idx, w, uniq = ip
+    pos = {int(p): i for i, p in enumerate(uniq)}
+    out = np.zeros((idx.shape[0], reg.shape[1]), np.float64)
+    for d in range(idx.shape[0]):
+        for f in range(idx.shape[1]):
+            out[d] += w[d, f] * reg[pos[int(idx[d, f])]]
+    return out
 
 
 # ------------------------------------------------------------------ regular level

@@ -93,3 +93,24 @@ def fwi_directions(m0, n, seed=33, frac=0.01):
     rng = np.random.default_rng(seed)
     return [np.stack([frac * m0[k].mean() * smooth_field(rng, m0.shape[1:]) for k in range(3)])
             for _ in range(n)]
+
+
+def oracle_nl_irregular(w, g, src_z, src_x, rec_z, rec_x, src_model, src_interp=("linear", 1),
+                        rec_interp=("linear", 1)):
+    """One NL shot through the oracle with any spatial interpolation: irregular sources ->
+    regular (interpolation adjoint, scaled by rhoxDtw / rhozDtw at the regular points or
+    2 dtw, and 1/(dx dz): nonlinearPropElasticGpu.cpp:109-144) -> oracle regular-level NL
+    forward -> irregular receivers.  Returns (5, nRec, nts) in FP64."""
+    med = O.media(g, w["elastic"])
+    sips = O.staggered_interp(g, src_z, src_x, *src_interp)
+    rips = O.staggered_interp(g, rec_z, rec_x, *rec_interp)
+    dtw = g.dts / g.sub
+    area = 1.0 / (g.dx * g.dz)
+    src = []
+    for k in range(5):
+        ip = sips[GRID_OF_COMP[k]]
+        reg = O.irreg_to_reg(ip, np.asarray(src_model[k], np.float64))
+        scale = med[k].reshape(-1)[ip[2]][:, None] if k < 2 else 2.0 * dtw
+        src.append(reg * scale * area)
+    dreg = O.nl_fwd_reg(g, med, [ip[2] for ip in sips], src, [ip[2] for ip in rips])
+    return np.stack([O.reg_to_irreg(rips[GRID_OF_COMP[k]], dreg[k]) for k in range(5)])

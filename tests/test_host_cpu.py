@@ -99,8 +99,46 @@ def test_space_interp_out_of_bounds():
     h = Hypercube.hypercube(axes=[Hypercube.axis(n=50, d=10.0), Hypercube.axis(n=60, d=10.0)])
     with pytest.raises(RuntimeError, match="out of bounds in the z"):
         nl.spaceInterpGpu(np.array([5.0], np.float32), np.array([300.0], np.float32), h, 10, "linear", 1, 0, 0, 0)
-    with pytest.raises(RuntimeError, match="not available"):
-        nl.spaceInterpGpu(np.array([200.0], np.float32), np.array([300.0], np.float32), h, 10, "sinc", 4, 0, 0, 0)
+    # sinc / gauss keep nFilt cells from the edges (spaceInterpGpu.cpp:378-403)
+    with pytest.raises(RuntimeError, match="out of bounds in the x"):
+        nl.spaceInterpGpu(np.array([200.0], np.float32), np.array([35.0], np.float32), h, 10, "sinc", 4, 0, 0, 0)
+    nl.spaceInterpGpu(np.array([200.0], np.float32), np.array([35.0], np.float32), h, 10, "linear", 4, 0, 0, 0)
+    with pytest.raises(RuntimeError, match="dipole acquisition not implemented"):
+        nl.spaceInterpGpu(np.array([200.0], np.float32), np.array([300.0], np.float32), h, 10, "gauss", 2, 1, 0, 0)
+    with pytest.raises(RuntimeError, match="method not defined"):
+        nl.spaceInterpGpu(np.array([200.0], np.float32), np.array([300.0], np.float32), h, 10, "cubic", 2, 0, 0, 0)
+
+
+@pytest.mark.parametrize("method,nfilt", [("sinc", 1), ("sinc", 4), ("gauss", 1), ("gauss", 3), ("linear", 1)])
+def test_sinc_gauss_interp_matches_oracle(method, nfilt):
+    """The C++ interpolator (pybind spaceInterpGpu) and the numpy one (torch FWI engine) against
+    the oracle's weights: same stencil, same unique regular list, weights to float rounding;
+    the C++ and numpy versions compute in the same float/double order and agree to an ulp."""
+    from elastic_isotropic_operators_amd import pyElastic_iso_float_nl as nl
+    from elastic_isotropic_operators_amd.sep import Hypercube
+    rng = np.random.default_rng(17)
+    nz, oz, dz, ox, dx = 70, -40.0, 10.0, -45.0, 12.5
+    z = (100.0 + 400.0 * rng.random(30)).astype(np.float32)
+    x = (100.0 + 500.0 * rng.random(30)).astype(np.float32)
+    z[0] = oz + 20 * dz  # on a node
+    h = Hypercube.hypercube(axes=[Hypercube.axis(n=nz, o=oz, d=dz), Hypercube.axis(n=80, o=ox, d=dx)])
+    cpp = nl.spaceInterpGpu(z, x, h, 10, method, nfilt, 0, 0, 0)
+    npy = G.Interp(nz, oz, dz, ox, dx, z, x, method, nfilt)
+    idx, w, uniq = O.interp(nz, oz, dz, ox, dx, z.astype(np.float64), x.astype(np.float64), method, nfilt)
+    nf = idx.shape[1]
+    assert list(cpp.getRegPosUnique()) == list(uniq) == list(npy.uniq)
+    assert np.array_equal(np.array(cpp.getRegPos()).reshape(-1, nf), idx)
+    assert np.array_equal(npy.idx, idx)
+    wc = np.array(cpp.getWeights(), np.float64).reshape(-1, nf)
+    # float positions (as the reference computes them) against the oracle's double ones
+    assert np.abs(wc - w).max() < 1e-5 * np.abs(w).max()
+    assert np.abs(wc - npy.w).max() <= 2 * np.spacing(np.float32(np.abs(w).max()))
+    # forward / adjoint of the C++ interpolator are transposes of each other on these weights
+    reg = rng.standard_normal((uniq.size, 10)).astype(np.float32)
+    irr = rng.standard_normal((30, 10)).astype(np.float32)
+    a = float(np.sum(npy.forward(reg).astype(np.float64) * irr))
+    b = float(np.sum(reg.astype(np.float64) * npy.adjoint(irr)))
+    assert abs(a - b) < 1e-5 * abs(a)
 
 
 def test_duplicate_gpu_ids_rejected_before_device_use(tmp_path):

@@ -206,3 +206,26 @@ def test_free_surface_operator_vs_oracle(tmp_path):
     (m, d, el, par, sig, sc, sx, sz, sxz, rc, rx, rz, rxz, _) = PF.BornOpInitFloat(args)
     with pytest.raises(RuntimeError, match="nonlinear operator only"):
         PF.BornElasticShotsGpu(m, d, el, par, sig, sc, sx, sz, sxz, rc, rx, rz, rxz)
+
+
+@pytest.mark.parametrize("smeth,rmeth", [(("sinc", 3), ("gauss", 2)), (("gauss", 1), ("sinc", 1)),
+                                         (("sinc", 4), ("linear", 1))])
+def test_sinc_gauss_acquisition_vs_oracle(tmp_path, smeth, rmeth):
+    """sourceInterpMethod / recInterpMethod = sinc or gauss (spaceInterpGpu.cpp:139-207)
+    through the drop-in NL operator against the oracle with the same stencil."""
+    from helpers import oracle_nl_irregular
+    w = _small(nts=120, seed=5)
+    extra = dict(sourceInterpMethod=smeth[0], sourceInterpNumFilters=smeth[1],
+                 recInterpMethod=rmeth[0], recInterpNumFilters=rmeth[1])
+    args, geo = operator_case(tmp_path, w, nExp=2, zSource=5, depthRec=3, extra=extra)
+    op, model, data, elastic = _nl_op(PF, args, PF.nonlinearOpInitFloat)
+    model.getNdArray()[0] = w["model"].astype(np.float32)
+    op.forward(False, model, data)
+    g = oracle_grid(w)
+    for e in range(2):
+        ref = oracle_nl_irregular(w, g, geo["sz"][e], geo["sx"][e], geo["rz"], geo["rx"], w["model"],
+                                  smeth, rmeth)
+        assert np.linalg.norm(ref) > 0
+        assert rel_l2(data.getNdArray()[e], ref) < 1e-5, (e, rel_l2(data.getNdArray()[e], ref))
+    op64, *_ = _nl_op(PD, args, PD.nonlinearOpInitDouble)
+    assert op64.dotTestCpp(False, 1e-12)

@@ -101,6 +101,76 @@ def test_interp_weights():
     assert len(uniq) == 8
 
 
+def test_sinc_gauss_interp_closed_forms():
+    """Sinc and Gaussian spatial weights (spaceInterpGpu.cpp:139-207):
+      * nfilt=1 sinc is exactly bilinear (sin(pi u)/(pi u) over sin(pi u)/(pi (1-u)) is
+        (1-u)/u), so it must equal the linear weights;
+      * nfilt=1 gauss (standard deviation 2 cells) is the logistic 1/(1 + exp((2u - 1)/8)) per
+        axis;
+      * sinc is exact on a grid node (weight 1 there, 0 at the other integer offsets);
+      * weights sum to 1 and are separable; mirroring the device about the cell centre
+        mirrors the stencil;
+      * an 8-point sinc interpolates a smooth plane wave to a few parts in 1e3 on average, and
+        a z/x-transposed stencil does markedly worse."""
+    rng = np.random.default_rng(3)
+    nz, oz, dz, ox, dx = 80, -40.0, 10.0, -45.0, 10.0
+    for _ in range(25):
+        z, x = 100.0 + 300.0 * rng.random(), 100.0 + 400.0 * rng.random()
+        a = O.interp(nz, oz, dz, ox, dx, [z], [x], "sinc", 1)
+        b = O.interp(nz, oz, dz, ox, dx, [z], [x], "linear", 1)
+        assert np.array_equal(a[0], b[0]) and np.allclose(a[1], b[1], atol=1e-12)
+        uz, ux = (z - oz) / dz % 1.0, (x - ox) / dx % 1.0
+        lz0 = 1.0 / (1.0 + np.exp((2 * uz - 1) / 8))
+        lx0 = 1.0 / (1.0 + np.exp((2 * ux - 1) / 8))
+        c = O.interp(nz, oz, dz, ox, dx, [z], [x], "gauss", 1)
+        exp = [lx0 * lz0, lx0 * (1 - lz0), (1 - lx0) * lz0, (1 - lx0) * (1 - lz0)]
+        assert np.array_equal(c[0], b[0]) and np.allclose(c[1][0], exp, atol=1e-12)
+        for method in ("sinc", "gauss"):
+            for nf in (2, 3, 4):
+                idx, w, _ = O.interp(nz, oz, dz, ox, dx, [z], [x], method, nf)
+                n2 = 2 * nf
+                W2 = w[0].reshape(n2, n2)               # [ix][iz]
+                assert abs(W2.sum() - 1.0) < 1e-12
+                assert np.allclose(W2, np.outer(W2.sum(axis=1), W2.sum(axis=0)), atol=1e-13)
+                zr, xr = int((z - oz) / dz), int((x - ox) / dx)
+                assert idx[0, 0] == (xr - nf + 1) * nz + zr - nf + 1 and idx[0, 1] == idx[0, 0] + 1
+                assert idx[0, n2] == idx[0, 0] + nz
+                zm = 2 * oz + (2 * zr + 1) * dz - z      # mirror about the cell centre
+                xm = 2 * ox + (2 * xr + 1) * dx - x
+                _, wm, _ = O.interp(nz, oz, dz, ox, dx, [zm], [xm], method, nf)
+                assert np.allclose(wm[0].reshape(n2, n2), W2[::-1, ::-1], atol=1e-12)
+    # on a node
+    idx, w, _ = O.interp(nz, oz, dz, ox, dx, [oz + 17 * dz], [ox + 23 * dx], "sinc", 4)
+    W2 = w[0].reshape(8, 8)
+    assert abs(W2[3, 3] - 1.0) < 1e-14 and np.abs(np.delete(W2.ravel(), 27)).max() < 1e-14
+    assert idx[0, 27] == 23 * nz + 17
+    # smooth field
+    kz, kx = 0.11 / dz, 0.07 / dx
+    errs, errs_t = [], []
+    for _ in range(40):
+        z, x = 200.0 + 200.0 * rng.random(), 200.0 + 200.0 * rng.random()
+        idx, w, _ = O.interp(nz, oz, dz, ox, dx, [z], [x], "sinc", 4)
+        pz = oz + (idx[0] % nz) * dz
+        px = ox + (idx[0] // nz) * dx
+        f = np.cos(kz * pz + kx * px)
+        wt = w[0].reshape(8, 8).T.ravel()
+        errs.append(abs(w[0] @ f - np.cos(kz * z + kx * x)))
+        errs_t.append(abs(wt @ f - np.cos(kz * z + kx * x)))
+    assert np.mean(errs) < 8e-3 and max(errs) < 2e-2 and np.mean(errs_t) > 1.5 * np.mean(errs)
+
+
+def test_irregular_pipeline_composition_matches_shot_oracle():
+    """The test-side composition used for sinc/gauss acquisitions (oracle interpolation +
+    regular-level propagator) reproduces the oracle's own shot-level NL forward for linear
+    interpolation, so only the weights differ between the two methods."""
+    from helpers import oracle_nl_irregular
+    w = W.small(nz=40, nx=56, nts=60, sub=1)
+    g = oracle_grid(w)
+    ref = O.nl_fwd_shot(g, w["elastic"], w["src_z"][0], w["src_x"][0], w["rec_z"], w["rec_x"], w["model"])
+    got = oracle_nl_irregular(w, g, w["src_z"][0], w["src_x"][0], w["rec_z"], w["rec_x"], w["model"])
+    assert np.linalg.norm(ref) > 0 and rel_l2(got, ref) < 1e-13
+
+
 def test_time_interp_linear_exact():
     nts, sub = 20, 4
     coarse = np.stack([np.arange(nts) * 2.0 + 1.0])
