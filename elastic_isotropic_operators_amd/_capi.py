@@ -103,7 +103,8 @@ EXPORTED = ("eio_last_error", "eio_ctx_create", "eio_ctx_destroy", "eio_ctx_set_
             "eio_ctx_num_variants", "eio_fwi_forward", "eio_fwi_gradient", "eio_fwi_info",
             "eio_fwi_release", "eio_ctx_num_time_block_variants", "eio_ctx_time_block_info",
             "eio_ctx_set_time_block", "eio_ctx_set_step_limit", "eio_nl_forward_wavefield",
-            "eio_nl_adjoint_wavefield")
+            "eio_nl_adjoint_wavefield", "eio_model_set_abc", "eio_nl_forward_snapshots",
+            "eio_nl_step_states")
 
 
 def check(rc):
@@ -288,6 +289,34 @@ class Context:
         w = ctypes.c_void_p(int(wavefield)) if device_ptrs else self._wfld_ptr(wavefield)
         check(lib().eio_nl_adjoint_wavefield(self.h, len(shots), arr, EIO_DEVICE_PTRS if device_ptrs else 0,
                                              int(shot), w))
+
+    def set_abc(self, mid, strip, weights=None, scale=None):
+        """Attach (strip > 0) or detach (strip = 0) a learned boundary: weights FP32, the four
+        edges' three layers in torch Conv2d layout (49600 values), scale[5]."""
+        if strip == 0:
+            check(lib().eio_model_set_abc(self.h, int(mid), 0, None, None))
+            return
+        w = np.ascontiguousarray(weights, dtype=np.float32).reshape(-1)
+        if w.size != 4 * (16 * 10 + 16 * 16 + 5 * 16) * 25:
+            raise EioError("learned-boundary weights must hold 49600 values")
+        sc = np.ascontiguousarray(scale, dtype=np.float32).reshape(-1)
+        if sc.size != 5:
+            raise EioError("learned-boundary scale must hold 5 values")
+        check(lib().eio_model_set_abc(self.h, int(mid), int(strip), w.ctypes.data_as(ctypes.c_void_p),
+                                      sc.ctypes.data_as(ctypes.c_void_p)))
+
+    def nl_forward_snapshots(self, shots, shot, steps, out_ptr):
+        """NL forward copying shot `shot`'s (older, newer) levels after the given fine steps to
+        the device buffer out_ptr ([len(steps)][2][5][nx][nz]); traces are device pointers."""
+        arr = self._shots(shots)
+        st = np.ascontiguousarray(steps, dtype=np.int32)
+        check(lib().eio_nl_forward_snapshots(self.h, len(shots), arr, EIO_DEVICE_PTRS, int(shot), int(st.size),
+                                             st.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(int(out_ptr))))
+
+    def step_states(self, mid, nstate, states_ptr):
+        """One fine step of nstate device states [nstate][2][5][nx][nz] of model mid, in place."""
+        check(lib().eio_nl_step_states(self.h, int(mid), int(nstate), ctypes.c_void_p(int(states_ptr)),
+                                       EIO_DEVICE_PTRS))
 
     def _wfld_ptr(self, a):
         if not isinstance(a, np.ndarray) or not a.flags.c_contiguous or a.dtype != self.dtype:

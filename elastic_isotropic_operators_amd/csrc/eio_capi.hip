@@ -25,6 +25,7 @@
 #include "../../include/eio.h"
 #include "eio_kernels.h"
 #include "eio_step2.h"
+#include "eio_abc.h"
 
 namespace {
 
@@ -88,6 +89,9 @@ struct ModelBase {
     void* damp = nullptr;   // minPad
     void* filt = nullptr;   // 2h + 1 (last entry: 1/(2 dts))
     double cz[4], cx[4];
+    int abc_strip = 0;      // learned absorbing boundary: strip width (0: off)
+    void* abc_w = nullptr;  // [4][3][25][64][4] bf16 B fragments
+    float abc_scale[5] = {1.f, 1.f, 1.f, 1.f, 1.f};
 };
 
 // Forward-wavefield store of the checkpointed FWI gradient: the batch state at the start of
@@ -220,7 +224,8 @@ void model_free(ModelBase& m) {
     if (m.med) (void)hipFree(m.med);
     if (m.damp) (void)hipFree(m.damp);
     if (m.filt) (void)hipFree(m.filt);
-    m.med = m.damp = m.filt = nullptr;
+    if (m.abc_w) (void)hipFree(m.abc_w);
+    m.med = m.damp = m.filt = m.abc_w = nullptr;
 }
 
 ModelBase& get_model(Ctx* c, int id) {
@@ -656,16 +661,137 @@ void attach_wavefield(Run<T>& run, int shot, void* out, int flags) {
                                  c->stream));
 }
 
+// Learned absorbing boundary of the shots of a run whose model has one attached: tile list
+// (grouped by ordered shot, so the tiles of the active shots are a prefix), per-shot network
+// parameters and correction buffers.
+struct AbcRun {
+    int ntile = 0;
+    std::vector<int> shot_end;  // per ordered shot: end of its tiles
+    std::vector<int> nsteps;
+    eio::AbcTile* d_tiles = nullptr;
+    eio::AbcShot* d_par = nullptr;
+    int active(int k) const {
+        int e = 0;
+        for (size_t i = 0; i < nsteps.size(); ++i)
+            if (k < nsteps[i]) e = shot_end[i];
+        return e;
+    }
+};
+
+bool any_abc(Ctx* c, int nshot, const eio_shot* shots) {
+    for (int i = 0; i < nshot && shots; ++i)
+        if (get_model(c, shots[i].model).abc_strip > 0) return true;
+    return false;
+}
+
 template <typename T>
-void nl_forward(Ctx* c, int nshot, const eio_shot* shots, int flags, int wshot = -1, void* wfld = nullptr) {
-    // the saved wavefield and the free surface are handled by the one-step kernel
+AbcRun abc_prepare(Run<T>& run, const eio_shot* shots) {
+    AbcRun a;
+    if constexpr (sizeof(T) != 4) {
+        return a;
+    } else {
+        Ctx* c = run.c;
+        const int ns = int(run.sd.size());
+        std::vector<eio::AbcTile> tiles;
+        std::vector<eio::AbcShot> par(ns);
+        size_t corr_elems = 0;
+        for (int oi = 0; oi < ns; ++oi) {
+            const ModelBase& m = get_model(c, shots[run.plan.order[oi]].model);
+            if (m.abc_strip > 0) corr_elems += 5 * size_t(m.n);
+        }
+        float* corr = corr_elems ? static_cast<float*>(ws_get(c, "abc_corr", corr_elems * sizeof(float))) : nullptr;
+        size_t co = 0;
+        a.shot_end.resize(ns);
+        a.nsteps = run.plan.nsteps;
+        const int F = eio::FAT;
+        for (int oi = 0; oi < ns; ++oi) {
+            const ModelBase& m = get_model(c, shots[run.plan.order[oi]].model);
+            const int S = m.abc_strip, nz = m.g.nz, nx = m.g.nx;
+            if (S > 0) {
+                eio::AbcShot& P = par[oi];
+                P.w = static_cast<const unsigned short*>(m.abc_w);
+                for (int f = 0; f < 5; ++f) {
+                    P.scale[f] = m.abc_scale[f];
+                    P.inv_scale[f] = 1.0f / m.abc_scale[f];
+                }
+                P.corr = corr + co;
+                co += 5 * size_t(m.n);
+                for (int e = 0; e < 2; ++e)  // x-min, x-max strips: every update row
+                    for (int z0 = F; z0 < nz - F; z0 += eio::ABC_MAXOZ)
+                        tiles.push_back({oi, e, e == 0 ? F : nx - F - S, z0, S, std::min(eio::ABC_MAXOZ, nz - F - z0)});
+                for (int e = 2; e < 4; ++e)  // z-min, z-max strips: the columns between
+                    for (int x0 = F + S; x0 < nx - F - S; x0 += eio::ABC_MAXOX)
+                        tiles.push_back({oi, e, x0, e == 2 ? F : nz - F - S, std::min(eio::ABC_MAXOX, nx - F - S - x0), S});
+            }
+            a.shot_end[oi] = int(tiles.size());
+        }
+        a.ntile = int(tiles.size());
+        if (a.ntile == 0) return a;
+        a.d_tiles = static_cast<eio::AbcTile*>(ws_get(c, "abc_tiles", sizeof(eio::AbcTile) * tiles.size()));
+        a.d_par = static_cast<eio::AbcShot*>(ws_get(c, "abc_par", sizeof(eio::AbcShot) * ns));
+        EIO_CHECK_HIP(hipMemcpyAsync(a.d_tiles, tiles.data(), sizeof(eio::AbcTile) * tiles.size(), hipMemcpyHostToDevice,
+                                     c->stream));
+        EIO_CHECK_HIP(hipMemcpyAsync(a.d_par, par.data(), sizeof(eio::AbcShot) * ns, hipMemcpyHostToDevice, c->stream));
+        return a;
+    }
+}
+
+// corrections of fine step k on the strips of the active shots
+template <typename T>
+void launch_abc(Run<T>& run, const AbcRun& a, int k) {
+    if constexpr (sizeof(T) == 4) {
+        const int nt = a.active(k);
+        if (nt <= 0) return;
+        hipLaunchKernelGGL((eio::k_abc_cnn<float>), dim3(nt), dim3(eio::ABC_NTHR), 0, run.c->stream, run.d_shots,
+                           a.d_par, a.d_tiles, nt, k);
+        hipLaunchKernelGGL((eio::k_abc_add<float>), dim3(nt), dim3(eio::ABC_NTHR), 0, run.c->stream, run.d_shots,
+                           a.d_par, a.d_tiles, nt, k);
+    }
+}
+
+// Fine-step snapshots of one shot: after step steps[i] (after the boundary correction) the
+// older then the newer level, [nsnap][2][5][nx*nz].
+template <typename T>
+struct Snap {
+    int oi = -1, nsnap = 0, next = 0;
+    const int* steps = nullptr;
+    T* out = nullptr;
+};
+
+template <typename T>
+void nl_forward(Ctx* c, int nshot, const eio_shot* shots, int flags, int wshot = -1, void* wfld = nullptr,
+                int snap_shot = -1, int nsnap = 0, const int* snap_steps = nullptr, void* snap_out = nullptr) {
+    // the saved wavefield, snapshots, the free surface and the learned boundary use the
+    // one-step kernel
     bool any_fs = false;
     for (int i = 0; i < nshot && shots; ++i) any_fs = any_fs || get_model(c, shots[i].model).g.surface;
-    const bool two = sizeof(T) == 4 && c->time_block == 2 && !wfld && !any_fs;
+    const bool abc = any_abc(c, nshot, shots);
+    const bool two = sizeof(T) == 4 && c->time_block == 2 && !wfld && !any_fs && !abc && nsnap == 0;
     Run<T> run = prepare<T>(c, nshot, shots, Op::NL_FWD, flags, nullptr, nullptr, 0, two);
     if (wfld) attach_wavefield(run, wshot, wfld, flags);
+    AbcRun ar;
+    if (abc) ar = abc_prepare(run, shots);
     int kmax = run.plan.nsteps.empty() ? 0 : run.plan.nsteps[0];
     if (c->step_limit > 0) kmax = std::min(kmax, c->step_limit);
+    Snap<T> sn;
+    size_t snap_n = 0;
+    if (nsnap > 0) {
+        if (!snap_steps || !snap_out) fail("snapshot steps or output pointer is NULL");
+        if (!(flags & EIO_DEVICE_PTRS)) fail("snapshots need EIO_DEVICE_PTRS (device output)");
+        if (snap_shot < 0 || snap_shot >= nshot) fail("snapshot shot index out of range");
+        int oi = 0;
+        while (run.plan.order[oi] != snap_shot) ++oi;
+        sn.oi = oi;
+        sn.nsnap = nsnap;
+        sn.steps = snap_steps;
+        sn.out = static_cast<T*>(snap_out);
+        snap_n = size_t(run.sd[oi].nz) * run.sd[oi].nx;
+        for (int i = 0; i < nsnap; ++i) {
+            if (snap_steps[i] < 0 || snap_steps[i] >= run.plan.nsteps[oi] || snap_steps[i] >= kmax)
+                fail("snapshot step out of range");
+            if (i > 0 && snap_steps[i] <= snap_steps[i - 1]) fail("snapshot steps must increase");
+        }
+    }
     long long nlaunch = 0;
     timing_begin<T>(c);
     EIO_CHECK_HIP(hipEventRecord(c->ev[2], c->stream));
@@ -678,11 +804,111 @@ void nl_forward(Ctx* c, int nshot, const eio_shot* shots, int flags, int wshot =
                 return;
             }
         }
-        for (int k = 0; k < kmax; ++k, ++nlaunch) launch_fwd<T, eio::M_NL>(run, active_tiles(run.plan, k), k, 0);
+        for (int k = 0; k < kmax; ++k, ++nlaunch) {
+            launch_fwd<T, eio::M_NL>(run, active_tiles(run.plan, k), k, 0);
+            if (abc) launch_abc(run, ar, k);
+            if (sn.next < sn.nsnap && sn.steps[sn.next] == k) {
+                const eio::ShotDev<T>& d = run.sd[sn.oi];
+                const T* newer = (k & 1) ? d.B : d.A;
+                const T* older = (k & 1) ? d.A : d.B;
+                T* o = sn.out + size_t(sn.next) * 10 * snap_n;
+                EIO_CHECK_HIP(hipMemcpyAsync(o, older, 5 * snap_n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+                EIO_CHECK_HIP(hipMemcpyAsync(o + 5 * snap_n, newer, 5 * snap_n * sizeof(T), hipMemcpyDeviceToDevice,
+                                             c->stream));
+                ++sn.next;
+            }
+        }
     });
     EIO_CHECK_HIP(hipEventRecord(c->ev[3], c->stream));
     timing_end<T>(c, nlaunch);
     finish(run);
+}
+
+// One fine step (no sources or receivers) of `nstate` states of one model, in place:
+// states [nstate][2][5][nx*nz] hold (older, newer); afterwards (newest, newer).  The
+// model's learned boundary, when attached, corrects the new level as in a propagation.
+template <typename T>
+void step_states(Ctx* c, int model, int nstate, void* states, int flags) {
+    if (nstate <= 0) fail("nstate must be > 0");
+    if (!states) fail("states pointer is NULL");
+    if (!(flags & EIO_DEVICE_PTRS)) fail("eio_nl_step_states needs EIO_DEVICE_PTRS (device states)");
+    const ModelBase& m = get_model(c, model);
+    std::vector<eio_shot> sh(nstate);
+    for (auto& s : sh) {
+        std::memset(&s, 0, sizeof(s));
+        s.model = model;
+    }
+    const int limit = c->step_limit;
+    c->step_limit = 0;
+    Run<T> run = prepare<T>(c, nstate, sh.data(), Op::NL_FWD, flags | EIO_DEVICE_PTRS, nullptr);
+    c->step_limit = limit;
+    T* st = static_cast<T*>(states);
+    for (int oi = 0; oi < nstate; ++oi) {
+        T* base = st + size_t(run.plan.order[oi]) * 10 * size_t(m.n);
+        run.sd[oi].A = base;
+        run.sd[oi].B = base + 5 * size_t(m.n);
+    }
+    EIO_CHECK_HIP(hipMemcpyAsync(run.d_shots, run.sd.data(), sizeof(eio::ShotDev<T>) * nstate, hipMemcpyHostToDevice,
+                                 c->stream));
+    AbcRun ar;
+    if (m.abc_strip > 0) ar = abc_prepare(run, sh.data());
+    launch_fwd<T, eio::M_NL>(run, int(run.plan.tiles.size()), 0, 0);
+    if (m.abc_strip > 0) launch_abc(run, ar, 0);
+    EIO_CHECK_HIP(hipGetLastError());
+    finish(run);
+}
+
+// Attach (strip > 0) or detach (strip == 0) a learned boundary to a model.  weights: FP32,
+// per edge (x-min, x-max, z-min, z-max) the three layers in torch Conv2d layout
+// [out][in][5][5] with (out, in) = (16, 10), (16, 16), (5, 16), the kernel's first axis
+// along x and second along z; scale: 5 per-field scales.
+void set_abc(Ctx* c, int model, int strip, const float* w, const float* scale) {
+    ModelBase& m = get_model(c, model);
+    if (c->prec != 4) fail("the learned boundary runs in FP32 contexts only");
+    if (strip < 0 || strip > eio::ABC_MAXOX) fail("strip must be in [0, 16]");
+    if (strip > 0) {
+        if (m.g.surface) fail("the learned boundary is not available with the free surface");
+        if (!w || !scale) fail("weights or scale is NULL");
+        if (m.g.nz - 2 * eio::FAT < 2 * strip || m.g.nx - 2 * eio::FAT < 2 * strip)
+            fail("grid too small for the boundary strip");
+        for (int f = 0; f < 5; ++f)
+            if (!(scale[f] > 0.f) || !std::isfinite(scale[f])) fail("scales must be positive and finite");
+    }
+    if (m.abc_w) {
+        EIO_CHECK_HIP(hipFree(m.abc_w));
+        m.abc_w = nullptr;
+    }
+    m.abc_strip = 0;
+    if (strip == 0) return;
+    const int cin[3] = {10, 16, 16}, cout[3] = {16, 16, 5};
+    std::vector<unsigned short> frag(4 * size_t(eio::ABC_WEDGE), 0);
+    auto bf = [](float f) {
+        unsigned u;
+        std::memcpy(&u, &f, 4);
+        u += 0x7FFFu + ((u >> 16) & 1u);
+        return (unsigned short)(u >> 16);
+    };
+    size_t wo = 0;
+    for (int e = 0; e < 4; ++e)
+        for (int l = 0; l < 3; ++l) {
+            for (int o = 0; o < cout[l]; ++o)
+                for (int i = 0; i < cin[l]; ++i)
+                    for (int dx = 0; dx < 5; ++dx)
+                        for (int dz = 0; dz < 5; ++dz) {
+                            const float v = w[wo + ((size_t(o) * cin[l] + i) * 5 + dx) * 5 + dz];
+                            const int tap = dx * 5 + dz;
+                            // B fragment: lane = 16 * (i / 4) + o, element i % 4
+                            const int lane = 16 * (i / 4) + o;
+                            frag[size_t(e) * eio::ABC_WEDGE + size_t(l) * eio::ABC_TAPS * eio::ABC_FRAG +
+                                 size_t(tap) * eio::ABC_FRAG + lane * 4 + (i % 4)] = bf(v);
+                        }
+            wo += size_t(cout[l]) * cin[l] * 25;
+        }
+    DeviceGuard dg(c->device);
+    EIO_CHECK_HIP(hipMalloc(&m.abc_w, frag.size() * sizeof(unsigned short)));
+    EIO_CHECK_HIP(hipMemcpy(m.abc_w, frag.data(), frag.size() * sizeof(unsigned short), hipMemcpyHostToDevice));
+    for (int f = 0; f < 5; ++f) m.abc_scale[f] = scale[f];
+    m.abc_strip = strip;
 }
 
 template <typename T>
@@ -1155,6 +1381,33 @@ int eio_nl_forward_wavefield(eio_ctx* ctx, int nshot, const eio_shot* shots, int
         if (!wavefield) fail("wavefield is NULL");
         if (c->prec == 4) nl_forward<float>(c, nshot, shots, flags, shot, wavefield);
         else nl_forward<double>(c, nshot, shots, flags, shot, wavefield);
+    });
+}
+
+int eio_nl_forward_snapshots(eio_ctx* ctx, int nshot, const eio_shot* shots, int flags, int shot, int nsnap,
+                             const int* steps, void* out) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        DeviceGuard dg(c->device);
+        if (nsnap <= 0) fail("nsnap must be > 0");
+        if (c->prec == 4) nl_forward<float>(c, nshot, shots, flags, -1, nullptr, shot, nsnap, steps, out);
+        else nl_forward<double>(c, nshot, shots, flags, -1, nullptr, shot, nsnap, steps, out);
+    });
+}
+
+int eio_nl_step_states(eio_ctx* ctx, int model, int nstate, void* states, int flags) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        DeviceGuard dg(c->device);
+        if (c->prec == 4) step_states<float>(c, model, nstate, states, flags);
+        else step_states<double>(c, model, nstate, states, flags);
+    });
+}
+
+int eio_model_set_abc(eio_ctx* ctx, int model, int strip, const float* weights, const float* scale) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        set_abc(c, model, strip, weights, scale);
     });
 }
 

@@ -129,6 +129,31 @@ int eio_nl_forward_wavefield(eio_ctx* ctx, int nshot, const eio_shot* shots, int
 int eio_nl_adjoint_wavefield(eio_ctx* ctx, int nshot, const eio_shot* shots, int flags, int shot,
                              void* wavefield);
 
+/* Learned absorbing boundary (the north-star method change; no counterpart in the reference,
+ * whose only boundary is the cosine sponge of kernelsGpuElastic.cu:1192-1221).  Attach
+ * (strip in 1..16) or detach (strip = 0) a small CNN that corrects the `strip`-cell edge band
+ * of the new level after every fine step of an FP32 NL forward of this model (the model's
+ * own sponge, minPad, stays active).  weights: FP32, per edge (x-min, x-max, z-min, z-max)
+ * the three 5x5 layers in torch Conv2d layout [out][in][5][5] with (out, in) = (16, 10),
+ * (16, 16), (5, 16), first kernel axis along x; inputs are the 5 fields of the new and the
+ * 5 of the previous level divided by scale[5], outputs times scale are added to the band.
+ * Rounded to BF16 on upload.  Not available with the free surface or in FP64 contexts. */
+int eio_model_set_abc(eio_ctx* ctx, int model_id, int strip, const float* weights, const float* scale);
+
+/* eio_nl_forward that also copies the raw state of shot `shot` after each fine step
+ * steps[i] (0-based, increasing): out [nsnap][2][5][nx][nz], the older then the newer level
+ * (device pointer; EIO_DEVICE_PTRS required).  The older level is stored as the fused step
+ * kernel keeps it: without its second sponge factor (applied when it is next read).
+ * No counterpart in the reference: training data of the learned boundary. */
+int eio_nl_forward_snapshots(eio_ctx* ctx, int nshot, const eio_shot* shots, int flags, int shot, int nsnap,
+                             const int* steps, void* out);
+
+/* One fine step without sources or receivers of nstate states of model_id, in place:
+ * states [nstate][2][5][nx][nz] (device, EIO_DEVICE_PTRS) hold (older, newer) as the
+ * snapshots do and become (newest, newer); an attached learned boundary is applied.
+ * No counterpart in the reference: training data of the learned boundary. */
+int eio_nl_step_states(eio_ctx* ctx, int model_id, int nstate, void* states, int flags);
+
 /* Born forward: dm = 5 staggered and scaled perturbations (drhox*rhoxDtw, drhoz*rhozDtw,
  * dlame*2dtw, dmu*2dtw, dmuxz*2dtw), nz*nx each, shared by all shots (all shots must use
  * the same model).  rec_tr is overwritten.

@@ -1,0 +1,114 @@
+"""Learned absorbing boundary on the GPU: the MFMA kernel against the torch network with BF16
+rounding at the same points, zero weights leave the propagation unchanged, snapshots and the
+one-step entry agree with the propagation, and a small end-to-end training run."""
+import numpy as np
+import pytest
+
+from helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _small_shot(ctx, strip=10, nz_i=40, nx_i=72, nts=200, seed=1):
+    from elastic_isotropic_operators_amd import abc_learn as A, workloads as W
+    rng = np.random.default_rng(seed)
+    vvr = W.layered_vvr(rng, nz_i, nx_i)
+    w = A.padded(vvr, strip)
+    return A.Shot(ctx, w, nts, 0.001, 2, 150.0, 300.0, np.full(20, 30.0), np.arange(20) * 30.0 + 20.0), w, vvr
+
+
+def test_abc_kernel_matches_bf16_torch_network():
+    import torch
+    from elastic_isotropic_operators_amd import _capi, abc_learn as A
+    ctx = _capi.Context(0, np.float32)
+    shot, w, _ = _small_shot(ctx)
+    nz, nx, S = w["nz"], w["nx"], 10
+    g = torch.Generator(device="cuda").manual_seed(7)
+    st = torch.randn((3, 2, 5, nx, nz), device="cuda", generator=g)
+    st[..., :4, :] = 0
+    st[..., -4:, :] = 0
+    st[..., :4] = 0
+    st[..., -4:] = 0
+    scale = np.array([0.5, 0.7, 3.0, 2.0, 1.5], np.float32)
+    net = A.AbcNet("cuda", seed=11)
+    with torch.no_grad():
+        for e in range(4):
+            net.W[e][2] *= 10.0  # corrections of the order of the fields
+    plain = st.clone()
+    ctx.step_states(shot.mid, 3, plain.data_ptr())
+    ctx.set_abc(shot.mid, S, net.flat(), scale)
+    learned = st.clone()
+    ctx.step_states(shot.mid, 3, learned.data_ptr())
+    torch.cuda.synchronize()
+    diff = (learned[:, 0] - plain[:, 0]).double()
+    assert torch.equal(learned[:, 1], plain[:, 1])
+    reg = A.band_regions(nz, nx, S)
+    mask = torch.zeros((nx, nz), dtype=torch.bool, device="cuda")
+    for (x0, x1, z0, z1) in reg:
+        mask[x0:x1, z0:z1] = True
+    assert float(diff[..., ~mask].abs().max()) == 0.0
+    sc = torch.tensor(scale, device="cuda")
+    for i in range(3):
+        feats = A.features(plain[i, 0], plain[i, 1], sc, reg)
+        with torch.no_grad():
+            for e in range(4):
+                ref = (net.edge_forward(e, feats[e][None], bf16=True)[0] * sc.view(5, 1, 1)).double()
+                got = A.band(diff[i], reg)[e]
+                err = float((got - ref).norm() / ref.norm())
+                assert err < 2e-3, (i, e, err)
+    ctx.close()
+
+
+def test_abc_zero_weights_leave_propagation_unchanged():
+    import torch
+    from elastic_isotropic_operators_amd import _capi, abc_learn as A
+    ctx = _capi.Context(0, np.float32)
+    ctx.set_time_block(1, 0)
+    shot, w, _ = _small_shot(ctx)
+    d0, _ = shot.forward()
+    zero = A.AbcNet("cuda", seed=0)
+    with torch.no_grad():
+        for e in range(4):
+            zero.W[e][2].zero_()
+    ctx.set_abc(shot.mid, 10, zero.flat(), np.ones(5, np.float32))
+    d1, _ = shot.forward()
+    assert np.array_equal(d0, d1)
+    ctx.set_abc(shot.mid, 0)
+    d2, _ = shot.forward()
+    assert np.array_equal(d0, d2)
+    ctx.close()
+
+
+def test_snapshots_and_one_step_entry_agree_with_propagation():
+    """After the source has ended, one step of a snapshot equals the next snapshot bit for bit."""
+    import torch
+    from elastic_isotropic_operators_amd import _capi
+    ctx = _capi.Context(0, np.float32)
+    shot, w, _ = _small_shot(ctx, nts=200)
+    for t in shot.src:
+        t[:, 120:] = 0.0
+    k = 2 * 2 * 130 + 1
+    snaps = shot.snapshots([k, k + 1, k + 7])
+    st = snaps[0:1].clone()
+    ctx.step_states(shot.mid, 1, st.data_ptr())
+    torch.cuda.synchronize()
+    assert torch.equal(st[0, 0], snaps[1, 1]) and torch.equal(st[0, 1], snaps[1, 0])
+    assert float(snaps[2].abs().max()) > 0
+    with pytest.raises(_capi.EioError):
+        shot.snapshots([k + 1, k])
+    ctx.close()
+
+
+def test_abc_small_training_run_and_reflection_test():
+    import torch
+    from elastic_isotropic_operators_amd import abc_learn as A, workloads as W
+    data, scale = A.generate(n_models=3, nz_i=40, nx_i=72, nts=500, pairs_per_model=4, first_step=300, log=None)
+    assert all(X.shape[0] == 12 for X, _ in data) and bool(torch.isfinite(scale).all())
+    net, val = A.train(data, scale, iters=60, batch=8, log=None)
+    assert all(np.isfinite(v) for v in val)
+    vvr = W.layered_vvr(np.random.default_rng(99), 40, 72)
+    res = A.reflection_test(vvr, net, scale.cpu().numpy(), nts=500, mid=30, wide=60, log=None)
+    assert set(res) == {"wide60", "plain10", "plain30", "learned10"}
+    assert res["wide60"]["refl"] == 0.0
+    assert res["plain10"]["refl"] > res["plain30"]["refl"] > 0
+    assert np.isfinite(res["learned10"]["refl"])
