@@ -176,6 +176,34 @@ def run_fwi_leg(args):
     raise RuntimeError("FWI leg failed (exit %d)" % r.returncode)
 
 
+ABC_WEIGHTS = os.path.join(ROOT, "elastic_isotropic_operators_amd", "data", "abc_strip10.npz")
+
+
+def abc_child(args):
+    """Learned-boundary leg (child process): one shot on the 1000 x 3000 layered model with
+    the wide (120-cell) sponge as reference, the plain 10- and 60-cell sponges and the learned
+    10-cell boundary; reflection energy at receivers inside the model and device time."""
+    import torch  # noqa: F401  (before the propagator library)
+    from elastic_isotropic_operators_amd import abc_learn as A
+    net, scale, strip, on = A.load(ABC_WEIGHTS)
+    vvr, nts = W.learned_abc_c4(scale=args.abc_scale)
+    res = A.reflection_test(vvr, net, scale, strip=strip, mid=60, wide=120, nts=nts, rec_every=4, on=on,
+                            time_block=2, log=log)
+    out = {"workload": "learned_abc_c4_%dx%d_nts%d" % (vvr.shape[2], vvr.shape[1], nts), "strip": strip,
+           "runs": res, "weights": os.path.relpath(ABC_WEIGHTS, ROOT)}
+    print("ABC_JSON " + json.dumps(out), flush=True)
+
+
+def run_abc_leg(args):
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--abc-child", "--abc-scale", str(args.abc_scale)]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=900)
+    for line in r.stdout.splitlines():
+        if line.startswith("ABC_JSON "):
+            return json.loads(line[len("ABC_JSON "):])
+    raise RuntimeError("learned-boundary leg failed (exit %d)" % r.returncode)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,9 +227,15 @@ def main():
     ap.add_argument("--fwi-warmup", type=int, default=1)
     ap.add_argument("--fwi-scale", type=float, default=1.0)
     ap.add_argument("--fwi-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--abc", type=int, default=1, help="run the learned-boundary leg (needs its weights)")
+    ap.add_argument("--abc-scale", type=float, default=1.0)
+    ap.add_argument("--abc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.fwi_child:
         fwi_child(args)
+        return
+    if args.abc_child:
+        abc_child(args)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box of this build has a single MI355X; "
@@ -313,6 +347,10 @@ def main():
         t0 = time.perf_counter()
         res["fwi_gradient"] = run_fwi_leg(args)
         log("fwi leg %.1f s" % (time.perf_counter() - t0))
+    if args.abc and os.path.exists(ABC_WEIGHTS):
+        t0 = time.perf_counter()
+        res["learned_boundary"] = run_abc_leg(args)
+        log("learned-boundary leg %.1f s" % (time.perf_counter() - t0))
     if not args.no_cpu:
         t0 = time.perf_counter()
         res["cpu_baseline"] = cpu_baseline(shots_w)

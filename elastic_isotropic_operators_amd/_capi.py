@@ -290,19 +290,21 @@ class Context:
         check(lib().eio_nl_adjoint_wavefield(self.h, len(shots), arr, EIO_DEVICE_PTRS if device_ptrs else 0,
                                              int(shot), w))
 
-    def set_abc(self, mid, strip, weights=None, scale=None):
-        """Attach (strip > 0) or detach (strip = 0) a learned boundary: weights FP32, the four
-        edges' three layers in torch Conv2d layout (49600 values), scale[5]."""
-        if strip == 0:
+    def set_abc(self, mid, on, weights=None, scale=None):
+        """Attach (on = 1: difference inputs, 2: level inputs; + 10 x depth to write only the
+        `depth` frame rows next to the update region) or detach (0) a learned boundary:
+        weights FP32, the four edges' three layers in torch Conv2d layout (49600 values),
+        scale[10] (new level, then differences or previous level)."""
+        if not on:
             check(lib().eio_model_set_abc(self.h, int(mid), 0, None, None))
             return
         w = np.ascontiguousarray(weights, dtype=np.float32).reshape(-1)
         if w.size != 4 * (16 * 10 + 16 * 16 + 5 * 16) * 25:
             raise EioError("learned-boundary weights must hold 49600 values")
         sc = np.ascontiguousarray(scale, dtype=np.float32).reshape(-1)
-        if sc.size != 5:
-            raise EioError("learned-boundary scale must hold 5 values")
-        check(lib().eio_model_set_abc(self.h, int(mid), int(strip), w.ctypes.data_as(ctypes.c_void_p),
+        if sc.size != 10:
+            raise EioError("learned-boundary scale must hold 10 values")
+        check(lib().eio_model_set_abc(self.h, int(mid), int(on), w.ctypes.data_as(ctypes.c_void_p),
                                       sc.ctypes.data_as(ctypes.c_void_p)))
 
     def nl_forward_snapshots(self, shots, shot, steps, out_ptr):

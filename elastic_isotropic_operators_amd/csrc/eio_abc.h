@@ -1,27 +1,35 @@
-// eio_abc.h -- learned absorbing boundary: a small CNN that corrects the edge strip of the
-// new time level after every fine step (BF16 MFMA, FP32 accumulation).
+// eio_abc.h -- learned absorbing boundary: a small CNN that supplies the field outside the
+// update region (the FAT frame the stencil reads) after every fine step (BF16 MFMA, FP32
+// accumulation).
 //
-// The model keeps a narrow plain cosine sponge of `strip` cells.  After the fused step
-// kernel has written the new level, each of the four edges runs its own 3-layer network on
-// the cells of its strip:
-//   input   16 channels per cell: the 5 fields of the new level and the 5 of the previous
-//           level, each divided by a per-field scale (6 zero channels)
+// Without it the stencil of the outermost update cells reads zeros in the 4-cell FAT frame:
+// the edge of the grid reflects, and only the cosine sponge in front of it hides that.  With
+// it, after the fused step kernel has written the new level, each of the four edges runs its
+// own 3-layer network that predicts the 5 fields of the new level on its side of the frame
+// (the values a wider grid would hold there), so the next step's stencil reads a continued
+// wavefield instead of zeros; the model's sponge stays active.
+//   input   16 channels per cell, on the update cells (zero elsewhere): the 5 fields of the
+//           new level divided by a per-field scale, and either the 5 differences new -
+//           previous (formed in FP32, before the BF16 rounding, which would otherwise swamp
+//           them: one fine step changes a field by about a percent) or the 5 fields of the
+//           previous level, divided by their own scale; 6 zero channels
 //   conv1   5x5, 16 -> 16, leaky ReLU (slope ABC_LEAK), no bias
 //   conv2   5x5, 16 -> 16, leaky ReLU, no bias
-//   conv3   5x5, 16 -> 5 (padded to 16), no bias; output times the field scale = correction
-// All three convolutions are "valid" convolutions on the field arrays extended by zeros
-// outside the array, so the receptive field of a strip cell is its 13 x 13 neighbourhood
-// and the network is positively homogeneous (a wavefield scaled by c > 0 gets its
-// correction scaled by c), which keeps it independent of the source amplitude.
+//   conv3   5x5, 16 -> 5 (padded to 16), no bias; output times the field scale = the frame
+// All three convolutions are "valid" convolutions on the field arrays extended by zeros, so a
+// frame cell sees the 13 x 13 neighbourhood around it, i.e. the first 6 update cells; the
+// network is positively homogeneous (a wavefield scaled by c > 0 gets its frame scaled by c).
 //
-// Ownership: the x-min and x-max strips own every update row; the z-min and z-max strips own
-// the columns between them.  A tile is an output rectangle of at most 16 columns and 24 rows;
-// the workgroup stages the tile's input (+6 cells) to LDS as BF16 and computes conv1 on
-// (+4), conv2 on (+2) and conv3 on the tile, one 16-row block of one column per MFMA chain:
-// for each of the 25 taps, A = 16 cells x 16 input channels (read from LDS), B = 16 input x
-// 16 output channels (the tap's weights, held in VGPRs), with v_mfma_f32_16x16x16_bf16.
-// Corrections go to a separate buffer (neighbouring tiles still read the uncorrected
-// level) and k_abc_add adds them afterwards.
+// Ownership: the x-min and x-max frames own the update rows, the z-min and z-max frames the
+// update columns (the frame corners are never read by the stencil).  A tile is an output
+// rectangle of the frame: 4 columns x at most 24 rows (x frames) or at most 16 columns x 4
+// rows (z frames); the workgroup stages the tile's input (+6 cells) to LDS as BF16 and
+// computes conv1 on (+4), conv2 on (+2) and conv3 on the tile, an MFMA block being 16 cells
+// along the frame (rows of one column for the x frames, columns of one row for the z frames,
+// whose LDS layout is transposed for that): for each of the 25 taps, A = 16 cells x 16 input
+// channels (read from LDS), B = 16 input x 16 output channels (the tap's weights, held in
+// VGPRs), with v_mfma_f32_16x16x16_bf16.  The network reads only update cells and writes only
+// frame cells, so tiles need no ordering among themselves: one launch per step.
 #pragma once
 #include "eio_kernels.h"
 
@@ -44,16 +52,17 @@ constexpr int ABC_LDS1 = (ABC_MAXOX + 8) * (ABC_MAXR3 + 8) * ABC_CH;
 struct AbcTile {
     int shot;    // ordered shot index
     int edge;    // 0 x-min, 1 x-max, 2 z-min, 3 z-max
-    int x0, z0;  // first output cell
+    int x0, z0;  // first output (frame) cell
     int ox, oz;  // output extent: ox <= ABC_MAXOX, oz <= ABC_MAXOZ
 };
 
 // Per-shot parameters of the network (device pointers), indexed by the ordered shot.
 struct AbcShot {
     const unsigned short* w;  // [4 edges][3 layers][25 taps][64 lanes][4] bf16, MFMA B-fragment order
-    float scale[5];           // per-field scale
-    float inv_scale[5];
-    float* corr;              // [5][nx*nz] corrections of the last step
+    float scale[5];           // per-field scale of the new level (and of the output)
+    float inv_scale[10];      // inverse scales of the 10 input channels
+    int diff;                 // 1: channels 5..9 are new - previous, 0: the previous level
+    int depth;                // frame rows written, counted from the update region (1..4; the rest stay 0)
 };
 
 typedef short abc_s4 __attribute__((ext_vector_type(4)));
@@ -66,53 +75,120 @@ __device__ __forceinline__ unsigned short abc_bf16(float f) {  // round to neare
 }
 __device__ __forceinline__ float abc_leaky(float v) { return v > 0.f ? v : ABC_LEAK * v; }
 
-// One convolution layer on LDS: input (cI x rI cells, 16 channels), output (cI-4) x rO where
-// rO = rI - 4.  LAST: write channels < 5 of the first ox x oz cells to the correction buffer.
-template <bool LAST>
-__device__ __forceinline__ void abc_layer(const unsigned short* __restrict__ in, int cI, int rI,
-                                          unsigned short* __restrict__ out, const unsigned short* __restrict__ wl,
-                                          int lane, int wave, float* corr, const AbcShot& P, unsigned n, int nz,
+// One convolution layer on LDS.  ZMAJ (x-min/x-max frames): cells stored [col][row], an MFMA
+// block is 16 consecutive rows of one column; XMAJ (z-min/z-max frames, 4 rows tall): cells
+// stored [row][col], a block is 16 consecutive columns of one row.  Input: nA x nB cells
+// (A = the slow axis of the layout: columns for ZMAJ, rows for XMAJ), 16 channels; output
+// (nA - 4) x (nB - 4).  Each wave runs two blocks' 25-tap MFMA chains interleaved.  LAST:
+// write channels < 5 of the first ox x oz cells, times the field scale, to `field`.
+__device__ __forceinline__ int nx_of(unsigned n, int nz) { return int(n / unsigned(nz)); }
+
+template <bool LAST, bool XMAJ, typename T>
+__device__ __forceinline__ void abc_layer(const unsigned short* __restrict__ in, int nA, int nB,
+                                          unsigned short* __restrict__ out, const abc_s4 (&b)[ABC_TAPS],
+                                          int lane, int wave, T* field, const AbcShot& P, unsigned n, int nz,
                                           int x0, int z0, int ox, int oz) {
-    const int cO = cI - 4, rO = rI - 4;
-    // B fragments of the 25 taps: lane holds W[tap][cin = 4*(lane/16) + j][cout = lane%16]
-    abc_s4 b[ABC_TAPS];
-#pragma unroll
-    for (int t = 0; t < ABC_TAPS; ++t) b[t] = *reinterpret_cast<const abc_s4*>(wl + (t * 64 + lane) * 4);
-    const int nbr = (rO + 15) >> 4;
-    const int nblk = cO * nbr;
+    constexpr int NW = ABC_NTHR / 64;
+    const int oA = nA - 4, oB = nB - 4;
+    const int nbb = (oB + 15) >> 4;
+    const int nblk = oA * nbb;
     const int i = lane & 15, g = lane >> 4;
-    for (int blk = wave; blk < nblk; blk += ABC_NTHR / 64) {
-        const int col = blk / nbr;
-        const int rb = blk - col * nbr;
-        const int row0 = min(rb * 16, rO - 16);
-        abc_f4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int blk = wave; blk < nblk; blk += 2 * NW) {
+        const int blk2 = blk + NW;
+        const bool two = blk2 < nblk;
+        const int a0 = blk / nbb, b0 = min((blk - a0 * nbb) * 16, oB - 16);
+        const int a1 = two ? blk2 / nbb : a0;
+        const int b1 = two ? min((blk2 - a1 * nbb) * 16, oB - 16) : b0;
+        abc_f4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int dx = 0; dx < 5; ++dx) {
 #pragma unroll
             for (int dz = 0; dz < 5; ++dz) {
-                const abc_s4 a = *reinterpret_cast<const abc_s4*>(
-                    in + ((col + dx) * rI + row0 + i + dz) * ABC_CH + 4 * g);
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[dx * 5 + dz], acc, 0, 0, 0);
+                // tap (dx, dz): cell (+dx along x, +dz along z) of the output cell
+                const int dA = XMAJ ? dz : dx, dB = XMAJ ? dx : dz;
+                const abc_s4 q0 = *reinterpret_cast<const abc_s4*>(in + ((a0 + dA) * nB + b0 + i + dB) * ABC_CH + 4 * g);
+                const abc_s4 q1 = *reinterpret_cast<const abc_s4*>(in + ((a1 + dA) * nB + b1 + i + dB) * ABC_CH + 4 * g);
+                acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(q0, b[dx * 5 + dz], acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(q1, b[dx * 5 + dz], acc1, 0, 0, 0);
             }
         }
-        // acc[j] = output cell (col, row0 + 4g + j), channel i
-        if constexpr (!LAST) {
+        // acc[j] = output cell (a, b + 4g + j), channel i
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-                out[(col * rO + row0 + 4 * g + j) * ABC_CH + i] = abc_bf16(abc_leaky(acc[j]));
-        } else {
-            if (i < 5 && col < ox) {
+        for (int h = 0; h < 2; ++h) {
+            if (h == 1 && !two) break;
+            const int a = h ? a1 : a0, bb = h ? b1 : b0;
+            const abc_f4 acc = h ? acc1 : acc0;
+            if constexpr (!LAST) {
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int r = row0 + 4 * g + j;
-                    if (r < oz) corr[size_t(i) * n + size_t(x0 + col) * nz + (z0 + r)] = acc[j] * P.scale[i];
+                for (int j = 0; j < 4; ++j) out[(a * oB + bb + 4 * g + j) * ABC_CH + i] = abc_bf16(abc_leaky(acc[j]));
+            } else {
+                if (i < 5) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int col = XMAJ ? bb + 4 * g + j : a, row = XMAJ ? a : bb + 4 * g + j;
+                        // distance of the frame cell from the update region, 0 = adjacent
+                        const int x = x0 + col, z = z0 + row;
+                        const int dist = XMAJ ? (z < FAT ? FAT - 1 - z : z - (nz - FAT))
+                                              : (x < FAT ? FAT - 1 - x : x - (nx_of(n, nz) - FAT));
+                        if (col < ox && row < oz && dist < P.depth)
+                            field[size_t(i) * n + size_t(x) * nz + z] = T(acc[j] * P.scale[i]);
+                    }
                 }
             }
         }
     }
 }
 
-// Corrections of fine step k (the new level is (k & 1) ? B : A, the previous one the other).
+template <bool XMAJ, typename T>
+__device__ __forceinline__ void abc_tile(const ShotDev<T>& S, const AbcShot& P, const AbcTile& tl, int k,
+                                         unsigned short* lds0, unsigned short* lds1,
+                                         const abc_s4 (&w1)[ABC_TAPS], const abc_s4 (&w2)[ABC_TAPS],
+                                         const abc_s4 (&w3)[ABC_TAPS]) {
+    const int nz = S.nz, nx = S.nx;
+    const unsigned n = unsigned(nz) * unsigned(nx);
+    T* nw = (k & 1) ? S.B : S.A;
+    const T* pv = (k & 1) ? S.A : S.B;
+    // layer-3 extent along the block axis is at least one block (16 cells)
+    const int c3 = XMAJ ? max(tl.ox, 16) : tl.ox;
+    const int r3 = XMAJ ? tl.oz : max(tl.oz, 16);
+    const int cin = c3 + 12, rin = r3 + 12;
+    const int nA = XMAJ ? rin : cin, nB = XMAJ ? cin : rin;
+    const int xs = tl.x0 - 6, zs = tl.z0 - 6;
+    // stage the input: update cells only (the frame and the outside read as zero); threads
+    // run along z for coalesced loads
+    for (int p = threadIdx.x; p < cin * rin; p += ABC_NTHR) {
+        const int c = p / rin, r = p - c * rin;
+        const int x = xs + c, z = zs + r;
+        unsigned short v[16];
+        const bool inside = x >= FAT && x < nx - FAT && z >= FAT && z < nz - FAT;
+        const size_t cell = inside ? size_t(x) * nz + z : 0;
+#pragma unroll
+        for (int f = 0; f < 5; ++f) {
+            const float a = inside ? float(nw[size_t(f) * n + cell]) : 0.f;
+            const float bb = inside ? float(pv[size_t(f) * n + cell]) : 0.f;
+            v[f] = abc_bf16(a * P.inv_scale[f]);
+            v[5 + f] = abc_bf16((P.diff ? a - bb : bb) * P.inv_scale[5 + f]);
+        }
+#pragma unroll
+        for (int f = 10; f < 16; ++f) v[f] = 0;
+        const int q = XMAJ ? r * cin + c : c * rin + r;
+        uint4* d = reinterpret_cast<uint4*>(lds0 + size_t(q) * ABC_CH);
+        d[0] = make_uint4(v[0] | (unsigned(v[1]) << 16), v[2] | (unsigned(v[3]) << 16), v[4] | (unsigned(v[5]) << 16),
+                          v[6] | (unsigned(v[7]) << 16));
+        d[1] = make_uint4(v[8] | (unsigned(v[9]) << 16), v[10] | (unsigned(v[11]) << 16),
+                          v[12] | (unsigned(v[13]) << 16), v[14] | (unsigned(v[15]) << 16));
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    abc_layer<false, XMAJ, T>(lds0, nA, nB, lds1, w1, lane, wave, nullptr, P, n, nz, 0, 0, 0, 0);
+    __syncthreads();
+    abc_layer<false, XMAJ, T>(lds1, nA - 4, nB - 4, lds0, w2, lane, wave, nullptr, P, n, nz, 0, 0, 0, 0);
+    __syncthreads();
+    abc_layer<true, XMAJ, T>(lds0, nA - 8, nB - 8, nullptr, w3, lane, wave, nw, P, n, nz, tl.x0, tl.z0, tl.ox, tl.oz);
+}
+
+// Frame of the new level of fine step k (the new level is (k & 1) ? B : A, the previous one
+// the other).
 template <typename T>
 __global__ __launch_bounds__(ABC_NTHR) void k_abc_cnn(const ShotDev<T>* __restrict__ shots,
                                                       const AbcShot* __restrict__ par,
@@ -122,69 +198,20 @@ __global__ __launch_bounds__(ABC_NTHR) void k_abc_cnn(const ShotDev<T>* __restri
     const int t = blockIdx.x;
     if (t >= ntile) return;
     const AbcTile tl = tiles[t];
-    const ShotDev<T>& S = shots[tl.shot];
     const AbcShot P = par[tl.shot];
-    const int nz = S.nz, nx = S.nx;
-    const unsigned n = unsigned(nz) * unsigned(nx);
-    const T* nw = (k & 1) ? S.B : S.A;
-    const T* pv = (k & 1) ? S.A : S.B;
-    const int r3 = max(tl.oz, 16);
-    const int cin = tl.ox + 12, rin = r3 + 12;
-    const int xs = tl.x0 - 6, zs = tl.z0 - 6;
-    // stage the input: zero outside the array
-    for (int p = threadIdx.x; p < cin * rin; p += ABC_NTHR) {
-        const int c = p / rin, r = p - c * rin;
-        const int x = xs + c, z = zs + r;
-        unsigned short v[16];
-        const bool inside = x >= 0 && x < nx && z >= 0 && z < nz;
-        const size_t cell = inside ? size_t(x) * nz + z : 0;
-#pragma unroll
-        for (int f = 0; f < 5; ++f) {
-            const float a = inside ? float(nw[size_t(f) * n + cell]) : 0.f;
-            const float b = inside ? float(pv[size_t(f) * n + cell]) : 0.f;
-            v[f] = abc_bf16(a * P.inv_scale[f]);
-            v[5 + f] = abc_bf16(b * P.inv_scale[f]);
-        }
-#pragma unroll
-        for (int f = 10; f < 16; ++f) v[f] = 0;
-        uint4* d = reinterpret_cast<uint4*>(lds0 + size_t(p) * ABC_CH);
-        d[0] = make_uint4(v[0] | (unsigned(v[1]) << 16), v[2] | (unsigned(v[3]) << 16), v[4] | (unsigned(v[5]) << 16),
-                          v[6] | (unsigned(v[7]) << 16));
-        d[1] = make_uint4(v[8] | (unsigned(v[9]) << 16), v[10] | (unsigned(v[11]) << 16),
-                          v[12] | (unsigned(v[13]) << 16), v[14] | (unsigned(v[15]) << 16));
-    }
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the three layers' B fragments: lane holds W[tap][cin = 4*(lane/16) + j][cout = lane%16];
+    // loaded before the staging so their latency overlaps it
+    const int lane = threadIdx.x & 63;
     const unsigned short* we = P.w + size_t(tl.edge) * ABC_WEDGE;
-    abc_layer<false>(lds0, cin, rin, lds1, we, lane, wave, nullptr, P, n, nz, 0, 0, 0, 0);
-    __syncthreads();
-    abc_layer<false>(lds1, cin - 4, rin - 4, lds0, we + ABC_TAPS * ABC_FRAG, lane, wave, nullptr, P, n, nz, 0, 0,
-                     0, 0);
-    __syncthreads();
-    abc_layer<true>(lds0, cin - 8, rin - 8, nullptr, we + 2 * ABC_TAPS * ABC_FRAG, lane, wave, P.corr, P, n, nz,
-                    tl.x0, tl.z0, tl.ox, tl.oz);
-}
-
-// new level += corrections on every tile's output cells
-template <typename T>
-__global__ __launch_bounds__(ABC_NTHR) void k_abc_add(const ShotDev<T>* __restrict__ shots,
-                                                      const AbcShot* __restrict__ par,
-                                                      const AbcTile* __restrict__ tiles, int ntile, int k) {
-    const int t = blockIdx.x;
-    if (t >= ntile) return;
-    const AbcTile tl = tiles[t];
-    const ShotDev<T>& S = shots[tl.shot];
-    const float* corr = par[tl.shot].corr;
-    const int nz = S.nz;
-    const size_t n = size_t(nz) * S.nx;
-    T* nw = (k & 1) ? S.B : S.A;
-    const int cells = tl.ox * tl.oz;
-    for (int p = threadIdx.x; p < 5 * cells; p += ABC_NTHR) {
-        const int f = p / cells, q = p - f * cells;
-        const int c = q / tl.oz, r = q - c * tl.oz;
-        const size_t idx = size_t(f) * n + size_t(tl.x0 + c) * nz + (tl.z0 + r);
-        nw[idx] += T(corr[idx]);
+    abc_s4 w1[ABC_TAPS], w2[ABC_TAPS], w3[ABC_TAPS];
+#pragma unroll
+    for (int q = 0; q < ABC_TAPS; ++q) {
+        w1[q] = *reinterpret_cast<const abc_s4*>(we + (q * 64 + lane) * 4);
+        w2[q] = *reinterpret_cast<const abc_s4*>(we + ABC_TAPS * ABC_FRAG + (q * 64 + lane) * 4);
+        w3[q] = *reinterpret_cast<const abc_s4*>(we + 2 * ABC_TAPS * ABC_FRAG + (q * 64 + lane) * 4);
     }
+    if (tl.edge >= 2) abc_tile<true, T>(shots[tl.shot], P, tl, k, lds0, lds1, w1, w2, w3);
+    else abc_tile<false, T>(shots[tl.shot], P, tl, k, lds0, lds1, w1, w2, w3);
 }
 
 }  // namespace eio

@@ -89,9 +89,10 @@ struct ModelBase {
     void* damp = nullptr;   // minPad
     void* filt = nullptr;   // 2h + 1 (last entry: 1/(2 dts))
     double cz[4], cx[4];
-    int abc_strip = 0;      // learned absorbing boundary: strip width (0: off)
+    int abc_on = 0;         // learned absorbing boundary attached: 1 difference inputs, 2 level inputs
+    int abc_depth = 4;      // frame rows the network writes
     void* abc_w = nullptr;  // [4][3][25][64][4] bf16 B fragments
-    float abc_scale[5] = {1.f, 1.f, 1.f, 1.f, 1.f};
+    float abc_scale[10] = {1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f};
 };
 
 // Forward-wavefield store of the checkpointed FWI gradient: the batch state at the start of
@@ -661,9 +662,9 @@ void attach_wavefield(Run<T>& run, int shot, void* out, int flags) {
                                  c->stream));
 }
 
-// Learned absorbing boundary of the shots of a run whose model has one attached: tile list
-// (grouped by ordered shot, so the tiles of the active shots are a prefix), per-shot network
-// parameters and correction buffers.
+// Learned absorbing boundary of the shots of a run whose model has one attached: frame tile
+// list (grouped by ordered shot, so the tiles of the active shots are a prefix) and per-shot
+// network parameters.
 struct AbcRun {
     int ntile = 0;
     std::vector<int> shot_end;  // per ordered shot: end of its tiles
@@ -680,7 +681,7 @@ struct AbcRun {
 
 bool any_abc(Ctx* c, int nshot, const eio_shot* shots) {
     for (int i = 0; i < nshot && shots; ++i)
-        if (get_model(c, shots[i].model).abc_strip > 0) return true;
+        if (get_model(c, shots[i].model).abc_on) return true;
     return false;
 }
 
@@ -694,34 +695,25 @@ AbcRun abc_prepare(Run<T>& run, const eio_shot* shots) {
         const int ns = int(run.sd.size());
         std::vector<eio::AbcTile> tiles;
         std::vector<eio::AbcShot> par(ns);
-        size_t corr_elems = 0;
-        for (int oi = 0; oi < ns; ++oi) {
-            const ModelBase& m = get_model(c, shots[run.plan.order[oi]].model);
-            if (m.abc_strip > 0) corr_elems += 5 * size_t(m.n);
-        }
-        float* corr = corr_elems ? static_cast<float*>(ws_get(c, "abc_corr", corr_elems * sizeof(float))) : nullptr;
-        size_t co = 0;
         a.shot_end.resize(ns);
         a.nsteps = run.plan.nsteps;
         const int F = eio::FAT;
         for (int oi = 0; oi < ns; ++oi) {
             const ModelBase& m = get_model(c, shots[run.plan.order[oi]].model);
-            const int S = m.abc_strip, nz = m.g.nz, nx = m.g.nx;
-            if (S > 0) {
+            const int nz = m.g.nz, nx = m.g.nx;
+            if (m.abc_on) {
                 eio::AbcShot& P = par[oi];
                 P.w = static_cast<const unsigned short*>(m.abc_w);
-                for (int f = 0; f < 5; ++f) {
-                    P.scale[f] = m.abc_scale[f];
-                    P.inv_scale[f] = 1.0f / m.abc_scale[f];
-                }
-                P.corr = corr + co;
-                co += 5 * size_t(m.n);
-                for (int e = 0; e < 2; ++e)  // x-min, x-max strips: every update row
+                for (int f = 0; f < 5; ++f) P.scale[f] = m.abc_scale[f];
+                for (int f = 0; f < 10; ++f) P.inv_scale[f] = 1.0f / m.abc_scale[f];
+                P.diff = m.abc_on == 1;
+                P.depth = m.abc_depth;
+                for (int e = 0; e < 2; ++e)  // x-min, x-max frames: the update rows
                     for (int z0 = F; z0 < nz - F; z0 += eio::ABC_MAXOZ)
-                        tiles.push_back({oi, e, e == 0 ? F : nx - F - S, z0, S, std::min(eio::ABC_MAXOZ, nz - F - z0)});
-                for (int e = 2; e < 4; ++e)  // z-min, z-max strips: the columns between
-                    for (int x0 = F + S; x0 < nx - F - S; x0 += eio::ABC_MAXOX)
-                        tiles.push_back({oi, e, x0, e == 2 ? F : nz - F - S, std::min(eio::ABC_MAXOX, nx - F - S - x0), S});
+                        tiles.push_back({oi, e, e == 0 ? 0 : nx - F, z0, F, std::min(eio::ABC_MAXOZ, nz - F - z0)});
+                for (int e = 2; e < 4; ++e)  // z-min, z-max frames: the update columns
+                    for (int x0 = F; x0 < nx - F; x0 += eio::ABC_MAXOX)
+                        tiles.push_back({oi, e, x0, e == 2 ? 0 : nz - F, std::min(eio::ABC_MAXOX, nx - F - x0), F});
             }
             a.shot_end[oi] = int(tiles.size());
         }
@@ -736,15 +728,13 @@ AbcRun abc_prepare(Run<T>& run, const eio_shot* shots) {
     }
 }
 
-// corrections of fine step k on the strips of the active shots
+// frames of the new level of fine step k of the active shots
 template <typename T>
 void launch_abc(Run<T>& run, const AbcRun& a, int k) {
     if constexpr (sizeof(T) == 4) {
         const int nt = a.active(k);
         if (nt <= 0) return;
         hipLaunchKernelGGL((eio::k_abc_cnn<float>), dim3(nt), dim3(eio::ABC_NTHR), 0, run.c->stream, run.d_shots,
-                           a.d_par, a.d_tiles, nt, k);
-        hipLaunchKernelGGL((eio::k_abc_add<float>), dim3(nt), dim3(eio::ABC_NTHR), 0, run.c->stream, run.d_shots,
                            a.d_par, a.d_tiles, nt, k);
     }
 }
@@ -851,35 +841,37 @@ void step_states(Ctx* c, int model, int nstate, void* states, int flags) {
     EIO_CHECK_HIP(hipMemcpyAsync(run.d_shots, run.sd.data(), sizeof(eio::ShotDev<T>) * nstate, hipMemcpyHostToDevice,
                                  c->stream));
     AbcRun ar;
-    if (m.abc_strip > 0) ar = abc_prepare(run, sh.data());
+    if (m.abc_on) ar = abc_prepare(run, sh.data());
     launch_fwd<T, eio::M_NL>(run, int(run.plan.tiles.size()), 0, 0);
-    if (m.abc_strip > 0) launch_abc(run, ar, 0);
+    if (m.abc_on) launch_abc(run, ar, 0);
     EIO_CHECK_HIP(hipGetLastError());
     finish(run);
 }
 
-// Attach (strip > 0) or detach (strip == 0) a learned boundary to a model.  weights: FP32,
-// per edge (x-min, x-max, z-min, z-max) the three layers in torch Conv2d layout
-// [out][in][5][5] with (out, in) = (16, 10), (16, 16), (5, 16), the kernel's first axis
-// along x and second along z; scale: 5 per-field scales.
-void set_abc(Ctx* c, int model, int strip, const float* w, const float* scale) {
+// Attach (on = 1: inputs new level and new - previous; on = 2: new and previous level; plus
+// 10 x depth to write only the `depth` frame rows next to the update region, default all 4)
+// or detach (on = 0) a learned boundary to a model.  weights: FP32, per edge
+// (x-min, x-max, z-min, z-max) the three layers in torch Conv2d layout [out][in][5][5] with
+// (out, in) = (16, 10), (16, 16), (5, 16), the kernel's first axis along x and second along z;
+// scale: 5 per-field scales of the new level (and output), then 5 of the differences.
+void set_abc(Ctx* c, int model, int on, const float* w, const float* scale) {
     ModelBase& m = get_model(c, model);
     if (c->prec != 4) fail("the learned boundary runs in FP32 contexts only");
-    if (strip < 0 || strip > eio::ABC_MAXOX) fail("strip must be in [0, 16]");
-    if (strip > 0) {
+    const int depth = on / 10 ? on / 10 : eio::FAT;
+    on %= 10;
+    if (on < 0 || on > 2 || depth < 1 || depth > eio::FAT) fail("on must be 0, 1 or 2, plus 10 x depth (1..4)");
+    if (on) {
         if (m.g.surface) fail("the learned boundary is not available with the free surface");
         if (!w || !scale) fail("weights or scale is NULL");
-        if (m.g.nz - 2 * eio::FAT < 2 * strip || m.g.nx - 2 * eio::FAT < 2 * strip)
-            fail("grid too small for the boundary strip");
-        for (int f = 0; f < 5; ++f)
+        for (int f = 0; f < 10; ++f)
             if (!(scale[f] > 0.f) || !std::isfinite(scale[f])) fail("scales must be positive and finite");
     }
     if (m.abc_w) {
         EIO_CHECK_HIP(hipFree(m.abc_w));
         m.abc_w = nullptr;
     }
-    m.abc_strip = 0;
-    if (strip == 0) return;
+    m.abc_on = 0;
+    if (!on) return;
     const int cin[3] = {10, 16, 16}, cout[3] = {16, 16, 5};
     std::vector<unsigned short> frag(4 * size_t(eio::ABC_WEDGE), 0);
     auto bf = [](float f) {
@@ -907,8 +899,9 @@ void set_abc(Ctx* c, int model, int strip, const float* w, const float* scale) {
     DeviceGuard dg(c->device);
     EIO_CHECK_HIP(hipMalloc(&m.abc_w, frag.size() * sizeof(unsigned short)));
     EIO_CHECK_HIP(hipMemcpy(m.abc_w, frag.data(), frag.size() * sizeof(unsigned short), hipMemcpyHostToDevice));
-    for (int f = 0; f < 5; ++f) m.abc_scale[f] = scale[f];
-    m.abc_strip = strip;
+    for (int f = 0; f < 10; ++f) m.abc_scale[f] = scale[f];
+    m.abc_on = on;
+    m.abc_depth = depth;
 }
 
 template <typename T>
@@ -1404,10 +1397,10 @@ int eio_nl_step_states(eio_ctx* ctx, int model, int nstate, void* states, int fl
     });
 }
 
-int eio_model_set_abc(eio_ctx* ctx, int model, int strip, const float* weights, const float* scale) {
+int eio_model_set_abc(eio_ctx* ctx, int model, int on, const float* weights, const float* scale) {
     return guarded([&]() {
         Ctx* c = as_ctx(ctx);
-        set_abc(c, model, strip, weights, scale);
+        set_abc(c, model, on, weights, scale);
     });
 }
 

@@ -12,6 +12,7 @@ Workloads (sizes are interior sizes, before padding):
   gaussian_c1             same grid, 8 shots, smoothed random perturbations (Born)
   layered_batch_c2        64 layered shots of uneven grid and trace length (batched NL)
   fwi_c3                  32-shot elastic FWI gradient on a faulted layered model with salt
+  learned_abc_c4          1000 x 3000 layered model for the learned-boundary comparison
 """
 import numpy as np
 
@@ -205,6 +206,15 @@ def fwi_c3(nshots=32, seed=3, scale=1.0, smooth_cells=25.0):
         w["name"] = "fwi_c3_%s_%dx%d_nts%d_%dshots" % (tag, w["nz"], w["nx"], nts, nshots)
         out.append(w)
     return out[0], out[1]
+
+
+def learned_abc_c4(seed=44, scale=1.0):
+    """Learned-boundary test model: a seeded layered model (the batched-shot ranges) on a
+    1000 x 3000 interior at 10 m, nts=4000 at 1 ms, sub=2; returns (vvr interior (3, nx, nz),
+    nts).  The networks are trained on other seeds."""
+    rng = np.random.default_rng(seed)
+    nz, nx, nts = int(1000 * scale), int(3000 * scale), int(4000 * scale)
+    return layered_vvr(rng, nz, nx), nts
 
 
 def cell_updates(w):

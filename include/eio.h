@@ -131,14 +131,18 @@ int eio_nl_adjoint_wavefield(eio_ctx* ctx, int nshot, const eio_shot* shots, int
 
 /* Learned absorbing boundary (the north-star method change; no counterpart in the reference,
  * whose only boundary is the cosine sponge of kernelsGpuElastic.cu:1192-1221).  Attach
- * (strip in 1..16) or detach (strip = 0) a small CNN that corrects the `strip`-cell edge band
- * of the new level after every fine step of an FP32 NL forward of this model (the model's
- * own sponge, minPad, stays active).  weights: FP32, per edge (x-min, x-max, z-min, z-max)
- * the three 5x5 layers in torch Conv2d layout [out][in][5][5] with (out, in) = (16, 10),
- * (16, 16), (5, 16), first kernel axis along x; inputs are the 5 fields of the new and the
- * 5 of the previous level divided by scale[5], outputs times scale are added to the band.
- * Rounded to BF16 on upload.  Not available with the free surface or in FP64 contexts. */
-int eio_model_set_abc(eio_ctx* ctx, int model_id, int strip, const float* weights, const float* scale);
+ * (on = 1 or 2) or detach (on = 0) a small CNN that, after every fine step of an FP32 NL forward
+ * of this model, writes the new level's 4-cell FAT frame (the cells outside the update
+ * region that the stencil reads, zero without it) from the first update cells of the new and
+ * the previous level; the model's own sponge (minPad) stays active.  weights: FP32, per edge
+ * (x-min, x-max, z-min, z-max) the three 5x5 layers in torch Conv2d layout [out][in][5][5]
+ * with (out, in) = (16, 10), (16, 16), (5, 16), first kernel axis along x; inputs are the 5
+ * fields of the new level divided by scale[0..4] and, divided by scale[5..9], the 5
+ * differences new - previous level (on = 1) or the 5 fields of the previous level (on = 2);
+ * outputs are times scale[0..4].  on + 10 * depth (depth 1..4) writes only the `depth` frame
+ * rows next to the update region and leaves the outer ones zero.  Rounded to BF16 on upload.  Not
+ * available with the free surface or in FP64. */
+int eio_model_set_abc(eio_ctx* ctx, int model_id, int on, const float* weights, const float* scale);
 
 /* eio_nl_forward that also copies the raw state of shot `shot` after each fine step
  * steps[i] (0-based, increasing): out [nsnap][2][5][nx][nz], the older then the newer level

@@ -18,44 +18,42 @@ def _small_shot(ctx, strip=10, nz_i=40, nx_i=72, nts=200, seed=1):
 
 
 def test_abc_kernel_matches_bf16_torch_network():
+    """One step with the network attached: the update cells equal the plain step's, the frame
+    cells equal the torch network (BF16 at the kernel's rounding points) on the plain step's
+    update cells."""
     import torch
     from elastic_isotropic_operators_amd import _capi, abc_learn as A
     ctx = _capi.Context(0, np.float32)
     shot, w, _ = _small_shot(ctx)
-    nz, nx, S = w["nz"], w["nx"], 10
+    nz, nx = w["nz"], w["nx"]
     g = torch.Generator(device="cuda").manual_seed(7)
     st = torch.randn((3, 2, 5, nx, nz), device="cuda", generator=g)
-    st[..., :4, :] = 0
-    st[..., -4:, :] = 0
-    st[..., :4] = 0
-    st[..., -4:] = 0
-    scale = np.array([0.5, 0.7, 3.0, 2.0, 1.5], np.float32)
+    scale = np.array([0.5, 0.7, 3.0, 2.0, 1.5, 0.01, 0.02, 0.05, 0.04, 0.03], np.float32)
     net = A.AbcNet("cuda", seed=11)
     with torch.no_grad():
         for e in range(4):
-            net.W[e][2] *= 10.0  # corrections of the order of the fields
+            net.W[e][2] *= 10.0
     plain = st.clone()
     ctx.step_states(shot.mid, 3, plain.data_ptr())
-    ctx.set_abc(shot.mid, S, net.flat(), scale)
+    ctx.set_abc(shot.mid, 1, net.flat(), scale)
     learned = st.clone()
     ctx.step_states(shot.mid, 3, learned.data_ptr())
     torch.cuda.synchronize()
-    diff = (learned[:, 0] - plain[:, 0]).double()
+    F = A.FAT
     assert torch.equal(learned[:, 1], plain[:, 1])
-    reg = A.band_regions(nz, nx, S)
-    mask = torch.zeros((nx, nz), dtype=torch.bool, device="cuda")
-    for (x0, x1, z0, z1) in reg:
-        mask[x0:x1, z0:z1] = True
-    assert float(diff[..., ~mask].abs().max()) == 0.0
+    assert torch.equal(learned[:, 0, :, F:nx - F, F:nz - F], plain[:, 0, :, F:nx - F, F:nz - F])
+    reg = A.frame_regions(nz, nx)
     sc = torch.tensor(scale, device="cuda")
     for i in range(3):
         feats = A.features(plain[i, 0], plain[i, 1], sc, reg)
         with torch.no_grad():
             for e in range(4):
-                ref = (net.edge_forward(e, feats[e][None], bf16=True)[0] * sc.view(5, 1, 1)).double()
-                got = A.band(diff[i], reg)[e]
+                ref = (net.edge_forward(e, feats[e][None], bf16=True)[0] * sc[:5].view(5, 1, 1)).double()
+                got = A.region(learned[i, 0], reg)[e].double()
                 err = float((got - ref).norm() / ref.norm())
                 assert err < 2e-3, (i, e, err)
+    # the frame corners are never written
+    assert torch.equal(learned[:, 0, :, :F, :F], st[:, 0, :, :F, :F])
     ctx.close()
 
 
@@ -70,7 +68,7 @@ def test_abc_zero_weights_leave_propagation_unchanged():
     with torch.no_grad():
         for e in range(4):
             zero.W[e][2].zero_()
-    ctx.set_abc(shot.mid, 10, zero.flat(), np.ones(5, np.float32))
+    ctx.set_abc(shot.mid, 1, zero.flat(), np.ones(10, np.float32))
     d1, _ = shot.forward()
     assert np.array_equal(d0, d1)
     ctx.set_abc(shot.mid, 0)
@@ -102,13 +100,37 @@ def test_snapshots_and_one_step_entry_agree_with_propagation():
 def test_abc_small_training_run_and_reflection_test():
     import torch
     from elastic_isotropic_operators_amd import abc_learn as A, workloads as W
-    data, scale = A.generate(n_models=3, nz_i=40, nx_i=72, nts=500, pairs_per_model=4, first_step=300, log=None)
+    data = A.generate(n_models=3, nz_i=40, nx_i=72, nts=500, snaps_per_model=4, first_step=300, log=None)
+    scale = A.input_scale(data, True)
     assert all(X.shape[0] == 12 for X, _ in data) and bool(torch.isfinite(scale).all())
     net, val = A.train(data, scale, iters=60, batch=8, log=None)
     assert all(np.isfinite(v) for v in val)
     vvr = W.layered_vvr(np.random.default_rng(99), 40, 72)
     res = A.reflection_test(vvr, net, scale.cpu().numpy(), nts=500, mid=30, wide=60, log=None)
-    assert set(res) == {"wide60", "plain10", "plain30", "learned10"}
+    assert set(res) == {"wide60", "plain10", "plain30", "learned10", "learned10_nosponge"}
     assert res["wide60"]["refl"] == 0.0
     assert res["plain10"]["refl"] > res["plain30"]["refl"] > 0
-    assert np.isfinite(res["learned10"]["refl"])
+    # a network trained for 60 iterations need not be stable: the run must complete
+    assert isinstance(res["learned10"]["refl"], float) and res["learned10"]["ms"] > 0
+
+
+def test_torch_training_step_matches_the_gpu_step():
+    """The differentiable step the network is fine-tuned through equals the propagator's step
+    (no sources, sponge on) to FP32 rounding."""
+    import torch
+    from elastic_isotropic_operators_amd import _capi, abc_learn as A
+    ctx = _capi.Context(0, np.float32)
+    shot, w, _ = _small_shot(ctx)
+    nz, nx = w["nz"], w["nx"]
+    g = torch.Generator(device="cuda").manual_seed(3)
+    st = torch.randn((2, 2, 5, nx, nz), device="cuda", generator=g)
+    ref = st.clone()
+    ctx.step_states(shot.mid, 2, ref.data_ptr())
+    torch.cuda.synchronize()
+    step = A.TorchStep(shot.media, nz, nx, w["dz"], w["dx"], w["minPad"])
+    got = step(st[:, 0], st[:, 1])
+    F = A.FAT
+    a = got[..., F:nx - F, F:nz - F].double()
+    b = ref[:, 0, :, F:nx - F, F:nz - F].double()
+    assert float((a - b).norm() / b.norm()) < 1e-6
+    ctx.close()
