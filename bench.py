@@ -238,9 +238,9 @@ def main():
         abc_child(args)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU only: every box of this build has a single MI355X; "
-              "shots would be partitioned across GPUs with one all-reduce of the gradient "
-              "(see DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU only: it runs on a single MI355X; over several GPUs the "
+              "shots would be partitioned (one batch per GPU) with one all-reduce of the FWI "
+              "gradient and misfit.", file=sys.stderr)
         sys.exit(2)
 
     t_build = time.perf_counter()

@@ -129,7 +129,7 @@ int eio_nl_forward_wavefield(eio_ctx* ctx, int nshot, const eio_shot* shots, int
 int eio_nl_adjoint_wavefield(eio_ctx* ctx, int nshot, const eio_shot* shots, int flags, int shot,
                              void* wavefield);
 
-/* Learned absorbing boundary (the north-star method change; no counterpart in the reference,
+/* Learned absorbing boundary (a method change beyond the reference; no counterpart there,
  * whose only boundary is the cosine sponge of kernelsGpuElastic.cu:1192-1221).  Attach
  * (on = 1 or 2) or detach (on = 0) a small CNN that, after every fine step of an FP32 NL forward
  * of this model, writes the new level's 4-cell FAT frame (the cells outside the update
