@@ -104,7 +104,7 @@ EXPORTED = ("eio_last_error", "eio_ctx_create", "eio_ctx_destroy", "eio_ctx_set_
             "eio_fwi_release", "eio_ctx_num_time_block_variants", "eio_ctx_time_block_info",
             "eio_ctx_set_time_block", "eio_ctx_set_step_limit", "eio_nl_forward_wavefield",
             "eio_nl_adjoint_wavefield", "eio_model_set_abc", "eio_nl_forward_snapshots",
-            "eio_nl_step_states")
+            "eio_nl_step_states", "eio_ctx_release_workspace")
 
 
 def check(rc):
@@ -137,6 +137,10 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def release_workspace(self):
+        """Free the context's cached device workspace (it is kept across calls otherwise)."""
+        check(lib().eio_ctx_release_workspace(self.h))
 
     def alloc(self, nbytes):
         p = ctypes.c_void_p()

@@ -105,6 +105,7 @@ struct FwiStore {
     void* ck = nullptr;          // (nseg-1) batch states
     void* ring = nullptr;        // per ordered shot: Vx then Vz, wcap slices each
     bool ring_fresh = false;     // the ring holds the last segment (true right after the forward)
+    bool ws = false;             // ring and checkpoints borrowed from the context workspace
     size_t bytes = 0;
 };
 
@@ -126,6 +127,7 @@ struct Ctx {
     int step_limit = 0;  // > 0: NL loops stop after this many fine steps (tuning runs)
     std::map<int, FwiStore> fwi;
     int next_fwi = 1;
+    bool fwi_ws_busy = false;  // a store holds the workspace's "fwi_ring" / "fwi_ck" buffers
 };
 
 void* ws_get(Ctx* c, const std::string& name, size_t bytes) {
@@ -1058,10 +1060,18 @@ unsigned long long geometry_hash(int nshot, const eio_shot* shots) {
     return h;
 }
 
-void fwi_free(FwiStore& f) {
-    if (f.ck) (void)hipFree(f.ck);
-    if (f.ring) (void)hipFree(f.ring);
+// A store's buffers: the first store alive borrows the context's grow-only workspace (no
+// allocation when consecutive stores fit: allocating and freeing tens of GB per gradient costs
+// seconds); stores made while it is held allocate their own.
+void fwi_free(Ctx* c, FwiStore& f) {
+    if (f.ws) {
+        c->fwi_ws_busy = false;
+    } else {
+        if (f.ck) (void)hipFree(f.ck);
+        if (f.ring) (void)hipFree(f.ring);
+    }
     f.ck = f.ring = nullptr;
+    f.ws = false;
 }
 
 template <typename T>
@@ -1095,12 +1105,26 @@ int fwi_forward(Ctx* c, int nshot, const eio_shot* shots, int flags, int segment
     const size_t ring_elems = size_t(nshot) * 2 * size_t(m.n) * f.wcap;
     const size_t ck_elems = size_t(f.nseg - 1) * f.state_elems;
     f.bytes = (ring_elems + ck_elems) * sizeof(T);
-    EIO_CHECK_HIP(hipMalloc(&f.ring, ring_elems * sizeof(T)));
-    if (ck_elems && hipMalloc(&f.ck, ck_elems * sizeof(T)) != hipSuccess) {
-        (void)hipGetLastError();
-        fwi_free(f);
-        throw EioError{EIO_ERR_RUNTIME, "out of device memory for " + std::to_string(f.nseg - 1) +
-                                            " checkpoints; pass a longer segment"};
+    const std::string oom = "out of device memory for the store (" + std::to_string(f.nseg - 1) +
+                            " checkpoints, " + std::to_string(f.wcap) + " coarse slices); pass another segment";
+    if (!c->fwi_ws_busy) {
+        f.ws = true;
+        c->fwi_ws_busy = true;
+        try {
+            f.ring = ws_get(c, "fwi_ring", ring_elems * sizeof(T));
+            if (ck_elems) f.ck = ws_get(c, "fwi_ck", ck_elems * sizeof(T));
+        } catch (const EioError&) {
+            (void)hipGetLastError();
+            fwi_free(c, f);
+            throw EioError{EIO_ERR_RUNTIME, oom};
+        }
+    } else {
+        if (hipMalloc(&f.ring, ring_elems * sizeof(T)) != hipSuccess ||
+            (ck_elems && hipMalloc(&f.ck, ck_elems * sizeof(T)) != hipSuccess)) {
+            (void)hipGetLastError();
+            fwi_free(c, f);
+            throw EioError{EIO_ERR_RUNTIME, oom};
+        }
     }
     try {
         Run<T> run = prepare<T>(c, nshot, shots, Op::FWI_FWD, flags, nullptr, static_cast<T*>(f.ring), f.wcap);
@@ -1125,7 +1149,7 @@ int fwi_forward(Ctx* c, int nshot, const eio_shot* shots, int flags, int segment
         finish(run);
     } catch (...) {
         (void)hipStreamSynchronize(c->stream);
-        fwi_free(f);
+        fwi_free(c, f);
         throw;
     }
     f.ring_fresh = true;
@@ -1248,7 +1272,7 @@ int eio_ctx_destroy(eio_ctx* ctx) {
         DeviceGuard dg(c->device);
         (void)hipStreamSynchronize(c->stream);
         for (auto& kv : c->models) model_free(kv.second);
-        for (auto& kv : c->fwi) fwi_free(kv.second);
+        for (auto& kv : c->fwi) fwi_free(c, kv.second);
         for (auto& kv : c->ws)
             if (kv.second.p) (void)hipFree(kv.second.p);
         for (auto& e : c->ev) (void)hipEventDestroy(e);
@@ -1483,8 +1507,25 @@ int eio_fwi_release(eio_ctx* ctx, int handle) {
         if (it == c->fwi.end()) fail("unknown FWI store handle " + std::to_string(handle));
         DeviceGuard dg(c->device);
         EIO_CHECK_HIP(hipStreamSynchronize(c->stream));
-        fwi_free(it->second);
+        fwi_free(c, it->second);
         c->fwi.erase(it);
+    });
+}
+
+int eio_ctx_release_workspace(eio_ctx* ctx) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        DeviceGuard dg(c->device);
+        EIO_CHECK_HIP(hipStreamSynchronize(c->stream));
+        for (auto it = c->ws.begin(); it != c->ws.end();) {
+            const bool held = c->fwi_ws_busy && (it->first == "fwi_ring" || it->first == "fwi_ck");
+            if (!held && it->second.p) {
+                EIO_CHECK_HIP(hipFree(it->second.p));
+                it = c->ws.erase(it);
+            } else {
+                ++it;
+            }
+        }
     });
 }
 

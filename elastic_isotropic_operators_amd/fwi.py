@@ -100,6 +100,19 @@ def vpvsrho_to_rholamemu(m):
     return torch.stack([rho, rho * (vp * vp - 2.0 * vs * vs), rho * vs * vs])
 
 
+def _datcomp_adjoint(sampled, components):
+    """ElasticDatComp.adjoint (dataCompModule.py:79-107) on (nShot, nComp, nRec, nts)."""
+    shape = (sampled.shape[0], 5) + tuple(sampled.shape[2:])
+    out = torch.zeros(shape, dtype=sampled.dtype, device=sampled.device)
+    for i, c in enumerate(components.lower().split(",")):
+        if c == "p":
+            out[:, 2] += 0.5 * sampled[:, i]
+            out[:, 3] += 0.5 * sampled[:, i]
+        else:
+            out[:, _COMP[c]] += sampled[:, i]
+    return out
+
+
 def datcomp(data, components="vx,vz,p"):
     """ElasticDatComp.forward (dataCompModule.py:45-77) on (nShot, 5, nRec, nts)."""
     out = []
@@ -208,11 +221,14 @@ class ElasticFwi:
         else:
             self.ctx.model_set_media(self.mid, device_ptrs=ptrs)
 
-    def _scaled_sources(self, media):
+    def _sel(self, sel):
+        return list(range(self.nshot)) if sel is None else list(sel)
+
+    def _scaled_sources(self, media, sel=None):
         """nonlinearPropElasticGpu.cpp:109-144: vx, vz by rhoxDtw, rhozDtw at the device
-        positions, stresses by 2*dtw, all by 1/(dx dz)."""
+        positions, stresses by 2*dtw, all by 1/(dx dz); for the shots `sel` (default all)."""
         out = []
-        for e in range(self.nshot):
+        for e in self._sel(sel):
             per = []
             for k in range(5):
                 r = self.src_reg[e][k]
@@ -224,11 +240,12 @@ class ElasticFwi:
             out.append(per)
         return out
 
-    def _shots(self, srcs, recs):
+    def _shots(self, srcs, recs, sel=None):
+        """C-ABI shots of the shots `sel`; srcs and recs are indexed by position in `sel`."""
         shots, keep = [], []
-        for e in range(self.nshot):
-            rec_ptr = [recs[k][e].data_ptr() if recs[k].shape[1] else 0 for k in range(5)]
-            src_ptr = [srcs[e][k].data_ptr() if srcs[e][k].numel() else 0 for k in range(5)]
+        for i, e in enumerate(self._sel(sel)):
+            rec_ptr = [recs[k][i].data_ptr() if recs[k].shape[1] else 0 for k in range(5)]
+            src_ptr = [srcs[i][k].data_ptr() if srcs[i][k].numel() else 0 for k in range(5)]
             s, kp = self.ctx.make_shot(self.mid, self.src_pos[e], self.rec_pos, src_ptr, rec_ptr)
             shots.append(s)
             keep.append(kp)
@@ -247,17 +264,19 @@ class ElasticFwi:
     def _irreg_to_reg(self, data):
         """spaceInterpGpu adjoint on (nShot, 5, nRec, nts)."""
         recs = []
+        ns = data.shape[0]
         for k in range(5):
             gi = GRID_OF_COMP[k]
             mp, w = self.rec_map[gi], self.rec_w[gi]
-            reg = torch.zeros((self.nshot, self.rec_pos[gi].size, self.nts), dtype=self.dtype, device=self.device)
-            contrib = (data[:, k, :, None, :] * w[None, :, :, None]).reshape(self.nshot, -1, self.nts)
+            reg = torch.zeros((ns, self.rec_pos[gi].size, self.nts), dtype=self.dtype, device=self.device)
+            contrib = (data[:, k, :, None, :] * w[None, :, :, None]).reshape(ns, -1, self.nts)
             reg.index_add_(1, mp.reshape(-1), contrib)
             recs.append(reg.contiguous())
         return recs
 
-    def _empty_recs(self):
-        return [torch.empty((self.nshot, self.rec_pos[GRID_OF_COMP[k]].size, self.nts), dtype=self.dtype,
+    def _empty_recs(self, ns=None):
+        ns = self.nshot if ns is None else ns
+        return [torch.empty((ns, self.rec_pos[GRID_OF_COMP[k]].size, self.nts), dtype=self.dtype,
                             device=self.device) for k in range(5)]
 
     # -------------------------------------------------------------- operations
@@ -307,6 +326,61 @@ class ElasticFwi:
         if torch.is_grad_enabled() and model.requires_grad:
             return ElasticFwiFunction.apply(model, self)
         return self.forward_data(model)[0]
+
+    def full_storage_group(self, margin=0.6):
+        """Shots per group whose full coarse Vx/Vz storage (2 nz nx nts values per shot) fits in
+        `margin` of the free device memory (0 when not even one shot fits)."""
+        free, _ = torch.cuda.mem_get_info(self.device)
+        per_shot = (2 * self.nts + 12) * self.nz * self.nx * torch.tensor([], dtype=self.dtype).element_size()
+        return int(margin * free // per_shot)
+
+    def misfit_and_gradient(self, model, d_obs, components=None, group=None):
+        """1/2 || C F(model) - d_obs ||^2 and its gradient with respect to `model` (rho, lambda,
+        mu), shot group by shot group: each group's forward keeps the whole coarse Vx/Vz
+        record (one segment, no checkpoints, the reference's full storage,
+        BornElasticGpuFunctions.cu:1128-1191, sized to the device) and its gradient follows at
+        once, so no segment is recomputed.  The misfit is a sum over shots, so the gradient of
+        each group needs only its own residual.  group: shots per group (default: as many as
+        fit in memory; falls back to checkpoints when even one does not)."""
+        model = model.detach().to(device=self.device, dtype=self.dtype)
+        d_obs = torch.as_tensor(d_obs, device=self.device, dtype=self.dtype)
+        media = scaled_media(model, self.dtw)
+        if group is None:
+            # sized once: later calls reuse the context's workspace, which free memory no longer shows
+            if getattr(self, "_group", None) is None:
+                self._group = self.full_storage_group()
+            group = self._group
+        g = group
+        segment = self.nts - 1 if g >= 1 else self.segment
+        g = max(1, min(g, self.nshot))
+        grad = torch.zeros((3, self.nx, self.nz), dtype=self.dtype, device=self.device)
+        phi = 0.0
+        for s0 in range(0, self.nshot, g):
+            sel = list(range(s0, min(self.nshot, s0 + g)))
+            srcs = self._scaled_sources(media, sel)
+            recs = self._empty_recs(len(sel))
+            self._torch_ready()
+            self._set_media(media)
+            shots, keep = self._shots(srcs, recs, sel)
+            h = self.ctx.fwi_forward(shots, segment, device_ptrs=True)
+            try:
+                data = self._reg_to_irreg(recs)
+                d = datcomp(data, components) if components else data
+                r = d - d_obs[sel]
+                phi += 0.5 * float(torch.sum(r.double() * r.double()))
+                gd = _datcomp_adjoint(r, components) if components else r
+                grecs = self._irreg_to_reg(gd.contiguous())
+                acc = torch.zeros((5, self.nx, self.nz), dtype=self.dtype, device=self.device)
+                self._torch_ready()
+                shots, keep = self._shots(srcs, grecs, sel)
+                n = self.nz * self.nx
+                ptrs = [acc.data_ptr() + k * n * acc.element_size() for k in range(5)]
+                self.ctx.fwi_gradient(h, shots, ptrs, device_ptrs=True)
+                self.last_store_info = self.ctx.fwi_info(h)
+            finally:
+                self.ctx.fwi_release(h)
+            grad += slices_to_gradient(acc, media, self.dtw)
+        return phi, grad
 
     def misfit(self, model, d_obs, components=None):
         """1/2 || C F(model) - d_obs ||^2 (C = ElasticDatComp when components is given)."""

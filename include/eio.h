@@ -73,6 +73,10 @@ const char* eio_last_error(void);
  * initBornGpu (nonlinearPropElasticGpuFunctions.cu:85-232, BornElasticGpuFunctions.cu:88-267). */
 int eio_ctx_create(int device, int precision, eio_ctx** out);
 int eio_ctx_destroy(eio_ctx* ctx);
+/* Free the context's device workspace (state, traces, FWI stores' buffers once released): the
+ * context keeps and reuses it across calls; the next call allocates again.  No counterpart in
+ * the reference, which allocates and frees per call (nonlinearPropElasticGpuFunctions.cu:385-403). */
+int eio_ctx_release_workspace(eio_ctx* ctx);
 /* Run on an external stream (e.g. torch's current stream); NULL = the context's own stream. */
 int eio_ctx_set_stream(eio_ctx* ctx, void* hip_stream);
 /* Timing of the last propagation call, from HIP events on the context stream:

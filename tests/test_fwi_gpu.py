@@ -233,3 +233,32 @@ def test_autograd_gradient_is_deterministic():
         eng.misfit(m, d_obs).backward()
         grads.append(m.grad.cpu().numpy())
     assert np.array_equal(grads[0], grads[1]) and np.array_equal(grads[0], grads[2])
+
+
+@pytest.mark.parametrize("tdtype", ["float64", "float32"])
+def test_grouped_full_storage_gradient_equals_checkpointed(tdtype):
+    """misfit_and_gradient (shot groups, full coarse storage, no recompute) gives the misfit and
+    gradient of the checkpointed autograd path."""
+    import torch
+    dtype = getattr(torch, tdtype)
+    w, g, m0, dtrue = fwi_case(nts=300, dts=0.001, nExp=3, nz=48, nx=72)
+    sz, sx, rz, rx, src = w["src_z"], w["src_x"], w["rec_z"], w["rec_x"], w["model"]
+    dobs = np.stack([O.nl_fwd_shot(g, m0 + dtrue, sz[e], sx[e], rz, rx, src) for e in range(3)])
+    eng = _engine(w, g, dtype, segment=7)
+    m = torch.tensor(m0, dtype=dtype, device="cuda", requires_grad=True)
+    d_obs = torch.tensor(dobs, dtype=dtype, device="cuda")
+    from elastic_isotropic_operators_amd.fwi import datcomp
+    comps = "vx,vz,p"
+    loss = eng.misfit(m, datcomp(d_obs, comps), comps)
+    loss.backward()
+    ref = m.grad.double()
+    tol = 1e-10 if dtype == torch.float64 else 1e-5
+    for group in (1, 2, None):
+        phi, grad = eng.misfit_and_gradient(m.detach(), datcomp(d_obs, comps), comps, group=group)
+        assert abs(phi - float(loss)) / float(loss) < tol
+        assert float(torch.linalg.norm(grad.double() - ref) / torch.linalg.norm(ref)) < tol, group
+    assert eng.last_store_info[1] == 0          # one segment: no checkpoints
+    # the stores borrow the context's workspace; freeing it between calls changes nothing
+    eng.ctx.release_workspace()
+    phi2, grad2 = eng.misfit_and_gradient(m.detach(), datcomp(d_obs, comps), comps, group=2)
+    assert float(torch.linalg.norm(grad2.double() - ref) / torch.linalg.norm(ref)) < tol
