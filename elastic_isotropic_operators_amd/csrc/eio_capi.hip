@@ -578,11 +578,12 @@ void launch_adj(Run<T>& run, int ntile, int k, int sspar) {
     }
 #undef EIO_L
 }
-void launch_fwd2(Run<float>& run, int ntile, int k, int par) {
+template <bool DEC = false>
+void launch_fwd2(Run<float>& run, int ntile, int k, int par, int dlo = 0, int dhi = -1) {
     hipStream_t s = run.c->stream;
-#define EIO_L2(R_, WX_, OCC_, NT_)                                                                   \
-    hipLaunchKernelGGL((eio::k_step2_fwd<float, R_, WX_, OCC_, NT_>), dim3(ntile), dim3(64 * WX_), 0, s, \
-                       run.d_shots, run.d_tiles, run.d_ents, ntile, k, par)
+#define EIO_L2(R_, WX_, OCC_, NT_)                                                                         \
+    hipLaunchKernelGGL((eio::k_step2_fwd<float, R_, WX_, OCC_, NT_, DEC>), dim3(ntile), dim3(64 * WX_), 0, s, \
+                       run.d_shots, run.d_tiles, run.d_ents, ntile, k, par, dlo, dhi)
     switch (run.c->variant2) {
         case 0: EIO_L2(2, 8, 2, 0); break;
         case 1: EIO_L2(2, 8, 4, 0); break;
@@ -1127,12 +1128,22 @@ int fwi_forward(Ctx* c, int nshot, const eio_shot* shots, int flags, int segment
         }
     }
     try {
-        Run<T> run = prepare<T>(c, nshot, shots, Op::FWI_FWD, flags, nullptr, static_cast<T*>(f.ring), f.wcap);
+        // one segment (the whole record kept): the two-step kernel with decimation; pairs of
+        // fine steps stay inside one coarse interval because the fine sub-sampling is even
+        const bool two = sizeof(T) == 4 && f.nseg == 1 && c->time_block == 2 && sub % 2 == 0;
+        Run<T> run = prepare<T>(c, nshot, shots, Op::FWI_FWD, flags, nullptr, static_cast<T*>(f.ring), f.wcap, two);
         const int ntile = int(run.plan.tiles.size());
         T* state = run.sd[0].A;  // the batch state is one contiguous buffer in plan order
         timing_begin<T>(c);
         EIO_CHECK_HIP(hipEventRecord(c->ev[2], c->stream));
         run_loop(c, [&]() {
+            if constexpr (sizeof(T) == 4) {
+                if (two) {
+                    for (int k = 0; k < (nts - 1) * sub; k += 2)
+                        launch_fwd2<true>(run, ntile, k, (k >> 1) & 1, 0, -1);
+                    return;
+                }
+            }
             for (int j = 0; j < f.nseg; ++j) {
                 if (j > 0) copy_dev<T>(c, static_cast<T*>(f.ck) + size_t(j - 1) * f.state_elems, state, f.state_elems);
                 const int k0 = seg_begin(f, j) * sub, k1 = seg_end(f, j) * sub;

@@ -18,6 +18,12 @@
 // Sources: each tile's list holds every source of the step-1 region (lcell relative to
 // (z0-4, x0-4)), injected into its own copy of u1; only owned cells are stored.  Receivers
 // are owned-only.  Arithmetic per cell is that of k_step_fwd in mode M_NL.
+//
+// DEC (the forward of the FWI gradient with its whole coarse record): both steps lie in the
+// same coarse interval (pairs start at even fine steps and the fine sub-sampling is even), so
+// their Vx, Vz contributions to slots its and its+1 (kernelsGpuElastic.cu:1337-1346) are
+// summed in registers and each slot is read-modified-written once per launch, with the
+// single-step kernel's store-first rule (dlo, dhi).
 #pragma once
 #include "eio_kernels.h"
 
@@ -33,11 +39,11 @@ __device__ __forceinline__ void bst_pol(rsrc_t r, T v, unsigned voff, unsigned s
     else bst(r, v, voff, soff);
 }
 
-// NTS: store u1 and u2 with the nontemporal cache policy
-template <typename T, int R2, int WX, int OCC, int NTS>
+// NTS: store u1 and u2 with the nontemporal cache policy; DEC: also decimate Vx, Vz
+template <typename T, int R2, int WX, int OCC, int NTS, bool DEC>
 __global__ __launch_bounds__(64 * WX) __attribute__((amdgpu_waves_per_eu(OCC)))
 void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict__ tiles,
-                 const Entry* __restrict__ ents, int ntiles, int k, int par) {
+                 const Entry* __restrict__ ents, int ntiles, int k, int par, int dlo, int dhi) {
     static_assert(8 % WX == 0, "WX must divide 8");
     constexpr int NT = 64 * WX;
     constexpr int TX = WX * R2;        // owned columns
@@ -208,6 +214,14 @@ void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
     const int ix2 = t.x0 + j2;
     const int lz2 = lane < FAT ? FAT : (lane > FAT + TZ2 - 1 ? FAT + TZ2 - 1 : lane);  // in-range LDS rows
     T v2[5][R2];
+    T u1x[R2], u1z[R2];  // u1's Vx, Vz on the owned cells (decimation)
+    if constexpr (DEC) {
+#pragma unroll
+        for (int r = 0; r < R2; ++r) {
+            u1x[r] = l1[0][j2 + r + FAT][lz2];
+            u1z[r] = l1[1][j2 + r + FAT][lz2];
+        }
+    }
     {
         T dx_sxx[R2], dx_sxz[R2], dx_vx[R2], dx_vz[R2];
         {
@@ -298,6 +312,22 @@ void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
             }
         }
     }
+    rsrc_t rwx0, rwx1, rwz0, rwz1;
+    T f1a = T(0), f1b = T(0), f2a = T(0), f2b = T(0);
+    bool sta = false, stb = false;
+    if constexpr (DEC) {
+        const size_t s0 = size_t(st1.its % S.wcap) * n, s1 = size_t((st1.its + 1) % S.wcap) * n;
+        rwx0 = make_rsrc(S.Wvx + s0, fbytes);
+        rwx1 = make_rsrc(S.Wvx + s1, fbytes);
+        rwz0 = make_rsrc(S.Wvz + s0, fbytes);
+        rwz1 = make_rsrc(S.Wvz + s1, fbytes);
+        f1a = S.filt[st1.it2];
+        f1b = S.filt[S.h + st1.it2];
+        f2a = S.filt[st2.it2];
+        f2b = S.filt[S.h + st2.it2];
+        sta = st1.it2 == 1 && st1.its == dlo;
+        stb = st1.it2 == 1 && st1.its + 1 != dhi;
+    }
 #pragma unroll
     for (int r = 0; r < R2; ++r) {
         const int ix = ix2 + r;
@@ -305,6 +335,22 @@ void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
             const unsigned vo = (unsigned(ix) * unsigned(nz) + unsigned(iz)) * SZ;
 #pragma unroll
             for (int f = 0; f < 5; ++f) bst_pol<T>(rw2, v2[f][r], vo, unsigned(f) * fbytes, NTS);
+            if constexpr (DEC) {
+                T xa = u1x[r] * f1a + v2[0][r] * f2a, za = u1z[r] * f1a + v2[1][r] * f2a;
+                T xb = u1x[r] * f1b + v2[0][r] * f2b, zb = u1z[r] * f1b + v2[1][r] * f2b;
+                if (!sta) {
+                    xa = bld<T>(rwx0, vo, 0) + xa;
+                    za = bld<T>(rwz0, vo, 0) + za;
+                }
+                if (!stb) {
+                    xb = bld<T>(rwx1, vo, 0) + xb;
+                    zb = bld<T>(rwz1, vo, 0) + zb;
+                }
+                bst(rwx0, xa, vo, 0u);
+                bst(rwz0, za, vo, 0u);
+                bst(rwx1, xb, vo, 0u);
+                bst(rwz1, zb, vo, 0u);
+            }
         }
     }
 }

@@ -154,10 +154,11 @@ class ElasticFwi:
     segment: checkpoint interval in coarse samples (0: ceil(sqrt(nts-1)))
     src_interp, rec_interp: (method, nfilt) of the spatial interpolation of sources and
              receivers: ("linear", 1), ("sinc", n) or ("gauss", n) (spaceInterpGpu)
+    time_block: (fine steps per launch, two-step kernel variant) of the FP32 forwards
     """
 
     def __init__(self, grid, src_z, src_x, sources, rec_z, rec_x, device=0, dtype=torch.float32,
-                 segment=0, variant=None, src_interp=("linear", 1), rec_interp=("linear", 1)):
+                 segment=0, variant=None, src_interp=("linear", 1), rec_interp=("linear", 1), time_block=(2, 4)):
         if not torch.cuda.is_available():
             if _capi._lib is not None:
                 raise RuntimeError("torch sees no GPU because libeio_hip.so was loaded before torch: "
@@ -173,6 +174,9 @@ class ElasticFwi:
         self.ctx = _capi.Context(device, np.float32 if dtype == torch.float32 else np.float64)
         if variant is not None:
             self.ctx.set_variant(variant)
+        # a forward that keeps its whole coarse record runs two fine steps per launch
+        # (time_block = (steps, two-step variant); (1, 0): one step per launch)
+        self.ctx.set_time_block(*time_block)
         self.mid = None
         self.last_store_info = None  # (interval, checkpoints, device bytes) of the last store
         src_z = np.atleast_2d(np.asarray(src_z, np.float64))

@@ -74,6 +74,46 @@ def test_checkpointed_gradient_matches_full_storage_oracle(dtype, segment):
         ctx.fwi_gradient(h, [shot2], out2)
 
 
+@pytest.mark.parametrize("variant2", [4, 0])
+def test_two_step_forward_with_whole_record_matches_oracle(variant2):
+    """FP32, one segment (the whole coarse record): the forward runs the two-step kernel with
+    the decimation fused; data and gradient against the FP64 oracle's full-storage Born
+    adjoint, at sub_fd 4 (two launches per coarse interval)."""
+    w, g, med, spos, rpos = _reg_case(sub=2, nshot=2)
+    rng = np.random.default_rng(47)
+    srcs = [random_traces(rng, spos[e], g.nts, np.float64) for e in range(2)]
+    res = [random_traces(rng, rpos, g.nts, np.float64) for _ in range(2)]
+    ctx = _capi.Context(0, np.float32)
+    ctx.set_time_block(2, variant2)
+    mid = ctx.model_add(g.nz, g.nx, g.nts, g.sub, g.dz, g.dx, g.dts, g.minPad, med, g.alphaCos)
+    shots, keep, recs = [], [], []
+    for e in range(2):
+        rec = _zeros_like_traces(rpos, g.nts, np.float32)
+        s, kp = ctx.make_shot(mid, spos[e], rpos, [np.ascontiguousarray(x, np.float32) for x in srcs[e]], rec)
+        shots.append(s)
+        keep.append(kp)
+        recs.append(rec)
+    h = ctx.fwi_forward(shots, g.nts - 1)
+    assert ctx.fwi_info(h)[1] == 0
+    ref_g = np.zeros((5, g.nx, g.nz))
+    for e in range(2):
+        ref_d = O.nl_fwd_reg(g, med, spos[e], srcs[e], rpos, np.float64)
+        for k in range(5):
+            assert rel_l2(recs[e][k], ref_d[k]) < 1e-5
+        src_fine = [O.time_interp(x, g.dts, g.sub) for x in srcs[e]]
+        ref_g += np.asarray(O.born_adj_reg(g, med, spos[e], src_fine, rpos, res[e], np.float64))
+    shots2 = []
+    for e in range(2):
+        s, kp = ctx.make_shot(mid, spos[e], rpos, [np.ascontiguousarray(x, np.float32) for x in srcs[e]],
+                              [np.ascontiguousarray(x, np.float32) for x in res[e]])
+        shots2.append(s)
+        keep.append(kp)
+    dm = np.zeros((5, g.nx, g.nz), np.float32)
+    ctx.fwi_gradient(h, shots2, dm)
+    ctx.fwi_release(h)
+    assert rel_l2(dm, ref_g) < 1e-4, rel_l2(dm, ref_g)
+
+
 def test_checkpointed_gradient_several_shots():
     """Three shots in one batch: stacked gradient equals the sum of per-shot oracle adjoints."""
     dtype = np.float32
