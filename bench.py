@@ -111,9 +111,11 @@ def fwi_child(args):
                 nts=w0["nts"], dts=w0["dts"], sub=w0["sub"], minPad=W.min_pad(w0), alphaCos=w0["alphaCos"])
     comps = "vx,vz,p"
 
+    adj = tuple(int(x) for x in args.fwi_adj.split(":"))
+
     def engine(dtype, sel):
         return ElasticFwi(grid, w0["src_z"][sel], w0["src_x"][sel], w0["model"], w0["rec_z"], w0["rec_x"],
-                          dtype=dtype)
+                          dtype=dtype, adj_time_block=adj)
 
     eng = engine(torch.float32, slice(None))
     with torch.no_grad():
@@ -162,6 +164,7 @@ def fwi_child(args):
            "gcell_per_s": round(grad_cells / t / 1e9, 2), "rel_l2_vs_fp64": rel, "fp64_check_shots": sel,
            "checkpoint_interval": L, "checkpoints": nck, "store_bytes": nbytes, "shots_per_group": group,
            "method": "shot groups with full coarse Vx/Vz storage sized to HBM, gradient per group",
+           "adjoint_time_block": list(adj),
            "full_storage_bytes": 2 * 4 * w0["nz"] * w0["nx"] * w0["nts"] * eng.nshot,
            "grad_norm": float(torch.linalg.norm(g32))}
     print("FWI_JSON " + json.dumps(out), flush=True)
@@ -171,7 +174,7 @@ def run_fwi_leg(args):
     import subprocess
     cmd = [sys.executable, os.path.abspath(__file__), "--fwi-child", "--fwi-shots", str(args.fwi_shots),
            "--fwi-steps", str(args.fwi_steps), "--fwi-warmup", str(args.fwi_warmup),
-           "--fwi-scale", str(args.fwi_scale)]
+           "--fwi-scale", str(args.fwi_scale), "--fwi-adj", args.fwi_adj]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=900)
     for line in r.stdout.splitlines():
         if line.startswith("FWI_JSON "):
@@ -229,6 +232,9 @@ def main():
     ap.add_argument("--fwi-steps", type=int, default=1)
     ap.add_argument("--fwi-warmup", type=int, default=1)
     ap.add_argument("--fwi-scale", type=float, default=1.0)
+    ap.add_argument("--fwi-adj", default="2:4",
+                    help="reverse fine steps per launch of the gradient's receiver pass and the two-step "
+                         "adjoint variant, as steps:variant")
     ap.add_argument("--fwi-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--abc", type=int, default=1, help="run the learned-boundary leg (needs its weights)")
     ap.add_argument("--abc-scale", type=float, default=1.0)

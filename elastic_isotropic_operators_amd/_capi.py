@@ -84,6 +84,10 @@ def lib():
             ("eio_ctx_time_block_info", [ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
             ("eio_ctx_set_time_block", [vp, ctypes.c_int, ctypes.c_int]),
+            ("eio_ctx_num_adjoint_time_block_variants", []),
+            ("eio_ctx_adjoint_time_block_info", [ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                                 ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+            ("eio_ctx_set_adjoint_time_block", [vp, ctypes.c_int, ctypes.c_int]),
             ("eio_ctx_set_step_limit", [vp, ctypes.c_int]),
             ("eio_nl_forward_wavefield", [vp, ctypes.c_int, ctypes.POINTER(Shot), ctypes.c_int, ctypes.c_int, vp]),
             ("eio_nl_adjoint_wavefield", [vp, ctypes.c_int, ctypes.POINTER(Shot), ctypes.c_int, ctypes.c_int, vp]),
@@ -104,7 +108,8 @@ EXPORTED = ("eio_last_error", "eio_ctx_create", "eio_ctx_destroy", "eio_ctx_set_
             "eio_fwi_release", "eio_ctx_num_time_block_variants", "eio_ctx_time_block_info",
             "eio_ctx_set_time_block", "eio_ctx_set_step_limit", "eio_nl_forward_wavefield",
             "eio_nl_adjoint_wavefield", "eio_model_set_abc", "eio_nl_forward_snapshots",
-            "eio_nl_step_states", "eio_ctx_release_workspace")
+            "eio_nl_step_states", "eio_ctx_release_workspace", "eio_ctx_num_adjoint_time_block_variants",
+            "eio_ctx_adjoint_time_block_info", "eio_ctx_set_adjoint_time_block")
 
 
 def check(rc):
@@ -190,6 +195,21 @@ class Context:
     def set_time_block(self, steps, variant=0):
         """Fine steps per launch of the FP32 NL forward (1 or 2) and the two-step variant."""
         check(lib().eio_ctx_set_time_block(self.h, int(steps), int(variant)))
+
+    def set_adjoint_time_block(self, steps, variant=0):
+        """Reverse fine steps per launch of the FP32 Born receiver pass (1 or 2) and the
+        two-step adjoint variant."""
+        check(lib().eio_ctx_set_adjoint_time_block(self.h, int(steps), int(variant)))
+
+    @staticmethod
+    def adjoint_time_block_variants():
+        """[(R2, WX, occ)] of the two-step adjoint kernel variants."""
+        out = []
+        for v in range(lib().eio_ctx_num_adjoint_time_block_variants()):
+            r, wx, oc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+            check(lib().eio_ctx_adjoint_time_block_info(v, ctypes.byref(r), ctypes.byref(wx), ctypes.byref(oc)))
+            out.append((r.value, wx.value, oc.value))
+        return out
 
     def set_step_limit(self, max_steps):
         """Stop NL loops after max_steps fine steps (0: no limit); for tuning runs."""

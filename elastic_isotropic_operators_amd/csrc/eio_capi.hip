@@ -69,6 +69,13 @@ struct Variant2 {
 constexpr Variant2 F32_VARIANTS2[] = {{2, 8, 2, 0}, {2, 8, 4, 0}, {4, 4, 2, 0}, {1, 8, 4, 0},
                                       {4, 8, 1, 0}, {2, 8, 4, 1}, {2, 8, 2, 1}, {4, 8, 1, 1}};
 constexpr int N_F32_VARIANTS2 = int(sizeof(F32_VARIANTS2) / sizeof(F32_VARIANTS2[0]));
+// Two-step adjoint (Born receiver pass, FWI gradient) variants: R2, WX, OCC as above.
+struct Variant2A {
+    int R2, WX, OCC, LATE;  // LATE: per-cell loads after the stage is written
+};
+constexpr Variant2A F32_VARIANTS2A[] = {{2, 8, 2, 0}, {4, 8, 1, 0}, {2, 8, 4, 0}, {4, 4, 2, 0},
+                                        {2, 8, 4, 1}, {2, 8, 2, 1}, {4, 8, 1, 1}};
+constexpr int N_F32_VARIANTS2A = int(sizeof(F32_VARIANTS2A) / sizeof(F32_VARIANTS2A[0]));
 
 template <typename T>
 int tile_x(int variant) {
@@ -124,6 +131,8 @@ struct Ctx {
     int variant = 1;  // best measured FP32 variant on the 64-shot layered batch
     int time_block = 1;  // fine steps per launch of the FP32 NL forward (1 or 2)
     int variant2 = 0;    // two-step kernel variant
+    int adj_time_block = 1;  // reverse fine steps per launch of the FP32 Born adjoint (1 or 2)
+    int variant2a = 0;       // two-step adjoint kernel variant
     int step_limit = 0;  // > 0: NL loops stop after this many fine steps (tuning runs)
     std::map<int, FwiStore> fwi;
     int next_fwi = 1;
@@ -315,10 +324,17 @@ void build_entries2(const ModelBase& m, const int* const pos[4], const int nreg[
     }
 }
 
+// Tile plans: one-step kernels (64 x TX tiles), the two-step forward (56 x TX tiles, source
+// entries on the step-1 region, receivers owned-only) and the two-step adjoint (56 x TX
+// tiles, receiver entries on the step-1 region: they are injected in both steps).
+enum class PlanKind { ONE, FWD2, ADJ2 };
+
 template <typename T>
-Plan make_plan(Ctx* c, int nshot, const eio_shot* shots, Op op, bool two_step = false) {
-    const int TX = two_step ? F32_VARIANTS2[c->variant2].R2 * F32_VARIANTS2[c->variant2].WX
-                            : tile_x<T>(c->variant);
+Plan make_plan(Ctx* c, int nshot, const eio_shot* shots, Op op, PlanKind kind = PlanKind::ONE) {
+    const bool two_step = kind != PlanKind::ONE;
+    const int TX = kind == PlanKind::FWD2   ? F32_VARIANTS2[c->variant2].R2 * F32_VARIANTS2[c->variant2].WX
+                   : kind == PlanKind::ADJ2 ? F32_VARIANTS2A[c->variant2a].R2 * F32_VARIANTS2A[c->variant2a].WX
+                                            : tile_x<T>(c->variant);
     const int TZ = two_step ? eio::TZ2 : eio::TZ;
     Plan p;
     p.order.resize(nshot);
@@ -332,9 +348,11 @@ Plan make_plan(Ctx* c, int nshot, const eio_shot* shots, Op op, bool two_step = 
         const int nbz = (m.g.nz - 2 * eio::FAT + TZ - 1) / TZ;
         const int nbx = (m.g.nx - 2 * eio::FAT + TX - 1) / TX;
         std::vector<std::vector<eio::Entry>> src_t(size_t(nbz) * nbx), rec_t(size_t(nbz) * nbx);
-        if (two_step) {
+        if (kind == PlanKind::FWD2) {
             build_entries2(m, s.src_pos, s.n_src, TX, true, src_t, nbz, nbx, "source");
             build_entries2(m, s.rec_pos, s.n_rec, TX, false, rec_t, nbz, nbx, "receiver");
+        } else if (kind == PlanKind::ADJ2) {
+            build_entries2(m, s.rec_pos, s.n_rec, TX, true, rec_t, nbz, nbx, "receiver");
         } else {
             build_entries<T>(m, s.src_pos, s.n_src, TX, src_t, nbz, "source");
             build_entries<T>(m, s.rec_pos, s.n_rec, TX, rec_t, nbz, "receiver");
@@ -372,6 +390,11 @@ struct Run {
     eio::ShotDev<T>* d_shots = nullptr;
     eio::TileDev* d_tiles = nullptr;
     eio::Entry* d_ents = nullptr;
+    // second plan of a Born adjoint / FWI gradient run with the two-step adjoint kernel
+    // (the one-step plan above serves the forward passes and the imaging condition)
+    Plan plan2;
+    eio::TileDev* d_tiles2 = nullptr;
+    eio::Entry* d_ents2 = nullptr;
     // host copies for the D2H of outputs
     struct Out {
         void* host;
@@ -383,16 +406,19 @@ struct Run {
 
 // Builds the plan, the per-shot descriptors and the device tables of one call, uploads
 // input traces and zeroes the state and the outputs.  FWI calls bring their own coarse
-// storage ring (`ring`, `wcap` slices per component and shot).
+// storage ring (`ring`, `wcap` slices per component and shot).  `two_step`: the NL forward
+// runs the two-step kernel; `adj2`: the Born adjoint pass runs the two-step adjoint kernel.
 template <typename T>
 Run<T> prepare(Ctx* c, int nshot, const eio_shot* shots, Op op, int flags,
-               const void* const dm_fwd[5], T* ring = nullptr, int wcap = 0, bool two_step = false) {
+               const void* const dm_fwd[5], T* ring = nullptr, int wcap = 0, bool two_step = false,
+               bool adj2 = false) {
     if (nshot <= 0) fail("nshot must be > 0");
     if (!shots) fail("shots is NULL");
     Run<T> run;
     run.c = c;
-    run.plan = make_plan<T>(c, nshot, shots, op, two_step);
-    const int nlev = two_step ? 4 : 2;  // state levels per shot
+    run.plan = make_plan<T>(c, nshot, shots, op, two_step ? PlanKind::FWD2 : PlanKind::ONE);
+    if (adj2) run.plan2 = make_plan<T>(c, nshot, shots, op, PlanKind::ADJ2);
+    const int nlev = (two_step || adj2) ? 4 : 2;  // state levels per shot
     const bool born = (op == Op::BORN_FWD || op == Op::BORN_ADJ);
     const bool fwi = (op == Op::FWI_FWD || op == Op::FWI_GRAD);
     const bool has_ss = born || op == Op::FWI_GRAD;
@@ -443,7 +469,7 @@ Run<T> prepare(Ctx* c, int nshot, const eio_shot* shots, Op op, int flags,
         for (int k = 0; k < 4; ++k) { d.cz[k] = T(m.cz[k]); d.cx[k] = T(m.cx[k]); }
         d.A = state + so;
         d.B = state + so + 5 * size_t(m.n);
-        if (two_step) {
+        if (two_step || adj2) {
             d.C = state + so + 10 * size_t(m.n);
             d.D = state + so + 15 * size_t(m.n);
         }
@@ -514,6 +540,14 @@ Run<T> prepare(Ctx* c, int nshot, const eio_shot* shots, Op op, int flags,
     EIO_CHECK_HIP(hipMemcpyAsync(run.d_tiles, run.plan.tiles.data(), sizeof(eio::TileDev) * run.plan.tiles.size(), h2d, c->stream));
     if (!run.plan.ents.empty())
         EIO_CHECK_HIP(hipMemcpyAsync(run.d_ents, run.plan.ents.data(), sizeof(eio::Entry) * run.plan.ents.size(), h2d, c->stream));
+    if (adj2) {
+        const Plan& p2 = run.plan2;
+        run.d_tiles2 = static_cast<eio::TileDev*>(ws_get(c, "tiles2", sizeof(eio::TileDev) * p2.tiles.size()));
+        run.d_ents2 = static_cast<eio::Entry*>(ws_get(c, "ents2", sizeof(eio::Entry) * std::max<size_t>(1, p2.ents.size())));
+        EIO_CHECK_HIP(hipMemcpyAsync(run.d_tiles2, p2.tiles.data(), sizeof(eio::TileDev) * p2.tiles.size(), h2d, c->stream));
+        if (!p2.ents.empty())
+            EIO_CHECK_HIP(hipMemcpyAsync(run.d_ents2, p2.ents.data(), sizeof(eio::Entry) * p2.ents.size(), h2d, c->stream));
+    }
     // ---- zero state (and Born buffers; the coarse storage needs no clearing: the
     // decimation overwrites each slot with its first contribution)
     fill0<T>(c, state, long(state_elems));
@@ -595,6 +629,41 @@ void launch_fwd2(Run<float>& run, int ntile, int k, int par, int dlo = 0, int dh
         default: EIO_L2(4, 8, 1, 1); break;
     }
 #undef EIO_L2
+}
+
+// Reverse fine steps k, k+1 (k even) of the Born adjoint pass; adjoint-field slots as in
+// launch_adj (sspar: slot of coarse step its).
+inline void launch_adj2(Run<float>& run, int k, int sspar) {
+    hipStream_t s = run.c->stream;
+    const int ntile = int(run.plan2.tiles.size());
+    const int par = (k >> 1) & 1;
+#define EIO_L2A(R_, WX_, OCC_, LATE_)                                                                   \
+    hipLaunchKernelGGL((eio::k_step2_adj<float, R_, WX_, OCC_, LATE_>), dim3(ntile), dim3(64 * WX_), 0, s, \
+                       run.d_shots, run.d_tiles2, run.d_ents2, ntile, k, par, sspar)
+    switch (run.c->variant2a) {
+        case 0: EIO_L2A(2, 8, 2, false); break;
+        case 1: EIO_L2A(4, 8, 1, false); break;
+        case 2: EIO_L2A(2, 8, 4, false); break;
+        case 3: EIO_L2A(4, 4, 2, false); break;
+        case 4: EIO_L2A(2, 8, 4, true); break;
+        case 5: EIO_L2A(2, 8, 2, true); break;
+        default: EIO_L2A(4, 8, 1, true); break;
+    }
+#undef EIO_L2A
+}
+
+// The Born adjoint pass over coarse interval its (reverse fine steps (nts-2-its)*sub ..),
+// adjoint fields of the interval into slots ss[p] (its) and ss[p^1] (its+1).
+template <typename T>
+void adjoint_interval(Run<T>& run, int ntile, bool two, int its, int nts, int sub, int p) {
+    const int k0 = (nts - 2 - its) * sub;
+    if constexpr (sizeof(T) == 4) {
+        if (two) {
+            for (int j = 0; j < sub; j += 2) launch_adj2(run, k0 + j, p);
+            return;
+        }
+    }
+    for (int j = 0; j < sub; ++j) launch_adj<T, eio::M_BADJ>(run, ntile, k0 + j, p);
 }
 
 template <typename T, bool ADJ>
@@ -982,9 +1051,10 @@ template <typename T>
 void born_adjoint(Ctx* c, int nshot, const eio_shot* shots, void* const dm[5], int flags) {
     if (!dm) fail("dm is NULL");
     const ModelBase& m = get_model(c, shots[0].model);
-    Run<T> run = prepare<T>(c, nshot, shots, Op::BORN_ADJ, flags, nullptr);
-    const int ntile = int(run.plan.tiles.size());
     const int kmax = m.ntw - 1, sub = m.g.sub_fd, nts = m.g.nts;
+    const bool two = sizeof(T) == 4 && c->adj_time_block == 2 && sub % 2 == 0;
+    Run<T> run = prepare<T>(c, nshot, shots, Op::BORN_ADJ, flags, nullptr, nullptr, 0, false, two);
+    const int ntile = int(run.plan.tiles.size());
     timing_begin<T>(c);
     EIO_CHECK_HIP(hipEventRecord(c->ev[2], c->stream));
     run_loop(c, [&]() {
@@ -992,11 +1062,14 @@ void born_adjoint(Ctx* c, int nshot, const eio_shot* shots, void* const dm[5], i
         for (int oi = 0; oi < nshot; ++oi) {
             fill0<T>(c, run.sd[oi].A, 5 * m.n);
             fill0<T>(c, run.sd[oi].B, 5 * m.n);
+            if (two) {
+                fill0<T>(c, run.sd[oi].C, 5 * m.n);
+                fill0<T>(c, run.sd[oi].D, 5 * m.n);
+            }
         }
         int p = 0;
         for (int its = nts - 2; its >= 0; --its) {
-            for (int it2 = sub - 1; it2 >= 0; --it2)
-                launch_adj<T, eio::M_BADJ>(run, ntile, (nts - 2 - its) * sub + (sub - 1 - it2), p);
+            adjoint_interval(run, ntile, two, its, nts, sub, p);
             launch_img<T, true>(run, ntile, its + 1, p ^ 1);
             p ^= 1;
         }
@@ -1179,7 +1252,9 @@ void fwi_gradient(Ctx* c, int handle, int nshot, const eio_shot* shots, void* co
         fail("shots differ from those of the FWI forward call of this store");
     const ModelBase& m = get_model(c, f.model);
     if (m.g.nts != f.nts || m.g.sub_fd != f.sub) fail("model time axis changed since the FWI forward call");
-    Run<T> run = prepare<T>(c, nshot, shots, Op::FWI_GRAD, flags, nullptr, static_cast<T*>(f.ring), f.wcap);
+    const bool two = sizeof(T) == 4 && c->adj_time_block == 2 && f.sub % 2 == 0;
+    Run<T> run = prepare<T>(c, nshot, shots, Op::FWI_GRAD, flags, nullptr, static_cast<T*>(f.ring), f.wcap,
+                            false, two);
     const int ntile = int(run.plan.tiles.size());
     const int nts = f.nts, sub = f.sub;
     // forward-recompute state and its shot table
@@ -1220,8 +1295,7 @@ void fwi_gradient(Ctx* c, int handle, int nshot, const eio_shot* shots, void* co
                 recompute(jn - 1);
                 --jn;
             }
-            for (int it2 = sub - 1; it2 >= 0; --it2)
-                launch_adj<T, eio::M_BADJ>(run, ntile, (nts - 2 - its) * sub + (sub - 1 - it2), p);
+            adjoint_interval(run, ntile, two, its, nts, sub, p);
             launch_img<T, true>(run, ntile, its + 1, p ^ 1);
             p ^= 1;
         }
@@ -1341,6 +1415,27 @@ int eio_ctx_time_block_info(int variant, int* R2, int* WX, int* occ) {
 }
 
 int eio_ctx_num_time_block_variants(void) { return N_F32_VARIANTS2; }
+
+int eio_ctx_set_adjoint_time_block(eio_ctx* ctx, int steps, int variant) {
+    return guarded([&]() {
+        if (steps != 1 && steps != 2) fail("adjoint time block must be 1 or 2 fine steps");
+        if (variant < 0 || variant >= N_F32_VARIANTS2A) fail("two-step adjoint variant out of range");
+        Ctx* c = as_ctx(ctx);
+        c->adj_time_block = steps;
+        c->variant2a = variant;
+    });
+}
+
+int eio_ctx_adjoint_time_block_info(int variant, int* R2, int* WX, int* occ) {
+    return guarded([&]() {
+        if (variant < 0 || variant >= N_F32_VARIANTS2A) fail("two-step adjoint variant out of range");
+        if (R2) *R2 = F32_VARIANTS2A[variant].R2;
+        if (WX) *WX = F32_VARIANTS2A[variant].WX;
+        if (occ) *occ = F32_VARIANTS2A[variant].OCC;
+    });
+}
+
+int eio_ctx_num_adjoint_time_block_variants(void) { return N_F32_VARIANTS2A; }
 
 int eio_ctx_set_step_limit(eio_ctx* ctx, int max_steps) {
     return guarded([&]() {

@@ -355,4 +355,310 @@ void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
     }
 }
 
+// ------------------------------------------------------------------ two-step adjoint
+// Two reverse fine steps per launch for the Born receiver pass and the FWI gradient (the
+// arithmetic of k_step_adj in mode M_BADJ).  The adjoint stencil takes derivatives of the
+// media-scaled level, so step 1 stages M u_cur on (TX+16) x 72 cells from (x0-8, z0-8)
+// (kernelsGpuElastic.cu:380-529), computes u1 on the (TX+8) x 64 step-1 region, and, once
+// the data of step 1 are injected, writes M u1 over that stage for step 2 on the owned
+// 56 x TX cells.  u_old, u_cur and the media are read once and u1, u2 written once: 100 B per
+// two cell-updates, as for the forward.
+//
+// Both reverse steps lie in one coarse interval (pairs start at even reverse steps and the
+// fine sub-sampling of the Born operator is even), so their adjoint-field contributions
+// (kernelsGpuElastic.cu:1369-1376) are summed in registers: slot its (ss[sspar]) is written
+// by the interval's first pair and read-modified-written by later ones, slot its+1
+// (ss[sspar^1]) read-modified-written once per pair.  Levels A,B -> C,D and back (par), as
+// for the forward.  LATE: the older level is loaded after the stage is written, the step-1
+// media after step 1 and u_cur of the owned cells after M u1 is formed (each set of values
+// is live only in the phase that uses it: fewer registers, more latency to hide).
+template <typename T, int R2, int WX, int OCC, bool LATE = false>
+__global__ __launch_bounds__(64 * WX) __attribute__((amdgpu_waves_per_eu(OCC)))
+void k_step2_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict__ tiles,
+                 const Entry* __restrict__ ents, int ntiles, int k, int par, int sspar) {
+    static_assert(8 % WX == 0, "WX must divide 8");
+    constexpr int NT = 64 * WX;
+    constexpr int TX = WX * R2;        // owned columns
+    constexpr int R1 = R2 + 8 / WX;    // step-1 columns per wave
+    constexpr int TX1 = TX + 8;        // step-1 columns
+    constexpr int TXC = TX + 16;       // staged columns
+    constexpr unsigned SZ = sizeof(T);
+    __shared__ T lc[5][TXC][ZC];       // M u_cur, then M u1 on the step-1 region
+    __shared__ T l1[5][TX1][Z1];       // u1, then u2 for the owned data injection
+
+    const TileDev t = tiles[xcd_remap(blockIdx.x, ntiles)];
+    const ShotDev<T>& S = shots[t.shot];
+    const int nz = S.nz, nx = S.nx;
+    const unsigned n = unsigned(nz) * unsigned(nx);
+    const unsigned fbytes = n * SZ;
+    const StepTime st1 = adj_time(k, S.sub, S.nts);
+    const StepTime st2 = adj_time(k + 1, S.sub, S.nts);   // same interval, it2 one lower
+    const T* oldp = par ? S.C : S.A;
+    const T* curp = par ? S.D : S.B;
+    T* out1 = par ? S.A : S.C;
+    T* out2 = par ? S.B : S.D;
+    const rsrc_t rold = make_rsrc(oldp, 5 * fbytes);
+    const rsrc_t rcur = make_rsrc(curp, 5 * fbytes);
+    const rsrc_t rmed = make_rsrc(S.med, 5 * fbytes);
+    const rsrc_t rw1 = make_rsrc(out1, 5 * fbytes);
+    const rsrc_t rw2 = make_rsrc(out2, 5 * fbytes);
+
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int iz = t.z0 - FAT + lane;                       // row of this lane (both steps)
+    const bool zin = iz >= FAT && iz < nz - FAT;            // update-region row
+    const bool zown = lane >= FAT && lane < FAT + TZ2 && zin;
+    const int j1 = w * R1;                                   // first step-1 column (rel x0-4)
+    const int ix1 = t.x0 - FAT + j1;
+    const int izl = iz < 0 ? 0 : iz;                         // masked lanes load row 0
+    const int j2 = w * R2;                                   // first owned column (rel x0)
+    const int ix2 = t.x0 + j2;
+    const int lz2 = lane < FAT ? FAT : (lane > FAT + TZ2 - 1 ? FAT + TZ2 - 1 : lane);
+
+    // ---- stage M u_cur (cells off the grid are 0); the older level and media of the step-1
+    // cells and u_cur of the owned cells are loaded behind the stage loads (or, LATE, after
+    // the stage is written)
+    constexpr int NFILL = (TXC * ZC + NT - 1) / NT;
+    T o[5][R1], m1[5][R1], oc[5][R2];
+    auto load_old = [&]() {
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {
+            const int ix = ix1 + r;
+            const unsigned vo = (unsigned(ix < 0 ? 0 : ix) * unsigned(nz) + unsigned(izl)) * SZ;
+#pragma unroll
+            for (int f = 0; f < 5; ++f) o[f][r] = bld<T>(rold, vo, unsigned(f) * fbytes);
+        }
+    };
+    auto load_med = [&]() {
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {
+            const int ix = ix1 + r;
+            const unsigned vo = (unsigned(ix < 0 ? 0 : ix) * unsigned(nz) + unsigned(izl)) * SZ;
+#pragma unroll
+            for (int f = 0; f < 5; ++f) m1[f][r] = bld<T>(rmed, vo, unsigned(f) * fbytes);
+        }
+    };
+    auto load_cur = [&]() {
+#pragma unroll
+        for (int r = 0; r < R2; ++r) {
+            const unsigned vo = (unsigned(ix2 + r) * unsigned(nz) + unsigned(izl)) * SZ;
+#pragma unroll
+            for (int f = 0; f < 5; ++f) oc[f][r] = bld<T>(rcur, vo, unsigned(f) * fbytes);
+        }
+    };
+    {
+        T hv_[NFILL][5], hm[NFILL][5];
+#pragma unroll
+        for (int it = 0; it < NFILL; ++it) {
+            const int e = threadIdx.x + it * NT;
+            const int c = e / ZC, z = e - c * ZC;
+            const int gx = t.x0 - 2 * FAT + c, gz = t.z0 - 2 * FAT + z;
+            const bool in = e < TXC * ZC && gx >= 0 && gx < nx && gz >= 0 && gz < nz;
+            const unsigned hv = in ? (unsigned(gx) * unsigned(nz) + unsigned(gz)) * SZ : 0u;
+#pragma unroll
+            for (int f = 0; f < 5; ++f) {
+                const T v = bld<T>(rcur, hv, unsigned(f) * fbytes);
+                hv_[it][f] = in ? v : T(0);
+            }
+#pragma unroll
+            for (int f = 0; f < 5; ++f) hm[it][f] = bld<T>(rmed, hv, unsigned(f) * fbytes);
+        }
+        if constexpr (!LATE) {
+            load_old();
+            load_med();
+            load_cur();
+        }
+#pragma unroll
+        for (int it = 0; it < NFILL; ++it) {
+            const int e = threadIdx.x + it * NT;
+            if (e < TXC * ZC) {
+                const int c = e / ZC, z = e - c * ZC;
+                lc[0][c][z] = hm[it][0] * hv_[it][0];
+                lc[1][c][z] = hm[it][1] * hv_[it][1];
+                lc[2][c][z] = hm[it][2] * hv_[it][2] + hm[it][3] * hv_[it][3];
+                lc[3][c][z] = hm[it][3] * hv_[it][2] + hm[it][2] * hv_[it][3];
+                lc[4][c][z] = hm[it][4] * hv_[it][4];
+            }
+        }
+    }
+    if constexpr (LATE) load_old();
+    __syncthreads();
+
+    const T* cz = S.cz;
+    const T* cx = S.cx;
+    // ---- step 1 on columns j1 .. j1+R1-1 (rel x0-4) = lc columns j1+4 ..; lane row = lc row lane+4
+    {
+        T dx_a[R1], dx_s6[R1], dx_s0[R1], dx_s1[R1];
+        {
+            T q[R1 + 8];
+#pragma unroll
+            for (int j = 0; j < R1 + 8; ++j) q[j] = lc[2][j1 + j][lane + FAT];
+#pragma unroll
+            for (int r = 0; r < R1; ++r) dx_a[r] = dm_q(q, r + 4, cx);
+#pragma unroll
+            for (int j = 0; j < R1 + 8; ++j) q[j] = lc[4][j1 + j][lane + FAT];
+#pragma unroll
+            for (int r = 0; r < R1; ++r) dx_s6[r] = dp_q(q, r + 4, cx);
+#pragma unroll
+            for (int j = 0; j < R1 + 8; ++j) q[j] = lc[0][j1 + j][lane + FAT];
+#pragma unroll
+            for (int r = 0; r < R1; ++r) dx_s0[r] = dp_q(q, r + 4, cx);
+#pragma unroll
+            for (int j = 0; j < R1 + 8; ++j) q[j] = lc[1][j1 + j][lane + FAT];
+#pragma unroll
+            for (int r = 0; r < R1; ++r) dx_s1[r] = dm_q(q, r + 4, cx);
+        }
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {
+            const int ix = ix1 + r;
+            const bool ok = zin && ix >= FAT && ix < nx - FAT;
+            const int c = j1 + r + FAT;
+            const T dz_s6 = dp_q(&lc[4][c][lane + FAT], 0, cz);
+            const T dz_b = dm_q(&lc[3][c][lane + FAT], 0, cz);
+            const T dz_s1 = dp_q(&lc[1][c][lane + FAT], 0, cz);
+            const T dz_s0 = dm_q(&lc[0][c][lane + FAT], 0, cz);
+            const T d = ok ? damp_at(S, iz, ix) : T(1);
+            const T v0 = d * (d * o[0][r] - (dx_a[r] + dz_s6));
+            const T v1 = d * (d * o[1][r] - (dz_b + dx_s6[r]));
+            const T v2 = d * (d * o[2][r] - dx_s0[r]);
+            const T v3 = d * (d * o[3][r] - dz_s1);
+            const T v4 = d * (d * o[4][r] - (dz_s0 + dx_s1[r]));
+            l1[0][j1 + r][lane] = ok ? v0 : T(0);
+            l1[1][j1 + r][lane] = ok ? v1 : T(0);
+            l1[2][j1 + r][lane] = ok ? v2 : T(0);
+            l1[3][j1 + r][lane] = ok ? v3 : T(0);
+            l1[4][j1 + r][lane] = ok ? v4 : T(0);
+        }
+    }
+    if constexpr (LATE) load_med();
+    __syncthreads();
+
+    const bool has_inj = t.rec_e > t.rec_b;
+    // ---- step-1 data injection on the whole step-1 region (kernelsGpuElastic.cu:191-229, it2+1)
+    if (has_inj) {
+        const T fa = S.filt[st1.it2 + 1], fb = S.filt[S.h + st1.it2 + 1];
+        for (int e = t.rec_b + threadIdx.x; e < t.rec_e; e += NT) {
+            const Entry en = ents[e];
+            const int lz = en.lcell & 255, lx = en.lcell >> 8;
+            const int comp = en.crow >> 24, row = en.crow & 0xffffff;
+            const T* tr = S.rec_tr[comp] + long(row) * S.nts + st1.its;
+            const T s = tr[0] * fa + tr[1] * fb;
+            l1[comp][lx][lz] += damp_at(S, t.z0 - FAT + lz, t.x0 - FAT + lx) * s;
+        }
+        __syncthreads();
+    }
+    // ---- M u1 over the stage (same cells: lc column j+4, row lane+4), u1 stored on owned cells
+#pragma unroll
+    for (int r = 0; r < R1; ++r) {
+        const int j = j1 + r, ix = ix1 + r;
+        T u[5];
+#pragma unroll
+        for (int f = 0; f < 5; ++f) u[f] = l1[f][j][lane];
+        lc[0][j + FAT][lane + FAT] = m1[0][r] * u[0];
+        lc[1][j + FAT][lane + FAT] = m1[1][r] * u[1];
+        lc[2][j + FAT][lane + FAT] = m1[2][r] * u[2] + m1[3][r] * u[3];
+        lc[3][j + FAT][lane + FAT] = m1[3][r] * u[2] + m1[2][r] * u[3];
+        lc[4][j + FAT][lane + FAT] = m1[4][r] * u[4];
+        if (zown && j >= FAT && j < FAT + TX && ix < nx - FAT) {
+            const unsigned vo = (unsigned(ix) * unsigned(nz) + unsigned(iz)) * SZ;
+#pragma unroll
+            for (int f = 0; f < 5; ++f) bst(rw1, u[f], vo, unsigned(f) * fbytes);
+        }
+    }
+    // u1 on the owned cells of this lane's step-2 columns (adjoint-field extraction)
+    T u1[5][R2];
+#pragma unroll
+    for (int r = 0; r < R2; ++r)
+#pragma unroll
+        for (int f = 0; f < 5; ++f) u1[f][r] = l1[f][j2 + r + FAT][lz2];
+    if constexpr (LATE) load_cur();
+    __syncthreads();
+
+    // ---- step 2 on owned columns j2 .. j2+R2-1 (rel x0) = lc columns j2+8 ..; rows lz2+4
+    T v2[5][R2];
+    {
+        T dx_a[R2], dx_s6[R2], dx_s0[R2], dx_s1[R2];
+        {
+            T q[R2 + 8];
+#pragma unroll
+            for (int j = 0; j < R2 + 8; ++j) q[j] = lc[2][j2 + FAT + j][lz2 + FAT];
+#pragma unroll
+            for (int r = 0; r < R2; ++r) dx_a[r] = dm_q(q, r + 4, cx);
+#pragma unroll
+            for (int j = 0; j < R2 + 8; ++j) q[j] = lc[4][j2 + FAT + j][lz2 + FAT];
+#pragma unroll
+            for (int r = 0; r < R2; ++r) dx_s6[r] = dp_q(q, r + 4, cx);
+#pragma unroll
+            for (int j = 0; j < R2 + 8; ++j) q[j] = lc[0][j2 + FAT + j][lz2 + FAT];
+#pragma unroll
+            for (int r = 0; r < R2; ++r) dx_s0[r] = dp_q(q, r + 4, cx);
+#pragma unroll
+            for (int j = 0; j < R2 + 8; ++j) q[j] = lc[1][j2 + FAT + j][lz2 + FAT];
+#pragma unroll
+            for (int r = 0; r < R2; ++r) dx_s1[r] = dm_q(q, r + 4, cx);
+        }
+#pragma unroll
+        for (int r = 0; r < R2; ++r) {
+            const int ix = ix2 + r;
+            const bool ok = zown && ix < nx - FAT;
+            const int c = j2 + r + 2 * FAT;
+            const T dz_s6 = dp_q(&lc[4][c][lz2 + FAT], 0, cz);
+            const T dz_b = dm_q(&lc[3][c][lz2 + FAT], 0, cz);
+            const T dz_s1 = dp_q(&lc[1][c][lz2 + FAT], 0, cz);
+            const T dz_s0 = dm_q(&lc[0][c][lz2 + FAT], 0, cz);
+            const T d = ok ? damp_at(S, iz, ix) : T(1);
+            v2[0][r] = d * (d * oc[0][r] - (dx_a[r] + dz_s6));
+            v2[1][r] = d * (d * oc[1][r] - (dz_b + dx_s6[r]));
+            v2[2][r] = d * (d * oc[2][r] - dx_s0[r]);
+            v2[3][r] = d * (d * oc[3][r] - dz_s1);
+            v2[4][r] = d * (d * oc[4][r] - (dz_s0 + dx_s1[r]));
+        }
+    }
+    if (has_inj) {  // step-2 data injection on owned cells, through the l1 area (uniform per tile)
+#pragma unroll
+        for (int r = 0; r < R2; ++r)
+#pragma unroll
+            for (int f = 0; f < 5; ++f)
+                if (zown) l1[f][j2 + r + FAT][lane] = v2[f][r];
+        __syncthreads();
+        const T fa = S.filt[st2.it2 + 1], fb = S.filt[S.h + st2.it2 + 1];
+        for (int e = t.rec_b + threadIdx.x; e < t.rec_e; e += NT) {
+            const Entry en = ents[e];
+            const int lz = en.lcell & 255, lx = en.lcell >> 8;
+            if (lz < FAT || lz >= FAT + TZ2 || lx < FAT || lx >= FAT + TX) continue;
+            const int comp = en.crow >> 24, row = en.crow & 0xffffff;
+            const T* tr = S.rec_tr[comp] + long(row) * S.nts + st2.its;
+            const T s = tr[0] * fa + tr[1] * fb;
+            l1[comp][lx][lz] += damp_at(S, t.z0 - FAT + lz, t.x0 - FAT + lx) * s;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R2; ++r)
+#pragma unroll
+            for (int f = 0; f < 5; ++f)
+                if (zown) v2[f][r] = l1[f][j2 + r + FAT][lane];
+    }
+    // ---- u2 and the pair's adjoint fields: slot its gets F[a] u1 + F[a-1] u2 and slot its+1
+    // F[h+a] u1 + F[h+a-1] u2 (a = st1.it2)
+    const rsrc_t rsl = make_rsrc(S.ss[sspar], 5 * fbytes);
+    const rsrc_t rsr = make_rsrc(S.ss[sspar ^ 1], 5 * fbytes);
+    const T ea1 = S.filt[st1.it2], ea2 = S.filt[st2.it2];
+    const T eb1 = S.filt[S.h + st1.it2], eb2 = S.filt[S.h + st2.it2];
+    const bool first = st1.it2 == S.sub - 1;
+#pragma unroll
+    for (int r = 0; r < R2; ++r) {
+        const int ix = ix2 + r;
+        if (zown && ix < nx - FAT) {
+            const unsigned vo = (unsigned(ix) * unsigned(nz) + unsigned(iz)) * SZ;
+#pragma unroll
+            for (int f = 0; f < 5; ++f) {
+                const unsigned sf = unsigned(f) * fbytes;
+                bst(rw2, v2[f][r], vo, sf);
+                const T l = u1[f][r] * ea1 + v2[f][r] * ea2;
+                bst(rsl, first ? l : bld<T>(rsl, vo, sf) + l, vo, sf);
+                bst(rsr, bld<T>(rsr, vo, sf) + (u1[f][r] * eb1 + v2[f][r] * eb2), vo, sf);
+            }
+        }
+    }
+}
+
 }  // namespace eio

@@ -155,10 +155,13 @@ class ElasticFwi:
     src_interp, rec_interp: (method, nfilt) of the spatial interpolation of sources and
              receivers: ("linear", 1), ("sinc", n) or ("gauss", n) (spaceInterpGpu)
     time_block: (fine steps per launch, two-step kernel variant) of the FP32 forwards
+    adj_time_block: (reverse fine steps per launch, two-step adjoint variant) of the FP32
+             gradient's receiver pass
     """
 
     def __init__(self, grid, src_z, src_x, sources, rec_z, rec_x, device=0, dtype=torch.float32,
-                 segment=0, variant=None, src_interp=("linear", 1), rec_interp=("linear", 1), time_block=(2, 4)):
+                 segment=0, variant=None, src_interp=("linear", 1), rec_interp=("linear", 1), time_block=(2, 4),
+                 adj_time_block=(2, 4)):
         if not torch.cuda.is_available():
             if _capi._lib is not None:
                 raise RuntimeError("torch sees no GPU because libeio_hip.so was loaded before torch: "
@@ -177,6 +180,7 @@ class ElasticFwi:
         # a forward that keeps its whole coarse record runs two fine steps per launch
         # (time_block = (steps, two-step variant); (1, 0): one step per launch)
         self.ctx.set_time_block(*time_block)
+        self.ctx.set_adjoint_time_block(*adj_time_block)
         self.mid = None
         self.last_store_info = None  # (interval, checkpoints, device bytes) of the last store
         src_z = np.atleast_2d(np.asarray(src_z, np.float64))

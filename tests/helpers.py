@@ -71,6 +71,18 @@ def operator_case(tmp_path, w, nExp=1, zSource=4, xSource=None, spacing=None, nR
     return args, dict(sz=sz, sx=sx, rz=rz, rx=rx)
 
 
+def scattered_positions(rng, nz, nx, count):
+    """`count` distinct update-region positions (ix*nz+iz) on each of the 4 grids, spread over
+    the whole grid so that many fall on or next to tile edges."""
+    out = []
+    for _ in range(4):
+        iz = rng.integers(4, nz - 4, size=4 * count)
+        ix = rng.integers(4, nx - 4, size=4 * count)
+        p = np.unique(ix * nz + iz)[:count]
+        out.append(rng.permutation(p).astype(np.int32))
+    return out
+
+
 def smooth_field(rng, shape, sigma=4.0):
     from scipy.ndimage import gaussian_filter
     f = gaussian_filter(rng.standard_normal(shape), sigma)

@@ -12,7 +12,7 @@ import oracle as O
 from elastic_isotropic_operators_amd import _capi
 from elastic_isotropic_operators_amd import workloads as W
 
-from helpers import GRID_OF_COMP, oracle_grid, reg_positions, rel_l2, random_traces
+from helpers import GRID_OF_COMP, oracle_grid, reg_positions, rel_l2, random_traces, scattered_positions
 
 pytestmark = pytest.mark.gpu
 
@@ -251,18 +251,6 @@ def test_born_adjoint_with_reused_storage_of_another_shape():
     assert rel_l2(out, ref) < 1e-4, rel_l2(out, ref)
 
 
-def _scattered_positions(rng, nz, nx, count):
-    """`count` distinct update-region positions (ix*nz+iz) on each of the 4 grids, spread over
-    the whole grid so that many fall on or next to tile edges."""
-    out = []
-    for _ in range(4):
-        iz = rng.integers(4, nz - 4, size=4 * count)
-        ix = rng.integers(4, nx - 4, size=4 * count)
-        p = np.unique(ix * nz + iz)[:count]
-        out.append(rng.permutation(p).astype(np.int32))
-    return out
-
-
 @pytest.mark.parametrize("v", range(5))
 @pytest.mark.parametrize("sub_fd", [2, 3])
 def test_two_step_nl_forward_matches_oracle(v, sub_fd):
@@ -277,8 +265,8 @@ def test_two_step_nl_forward_matches_oracle(v, sub_fd):
                w["alphaCos"], w["oz"], w["ox"])
     med = O.media(g, w["elastic"], np.float64)
     rng = np.random.default_rng(61 + v)
-    spos = _scattered_positions(rng, g.nz, g.nx, 40)
-    rpos = _scattered_positions(rng, g.nz, g.nx, 60)
+    spos = scattered_positions(rng, g.nz, g.nx, 40)
+    rpos = scattered_positions(rng, g.nz, g.nx, 60)
     src = random_traces(rng, spos, g.nts, np.float64)
     ref = O.nl_fwd_reg(g, med, spos, src, rpos, np.float64)
     ctx = _capi.Context(0, np.float32)
@@ -288,6 +276,59 @@ def test_two_step_nl_forward_matches_oracle(v, sub_fd):
     for k in range(5):
         assert np.linalg.norm(ref[k]) > 0
         assert rel_l2(out[k], ref[k]) < 1e-5, (v, k, rel_l2(out[k], ref[k]))
+
+
+@pytest.mark.parametrize("va", range(7))
+@pytest.mark.parametrize("sub_fd", [2, 4, 3])
+def test_two_step_born_adjoint_matches_oracle(va, sub_fd):
+    """The Born adjoint with two reverse fine steps per launch of the receiver pass (four state
+    levels, step 1 recomputed on the tile halo, data injected on the halo in step 1 and on
+    owned cells in step 2, the pair's adjoint fields summed in registers) against the FP64
+    oracle and against one step per launch, with 300 receivers per grid scattered over a
+    grid of several tiles each way.  sub_fd 2: one pair per coarse interval, 4: two (slot its
+    read-modified-written by the second), 3: odd, one step per launch."""
+    if va >= len(_capi.Context.adjoint_time_block_variants()):
+        pytest.skip("variant not compiled")
+    w = W.small(nz=150, nx=130, nts=50, sub=1, seed=80)
+    g = O.grid(w["nz"], w["nx"], w["nts"], sub_fd, w["dz"], w["dx"], w["dts"], W.min_pad(w),
+               w["alphaCos"], w["oz"], w["ox"])
+    med = O.media(g, w["elastic"], np.float64)
+    rng = np.random.default_rng(81 + va)
+    spos = scattered_positions(rng, g.nz, g.nx, 20)
+    rpos = scattered_positions(rng, g.nz, g.nx, 300)
+    src_fine, _ = _born_inputs(w, g, spos, 82)
+    dat = random_traces(rng, rpos, g.nts, np.float64)
+    ref = O.born_adj_reg(g, med, spos, src_fine, rpos, dat, np.float64)
+    ctx = _capi.Context(0, np.float32)
+    mid = _model(ctx, w, g, med)
+    one = _gpu_born_adj(ctx, mid, spos, rpos, src_fine, dat, g.nx, g.nz, np.float32)
+    ctx.set_adjoint_time_block(2, va)
+    two = _gpu_born_adj(ctx, mid, spos, rpos, src_fine, dat, g.nx, g.nz, np.float32)
+    for k in range(5):
+        assert np.linalg.norm(ref[k]) > 0
+        assert rel_l2(two[k], ref[k]) < 1e-4, (va, k, rel_l2(two[k], ref[k]))
+        assert rel_l2(two[k], one[k]) < 1e-5, (va, k, rel_l2(two[k], one[k]))
+    # a second call reuses the four levels and the slots: same result
+    again = _gpu_born_adj(ctx, mid, spos, rpos, src_fine, dat, g.nx, g.nz, np.float32)
+    assert np.array_equal(again, two)
+
+
+def test_two_step_born_adjoint_dot_product():
+    """<B m, d> = <m, B^T d> with the two-step receiver pass (FP32 bar 1e-5)."""
+    w, g, med, spos, rpos = _case(nz=120, nx=100, nts=80, sub=1, seed=85)
+    src_fine, dm5 = _born_inputs(w, g, spos, 86)
+    rng = np.random.default_rng(87)
+    d = [rng.standard_normal((rpos[GRID_OF_COMP[k]].size, g.nts)) for k in range(5)]
+    ctx = _capi.Context(0, np.float32)
+    ctx.set_adjoint_time_block(2, 0)
+    mid = _model(ctx, w, g, med)
+    Bm = _gpu_born_fwd(ctx, mid, spos, rpos, src_fine, dm5, g.nts, np.float32)
+    Btd = _gpu_born_adj(ctx, mid, spos, rpos, src_fine, [x.astype(np.float32) for x in d], g.nx, g.nz,
+                        np.float32)
+    dm5c = dm5.astype(np.float32).astype(np.float64)
+    a = sum(np.sum(Bm[k].astype(np.float64) * d[k].astype(np.float32)) for k in range(5))
+    b = np.sum(dm5c * Btd.astype(np.float64))
+    assert abs(a - b) / abs(b) < DOT_TOL[np.float32], (a, b)
 
 
 def test_two_step_uneven_batch_matches_single_step():
@@ -397,7 +438,7 @@ def test_free_surface_nl_forward_adjoint_match_oracle(dtype, variant):
     the GPU adjoint passes the dot test."""
     w, g, med = _fs_case()
     rng = np.random.default_rng(100 + variant)
-    sp, rp = _scattered_positions(rng, g.nz, g.nx, 20), _scattered_positions(rng, g.nz, g.nx, 30)
+    sp, rp = scattered_positions(rng, g.nz, g.nx, 20), scattered_positions(rng, g.nz, g.nx, 30)
     src = random_traces(rng, sp, g.nts, np.float64)
     dat = random_traces(rng, rp, g.nts, np.float64)
     ref_d = O.nl_fwd_reg(g, med, sp, src, rp, np.float64)

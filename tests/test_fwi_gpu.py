@@ -12,7 +12,8 @@ import pytest
 import oracle as O
 from elastic_isotropic_operators_amd import _capi
 
-from helpers import GRID_OF_COMP, fwi_case, fwi_directions, oracle_grid, random_traces, reg_positions, rel_l2
+from helpers import (GRID_OF_COMP, fwi_case, fwi_directions, oracle_grid, random_traces, reg_positions, rel_l2,
+                     scattered_positions)
 
 pytestmark = pytest.mark.gpu
 
@@ -112,6 +113,50 @@ def test_two_step_forward_with_whole_record_matches_oracle(variant2):
     ctx.fwi_gradient(h, shots2, dm)
     ctx.fwi_release(h)
     assert rel_l2(dm, ref_g) < 1e-4, rel_l2(dm, ref_g)
+
+
+@pytest.mark.parametrize("va,sub,segment", [(0, 1, 0), (1, 2, 59), (2, 1, 59), (3, 2, 0), (1, 1, 0), (4, 1, 0),
+                                             (4, 2, 59), (5, 2, 0), (6, 1, 59)])
+def test_two_step_adjoint_gradient_matches_oracle(va, sub, segment):
+    """FP32 FWI gradient with two reverse fine steps per launch of the receiver pass, over the
+    checkpointed store (segment 0 -> 8: recomputed segments, one-step tiles) and the whole
+    record (59): two shots, residuals at 200 receivers per grid scattered over several tiles,
+    against the FP64 oracle's full-storage Born adjoint and against one step per launch.
+    sub = 1 -> sub_fd 2 (one pair per coarse interval), 2 -> sub_fd 4 (two pairs)."""
+    from elastic_isotropic_operators_amd import workloads as W
+    w = W.small(nz=130, nx=120, nts=60, sub=sub, seed=90)
+    g = oracle_grid(w)
+    med = O.media(g, w["elastic"], np.float64)
+    rng = np.random.default_rng(91 + va)
+    spos = [scattered_positions(rng, g.nz, g.nx, 3) for _ in range(2)]
+    rpos = scattered_positions(rng, g.nz, g.nx, 200)
+    srcs = [random_traces(rng, spos[e], g.nts, np.float64) for e in range(2)]
+    res = [random_traces(rng, rpos, g.nts, np.float64) for _ in range(2)]
+    ref_g = np.zeros((5, g.nx, g.nz))
+    for e in range(2):
+        src_fine = [O.time_interp(x, g.dts, g.sub) for x in srcs[e]]
+        ref_g += np.asarray(O.born_adj_reg(g, med, spos[e], src_fine, rpos, res[e], np.float64))
+    ctx = _capi.Context(0, np.float32)
+    mid = ctx.model_add(g.nz, g.nx, g.nts, g.sub, g.dz, g.dx, g.dts, g.minPad, med, g.alphaCos)
+    keep, shots, shots2 = [], [], []
+    for e in range(2):
+        s, kp = ctx.make_shot(mid, spos[e], rpos, [np.ascontiguousarray(x, np.float32) for x in srcs[e]],
+                              _zeros_like_traces(rpos, g.nts, np.float32))
+        shots.append(s)
+        keep.append(kp)
+        s, kp = ctx.make_shot(mid, spos[e], rpos, [np.ascontiguousarray(x, np.float32) for x in srcs[e]],
+                              [np.ascontiguousarray(x, np.float32) for x in res[e]])
+        shots2.append(s)
+        keep.append(kp)
+    h = ctx.fwi_forward(shots, segment)
+    one = np.zeros((5, g.nx, g.nz), np.float32)
+    ctx.fwi_gradient(h, shots2, one)
+    ctx.set_adjoint_time_block(2, va)
+    two = np.zeros((5, g.nx, g.nz), np.float32)
+    ctx.fwi_gradient(h, shots2, two)
+    ctx.fwi_release(h)
+    assert rel_l2(two, ref_g) < 1e-4, rel_l2(two, ref_g)
+    assert rel_l2(two, one) < 1e-5, rel_l2(two, one)
 
 
 def test_checkpointed_gradient_several_shots():
