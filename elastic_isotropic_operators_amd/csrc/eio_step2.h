@@ -23,7 +23,8 @@
 // same coarse interval (pairs start at even fine steps and the fine sub-sampling is even), so
 // their Vx, Vz contributions to slots its and its+1 (kernelsGpuElastic.cu:1337-1346) are
 // summed in registers and each slot is read-modified-written once per launch, with the
-// single-step kernel's store-first rule (dlo, dhi).
+// single-step kernel's store-first rule (dlo, dhi).  Only this tile writes the slots of its
+// owned cells, so their old values are loaded with the stage, not after step 2.
 #pragma once
 #include "eio_kernels.h"
 
@@ -109,6 +110,33 @@ void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
         for (int f = 0; f < 5; ++f) o[f][r] = bld<T>(rold, vo, unsigned(f) * fbytes);
 #pragma unroll
         for (int f = 0; f < 5; ++f) m[f][r] = bld<T>(rmed, vo, unsigned(f) * fbytes);
+    }
+    // decimation slots its (x0, z0) and its+1 (x1, z1) of the owned cells, with the stage
+    const int j2 = w * R2;
+    const int ix2 = t.x0 + j2;
+    rsrc_t rwx0, rwx1, rwz0, rwz1;
+    T f1a = T(0), f1b = T(0), f2a = T(0), f2b = T(0);
+    T px0[R2], pz0[R2], px1[R2], pz1[R2];
+    if constexpr (DEC) {
+        const size_t s0 = size_t(st1.its % S.wcap) * n, s1 = size_t((st1.its + 1) % S.wcap) * n;
+        rwx0 = make_rsrc(S.Wvx + s0, fbytes);
+        rwx1 = make_rsrc(S.Wvx + s1, fbytes);
+        rwz0 = make_rsrc(S.Wvz + s0, fbytes);
+        rwz1 = make_rsrc(S.Wvz + s1, fbytes);
+        f1a = S.filt[st1.it2];
+        f1b = S.filt[S.h + st1.it2];
+        f2a = S.filt[st2.it2];
+        f2b = S.filt[S.h + st2.it2];
+        const bool sta = st1.it2 == 1 && st1.its == dlo;
+        const bool stb = st1.it2 == 1 && st1.its + 1 != dhi;
+#pragma unroll
+        for (int r = 0; r < R2; ++r) {
+            const unsigned vo = (unsigned(ix2 + r) * unsigned(nz) + unsigned(izl)) * SZ;
+            px0[r] = sta ? T(0) : bld<T>(rwx0, vo, 0);
+            pz0[r] = sta ? T(0) : bld<T>(rwz0, vo, 0);
+            px1[r] = stb ? T(0) : bld<T>(rwx1, vo, 0);
+            pz1[r] = stb ? T(0) : bld<T>(rwz1, vo, 0);
+        }
     }
 #pragma unroll
     for (int it = 0; it < NFILL; ++it) {
@@ -210,8 +238,6 @@ void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
     if (!two) return;
 
     // ---- step 2 on owned columns j2 .. j2+R2-1 (rel x0) = l1 columns j2+4 ..
-    const int j2 = w * R2;
-    const int ix2 = t.x0 + j2;
     const int lz2 = lane < FAT ? FAT : (lane > FAT + TZ2 - 1 ? FAT + TZ2 - 1 : lane);  // in-range LDS rows
     T v2[5][R2];
     T u1x[R2], u1z[R2];  // u1's Vx, Vz on the owned cells (decimation)
@@ -312,22 +338,6 @@ void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
             }
         }
     }
-    rsrc_t rwx0, rwx1, rwz0, rwz1;
-    T f1a = T(0), f1b = T(0), f2a = T(0), f2b = T(0);
-    bool sta = false, stb = false;
-    if constexpr (DEC) {
-        const size_t s0 = size_t(st1.its % S.wcap) * n, s1 = size_t((st1.its + 1) % S.wcap) * n;
-        rwx0 = make_rsrc(S.Wvx + s0, fbytes);
-        rwx1 = make_rsrc(S.Wvx + s1, fbytes);
-        rwz0 = make_rsrc(S.Wvz + s0, fbytes);
-        rwz1 = make_rsrc(S.Wvz + s1, fbytes);
-        f1a = S.filt[st1.it2];
-        f1b = S.filt[S.h + st1.it2];
-        f2a = S.filt[st2.it2];
-        f2b = S.filt[S.h + st2.it2];
-        sta = st1.it2 == 1 && st1.its == dlo;
-        stb = st1.it2 == 1 && st1.its + 1 != dhi;
-    }
 #pragma unroll
     for (int r = 0; r < R2; ++r) {
         const int ix = ix2 + r;
@@ -336,16 +346,10 @@ void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
 #pragma unroll
             for (int f = 0; f < 5; ++f) bst_pol<T>(rw2, v2[f][r], vo, unsigned(f) * fbytes, NTS);
             if constexpr (DEC) {
-                T xa = u1x[r] * f1a + v2[0][r] * f2a, za = u1z[r] * f1a + v2[1][r] * f2a;
-                T xb = u1x[r] * f1b + v2[0][r] * f2b, zb = u1z[r] * f1b + v2[1][r] * f2b;
-                if (!sta) {
-                    xa = bld<T>(rwx0, vo, 0) + xa;
-                    za = bld<T>(rwz0, vo, 0) + za;
-                }
-                if (!stb) {
-                    xb = bld<T>(rwx1, vo, 0) + xb;
-                    zb = bld<T>(rwz1, vo, 0) + zb;
-                }
+                const T xa = px0[r] + (u1x[r] * f1a + v2[0][r] * f2a);
+                const T za = pz0[r] + (u1z[r] * f1a + v2[1][r] * f2a);
+                const T xb = px1[r] + (u1x[r] * f1b + v2[0][r] * f2b);
+                const T zb = pz1[r] + (u1z[r] * f1b + v2[1][r] * f2b);
                 bst(rwx0, xa, vo, 0u);
                 bst(rwz0, za, vo, 0u);
                 bst(rwx1, xb, vo, 0u);
@@ -368,8 +372,9 @@ void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
 // fine sub-sampling of the Born operator is even), so their adjoint-field contributions
 // (kernelsGpuElastic.cu:1369-1376) are summed in registers: slot its (ss[sspar]) is written
 // by the interval's first pair and read-modified-written by later ones, slot its+1
-// (ss[sspar^1]) read-modified-written once per pair.  Levels A,B -> C,D and back (par), as
-// for the forward.  LATE: the older level is loaded after the stage is written, the step-1
+// (ss[sspar^1]) read-modified-written once per pair; only this tile writes the slots of its
+// owned cells, so their old values are loaded with the stage.  Levels A,B -> C,D and back
+// (par), as for the forward.  LATE: the older level is loaded after the stage is written, the step-1
 // media after step 1 and u_cur of the owned cells after M u1 is formed (each set of values
 // is live only in the phase that uses it: fewer registers, more latency to hide).
 template <typename T, int R2, int WX, int OCC, bool LATE = false>
@@ -419,6 +424,22 @@ void k_step2_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
     // the stage is written)
     constexpr int NFILL = (TXC * ZC + NT - 1) / NT;
     T o[5][R1], m1[5][R1], oc[5][R2];
+    // adjoint-field slots its (p0; not read by the interval's first pair) and its+1 (p1)
+    const rsrc_t rsl = make_rsrc(S.ss[sspar], 5 * fbytes);
+    const rsrc_t rsr = make_rsrc(S.ss[sspar ^ 1], 5 * fbytes);
+    const bool first = st1.it2 == S.sub - 1;
+    T p0[5][R2], p1[5][R2];
+    auto load_slots = [&]() {
+#pragma unroll
+        for (int r = 0; r < R2; ++r) {
+            const unsigned vo = (unsigned(ix2 + r) * unsigned(nz) + unsigned(izl)) * SZ;
+#pragma unroll
+            for (int f = 0; f < 5; ++f) {
+                p0[f][r] = first ? T(0) : bld<T>(rsl, vo, unsigned(f) * fbytes);
+                p1[f][r] = bld<T>(rsr, vo, unsigned(f) * fbytes);
+            }
+        }
+    };
     auto load_old = [&]() {
 #pragma unroll
         for (int r = 0; r < R1; ++r) {
@@ -466,6 +487,7 @@ void k_step2_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
             load_old();
             load_med();
             load_cur();
+            load_slots();
         }
 #pragma unroll
         for (int it = 0; it < NFILL; ++it) {
@@ -480,7 +502,10 @@ void k_step2_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
             }
         }
     }
-    if constexpr (LATE) load_old();
+    if constexpr (LATE) {
+        load_old();
+        load_slots();
+    }
     __syncthreads();
 
     const T* cz = S.cz;
@@ -639,11 +664,8 @@ void k_step2_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
     }
     // ---- u2 and the pair's adjoint fields: slot its gets F[a] u1 + F[a-1] u2 and slot its+1
     // F[h+a] u1 + F[h+a-1] u2 (a = st1.it2)
-    const rsrc_t rsl = make_rsrc(S.ss[sspar], 5 * fbytes);
-    const rsrc_t rsr = make_rsrc(S.ss[sspar ^ 1], 5 * fbytes);
     const T ea1 = S.filt[st1.it2], ea2 = S.filt[st2.it2];
     const T eb1 = S.filt[S.h + st1.it2], eb2 = S.filt[S.h + st2.it2];
-    const bool first = st1.it2 == S.sub - 1;
 #pragma unroll
     for (int r = 0; r < R2; ++r) {
         const int ix = ix2 + r;
@@ -653,9 +675,8 @@ void k_step2_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
             for (int f = 0; f < 5; ++f) {
                 const unsigned sf = unsigned(f) * fbytes;
                 bst(rw2, v2[f][r], vo, sf);
-                const T l = u1[f][r] * ea1 + v2[f][r] * ea2;
-                bst(rsl, first ? l : bld<T>(rsl, vo, sf) + l, vo, sf);
-                bst(rsr, bld<T>(rsr, vo, sf) + (u1[f][r] * eb1 + v2[f][r] * eb2), vo, sf);
+                bst(rsl, p0[f][r] + (u1[f][r] * ea1 + v2[f][r] * ea2), vo, sf);
+                bst(rsr, p1[f][r] + (u1[f][r] * eb1 + v2[f][r] * eb2), vo, sf);
             }
         }
     }
