@@ -34,6 +34,20 @@ constexpr int TZ2 = 56;       // owned z rows of a two-step tile
 constexpr int Z1 = 64;        // rows of u1 (= lanes)
 constexpr int ZC = 72;        // rows of u_cur staged in LDS
 
+// Four consecutive floats from a buffer (a 4-byte-aligned offset is enough): the LLVM
+// intrinsic itself, since this toolchain's __builtin_amdgcn_raw_buffer_load_b128 is lowered
+// to a 32-bit load.
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+__device__ f32x4_t raw_buffer_load_f32x4(rsrc_t r, int voff, int soff, int aux)
+    __asm("llvm.amdgcn.raw.ptr.buffer.load.v4f32");
+__device__ __forceinline__ void bld4(rsrc_t r, unsigned voff, unsigned soff, float v[4]) {
+    const f32x4_t q = raw_buffer_load_f32x4(r, int(voff), int(soff), 0);
+    v[0] = q.x;
+    v[1] = q.y;
+    v[2] = q.z;
+    v[3] = q.w;
+}
+
 template <typename T>
 __device__ __forceinline__ void bst_pol(rsrc_t r, T v, unsigned voff, unsigned soff, bool nt) {
     if (nt) bst_nt(r, v, voff, soff);
@@ -46,13 +60,14 @@ __global__ __launch_bounds__(64 * WX) __attribute__((amdgpu_waves_per_eu(OCC)))
 void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict__ tiles,
                  const Entry* __restrict__ ents, int ntiles, int k, int par, int dlo, int dhi) {
     static_assert(8 % WX == 0, "WX must divide 8");
+    static_assert(sizeof(T) == 4, "the two-step forward is FP32");
     constexpr int NT = 64 * WX;
     constexpr int TX = WX * R2;        // owned columns
     constexpr int R1 = R2 + 8 / WX;    // step-1 columns per wave
     constexpr int TX1 = TX + 8;        // step-1 columns
     constexpr int TXC = TX + 16;       // staged u_cur columns
     constexpr unsigned SZ = sizeof(T);
-    __shared__ T lc[5][TXC][ZC];
+    __shared__ __attribute__((aligned(16))) T lc[5][TXC][ZC];
     __shared__ T l1[5][TX1][Z1];
 
     const TileDev t = tiles[xcd_remap(blockIdx.x, ntiles)];
@@ -83,23 +98,26 @@ void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
     // that are masked, so their loads are pointed at row FAT instead.
     const int izl = iz < 0 ? 0 : iz;
 
-    // ---- u_cur on (TX+16) x 72 cells from (x0-8, z0-8) (cells off the grid are 0): all
-    // loads are issued first, then the older level and media of the step-1 cells, and only
-    // then are the staged values written to LDS, so every load of the tile is in flight at once
-    constexpr int NFILL = (TXC * ZC + NT - 1) / NT;
-    T hf[NFILL][5];
+    // ---- u_cur on (TX+16) x 72 cells from (x0-8, z0-8) (cells off the grid are 0), four rows
+    // per 16-byte load: all loads are issued first, then the older level and media of the
+    // step-1 cells, and only then are the staged values written to LDS, so every load of the
+    // tile is in flight at once.  Row quads start at z0-8+4q = 56 bz-4+4q, a multiple of 4:
+    // a quad is wholly above row 0 or wholly below it, and only the bottom edge cuts one.
+    constexpr int ZQ = ZC / 4;
+    constexpr int NFILL = (TXC * ZQ + NT - 1) / NT;
+    T hf[NFILL][5][4];
+    unsigned hm[NFILL];  // bit k: row k of the quad is on the grid
 #pragma unroll
     for (int it = 0; it < NFILL; ++it) {
         const int e = threadIdx.x + it * NT;
-        const int c = e / ZC, z = e - c * ZC;
-        const int gx = t.x0 - 2 * FAT + c, gz = t.z0 - 2 * FAT + z;
-        const bool in = e < TXC * ZC && gx >= 0 && gx < nx && gz >= 0 && gz < nz;
+        const int c = e / ZQ, q = e - c * ZQ;
+        const int gx = t.x0 - 2 * FAT + c, gz = t.z0 - 2 * FAT + 4 * q;
+        const bool in = e < TXC * ZQ && gx >= 0 && gx < nx && gz >= 0 && gz < nz;
+        const int nin = in ? (nz - gz < 4 ? nz - gz : 4) : 0;
+        hm[it] = (1u << nin) - 1u;
         const unsigned hv = in ? (unsigned(gx) * unsigned(nz) + unsigned(gz)) * SZ : 0u;
 #pragma unroll
-        for (int f = 0; f < 5; ++f) {
-            const T v = bld<T>(rcur, hv, unsigned(f) * fbytes);
-            hf[it][f] = in ? v : T(0);
-        }
+        for (int f = 0; f < 5; ++f) bld4(rcur, hv, unsigned(f) * fbytes, hf[it][f]);
     }
     T o[5][R1], m[5][R1];
 #pragma unroll
@@ -141,10 +159,17 @@ void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
 #pragma unroll
     for (int it = 0; it < NFILL; ++it) {
         const int e = threadIdx.x + it * NT;
-        if (e < TXC * ZC) {
-            const int c = e / ZC, z = e - c * ZC;
+        if (e < TXC * ZQ) {
+            const int c = e / ZQ, q = e - c * ZQ;
 #pragma unroll
-            for (int f = 0; f < 5; ++f) lc[f][c][z] = hf[it][f];
+            for (int f = 0; f < 5; ++f) {
+                float4 v;
+                v.x = (hm[it] & 1u) ? hf[it][f][0] : 0.0f;
+                v.y = (hm[it] & 2u) ? hf[it][f][1] : 0.0f;
+                v.z = (hm[it] & 4u) ? hf[it][f][2] : 0.0f;
+                v.w = (hm[it] & 8u) ? hf[it][f][3] : 0.0f;
+                *reinterpret_cast<float4*>(&lc[f][c][4 * q]) = v;
+            }
         }
     }
     __syncthreads();
