@@ -407,13 +407,14 @@ __global__ __launch_bounds__(64 * WX) __attribute__((amdgpu_waves_per_eu(OCC)))
 void k_step2_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict__ tiles,
                  const Entry* __restrict__ ents, int ntiles, int k, int par, int sspar) {
     static_assert(8 % WX == 0, "WX must divide 8");
+    static_assert(sizeof(T) == 4, "the two-step adjoint is FP32");
     constexpr int NT = 64 * WX;
     constexpr int TX = WX * R2;        // owned columns
     constexpr int R1 = R2 + 8 / WX;    // step-1 columns per wave
     constexpr int TX1 = TX + 8;        // step-1 columns
     constexpr int TXC = TX + 16;       // staged columns
     constexpr unsigned SZ = sizeof(T);
-    __shared__ T lc[5][TXC][ZC];       // M u_cur, then M u1 on the step-1 region
+    __shared__ __attribute__((aligned(16))) T lc[5][TXC][ZC];  // M u_cur, then M u1 on the step-1 region
     __shared__ T l1[5][TX1][Z1];       // u1, then u2 for the owned data injection
 
     const TileDev t = tiles[xcd_remap(blockIdx.x, ntiles)];
@@ -447,7 +448,6 @@ void k_step2_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
     // ---- stage M u_cur (cells off the grid are 0); the older level and media of the step-1
     // cells and u_cur of the owned cells are loaded behind the stage loads (or, LATE, after
     // the stage is written)
-    constexpr int NFILL = (TXC * ZC + NT - 1) / NT;
     T o[5][R1], m1[5][R1], oc[5][R2];
     // adjoint-field slots its (p0; not read by the interval's first pair) and its+1 (p1)
     const rsrc_t rsl = make_rsrc(S.ss[sspar], 5 * fbytes);
@@ -492,21 +492,25 @@ void k_step2_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
         }
     };
     {
-        T hv_[NFILL][5], hm[NFILL][5];
+        // u_cur and media four rows per 16-byte load (row quads start at z0-8+4q, a multiple
+        // of 4: wholly above row 0 or not, and only the bottom edge cuts one)
+        constexpr int ZQ = ZC / 4;
+        constexpr int NQ = (TXC * ZQ + NT - 1) / NT;
+        T hv_[NQ][5][4], hm[NQ][5][4];
+        unsigned hk[NQ];  // bit k: row k of the quad is on the grid
 #pragma unroll
-        for (int it = 0; it < NFILL; ++it) {
+        for (int it = 0; it < NQ; ++it) {
             const int e = threadIdx.x + it * NT;
-            const int c = e / ZC, z = e - c * ZC;
-            const int gx = t.x0 - 2 * FAT + c, gz = t.z0 - 2 * FAT + z;
-            const bool in = e < TXC * ZC && gx >= 0 && gx < nx && gz >= 0 && gz < nz;
+            const int c = e / ZQ, q = e - c * ZQ;
+            const int gx = t.x0 - 2 * FAT + c, gz = t.z0 - 2 * FAT + 4 * q;
+            const bool in = e < TXC * ZQ && gx >= 0 && gx < nx && gz >= 0 && gz < nz;
+            const int nin = in ? (nz - gz < 4 ? nz - gz : 4) : 0;
+            hk[it] = (1u << nin) - 1u;
             const unsigned hv = in ? (unsigned(gx) * unsigned(nz) + unsigned(gz)) * SZ : 0u;
 #pragma unroll
-            for (int f = 0; f < 5; ++f) {
-                const T v = bld<T>(rcur, hv, unsigned(f) * fbytes);
-                hv_[it][f] = in ? v : T(0);
-            }
+            for (int f = 0; f < 5; ++f) bld4(rcur, hv, unsigned(f) * fbytes, hv_[it][f]);
 #pragma unroll
-            for (int f = 0; f < 5; ++f) hm[it][f] = bld<T>(rmed, hv, unsigned(f) * fbytes);
+            for (int f = 0; f < 5; ++f) bld4(rmed, hv, unsigned(f) * fbytes, hm[it][f]);
         }
         if constexpr (!LATE) {
             load_old();
@@ -515,15 +519,25 @@ void k_step2_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
             load_slots();
         }
 #pragma unroll
-        for (int it = 0; it < NFILL; ++it) {
+        for (int it = 0; it < NQ; ++it) {
             const int e = threadIdx.x + it * NT;
-            if (e < TXC * ZC) {
-                const int c = e / ZC, z = e - c * ZC;
-                lc[0][c][z] = hm[it][0] * hv_[it][0];
-                lc[1][c][z] = hm[it][1] * hv_[it][1];
-                lc[2][c][z] = hm[it][2] * hv_[it][2] + hm[it][3] * hv_[it][3];
-                lc[3][c][z] = hm[it][3] * hv_[it][2] + hm[it][2] * hv_[it][3];
-                lc[4][c][z] = hm[it][4] * hv_[it][4];
+            if (e < TXC * ZQ) {
+                const int c = e / ZQ, q = e - c * ZQ;
+                f32x4_t p[5];
+#pragma unroll
+                for (int k2 = 0; k2 < 4; ++k2) {
+                    const bool in = (hk[it] >> k2) & 1u;
+                    const T u0 = in ? hv_[it][0][k2] : T(0), u1_ = in ? hv_[it][1][k2] : T(0);
+                    const T u2 = in ? hv_[it][2][k2] : T(0), u3 = in ? hv_[it][3][k2] : T(0);
+                    const T u4 = in ? hv_[it][4][k2] : T(0);
+                    p[0][k2] = hm[it][0][k2] * u0;
+                    p[1][k2] = hm[it][1][k2] * u1_;
+                    p[2][k2] = hm[it][2][k2] * u2 + hm[it][3][k2] * u3;
+                    p[3][k2] = hm[it][3][k2] * u2 + hm[it][2][k2] * u3;
+                    p[4][k2] = hm[it][4][k2] * u4;
+                }
+#pragma unroll
+                for (int f = 0; f < 5; ++f) *reinterpret_cast<f32x4_t*>(&lc[f][c][4 * q]) = p[f];
             }
         }
     }
