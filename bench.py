@@ -232,7 +232,7 @@ def main():
     ap.add_argument("--fwi-steps", type=int, default=1)
     ap.add_argument("--fwi-warmup", type=int, default=1)
     ap.add_argument("--fwi-scale", type=float, default=1.0)
-    ap.add_argument("--fwi-adj", default="2:5",
+    ap.add_argument("--fwi-adj", default="2:7",
                     help="reverse fine steps per launch of the gradient's receiver pass and the two-step "
                          "adjoint variant, as steps:variant")
     ap.add_argument("--fwi-child", action="store_true", help=argparse.SUPPRESS)

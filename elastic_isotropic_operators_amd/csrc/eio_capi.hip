@@ -71,10 +71,11 @@ constexpr Variant2 F32_VARIANTS2[] = {{2, 8, 2, 0}, {2, 8, 4, 0}, {4, 4, 2, 0}, 
 constexpr int N_F32_VARIANTS2 = int(sizeof(F32_VARIANTS2) / sizeof(F32_VARIANTS2[0]));
 // Two-step adjoint (Born receiver pass, FWI gradient) variants: R2, WX, OCC as above.
 struct Variant2A {
-    int R2, WX, OCC, LATE;  // LATE: per-cell loads after the stage is written
+    int R2, WX, OCC, LATE, PS;  // LATE: per-cell loads after the stage is written; PS: slots with the stage
 };
-constexpr Variant2A F32_VARIANTS2A[] = {{2, 8, 2, 0}, {4, 8, 1, 0}, {2, 8, 4, 0}, {4, 4, 2, 0},
-                                        {2, 8, 4, 1}, {2, 8, 2, 1}, {4, 8, 1, 1}};
+constexpr Variant2A F32_VARIANTS2A[] = {{2, 8, 2, 0, 1}, {4, 8, 1, 0, 1}, {2, 8, 4, 0, 1}, {4, 4, 2, 0, 1},
+                                        {2, 8, 4, 1, 1}, {2, 8, 2, 1, 1}, {4, 8, 1, 1, 1}, {4, 8, 1, 1, 0},
+                                        {4, 8, 1, 0, 0}, {2, 8, 2, 1, 0}};
 constexpr int N_F32_VARIANTS2A = int(sizeof(F32_VARIANTS2A) / sizeof(F32_VARIANTS2A[0]));
 
 template <typename T>
@@ -637,17 +638,20 @@ inline void launch_adj2(Run<float>& run, int k, int sspar) {
     hipStream_t s = run.c->stream;
     const int ntile = int(run.plan2.tiles.size());
     const int par = (k >> 1) & 1;
-#define EIO_L2A(R_, WX_, OCC_, LATE_)                                                                   \
-    hipLaunchKernelGGL((eio::k_step2_adj<float, R_, WX_, OCC_, LATE_>), dim3(ntile), dim3(64 * WX_), 0, s, \
-                       run.d_shots, run.d_tiles2, run.d_ents2, ntile, k, par, sspar)
+#define EIO_L2A(R_, WX_, OCC_, LATE_, PS_)                                                              \
+    hipLaunchKernelGGL((eio::k_step2_adj<float, R_, WX_, OCC_, LATE_, PS_>), dim3(ntile), dim3(64 * WX_), 0, \
+                       s, run.d_shots, run.d_tiles2, run.d_ents2, ntile, k, par, sspar)
     switch (run.c->variant2a) {
-        case 0: EIO_L2A(2, 8, 2, false); break;
-        case 1: EIO_L2A(4, 8, 1, false); break;
-        case 2: EIO_L2A(2, 8, 4, false); break;
-        case 3: EIO_L2A(4, 4, 2, false); break;
-        case 4: EIO_L2A(2, 8, 4, true); break;
-        case 5: EIO_L2A(2, 8, 2, true); break;
-        default: EIO_L2A(4, 8, 1, true); break;
+        case 0: EIO_L2A(2, 8, 2, false, true); break;
+        case 1: EIO_L2A(4, 8, 1, false, true); break;
+        case 2: EIO_L2A(2, 8, 4, false, true); break;
+        case 3: EIO_L2A(4, 4, 2, false, true); break;
+        case 4: EIO_L2A(2, 8, 4, true, true); break;
+        case 5: EIO_L2A(2, 8, 2, true, true); break;
+        case 6: EIO_L2A(4, 8, 1, true, true); break;
+        case 7: EIO_L2A(4, 8, 1, true, false); break;
+        case 8: EIO_L2A(4, 8, 1, false, false); break;
+        default: EIO_L2A(2, 8, 2, true, false); break;
     }
 #undef EIO_L2A
 }

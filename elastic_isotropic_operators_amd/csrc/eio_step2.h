@@ -402,7 +402,8 @@ void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
 // (par), as for the forward.  LATE: the older level is loaded after the stage is written, the step-1
 // media after step 1 and u_cur of the owned cells after M u1 is formed (each set of values
 // is live only in the phase that uses it: fewer registers, more latency to hide).
-template <typename T, int R2, int WX, int OCC, bool LATE = false>
+// PS: load the slots with the stage (else after step 2: fewer registers for wide tiles).
+template <typename T, int R2, int WX, int OCC, bool LATE = false, bool PS = true>
 __global__ __launch_bounds__(64 * WX) __attribute__((amdgpu_waves_per_eu(OCC)))
 void k_step2_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict__ tiles,
                  const Entry* __restrict__ ents, int ntiles, int k, int par, int sspar) {
@@ -516,7 +517,7 @@ void k_step2_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
             load_old();
             load_med();
             load_cur();
-            load_slots();
+            if constexpr (PS) load_slots();
         }
 #pragma unroll
         for (int it = 0; it < NQ; ++it) {
@@ -543,7 +544,7 @@ void k_step2_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
     }
     if constexpr (LATE) {
         load_old();
-        load_slots();
+        if constexpr (PS) load_slots();
     }
     __syncthreads();
 
@@ -705,6 +706,7 @@ void k_step2_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
     // F[h+a] u1 + F[h+a-1] u2 (a = st1.it2)
     const T ea1 = S.filt[st1.it2], ea2 = S.filt[st2.it2];
     const T eb1 = S.filt[S.h + st1.it2], eb2 = S.filt[S.h + st2.it2];
+    if constexpr (!PS) load_slots();
 #pragma unroll
     for (int r = 0; r < R2; ++r) {
         const int ix = ix2 + r;

@@ -437,10 +437,10 @@ struct Device {
         check_eio(eio_ctx_create(device, prec, &ctx));
         if (prec == 4) {
             // FP32: two fine steps per launch for the NL forward (the 56 x 32 tile) and the
-            // Born receiver pass (56 x 16); passes these kernels do not cover (wavefield saving,
+            // Born receiver pass (56 x 32); passes these kernels do not cover (wavefield saving,
             // free surface, odd fine sub-sampling) fall back to one step per launch by themselves
             check_eio(eio_ctx_set_time_block(ctx, 2, 4));
-            check_eio(eio_ctx_set_adjoint_time_block(ctx, 2, 5));
+            check_eio(eio_ctx_set_adjoint_time_block(ctx, 2, 7));
         }
     }
     ~Device() {

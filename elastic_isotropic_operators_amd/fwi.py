@@ -161,7 +161,7 @@ class ElasticFwi:
 
     def __init__(self, grid, src_z, src_x, sources, rec_z, rec_x, device=0, dtype=torch.float32,
                  segment=0, variant=None, src_interp=("linear", 1), rec_interp=("linear", 1), time_block=(2, 4),
-                 adj_time_block=(2, 5)):
+                 adj_time_block=(2, 7)):
         if not torch.cuda.is_available():
             if _capi._lib is not None:
                 raise RuntimeError("torch sees no GPU because libeio_hip.so was loaded before torch: "

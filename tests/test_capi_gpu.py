@@ -278,7 +278,7 @@ def test_two_step_nl_forward_matches_oracle(v, sub_fd):
         assert rel_l2(out[k], ref[k]) < 1e-5, (v, k, rel_l2(out[k], ref[k]))
 
 
-@pytest.mark.parametrize("va", range(7))
+@pytest.mark.parametrize("va", range(10))
 @pytest.mark.parametrize("sub_fd", [2, 4, 3])
 def test_two_step_born_adjoint_matches_oracle(va, sub_fd):
     """The Born adjoint with two reverse fine steps per launch of the receiver pass (four state
