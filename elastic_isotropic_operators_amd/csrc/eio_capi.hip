@@ -327,7 +327,8 @@ void build_entries2(const ModelBase& m, const int* const pos[4], const int nreg[
 
 // Tile plans: one-step kernels (64 x TX tiles), the two-step forward (56 x TX tiles, source
 // entries on the step-1 region, receivers owned-only) and the two-step adjoint (56 x TX
-// tiles, receiver entries on the step-1 region: they are injected in both steps).
+// tiles, receiver entries on the step-1 region: they are injected in both steps; source
+// entries owned-only: the NL adjoint reads them in both steps).
 enum class PlanKind { ONE, FWD2, ADJ2 };
 
 template <typename T>
@@ -353,6 +354,7 @@ Plan make_plan(Ctx* c, int nshot, const eio_shot* shots, Op op, PlanKind kind = 
             build_entries2(m, s.src_pos, s.n_src, TX, true, src_t, nbz, nbx, "source");
             build_entries2(m, s.rec_pos, s.n_rec, TX, false, rec_t, nbz, nbx, "receiver");
         } else if (kind == PlanKind::ADJ2) {
+            build_entries2(m, s.src_pos, s.n_src, TX, false, src_t, nbz, nbx, "source");
             build_entries2(m, s.rec_pos, s.n_rec, TX, true, rec_t, nbz, nbx, "receiver");
         } else {
             build_entries<T>(m, s.src_pos, s.n_src, TX, src_t, nbz, "source");
@@ -632,14 +634,15 @@ void launch_fwd2(Run<float>& run, int ntile, int k, int par, int dlo = 0, int dh
 #undef EIO_L2
 }
 
-// Reverse fine steps k, k+1 (k even) of the Born adjoint pass; adjoint-field slots as in
-// launch_adj (sspar: slot of coarse step its).
-inline void launch_adj2(Run<float>& run, int k, int sspar) {
+// Reverse fine steps k, k+1 (k even) of the Born receiver pass (adjoint-field slots as in
+// launch_adj, sspar: slot of coarse step its) or of the NL adjoint.
+template <int MODE = eio::M_BADJ>
+void launch_adj2(Run<float>& run, int k, int sspar) {
     hipStream_t s = run.c->stream;
     const int ntile = int(run.plan2.tiles.size());
     const int par = (k >> 1) & 1;
 #define EIO_L2A(R_, WX_, OCC_, LATE_, PS_)                                                              \
-    hipLaunchKernelGGL((eio::k_step2_adj<float, R_, WX_, OCC_, LATE_, PS_>), dim3(ntile), dim3(64 * WX_), 0, \
+    hipLaunchKernelGGL((eio::k_step2_adj<float, R_, WX_, OCC_, LATE_, PS_, MODE>), dim3(ntile), dim3(64 * WX_), 0, \
                        s, run.d_shots, run.d_tiles2, run.d_ents2, ntile, k, par, sspar)
     switch (run.c->variant2a) {
         case 0: EIO_L2A(2, 8, 2, false, true); break;
@@ -982,7 +985,14 @@ void set_abc(Ctx* c, int model, int on, const float* w, const float* scale) {
 
 template <typename T>
 void nl_adjoint(Ctx* c, int nshot, const eio_shot* shots, int flags, int wshot = -1, void* wfld = nullptr) {
-    Run<T> run = prepare<T>(c, nshot, shots, Op::NL_ADJ, flags, nullptr);
+    if (nshot <= 0 || !shots) fail("no shots");
+    // FP32 without free surface or saved wavefield: two reverse fine steps per launch
+    bool any_fs = false;
+    for (int i = 0; i < nshot; ++i) any_fs = any_fs || get_model(c, shots[i].model).g.surface;
+    int kmax0 = get_model(c, shots[0].model).ntw - 1;
+    if (c->step_limit > 0) kmax0 = std::min(kmax0, c->step_limit);
+    const bool two = sizeof(T) == 4 && c->adj_time_block == 2 && !wfld && !any_fs && kmax0 % 2 == 0;
+    Run<T> run = prepare<T>(c, nshot, shots, Op::NL_ADJ, flags, nullptr, nullptr, 0, false, two);
     if (wfld) attach_wavefield(run, wshot, wfld, flags);
     // The adjoint runs backwards from each shot's own last sample: shots of different
     // lengths are aligned at their end, so a shot joins the batch when its remaining step
@@ -995,6 +1005,12 @@ void nl_adjoint(Ctx* c, int nshot, const eio_shot* shots, int flags, int wshot =
     timing_begin<T>(c);
     EIO_CHECK_HIP(hipEventRecord(c->ev[2], c->stream));
     run_loop(c, [&]() {
+        if constexpr (sizeof(T) == 4) {
+            if (two) {
+                for (int k = 0; k < kmax; k += 2) launch_adj2<eio::M_NLADJ>(run, k, 0);
+                return;
+            }
+        }
         for (int k = 0; k < kmax; ++k) launch_adj<T, eio::M_NLADJ>(run, ntile, k, 0);
     });
     EIO_CHECK_HIP(hipEventRecord(c->ev[3], c->stream));

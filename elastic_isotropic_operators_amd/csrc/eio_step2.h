@@ -403,7 +403,10 @@ void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
 // media after step 1 and u_cur of the owned cells after M u1 is formed (each set of values
 // is live only in the phase that uses it: fewer registers, more latency to hide).
 // PS: load the slots with the stage (else after step 2: fewer registers for wide tiles).
-template <typename T, int R2, int WX, int OCC, bool LATE = false, bool PS = true>
+// MODE M_NLADJ (the NL adjoint, kernelsGpuElastic.cu:100-116): no adjoint-field slots; the
+// source traces are extracted from u1 and u2 at the tile's owned source cells (its source
+// entries are owned-only: each source cell is read once per step).
+template <typename T, int R2, int WX, int OCC, bool LATE = false, bool PS = true, int MODE = M_BADJ>
 __global__ __launch_bounds__(64 * WX) __attribute__((amdgpu_waves_per_eu(OCC)))
 void k_step2_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict__ tiles,
                  const Entry* __restrict__ ents, int ntiles, int k, int par, int sspar) {
@@ -517,7 +520,7 @@ void k_step2_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
             load_old();
             load_med();
             load_cur();
-            if constexpr (PS) load_slots();
+            if constexpr (PS && MODE == M_BADJ) load_slots();
         }
 #pragma unroll
         for (int it = 0; it < NQ; ++it) {
@@ -544,7 +547,7 @@ void k_step2_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
     }
     if constexpr (LATE) {
         load_old();
-        if constexpr (PS) load_slots();
+        if constexpr (PS && MODE == M_BADJ) load_slots();
     }
     __syncthreads();
 
@@ -598,6 +601,20 @@ void k_step2_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
     __syncthreads();
 
     const bool has_inj = t.rec_e > t.rec_b;
+    const bool has_ext = MODE == M_NLADJ && t.src_e > t.src_b;
+    // source-trace extraction of one step from the l1 area (owned source cells)
+    auto extract = [&](const StepTime& st) {
+        const T fa = S.filt[st.it2], fb = S.filt[S.h + st.it2];
+        for (int e = t.src_b + threadIdx.x; e < t.src_e; e += NT) {
+            const Entry en = ents[e];
+            const int lz = en.lcell & 255, lx = en.lcell >> 8;
+            const int comp = en.crow >> 24, row = en.crow & 0xffffff;
+            const T v = l1[comp][lx][lz];
+            T* tr = S.src_tr[comp] + long(row) * S.nts + st.its;
+            tr[0] += v * fa;
+            tr[1] += v * fb;
+        }
+    };
     // ---- step-1 data injection on the whole step-1 region (kernelsGpuElastic.cu:191-229, it2+1)
     if (has_inj) {
         const T fa = S.filt[st1.it2 + 1], fb = S.filt[S.h + st1.it2 + 1];
@@ -611,6 +628,7 @@ void k_step2_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
         }
         __syncthreads();
     }
+    if (has_ext) extract(st1);  // l1 is read only until the next barrier
     // ---- M u1 over the stage (same cells: lc column j+4, row lane+4), u1 stored on owned cells
 #pragma unroll
     for (int r = 0; r < R1; ++r) {
@@ -631,10 +649,12 @@ void k_step2_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
     }
     // u1 on the owned cells of this lane's step-2 columns (adjoint-field extraction)
     T u1[5][R2];
+    if constexpr (MODE == M_BADJ) {
 #pragma unroll
-    for (int r = 0; r < R2; ++r)
+        for (int r = 0; r < R2; ++r)
 #pragma unroll
-        for (int f = 0; f < 5; ++f) u1[f][r] = l1[f][j2 + r + FAT][lz2];
+            for (int f = 0; f < 5; ++f) u1[f][r] = l1[f][j2 + r + FAT][lz2];
+    }
     if constexpr (LATE) load_cur();
     __syncthreads();
 
@@ -678,35 +698,38 @@ void k_step2_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
             v2[4][r] = d * (d * oc[4][r] - (dz_s0 + dx_s1[r]));
         }
     }
-    if (has_inj) {  // step-2 data injection on owned cells, through the l1 area (uniform per tile)
+    if (has_inj || has_ext) {  // step-2 data through the l1 area (uniform per tile), owned cells
 #pragma unroll
         for (int r = 0; r < R2; ++r)
 #pragma unroll
             for (int f = 0; f < 5; ++f)
                 if (zown) l1[f][j2 + r + FAT][lane] = v2[f][r];
         __syncthreads();
-        const T fa = S.filt[st2.it2 + 1], fb = S.filt[S.h + st2.it2 + 1];
-        for (int e = t.rec_b + threadIdx.x; e < t.rec_e; e += NT) {
-            const Entry en = ents[e];
-            const int lz = en.lcell & 255, lx = en.lcell >> 8;
-            if (lz < FAT || lz >= FAT + TZ2 || lx < FAT || lx >= FAT + TX) continue;
-            const int comp = en.crow >> 24, row = en.crow & 0xffffff;
-            const T* tr = S.rec_tr[comp] + long(row) * S.nts + st2.its;
-            const T s = tr[0] * fa + tr[1] * fb;
-            l1[comp][lx][lz] += damp_at(S, t.z0 - FAT + lz, t.x0 - FAT + lx) * s;
+        if (has_inj) {
+            const T fa = S.filt[st2.it2 + 1], fb = S.filt[S.h + st2.it2 + 1];
+            for (int e = t.rec_b + threadIdx.x; e < t.rec_e; e += NT) {
+                const Entry en = ents[e];
+                const int lz = en.lcell & 255, lx = en.lcell >> 8;
+                if (lz < FAT || lz >= FAT + TZ2 || lx < FAT || lx >= FAT + TX) continue;
+                const int comp = en.crow >> 24, row = en.crow & 0xffffff;
+                const T* tr = S.rec_tr[comp] + long(row) * S.nts + st2.its;
+                const T s = tr[0] * fa + tr[1] * fb;
+                l1[comp][lx][lz] += damp_at(S, t.z0 - FAT + lz, t.x0 - FAT + lx) * s;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < R2; ++r)
+#pragma unroll
+                for (int f = 0; f < 5; ++f)
+                    if (zown) v2[f][r] = l1[f][j2 + r + FAT][lane];
         }
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < R2; ++r)
-#pragma unroll
-            for (int f = 0; f < 5; ++f)
-                if (zown) v2[f][r] = l1[f][j2 + r + FAT][lane];
+        if (has_ext) extract(st2);
     }
     // ---- u2 and the pair's adjoint fields: slot its gets F[a] u1 + F[a-1] u2 and slot its+1
     // F[h+a] u1 + F[h+a-1] u2 (a = st1.it2)
     const T ea1 = S.filt[st1.it2], ea2 = S.filt[st2.it2];
     const T eb1 = S.filt[S.h + st1.it2], eb2 = S.filt[S.h + st2.it2];
-    if constexpr (!PS) load_slots();
+    if constexpr (!PS && MODE == M_BADJ) load_slots();
 #pragma unroll
     for (int r = 0; r < R2; ++r) {
         const int ix = ix2 + r;
@@ -716,8 +739,10 @@ void k_step2_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
             for (int f = 0; f < 5; ++f) {
                 const unsigned sf = unsigned(f) * fbytes;
                 bst(rw2, v2[f][r], vo, sf);
-                bst(rsl, p0[f][r] + (u1[f][r] * ea1 + v2[f][r] * ea2), vo, sf);
-                bst(rsr, p1[f][r] + (u1[f][r] * eb1 + v2[f][r] * eb2), vo, sf);
+                if constexpr (MODE == M_BADJ) {
+                    bst(rsl, p0[f][r] + (u1[f][r] * ea1 + v2[f][r] * ea2), vo, sf);
+                    bst(rsr, p1[f][r] + (u1[f][r] * eb1 + v2[f][r] * eb2), vo, sf);
+                }
             }
         }
     }

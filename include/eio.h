@@ -101,13 +101,13 @@ int eio_ctx_set_variant(eio_ctx* ctx, int variant);
 int eio_ctx_num_time_block_variants(void);
 int eio_ctx_time_block_info(int variant, int* R2, int* WX, int* occ);
 int eio_ctx_set_time_block(eio_ctx* ctx, int steps, int variant);
-/* Reverse fine steps per launch of the FP32 Born receiver pass (eio_born_adjoint and
- * eio_fwi_gradient): 1 (one fused step per launch, default) or 2 (both steps of a pair in one
- * launch, four state levels per shot; the adjoint fields of the pair are extracted with one
- * write of coarse slot its and one read-modify-write of slot its+1), and the variant of the
- * two-step adjoint kernel (owned tile 56 x WX*R2).  FP64 contexts ignore it.  No counterpart
- * in the reference (its receiver pass, BornElasticGpuFunctions.cu:262-370, takes one step per
- * launch). */
+/* Reverse fine steps per launch of the FP32 adjoint passes (eio_born_adjoint,
+ * eio_fwi_gradient and eio_nl_adjoint without free surface or saved wavefield): 1 (one fused
+ * step per launch, default) or 2 (both steps of a pair in one launch, four state levels per
+ * shot; the Born adjoint fields of the pair are extracted with at most one read-modify-write
+ * of each coarse slot), and the variant of the two-step adjoint kernel (owned tile
+ * 56 x WX*R2).  FP64 contexts ignore it.  No counterpart in the reference (its receiver pass,
+ * BornElasticGpuFunctions.cu:262-370, and its NL adjoint take one step per launch). */
 int eio_ctx_num_adjoint_time_block_variants(void);
 int eio_ctx_adjoint_time_block_info(int variant, int* R2, int* WX, int* occ);
 int eio_ctx_set_adjoint_time_block(eio_ctx* ctx, int steps, int variant);

@@ -313,6 +313,50 @@ def test_two_step_born_adjoint_matches_oracle(va, sub_fd):
     assert np.array_equal(again, two)
 
 
+@pytest.mark.parametrize("va", range(10))
+def test_two_step_nl_adjoint_matches_oracle(va):
+    """The NL adjoint with two reverse fine steps per launch (data injected on the step-1 halo,
+    source traces extracted from u1 and u2 at owned source cells) against the FP64 oracle and
+    against one step per launch: 120 sources and 200 receivers per grid scattered over a grid
+    of several tiles each way, nts 50, sub_fd 4."""
+    if va >= len(_capi.Context.adjoint_time_block_variants()):
+        pytest.skip("variant not compiled")
+    w = W.small(nz=150, nx=130, nts=50, sub=2, seed=95)
+    g = oracle_grid(w)
+    med = O.media(g, w["elastic"], np.float64)
+    rng = np.random.default_rng(96 + va)
+    spos = scattered_positions(rng, g.nz, g.nx, 120)
+    rpos = scattered_positions(rng, g.nz, g.nx, 200)
+    dat = random_traces(rng, rpos, g.nts, np.float64)
+    ref = O.nl_adj_reg(g, med, spos, rpos, dat, np.float64)
+    ctx = _capi.Context(0, np.float32)
+    mid = _model(ctx, w, g, med)
+    one = _gpu_nl_adj(ctx, mid, spos, rpos, dat, g.nts, np.float32)
+    ctx.set_adjoint_time_block(2, va)
+    two = _gpu_nl_adj(ctx, mid, spos, rpos, dat, g.nts, np.float32)
+    for k in range(5):
+        assert np.linalg.norm(ref[k]) > 0
+        assert rel_l2(two[k], ref[k]) < 1e-5, (va, k, rel_l2(two[k], ref[k]))
+        assert rel_l2(two[k], one[k]) < 1e-5, (va, k, rel_l2(two[k], one[k]))
+
+
+def test_two_step_nl_dot_product():
+    """<F m, d> = <m, F^T d> with two fine steps per launch both ways (FP32 bar 1e-5)."""
+    w, g, med, spos, rpos = _case(nz=120, nx=100, nts=80, sub=1, seed=97)
+    rng = np.random.default_rng(98)
+    m = [rng.standard_normal((spos[GRID_OF_COMP[k]].size, g.nts)).astype(np.float32) for k in range(5)]
+    d = [rng.standard_normal((rpos[GRID_OF_COMP[k]].size, g.nts)).astype(np.float32) for k in range(5)]
+    ctx = _capi.Context(0, np.float32)
+    ctx.set_time_block(2, 4)
+    ctx.set_adjoint_time_block(2, 7)
+    mid = _model(ctx, w, g, med)
+    Fm = _gpu_nl_fwd(ctx, mid, spos, rpos, m, g.nts, np.float32)
+    Ftd = _gpu_nl_adj(ctx, mid, spos, rpos, d, g.nts, np.float32)
+    a = sum(np.sum(Fm[k].astype(np.float64) * d[k]) for k in range(5))
+    b = sum(np.sum(m[k].astype(np.float64) * Ftd[k]) for k in range(5))
+    assert abs(a - b) / abs(b) < DOT_TOL[np.float32], (a, b)
+
+
 def test_two_step_born_adjoint_dot_product():
     """<B m, d> = <m, B^T d> with the two-step receiver pass (FP32 bar 1e-5)."""
     w, g, med, spos, rpos = _case(nz=120, nx=100, nts=80, sub=1, seed=85)

@@ -168,7 +168,9 @@ def test_forward_and_adjoint_wavefield(tmp_path):
     si = G.staggered_interps(w["nz"], w["oz"], w["dz"], w["ox"], w["dx"], geo["sz"][1], geo["sx"][1])
     mm1 = model.clone()
     op.adjoint(False, mm1, dd)
-    assert np.array_equal(mm1.getNdArray(), mm.getNdArray())
+    # the plain adjoint takes two reverse fine steps per launch, the wavefield-saving one one
+    # step: the same operator within FP32 rounding
+    assert rel_l2(mm1.getNdArray(), mm.getNdArray()) < 1e-6
     # shot 1's own contribution to the (shot-stacked) adjoint model
     dd1 = dd.clone()
     dd1.getNdArray()[...] = 0
