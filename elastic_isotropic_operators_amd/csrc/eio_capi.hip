@@ -398,6 +398,10 @@ struct Run {
     Plan plan2;
     eio::TileDev* d_tiles2 = nullptr;
     eio::Entry* d_ents2 = nullptr;
+    // two-step forward plan of the Born source pass (Born runs whose other passes take one step)
+    Plan plan3;
+    eio::TileDev* d_tiles3 = nullptr;
+    eio::Entry* d_ents3 = nullptr;
     // host copies for the D2H of outputs
     struct Out {
         void* host;
@@ -410,18 +414,20 @@ struct Run {
 // Builds the plan, the per-shot descriptors and the device tables of one call, uploads
 // input traces and zeroes the state and the outputs.  FWI calls bring their own coarse
 // storage ring (`ring`, `wcap` slices per component and shot).  `two_step`: the NL forward
-// runs the two-step kernel; `adj2`: the Born adjoint pass runs the two-step adjoint kernel.
+// runs the two-step kernel; `adj2`: the adjoint passes run the two-step adjoint kernel;
+// `src2`: the Born source pass runs the two-step forward kernel.
 template <typename T>
 Run<T> prepare(Ctx* c, int nshot, const eio_shot* shots, Op op, int flags,
                const void* const dm_fwd[5], T* ring = nullptr, int wcap = 0, bool two_step = false,
-               bool adj2 = false) {
+               bool adj2 = false, bool src2 = false) {
     if (nshot <= 0) fail("nshot must be > 0");
     if (!shots) fail("shots is NULL");
     Run<T> run;
     run.c = c;
     run.plan = make_plan<T>(c, nshot, shots, op, two_step ? PlanKind::FWD2 : PlanKind::ONE);
     if (adj2) run.plan2 = make_plan<T>(c, nshot, shots, op, PlanKind::ADJ2);
-    const int nlev = (two_step || adj2) ? 4 : 2;  // state levels per shot
+    if (src2) run.plan3 = make_plan<T>(c, nshot, shots, op, PlanKind::FWD2);
+    const int nlev = (two_step || adj2 || src2) ? 4 : 2;  // state levels per shot
     const bool born = (op == Op::BORN_FWD || op == Op::BORN_ADJ);
     const bool fwi = (op == Op::FWI_FWD || op == Op::FWI_GRAD);
     const bool has_ss = born || op == Op::FWI_GRAD;
@@ -472,7 +478,7 @@ Run<T> prepare(Ctx* c, int nshot, const eio_shot* shots, Op op, int flags,
         for (int k = 0; k < 4; ++k) { d.cz[k] = T(m.cz[k]); d.cx[k] = T(m.cx[k]); }
         d.A = state + so;
         d.B = state + so + 5 * size_t(m.n);
-        if (two_step || adj2) {
+        if (two_step || adj2 || src2) {
             d.C = state + so + 10 * size_t(m.n);
             d.D = state + so + 15 * size_t(m.n);
         }
@@ -543,6 +549,14 @@ Run<T> prepare(Ctx* c, int nshot, const eio_shot* shots, Op op, int flags,
     EIO_CHECK_HIP(hipMemcpyAsync(run.d_tiles, run.plan.tiles.data(), sizeof(eio::TileDev) * run.plan.tiles.size(), h2d, c->stream));
     if (!run.plan.ents.empty())
         EIO_CHECK_HIP(hipMemcpyAsync(run.d_ents, run.plan.ents.data(), sizeof(eio::Entry) * run.plan.ents.size(), h2d, c->stream));
+    if (src2) {
+        const Plan& p3 = run.plan3;
+        run.d_tiles3 = static_cast<eio::TileDev*>(ws_get(c, "tiles3", sizeof(eio::TileDev) * p3.tiles.size()));
+        run.d_ents3 = static_cast<eio::Entry*>(ws_get(c, "ents3", sizeof(eio::Entry) * std::max<size_t>(1, p3.ents.size())));
+        EIO_CHECK_HIP(hipMemcpyAsync(run.d_tiles3, p3.tiles.data(), sizeof(eio::TileDev) * p3.tiles.size(), h2d, c->stream));
+        if (!p3.ents.empty())
+            EIO_CHECK_HIP(hipMemcpyAsync(run.d_ents3, p3.ents.data(), sizeof(eio::Entry) * p3.ents.size(), h2d, c->stream));
+    }
     if (adj2) {
         const Plan& p2 = run.plan2;
         run.d_tiles2 = static_cast<eio::TileDev*>(ws_get(c, "tiles2", sizeof(eio::TileDev) * p2.tiles.size()));
@@ -615,12 +629,16 @@ void launch_adj(Run<T>& run, int ntile, int k, int sspar) {
     }
 #undef EIO_L
 }
-template <bool DEC = false>
-void launch_fwd2(Run<float>& run, int ntile, int k, int par, int dlo = 0, int dhi = -1) {
+// `tiles`, `ents`: another plan's tables (the Born source pass of a one-step run)
+template <bool DEC = false, bool FINE = false>
+void launch_fwd2(Run<float>& run, int ntile, int k, int par, int dlo = 0, int dhi = -1,
+                 const eio::TileDev* tiles = nullptr, const eio::Entry* ents = nullptr) {
     hipStream_t s = run.c->stream;
+    const eio::TileDev* tt = tiles ? tiles : run.d_tiles;
+    const eio::Entry* te = ents ? ents : run.d_ents;
 #define EIO_L2(R_, WX_, OCC_, NT_)                                                                         \
-    hipLaunchKernelGGL((eio::k_step2_fwd<float, R_, WX_, OCC_, NT_, DEC>), dim3(ntile), dim3(64 * WX_), 0, s, \
-                       run.d_shots, run.d_tiles, run.d_ents, ntile, k, par, dlo, dhi)
+    hipLaunchKernelGGL((eio::k_step2_fwd<float, R_, WX_, OCC_, NT_, DEC, FINE>), dim3(ntile), dim3(64 * WX_), 0, \
+                       s, run.d_shots, tt, te, ntile, k, par, dlo, dhi)
     switch (run.c->variant2) {
         case 0: EIO_L2(2, 8, 2, 0); break;
         case 1: EIO_L2(2, 8, 4, 0); break;
@@ -1018,8 +1036,18 @@ void nl_adjoint(Ctx* c, int nshot, const eio_shot* shots, int flags, int wshot =
     finish(run);
 }
 
+// The Born source pass (fine sources, Vx/Vz decimated into the whole record): two fine
+// steps per launch on the run's third plan when it has one (FP32, even step count).
 template <typename T>
 void born_source_pass(Run<T>& run, int ntile, int kmax) {
+    if constexpr (sizeof(T) == 4) {
+        if (!run.plan3.tiles.empty()) {
+            const int nt3 = int(run.plan3.tiles.size());
+            for (int k = 0; k < kmax; k += 2)
+                launch_fwd2<true, true>(run, nt3, k, (k >> 1) & 1, 0, -1, run.d_tiles3, run.d_ents3);
+            return;
+        }
+    }
     for (int k = 0; k < kmax; ++k) launch_fwd<T, eio::M_BSRC>(run, ntile, k, 0, 0, -1);
 }
 
@@ -1036,9 +1064,10 @@ void born_forward(Ctx* c, int nshot, const eio_shot* shots, const void* const dm
         EIO_CHECK_HIP(hipMemcpyAsync(b + size_t(f) * m.n, dm[f], nb, kind, c->stream));
     }
     const void* dmp[5] = {b, b + m.n, b + 2 * m.n, b + 3 * m.n, b + 4 * m.n};
-    Run<T> run = prepare<T>(c, nshot, shots, Op::BORN_FWD, flags, dmp);
-    const int ntile = int(run.plan.tiles.size());
     const int kmax = m.ntw - 1, sub = m.g.sub_fd, nts = m.g.nts;
+    const bool src2 = sizeof(T) == 4 && c->time_block == 2 && kmax % 2 == 0;
+    Run<T> run = prepare<T>(c, nshot, shots, Op::BORN_FWD, flags, dmp, nullptr, 0, false, false, src2);
+    const int ntile = int(run.plan.tiles.size());
     long long nstep = 0;
     timing_begin<T>(c);
     EIO_CHECK_HIP(hipEventRecord(c->ev[2], c->stream));
@@ -1073,7 +1102,8 @@ void born_adjoint(Ctx* c, int nshot, const eio_shot* shots, void* const dm[5], i
     const ModelBase& m = get_model(c, shots[0].model);
     const int kmax = m.ntw - 1, sub = m.g.sub_fd, nts = m.g.nts;
     const bool two = sizeof(T) == 4 && c->adj_time_block == 2 && sub % 2 == 0;
-    Run<T> run = prepare<T>(c, nshot, shots, Op::BORN_ADJ, flags, nullptr, nullptr, 0, false, two);
+    const bool src2 = sizeof(T) == 4 && c->time_block == 2 && kmax % 2 == 0;
+    Run<T> run = prepare<T>(c, nshot, shots, Op::BORN_ADJ, flags, nullptr, nullptr, 0, false, two, src2);
     const int ntile = int(run.plan.tiles.size());
     timing_begin<T>(c);
     EIO_CHECK_HIP(hipEventRecord(c->ev[2], c->stream));

@@ -54,8 +54,10 @@ __device__ __forceinline__ void bst_pol(rsrc_t r, T v, unsigned voff, unsigned s
     else bst(r, v, voff, soff);
 }
 
-// NTS: store u1 and u2 with the nontemporal cache policy; DEC: also decimate Vx, Vz
-template <typename T, int R2, int WX, int OCC, int NTS, bool DEC>
+// NTS: store u1 and u2 with the nontemporal cache policy; DEC: also decimate Vx, Vz;
+// FINE: the Born source pass (mode M_BSRC of k_step_fwd): sources from pre-fine-sampled traces
+// (kernelsGpuElastic.cu:121-168), no receiver recording.
+template <typename T, int R2, int WX, int OCC, int NTS, bool DEC, bool FINE = false>
 __global__ __launch_bounds__(64 * WX) __attribute__((amdgpu_waves_per_eu(OCC)))
 void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict__ tiles,
                  const Entry* __restrict__ ents, int ntiles, int k, int par, int dlo, int dhi) {
@@ -223,7 +225,7 @@ void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
     __syncthreads();
 
     const bool has_src = t.src_e > t.src_b;
-    const bool has_rec = t.rec_e > t.rec_b;
+    const bool has_rec = !FINE && t.rec_e > t.rec_b;
     // ---- step-1 sources on the whole step-1 region (kernelsGpuElastic.cu:172-189)
     if (has_src) {
         for (int e = t.src_b + threadIdx.x; e < t.src_e; e += NT) {
@@ -231,8 +233,13 @@ void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
             const int lz = en.lcell & 255, lx = en.lcell >> 8;
             const int comp = en.crow >> 24, row = en.crow & 0xffffff;
             const T* tr = S.src_tr[comp];
-            const long b = long(row) * S.nts + st1.its;
-            const T s = tr[b] * S.filt[st1.it2 - 1] + tr[b + 1] * S.filt[S.h + st1.it2 - 1];
+            T s;
+            if constexpr (FINE) {
+                s = tr[long(row) * S.ntw + k];
+            } else {
+                const long b = long(row) * S.nts + st1.its;
+                s = tr[b] * S.filt[st1.it2 - 1] + tr[b + 1] * S.filt[S.h + st1.it2 - 1];
+            }
             l1[comp][lx][lz] += damp_at(S, t.z0 - FAT + lz, t.x0 - FAT + lx) * s;
         }
         __syncthreads();
@@ -339,8 +346,13 @@ void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
                 if (lz < FAT || lz >= FAT + TZ2 || lx < FAT || lx >= FAT + TX) continue;
                 const int comp = en.crow >> 24, row = en.crow & 0xffffff;
                 const T* tr = S.src_tr[comp];
-                const long b = long(row) * S.nts + st2.its;
-                const T s = tr[b] * S.filt[st2.it2 - 1] + tr[b + 1] * S.filt[S.h + st2.it2 - 1];
+                T s;
+                if constexpr (FINE) {
+                    s = tr[long(row) * S.ntw + k + 1];
+                } else {
+                    const long b = long(row) * S.nts + st2.its;
+                    s = tr[b] * S.filt[st2.it2 - 1] + tr[b + 1] * S.filt[S.h + st2.it2 - 1];
+                }
                 l1[comp][lx][lz] += damp_at(S, t.z0 - FAT + lz, t.x0 - FAT + lx) * s;
             }
             __syncthreads();
