@@ -629,16 +629,17 @@ void launch_adj(Run<T>& run, int ntile, int k, int sspar) {
     }
 #undef EIO_L
 }
-// `tiles`, `ents`: another plan's tables (the Born source pass of a one-step run)
-template <bool DEC = false, bool FINE = false>
+// `tiles`, `ents`: another plan's tables (the Born passes of a one-step run); `sspar`: the
+// secondary-source slot of coarse step its (SS)
+template <bool DEC = false, bool FINE = false, bool SS = false>
 void launch_fwd2(Run<float>& run, int ntile, int k, int par, int dlo = 0, int dhi = -1,
-                 const eio::TileDev* tiles = nullptr, const eio::Entry* ents = nullptr) {
+                 const eio::TileDev* tiles = nullptr, const eio::Entry* ents = nullptr, int sspar = 0) {
     hipStream_t s = run.c->stream;
     const eio::TileDev* tt = tiles ? tiles : run.d_tiles;
     const eio::Entry* te = ents ? ents : run.d_ents;
 #define EIO_L2(R_, WX_, OCC_, NT_)                                                                         \
-    hipLaunchKernelGGL((eio::k_step2_fwd<float, R_, WX_, OCC_, NT_, DEC, FINE>), dim3(ntile), dim3(64 * WX_), 0, \
-                       s, run.d_shots, tt, te, ntile, k, par, dlo, dhi)
+    hipLaunchKernelGGL((eio::k_step2_fwd<float, R_, WX_, OCC_, NT_, DEC, FINE, SS>), dim3(ntile),            \
+                       dim3(64 * WX_), 0, s, run.d_shots, tt, te, ntile, k, par, dlo, dhi, sspar)
     switch (run.c->variant2) {
         case 0: EIO_L2(2, 8, 2, 0); break;
         case 1: EIO_L2(2, 8, 4, 0); break;
@@ -1036,6 +1037,23 @@ void nl_adjoint(Ctx* c, int nshot, const eio_shot* shots, int flags, int wshot =
     finish(run);
 }
 
+// The Born scattered pass over coarse interval its (fine steps its*sub ..): two fine steps
+// per launch on the run's third plan (`two`), secondary sources from slots ss[p], ss[p^1].
+template <typename T>
+void scattered_interval(Run<T>& run, int ntile, bool two, int its, int sub, int p) {
+    if constexpr (sizeof(T) == 4) {
+        if (two) {
+            const int nt3 = int(run.plan3.tiles.size());
+            for (int j = 0; j < sub; j += 2) {
+                const int k = its * sub + j;
+                launch_fwd2<false, false, true>(run, nt3, k, (k >> 1) & 1, 0, -1, run.d_tiles3, run.d_ents3, p);
+            }
+            return;
+        }
+    }
+    for (int it2 = 1; it2 <= sub; ++it2) launch_fwd<T, eio::M_BSCAT>(run, ntile, its * sub + it2 - 1, p);
+}
+
 // The Born source pass (fine sources, Vx/Vz decimated into the whole record): two fine
 // steps per launch on the run's third plan when it has one (FP32, even step count).
 template <typename T>
@@ -1077,13 +1095,16 @@ void born_forward(Ctx* c, int nshot, const eio_shot* shots, const void* const dm
         for (int oi = 0; oi < nshot; ++oi) {
             fill0<T>(c, run.sd[oi].A, 5 * m.n);
             fill0<T>(c, run.sd[oi].B, 5 * m.n);
+            if (src2) {
+                fill0<T>(c, run.sd[oi].C, 5 * m.n);
+                fill0<T>(c, run.sd[oi].D, 5 * m.n);
+            }
         }
         int p = 0;
         launch_img<T, false>(run, ntile, 0, p);
         for (int its = 0; its < nts - 1; ++its) {
             launch_img<T, false>(run, ntile, its + 1, p ^ 1);
-            for (int it2 = 1; it2 <= sub; ++it2)
-                launch_fwd<T, eio::M_BSCAT>(run, ntile, its * sub + it2 - 1, p);
+            scattered_interval(run, ntile, src2, its, sub, p);
             p ^= 1;
         }
     });

@@ -56,11 +56,16 @@ __device__ __forceinline__ void bst_pol(rsrc_t r, T v, unsigned voff, unsigned s
 
 // NTS: store u1 and u2 with the nontemporal cache policy; DEC: also decimate Vx, Vz;
 // FINE: the Born source pass (mode M_BSRC of k_step_fwd): sources from pre-fine-sampled traces
-// (kernelsGpuElastic.cu:121-168), no receiver recording.
-template <typename T, int R2, int WX, int OCC, int NTS, bool DEC, bool FINE = false>
+// (kernelsGpuElastic.cu:121-168), no receiver recording.  SS: the Born scattered pass (mode
+// M_BSCAT): no point sources; every cell of both steps adds the secondary source of slots
+// ss[sspar] (coarse its) and ss[sspar^1] (its+1) with the weights of its own fine step
+// (kernelsGpuElastic.cu:1363-1368); both steps lie in one coarse interval, so they share the
+// two slots.
+template <typename T, int R2, int WX, int OCC, int NTS, bool DEC, bool FINE = false, bool SS = false>
 __global__ __launch_bounds__(64 * WX) __attribute__((amdgpu_waves_per_eu(OCC)))
 void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict__ tiles,
-                 const Entry* __restrict__ ents, int ntiles, int k, int par, int dlo, int dhi) {
+                 const Entry* __restrict__ ents, int ntiles, int k, int par, int dlo, int dhi,
+                 int sspar) {
     static_assert(8 % WX == 0, "WX must divide 8");
     static_assert(sizeof(T) == 4, "the two-step forward is FP32");
     constexpr int NT = 64 * WX;
@@ -178,6 +183,17 @@ void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
 
     const T* cz = S.cz;
     const T* cx = S.cx;
+    // SS: secondary-source slots and the weights of the two steps (sampled at it2-1)
+    rsrc_t rssl, rssr;
+    T ssf1a = T(0), ssf1b = T(0), ssf2a = T(0), ssf2b = T(0);
+    if constexpr (SS) {
+        rssl = make_rsrc(S.ss[sspar], 5 * fbytes);
+        rssr = make_rsrc(S.ss[sspar ^ 1], 5 * fbytes);
+        ssf1a = S.filt[st1.it2 - 1];
+        ssf1b = S.filt[S.h + st1.it2 - 1];
+        ssf2a = S.filt[st2.it2 - 1];
+        ssf2b = S.filt[S.h + st2.it2 - 1];
+    }
     // ---- step 1 on columns j1 .. j1+R1-1 (rel x0-4) = lc columns j1+4 .. ; lane row = lc row lane+4
     {
         T dx_sxx[R1], dx_sxz[R1], dx_vx[R1], dx_vz[R1];
@@ -210,11 +226,25 @@ void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
             const T dz_vz = dp_q(&lc[1][c][lane + FAT], 0, cz);
             const T dz_vx = dm_q(&lc[0][c][lane + FAT], 0, cz);
             const T d = ok ? damp_at(S, iz, ix) : T(1);
-            const T a0 = d * o[0][r] + m[0][r] * (dx_sxx[r] + dz_sxz);
-            const T a1 = d * o[1][r] + m[1][r] * (dz_szz + dx_sxz[r]);
-            const T a2 = d * o[2][r] + (m[2][r] * dx_vx[r] + m[3][r] * dz_vz);
-            const T a3 = d * o[3][r] + (m[3][r] * dx_vx[r] + m[2][r] * dz_vz);
-            const T a4 = d * o[4][r] + m[4][r] * (dz_vx + dx_vz[r]);
+            T a0 = d * o[0][r] + m[0][r] * (dx_sxx[r] + dz_sxz);
+            T a1 = d * o[1][r] + m[1][r] * (dz_szz + dx_sxz[r]);
+            T a2 = d * o[2][r] + (m[2][r] * dx_vx[r] + m[3][r] * dz_vz);
+            T a3 = d * o[3][r] + (m[3][r] * dx_vx[r] + m[2][r] * dz_vz);
+            T a4 = d * o[4][r] + m[4][r] * (dz_vx + dx_vz[r]);
+            if constexpr (SS) {
+                const unsigned vo = (unsigned(ix < 0 ? 0 : ix) * unsigned(nz) + unsigned(izl)) * SZ;
+                T sl[5], sr[5];
+#pragma unroll
+                for (int f = 0; f < 5; ++f) {
+                    sl[f] = bld<T>(rssl, vo, unsigned(f) * fbytes);
+                    sr[f] = bld<T>(rssr, vo, unsigned(f) * fbytes);
+                }
+                a0 += sl[0] * ssf1a + sr[0] * ssf1b;
+                a1 += sl[1] * ssf1a + sr[1] * ssf1b;
+                a2 += sl[2] * ssf1a + sr[2] * ssf1b;
+                a3 += sl[3] * ssf1a + sr[3] * ssf1b;
+                a4 += sl[4] * ssf1a + sr[4] * ssf1b;
+            }
             l1[0][j1 + r][lane] = ok ? d * a0 : T(0);
             l1[1][j1 + r][lane] = ok ? d * a1 : T(0);
             l1[2][j1 + r][lane] = ok ? d * a2 : T(0);
@@ -224,7 +254,7 @@ void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
     }
     __syncthreads();
 
-    const bool has_src = t.src_e > t.src_b;
+    const bool has_src = !SS && t.src_e > t.src_b;
     const bool has_rec = !FINE && t.rec_e > t.rec_b;
     // ---- step-1 sources on the whole step-1 region (kernelsGpuElastic.cu:172-189)
     if (has_src) {
@@ -319,11 +349,24 @@ void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
             const T dz_vz = dp_q(&l1[1][c][lz2], 0, cz);
             const T dz_vx = dm_q(&l1[0][c][lz2], 0, cz);
             const T d = ok ? damp_at(S, iz, ix) : T(1);
-            const T a0 = d * oo[0] + mm[0] * (dx_sxx[r] + dz_sxz);
-            const T a1 = d * oo[1] + mm[1] * (dz_szz + dx_sxz[r]);
-            const T a2 = d * oo[2] + (mm[2] * dx_vx[r] + mm[3] * dz_vz);
-            const T a3 = d * oo[3] + (mm[3] * dx_vx[r] + mm[2] * dz_vz);
-            const T a4 = d * oo[4] + mm[4] * (dz_vx + dx_vz[r]);
+            T a0 = d * oo[0] + mm[0] * (dx_sxx[r] + dz_sxz);
+            T a1 = d * oo[1] + mm[1] * (dz_szz + dx_sxz[r]);
+            T a2 = d * oo[2] + (mm[2] * dx_vx[r] + mm[3] * dz_vz);
+            T a3 = d * oo[3] + (mm[3] * dx_vx[r] + mm[2] * dz_vz);
+            T a4 = d * oo[4] + mm[4] * (dz_vx + dx_vz[r]);
+            if constexpr (SS) {
+                T sl[5], sr[5];
+#pragma unroll
+                for (int f = 0; f < 5; ++f) {
+                    sl[f] = bld<T>(rssl, vo, unsigned(f) * fbytes);
+                    sr[f] = bld<T>(rssr, vo, unsigned(f) * fbytes);
+                }
+                a0 += sl[0] * ssf2a + sr[0] * ssf2b;
+                a1 += sl[1] * ssf2a + sr[1] * ssf2b;
+                a2 += sl[2] * ssf2a + sr[2] * ssf2b;
+                a3 += sl[3] * ssf2a + sr[3] * ssf2b;
+                a4 += sl[4] * ssf2a + sr[4] * ssf2b;
+            }
             v2[0][r] = d * a0;
             v2[1][r] = d * a1;
             v2[2][r] = d * a2;

@@ -97,8 +97,8 @@ int eio_ctx_set_variant(eio_ctx* ctx, int variant);
  * steps per launch, temporally blocked: 50 instead of 80 bytes of HBM traffic per
  * cell-update, four state levels per shot), and the variant of the two-step kernel
  * (owned tile 56 x WX*R2).  It applies to the NL forward (without saved wavefield, free
- * surface or learned boundary), the whole-record FWI forward and the Born source pass; the
- * Born scattered pass takes one step per launch.  No counterpart in the reference. */
+ * surface or learned boundary), the whole-record FWI forward and the Born source and
+ * scattered passes.  No counterpart in the reference. */
 int eio_ctx_num_time_block_variants(void);
 int eio_ctx_time_block_info(int variant, int* R2, int* WX, int* occ);
 int eio_ctx_set_time_block(eio_ctx* ctx, int steps, int variant);

@@ -359,10 +359,11 @@ def test_two_step_nl_dot_product():
 
 @pytest.mark.parametrize("v", [0, 1, 4, 7])
 def test_two_step_born_source_pass_matches_oracle(v):
-    """Born forward and adjoint whose source pass takes two fine steps per launch (fine-sampled
-    sources injected on the step-1 halo, Vx/Vz decimated into the whole record, no receiver
-    recording): the FP64 oracle and one step per launch, with 30 sources and 200 receivers per
-    grid scattered over several tiles each way; sub_fd 4."""
+    """Born forward and adjoint whose source and scattered passes take two fine steps per launch
+    (fine-sampled sources injected on the step-1 halo and Vx/Vz decimated into the whole
+    record; secondary sources added at every cell of both steps, receivers recorded): the FP64
+    oracle and one step per launch, with 30 sources and 200 receivers per grid scattered over
+    several tiles each way; sub_fd 4."""
     w = W.small(nz=150, nx=130, nts=50, sub=2, seed=100)
     g = oracle_grid(w)
     med = O.media(g, w["elastic"], np.float64)
