@@ -151,7 +151,9 @@ class ElasticFwi:
     src_z, src_x: (nShot, nSrc) source coordinates
     sources: (5, nSrc, nts) shared by all shots, or (nShot, 5, nSrc, nts)
     rec_z, rec_x: (nRec,) receiver coordinates shared by all shots
-    segment: checkpoint interval in coarse samples (0: ceil(sqrt(nts-1)))
+    segment: checkpoint interval in coarse samples; 0: the whole record (no checkpoint, nothing
+             recomputed) when the batch's full coarse Vx/Vz storage fits in 60 % of the free
+             device memory, else ceil(sqrt(nts-1))
     src_interp, rec_interp: (method, nfilt) of the spatial interpolation of sources and
              receivers: ("linear", 1), ("sinc", n) or ("gauss", n) (spaceInterpGpu)
     time_block: (fine steps per launch, two-step kernel variant) of the FP32 forwards
@@ -300,7 +302,10 @@ class ElasticFwi:
         shots, keep = self._shots(srcs, recs)
         store = None
         if checkpoint:
-            h = self.ctx.fwi_forward(shots, self.segment, device_ptrs=True)
+            seg = self.segment
+            if seg == 0 and self.full_storage_group() >= self.nshot:
+                seg = self.nts - 1
+            h = self.ctx.fwi_forward(shots, seg, device_ptrs=True)
             store = _Store(self, h, srcs)
             self.last_store_info = self.ctx.fwi_info(h)
         else:
