@@ -435,6 +435,39 @@ def test_two_step_uneven_batch_matches_single_step():
             assert rel_l2(b[k], a[k]) < 2e-6, (k, rel_l2(b[k], a[k]))
 
 
+def test_two_step_nl_adjoint_batch_of_different_grids():
+    """One NL adjoint call over shots on three grids of different shape (same time axis, as the
+    adjoint requires): two reverse steps per launch give each shot's one-step result, and a
+    shot's result does not depend on the batch it runs in."""
+    cases = [_case(nz=70, nx=96, nts=60, sub=1, seed=121), _case(nz=96, nx=70, nts=60, sub=1, seed=122),
+             _case(nz=45, nx=120, nts=60, sub=1, seed=123)]
+    outs = {}
+    for tb in (1, 2, "alone"):
+        ctx = _capi.Context(0, np.float32)
+        ctx.set_adjoint_time_block(1 if tb == 1 else 2, 7)
+        srcs = []
+        batches = [[0, 1, 2]] if tb != "alone" else [[0], [1], [2]]
+        per = {}
+        for batch in batches:
+            shots, keep = [], []
+            for i in batch:
+                w, g, med, spos, rpos = cases[i]
+                mid = _model(ctx, w, g, med)
+                dat = random_traces(np.random.default_rng(130 + i), rpos, g.nts, np.float32)
+                src = [np.zeros((spos[GRID_OF_COMP[k]].size, g.nts), np.float32) for k in range(5)]
+                s, kp = ctx.make_shot(mid, spos, rpos, src, dat)
+                shots.append(s)
+                keep.append((kp, dat))
+                per[i] = src
+            ctx.nl_adjoint(shots)
+        outs[tb] = [per[i] for i in range(3)]
+    for i in range(3):
+        for k in range(5):
+            assert np.linalg.norm(outs[1][i][k]) > 0
+            assert rel_l2(outs[2][i][k], outs[1][i][k]) < 2e-6, (i, k)
+            assert np.array_equal(outs[2][i][k], outs["alone"][i][k]), (i, k)
+
+
 def test_autotuner_choice_keeps_parity():
     """The autotuner times every candidate kernel on the caller's batch with a step limit,
     leaves the fastest set on the context and lifts the limit: the next full call is exact."""
