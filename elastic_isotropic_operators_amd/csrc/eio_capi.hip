@@ -631,15 +631,17 @@ void launch_adj(Run<T>& run, int ntile, int k, int sspar) {
 }
 // `tiles`, `ents`: another plan's tables (the Born passes of a one-step run); `sspar`: the
 // secondary-source slot of coarse step its (SS)
-template <bool DEC = false, bool FINE = false, bool SS = false>
+template <bool DEC = false, bool FINE = false, bool SS = false, bool NOREC = false>
 void launch_fwd2(Run<float>& run, int ntile, int k, int par, int dlo = 0, int dhi = -1,
-                 const eio::TileDev* tiles = nullptr, const eio::Entry* ents = nullptr, int sspar = 0) {
+                 const eio::TileDev* tiles = nullptr, const eio::Entry* ents = nullptr, int sspar = 0,
+                 const eio::ShotDev<float>* dsh = nullptr) {
     hipStream_t s = run.c->stream;
     const eio::TileDev* tt = tiles ? tiles : run.d_tiles;
     const eio::Entry* te = ents ? ents : run.d_ents;
+    const eio::ShotDev<float>* sh = dsh ? dsh : run.d_shots;
 #define EIO_L2(R_, WX_, OCC_, NT_)                                                                         \
-    hipLaunchKernelGGL((eio::k_step2_fwd<float, R_, WX_, OCC_, NT_, DEC, FINE, SS>), dim3(ntile),            \
-                       dim3(64 * WX_), 0, s, run.d_shots, tt, te, ntile, k, par, dlo, dhi, sspar)
+    hipLaunchKernelGGL((eio::k_step2_fwd<float, R_, WX_, OCC_, NT_, DEC, FINE, SS, NOREC>), dim3(ntile),     \
+                       dim3(64 * WX_), 0, s, sh, tt, te, ntile, k, par, dlo, dhi, sspar)
     switch (run.c->variant2) {
         case 0: EIO_L2(2, 8, 2, 0); break;
         case 1: EIO_L2(2, 8, 4, 0); break;
@@ -1272,9 +1274,10 @@ int fwi_forward(Ctx* c, int nshot, const eio_shot* shots, int flags, int segment
         }
     }
     try {
-        // one segment (the whole record kept): the two-step kernel with decimation; pairs of
-        // fine steps stay inside one coarse interval because the fine sub-sampling is even
-        const bool two = sizeof(T) == 4 && f.nseg == 1 && c->time_block == 2 && sub % 2 == 0;
+        // FP32: the two-step kernel (with the decimation in the last segment); pairs of fine
+        // steps stay inside one coarse interval because the fine sub-sampling is even, and
+        // segments start at even fine steps
+        const bool two = sizeof(T) == 4 && c->time_block == 2 && sub % 2 == 0;
         Run<T> run = prepare<T>(c, nshot, shots, Op::FWI_FWD, flags, nullptr, static_cast<T*>(f.ring), f.wcap, two);
         const int ntile = int(run.plan.tiles.size());
         T* state = run.sd[0].A;  // the batch state is one contiguous buffer in plan order
@@ -1283,8 +1286,23 @@ int fwi_forward(Ctx* c, int nshot, const eio_shot* shots, int flags, int segment
         run_loop(c, [&]() {
             if constexpr (sizeof(T) == 4) {
                 if (two) {
-                    for (int k = 0; k < (nts - 1) * sub; k += 2)
-                        launch_fwd2<true>(run, ntile, k, (k >> 1) & 1, 0, -1);
+                    for (int j = 0; j < f.nseg; ++j) {
+                        const int k0 = seg_begin(f, j) * sub, k1 = seg_end(f, j) * sub;
+                        // checkpoint (older, current level) of each shot: levels A,B before
+                        // an even pair, C,D before an odd one
+                        if (j > 0)
+                            for (int oi = 0; oi < nshot; ++oi) {
+                                const size_t n10 = 10 * size_t(run.sd[oi].nz) * run.sd[oi].nx;
+                                copy_dev<T>(c, static_cast<T*>(f.ck) + size_t(j - 1) * f.state_elems + oi * n10,
+                                            ((k0 >> 1) & 1) ? run.sd[oi].C : run.sd[oi].A, n10);
+                            }
+                        if (j + 1 < f.nseg) {
+                            for (int k = k0; k < k1; k += 2) launch_fwd2<false>(run, ntile, k, (k >> 1) & 1);
+                        } else {  // the last segment also fills the ring
+                            for (int k = k0; k < k1; k += 2)
+                                launch_fwd2<true>(run, ntile, k, (k >> 1) & 1, seg_begin(f, j), -1);
+                        }
+                    }
                     return;
                 }
             }
@@ -1324,19 +1342,25 @@ void fwi_gradient(Ctx* c, int handle, int nshot, const eio_shot* shots, void* co
     const ModelBase& m = get_model(c, f.model);
     if (m.g.nts != f.nts || m.g.sub_fd != f.sub) fail("model time axis changed since the FWI forward call");
     const bool two = sizeof(T) == 4 && c->adj_time_block == 2 && f.sub % 2 == 0;
+    // FP32: segments are recomputed two fine steps per launch (four levels per shot), the
+    // kernel of the forward, so that a recomputed ring equals the forward's bit for bit
+    const bool rtwo = sizeof(T) == 4 && c->time_block == 2 && f.sub % 2 == 0;
     Run<T> run = prepare<T>(c, nshot, shots, Op::FWI_GRAD, flags, nullptr, static_cast<T*>(f.ring), f.wcap,
-                            false, two);
+                            false, two, rtwo);
     const int ntile = int(run.plan.tiles.size());
     const int nts = f.nts, sub = f.sub;
     // forward-recompute state and its shot table
-    T* fstate = static_cast<T*>(ws_get(c, "fstate", f.state_elems * sizeof(T)));
+    const size_t fstate_elems = (rtwo ? 2 : 1) * f.state_elems;
+    T* fstate = static_cast<T*>(ws_get(c, "fstate", fstate_elems * sizeof(T)));
     std::vector<eio::ShotDev<T>> sdf = run.sd;
     size_t off = 0;
     for (int oi = 0; oi < nshot; ++oi) {
         const size_t n = size_t(sdf[oi].nz) * sdf[oi].nx;
         sdf[oi].A = fstate + off;
         sdf[oi].B = fstate + off + 5 * n;
-        off += 10 * n;
+        sdf[oi].C = rtwo ? fstate + off + 10 * n : nullptr;
+        sdf[oi].D = rtwo ? fstate + off + 15 * n : nullptr;
+        off += (rtwo ? 20 : 10) * n;
     }
     auto* d_sdf = static_cast<eio::ShotDev<T>*>(ws_get(c, "shots_f", sizeof(eio::ShotDev<T>) * nshot));
     EIO_CHECK_HIP(hipMemcpyAsync(d_sdf, sdf.data(), sizeof(eio::ShotDev<T>) * nshot, hipMemcpyHostToDevice,
@@ -1346,16 +1370,38 @@ void fwi_gradient(Ctx* c, int handle, int nshot, const eio_shot* shots, void* co
     timing_begin<T>(c);
     EIO_CHECK_HIP(hipEventRecord(c->ev[2], c->stream));
     auto recompute = [&](int j) {
+        const int dhi = (j + 1 < f.nseg) ? seg_end(f, j) : -1;
+        const int k0 = seg_begin(f, j) * sub, k1 = seg_end(f, j) * sub;
+        if constexpr (sizeof(T) == 4) {
+            if (rtwo) {
+                // the checkpoint (older, current) goes to the level pair the segment's first
+                // pair reads; the other pair is overwritten on the update region, and every
+                // level's cells outside it stay zero from the clear before the sweep
+                if (j == 0) {
+                    fill0<T>(c, fstate, long(fstate_elems));
+                } else {
+                    for (int oi = 0; oi < nshot; ++oi) {
+                        const size_t n10 = 10 * size_t(sdf[oi].nz) * sdf[oi].nx;
+                        copy_dev<T>(c, ((k0 >> 1) & 1) ? sdf[oi].C : sdf[oi].A,
+                                    static_cast<const T*>(f.ck) + size_t(j - 1) * f.state_elems + oi * n10, n10);
+                    }
+                }
+                const int nt3 = int(run.plan3.tiles.size());
+                for (int k = k0; k < k1; k += 2)
+                    launch_fwd2<true, false, false, true>(run, nt3, k, (k >> 1) & 1, seg_begin(f, j), dhi,
+                                                          run.d_tiles3, run.d_ents3, 0, d_sdf);
+                return;
+            }
+        }
         if (j == 0) fill0<T>(c, fstate, long(f.state_elems));
         else copy_dev<T>(c, fstate, static_cast<const T*>(f.ck) + size_t(j - 1) * f.state_elems, f.state_elems);
-        const int dhi = (j + 1 < f.nseg) ? seg_end(f, j) : -1;
-        for (int k = seg_begin(f, j) * sub; k < seg_end(f, j) * sub; ++k)
-            launch_fwd<T, eio::M_RECOMP>(run, ntile, k, 0, seg_begin(f, j), dhi, d_sdf);
+        for (int k = k0; k < k1; ++k) launch_fwd<T, eio::M_RECOMP>(run, ntile, k, 0, seg_begin(f, j), dhi, d_sdf);
     };
     const bool fresh = f.ring_fresh;
     f.ring_fresh = false;
     if (!fresh) nrecomp += (long long)(seg_end(f, f.nseg - 1) - seg_begin(f, f.nseg - 1)) * sub;
     run_loop(c, [&]() {
+        if (rtwo) fill0<T>(c, fstate, long(fstate_elems));
         // a store used before no longer holds the last segment in its ring
         if (!fresh) recompute(f.nseg - 1);
         int p = 0;

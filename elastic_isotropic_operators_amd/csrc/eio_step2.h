@@ -60,8 +60,9 @@ __device__ __forceinline__ void bst_pol(rsrc_t r, T v, unsigned voff, unsigned s
 // M_BSCAT): no point sources; every cell of both steps adds the secondary source of slots
 // ss[sspar] (coarse its) and ss[sspar^1] (its+1) with the weights of its own fine step
 // (kernelsGpuElastic.cu:1363-1368); both steps lie in one coarse interval, so they share the
-// two slots.
-template <typename T, int R2, int WX, int OCC, int NTS, bool DEC, bool FINE = false, bool SS = false>
+// two slots.  NOREC: no receiver recording (the segment recompute of the FWI gradient).
+template <typename T, int R2, int WX, int OCC, int NTS, bool DEC, bool FINE = false, bool SS = false,
+          bool NOREC = false>
 __global__ __launch_bounds__(64 * WX) __attribute__((amdgpu_waves_per_eu(OCC)))
 void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict__ tiles,
                  const Entry* __restrict__ ents, int ntiles, int k, int par, int dlo, int dhi,
@@ -255,7 +256,7 @@ void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
     __syncthreads();
 
     const bool has_src = !SS && t.src_e > t.src_b;
-    const bool has_rec = !FINE && t.rec_e > t.rec_b;
+    const bool has_rec = !FINE && !NOREC && t.rec_e > t.rec_b;
     // ---- step-1 sources on the whole step-1 region (kernelsGpuElastic.cu:172-189)
     if (has_src) {
         for (int e = t.src_b + threadIdx.x; e < t.src_e; e += NT) {
