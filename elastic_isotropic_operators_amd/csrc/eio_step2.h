@@ -1,4 +1,7 @@
-// eio_step2.h -- two fine steps per launch for the NL forward (temporal blocking).
+// eio_step2.h -- two fine steps per launch (temporal blocking): k_step2_fwd for the NL
+// forward, the FWI forward and its segment recompute (DEC) and the Born source (FINE) and
+// scattered (SS) passes; k_step2_adj for the Born receiver pass / FWI gradient and the NL
+// adjoint.
 //
 // The single-step kernel moves 80 B per FP32 cell-update (old, current, media read; new
 // written).  Two leapfrog steps u1 = d(d u_old + A u_cur + s_k), u2 = d(d u_cur + A u1 + s_k+1)
@@ -17,9 +20,9 @@
 //
 // Sources: each tile's list holds every source of the step-1 region (lcell relative to
 // (z0-4, x0-4)), injected into its own copy of u1; only owned cells are stored.  Receivers
-// are owned-only.  Arithmetic per cell is that of k_step_fwd in mode M_NL.
+// are owned-only.  Arithmetic per cell is that of k_step_fwd in the matching mode.
 //
-// DEC (the forward of the FWI gradient with its whole coarse record): both steps lie in the
+// DEC (the FWI forward and the Born source pass, which keep a coarse record): both steps lie in the
 // same coarse interval (pairs start at even fine steps and the fine sub-sampling is even), so
 // their Vx, Vz contributions to slots its and its+1 (kernelsGpuElastic.cu:1337-1346) are
 // summed in registers and each slot is read-modified-written once per launch, with the
