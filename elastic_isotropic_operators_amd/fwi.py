@@ -344,7 +344,9 @@ class ElasticFwi:
         """Shots per group whose full coarse Vx/Vz storage (2 nz nx nts values per shot) fits in
         `margin` of the free device memory (0 when not even one shot fits)."""
         free, _ = torch.cuda.mem_get_info(self.device)
-        per_shot = (2 * self.nts + 12) * self.nz * self.nx * torch.tensor([], dtype=self.dtype).element_size()
+        # the record (2 nts fields) + state levels (4 x 5), recompute state (4 x 5), adjoint-field
+        # slots (2 x 5), accumulators (5) and traces, rounded up
+        per_shot = (2 * self.nts + 60) * self.nz * self.nx * torch.tensor([], dtype=self.dtype).element_size()
         return int(margin * free // per_shot)
 
     def misfit_and_gradient(self, model, d_obs, components=None, group=None):
