@@ -182,6 +182,76 @@ def run_fwi_leg(args):
     raise RuntimeError("FWI leg failed (exit %d)" % r.returncode)
 
 
+def born_child(args):
+    """Born leg (child process): workload C1 (the C0 grid and background, 8 shots at 50-cell
+    spacing, the seeded 5 % perturbation) through the drop-in operators (parfile ->
+    BornOpInit{Float,Double} -> BornElasticShotsGpu), as a reference script runs them: Born
+    forward of the perturbation, then the Born adjoint of that data.  Wall time of each call
+    (each returns synchronised), after one warm-up call; FP32 against the double operator."""
+    import tempfile
+    from elastic_isotropic_operators_amd import Elastic_iso_double_prop as PD
+    from elastic_isotropic_operators_amd import Elastic_iso_float_prop as PF
+    from elastic_isotropic_operators_amd.sep import Hypercube, SepVector, genericIO
+    w = W.gaussian_c1()
+    dm = w["dm"]
+    res = {}
+    with tempfile.TemporaryDirectory() as td:
+        hyp = Hypercube.hypercube(axes=[Hypercube.axis(n=w["nz"], o=w["oz"], d=w["dz"]),
+                                        Hypercube.axis(n=w["nx"], o=w["ox"], d=w["dx"]), Hypercube.axis(n=3)])
+        mod = SepVector.getSepVector(hyp)
+        mod.getNdArray()[...] = w["vvr"]
+        mfile = os.path.join(td, "model.H")
+        genericIO.defaultIO.writeVector(mfile, mod)
+        src = SepVector.getSepVector(Hypercube.hypercube(
+            axes=[Hypercube.axis(n=w["nts"], d=w["dts"]), Hypercube.axis(n=1), Hypercube.axis(n=5)]))
+        src.getNdArray()[...] = w["model"]
+        sfile = os.path.join(td, "sources.H")
+        genericIO.defaultIO.writeVector(sfile, src)
+        par = dict(nts=w["nts"], dts=w["dts"], sub=w["sub"], nz=w["nz"], nx=w["nx"], dz=w["dz"], dx=w["dx"],
+                   fMax=w["fMax"], mod_par=1, zPadMinus=w["zPadMinus"], zPadPlus=w["zPadPlus"],
+                   xPadMinus=w["xPadMinus"], xPadPlus=w["xPadPlus"], nExp=8, zSource=3, xSource=26,
+                   spacingShots=50, nReceiver=200, depthReceiver=1, oReceiver=1, dReceiver=2, iGpu=0,
+                   blockSize=16, elasticParam=mfile, sources=sfile)
+        cargs = ["%s=%s" % kv for kv in par.items()]
+        for P, init, name in ((PF, PF.BornOpInitFloat, "f32"), (PD, PD.BornOpInitDouble, "f64")):
+            (model, data, elastic, parobj, sig, sc, sx, sz, sxz, rc, rx, rz, rxz, _) = init(cargs)
+            op = P.BornElasticShotsGpu(model, data, elastic, parobj, sig, sc, sx, sz, sxz, rc, rx, rz, rxz)
+            model.getNdArray()[...] = dm
+            op.forward(False, model, data)
+            t0 = time.perf_counter()
+            op.forward(False, model, data)
+            tf = time.perf_counter() - t0
+            img = model.clone()
+            op.adjoint(False, img, data)
+            t0 = time.perf_counter()
+            op.adjoint(False, img, data)
+            ta = time.perf_counter() - t0
+            res[name] = (tf, ta, data.getNdArray().astype(np.float64), img.getNdArray().astype(np.float64))
+            log("born %s: forward %.3f s, adjoint %.3f s" % (name, tf, ta))
+    ntw = (w["nts"] - 1) * 2 * w["sub"] + 1
+    cells = 8 * (w["nz"] - 8) * (w["nx"] - 8) * (ntw - 1)   # one pass of the 8 shots
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    tf, ta, d32, i32 = res["f32"]
+    out = {"workload": w["name"], "shots": 8, "dtype": "f32", "cell_updates_per_pass": cells,
+           "forward_s": round(tf, 4), "adjoint_s": round(ta, 4),
+           "forward_gcell_per_s": round(2 * cells / tf / 1e9, 2),   # source + scattered pass
+           "adjoint_gcell_per_s": round(2 * cells / ta / 1e9, 2),   # source + receiver pass
+           "f64_forward_s": round(res["f64"][0], 4), "f64_adjoint_s": round(res["f64"][1], 4),
+           "rel_l2_data_f32_vs_f64": rel(d32, res["f64"][2]), "rel_l2_image_f32_vs_f64": rel(i32, res["f64"][3]),
+           "timing": "wall time of the operator call (host interpolation and transfers included)"}
+    print("BORN_JSON " + json.dumps(out), flush=True)
+
+
+def run_born_leg(args):
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--born-child"]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=600)
+    for line in r.stdout.splitlines():
+        if line.startswith("BORN_JSON "):
+            return json.loads(line[len("BORN_JSON "):])
+    raise RuntimeError("Born leg failed (exit %d)" % r.returncode)
+
+
 ABC_WEIGHTS = os.path.join(ROOT, "elastic_isotropic_operators_amd", "data", "abc_strip10.npz")
 
 
@@ -237,11 +307,16 @@ def main():
                          "adjoint variant, as steps:variant")
     ap.add_argument("--fwi-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--abc", type=int, default=1, help="run the learned-boundary leg (needs its weights)")
+    ap.add_argument("--born", type=int, default=1, help="run the Born operator leg (workload C1)")
+    ap.add_argument("--born-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--abc-scale", type=float, default=1.0)
     ap.add_argument("--abc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.fwi_child:
         fwi_child(args)
+        return
+    if args.born_child:
+        born_child(args)
         return
     if args.abc_child:
         abc_child(args)
@@ -356,6 +431,10 @@ def main():
         t0 = time.perf_counter()
         res["fwi_gradient"] = run_fwi_leg(args)
         log("fwi leg %.1f s" % (time.perf_counter() - t0))
+    if args.born:
+        t0 = time.perf_counter()
+        res["born"] = run_born_leg(args)
+        log("born leg %.1f s" % (time.perf_counter() - t0))
     if args.abc and os.path.exists(ABC_WEIGHTS):
         t0 = time.perf_counter()
         res["learned_boundary"] = run_abc_leg(args)

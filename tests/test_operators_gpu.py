@@ -114,6 +114,30 @@ def test_born_forward_adjoint_vs_oracle(tmp_path):
     assert rel_l2(mm.getNdArray(), ref) < 1e-5, rel_l2(mm.getNdArray(), ref)
 
 
+def test_c1_born_forward_and_adjoint_vs_fp64_oracle(tmp_path):
+    """Workload C1 at full size (the C0 grid, 8 shots at 50-cell spacing, the seeded 5 %
+    perturbation) through the float Born operator: the Born data of the perturbation against
+    the FP64 oracle (1e-5) and the Born adjoint of that data (the imaging condition) against
+    the oracle's (1e-4)."""
+    w = W.gaussian_c1()
+    dm = w["dm"]
+    args, geo = operator_case(tmp_path, w, nExp=8, zSource=3, xSource=26, spacing=50, nRec=200,
+                              depthRec=1, oRec=1, dRec=2)
+    op, model, data, elastic, sig = _born(PF, args, PF.BornOpInitFloat)
+    model.getNdArray()[...] = dm
+    op.forward(False, model, data)
+    g = oracle_grid(w)
+    ref = np.stack([O.born_fwd_shot(g, w["elastic"], geo["sz"][e], geo["sx"][e], w["model"], geo["rz"],
+                                    geo["rx"], dm) for e in range(8)])
+    assert np.linalg.norm(ref) > 0
+    assert rel_l2(data.getNdArray(), ref) < 1e-5, rel_l2(data.getNdArray(), ref)
+    img = model.clone()
+    op.adjoint(False, img, data)
+    ref_img = O.born_adj_shots(g, w["elastic"], geo["sz"], geo["sx"], w["model"], geo["rz"], geo["rx"],
+                               data.getNdArray().astype(np.float64))
+    assert rel_l2(img.getNdArray(), ref_img) < 1e-4, rel_l2(img.getNdArray(), ref_img)
+
+
 def test_born_dot_test_fp64(tmp_path):
     w = _small(nts=100, seed=7)
     args, _ = operator_case(tmp_path, w, nExp=2)
