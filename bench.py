@@ -8,15 +8,21 @@ one full batched propagation of the 64 shots.  Inputs (media, source traces) are
 in HBM before the timed region; outputs stay in HBM.
 
 FWI leg (fwi_gradient, s/shot): the 32-shot elastic FWI gradient (600x1592 padded grid,
-nts=3000, sub=2) through the torch autograd layer with checkpointed forward wavefields,
-plus the relative L2 distance of the FP32 gradient from the FP64 gradient on two of the
-shots.  It runs in a child process because torch must be imported before the propagator
-library is loaded (both then share torch's HIP runtime), while the headline leg and its
-profiles use the system runtime without torch.
+nts=3000, sub=2) through the torch FWI engine (ElasticFwi.misfit_and_gradient: shot groups
+sized to HBM keep their whole coarse record, two fine steps per launch in the forward and the
+receiver pass), plus the relative L2 distance of the FP32 gradient from the FP64 gradient on
+two of the shots.  It runs in a child process because torch must be imported before the
+propagator library is loaded (both then share torch's HIP runtime), while the headline leg and
+its profiles use the system runtime without torch.
+
+Born leg (born): workload C1 (8 shots on the 280x472 grid) through the drop-in float and
+double Born operators, forward and adjoint wall times and FP32 vs FP64.  Learned-boundary
+leg (learned_boundary): reflection energy and device time of the plain and learned absorbing
+boundaries on the 1000x3000 model (needs the committed weights).
 
 Prints ONE JSON line with GCell/s (cell-update = one update-region cell for one fine step),
 the HBM roofline of the fused step kernel (80 algorithmic bytes per FP32 cell-update,
-timed with HIP events on the kernel's stream), the FWI leg and a CPU baseline: the FP32
+timed with HIP events on the kernel's stream), the legs and a CPU baseline: the FP32
 OpenMP oracle on a bounded sample of the same workload on this host's cores.
 """
 import argparse
