@@ -101,9 +101,16 @@ def cpu_baseline(shots_w, target_s=12.0):
     rate = cells / max(el, 1e-9)
     nts = int(min(w["nts"], max(20, target_s * rate / ((w["nz"] - 8) * (w["nx"] - 8) * 2 * w["sub"]))))
     el, cells = run(nts)
+    model = ""
+    try:  # the host's CPU model string (as lscpu prints it)
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
     return dict(value=cells / el / 1e9, unit="GCell/s", cores=O.num_threads(), kind="port",
                 sample="shot 0 of the workload (%dx%d), first %d of %d coarse samples, FP32 oracle "
-                       "with flush-to-zero, %.1f s" % (w["nz"], w["nx"], nts, w["nts"], el))
+                       "with flush-to-zero, %.1f s" % (w["nz"], w["nx"], nts, w["nts"], el),
+                host_cpu=model, host_nproc=os.cpu_count())
 
 
 def fwi_child(args):
