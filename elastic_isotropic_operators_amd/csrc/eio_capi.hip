@@ -1085,7 +1085,8 @@ void born_forward(Ctx* c, int nshot, const eio_shot* shots, const void* const dm
     }
     const void* dmp[5] = {b, b + m.n, b + 2 * m.n, b + 3 * m.n, b + 4 * m.n};
     const int kmax = m.ntw - 1, sub = m.g.sub_fd, nts = m.g.nts;
-    const bool src2 = sizeof(T) == 4 && c->time_block == 2 && kmax % 2 == 0;
+    // pairs must stay inside a coarse interval (decimation, secondary-source slots): even sub_fd
+    const bool src2 = sizeof(T) == 4 && c->time_block == 2 && sub % 2 == 0;
     Run<T> run = prepare<T>(c, nshot, shots, Op::BORN_FWD, flags, dmp, nullptr, 0, false, false, src2);
     const int ntile = int(run.plan.tiles.size());
     long long nstep = 0;
@@ -1125,7 +1126,8 @@ void born_adjoint(Ctx* c, int nshot, const eio_shot* shots, void* const dm[5], i
     const ModelBase& m = get_model(c, shots[0].model);
     const int kmax = m.ntw - 1, sub = m.g.sub_fd, nts = m.g.nts;
     const bool two = sizeof(T) == 4 && c->adj_time_block == 2 && sub % 2 == 0;
-    const bool src2 = sizeof(T) == 4 && c->time_block == 2 && kmax % 2 == 0;
+    // pairs must stay inside a coarse interval (decimation, secondary-source slots): even sub_fd
+    const bool src2 = sizeof(T) == 4 && c->time_block == 2 && sub % 2 == 0;
     Run<T> run = prepare<T>(c, nshot, shots, Op::BORN_ADJ, flags, nullptr, nullptr, 0, false, two, src2);
     const int ntile = int(run.plan.tiles.size());
     timing_begin<T>(c);

@@ -389,6 +389,32 @@ def test_two_step_born_source_pass_matches_oracle(v):
     assert rel_l2(two_a, one_a) < 1e-5, rel_l2(two_a, one_a)
 
 
+def test_born_with_odd_fine_subsampling_and_time_block_two():
+    """A model with an odd fine sub-sampling (sub_fd 3) and an even step count: the Born passes
+    whose pairs would straddle coarse intervals take one step per launch, so forward and
+    adjoint still equal the oracle with the two-step setting on."""
+    w = W.small(nz=60, nx=70, nts=51, sub=1, seed=150)
+    g = O.grid(w["nz"], w["nx"], w["nts"], 3, w["dz"], w["dx"], w["dts"], W.min_pad(w),
+               w["alphaCos"], w["oz"], w["ox"])
+    med = O.media(g, w["elastic"], np.float64)
+    rng = np.random.default_rng(151)
+    spos = scattered_positions(rng, g.nz, g.nx, 8)
+    rpos = scattered_positions(rng, g.nz, g.nx, 30)
+    src_fine, dm5 = _born_inputs(w, g, spos, 152)
+    dat = random_traces(rng, rpos, g.nts, np.float64)
+    ref_f = O.born_fwd_reg(g, med, spos, src_fine, dm5, rpos, np.float64)
+    ref_a = O.born_adj_reg(g, med, spos, src_fine, rpos, dat, np.float64)
+    ctx = _capi.Context(0, np.float32)
+    ctx.set_time_block(2, 4)
+    ctx.set_adjoint_time_block(2, 7)
+    mid = _model(ctx, w, g, med)
+    out_f = _gpu_born_fwd(ctx, mid, spos, rpos, src_fine, dm5, g.nts, np.float32)
+    out_a = _gpu_born_adj(ctx, mid, spos, rpos, src_fine, dat, g.nx, g.nz, np.float32)
+    for k in range(5):
+        assert rel_l2(out_f[k], ref_f[k]) < 1e-5, (k, rel_l2(out_f[k], ref_f[k]))
+    assert rel_l2(out_a, ref_a) < 1e-4, rel_l2(out_a, ref_a)
+
+
 def test_two_step_born_adjoint_dot_product():
     """<B m, d> = <m, B^T d> with the two-step receiver pass (FP32 bar 1e-5)."""
     w, g, med, spos, rpos = _case(nz=120, nx=100, nts=80, sub=1, seed=85)
