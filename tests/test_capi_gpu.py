@@ -340,6 +340,26 @@ def test_two_step_nl_adjoint_matches_oracle(va):
         assert rel_l2(two[k], one[k]) < 1e-5, (va, k, rel_l2(two[k], one[k]))
 
 
+def test_two_step_nl_adjoint_pairs_across_coarse_intervals():
+    """sub_fd 3 and an even step count (nts 51): pairs of reverse steps straddle coarse
+    intervals, which the NL adjoint allows (each step injects and extracts at its own time)."""
+    w = W.small(nz=60, nx=70, nts=51, sub=1, seed=160)
+    g = O.grid(w["nz"], w["nx"], w["nts"], 3, w["dz"], w["dx"], w["dts"], W.min_pad(w),
+               w["alphaCos"], w["oz"], w["ox"])
+    med = O.media(g, w["elastic"], np.float64)
+    rng = np.random.default_rng(161)
+    spos = scattered_positions(rng, g.nz, g.nx, 20)
+    rpos = scattered_positions(rng, g.nz, g.nx, 30)
+    dat = random_traces(rng, rpos, g.nts, np.float64)
+    ref = O.nl_adj_reg(g, med, spos, rpos, dat, np.float64)
+    ctx = _capi.Context(0, np.float32)
+    ctx.set_adjoint_time_block(2, 7)
+    mid = _model(ctx, w, g, med)
+    out = _gpu_nl_adj(ctx, mid, spos, rpos, dat, g.nts, np.float32)
+    for k in range(5):
+        assert rel_l2(out[k], ref[k]) < 1e-5, (k, rel_l2(out[k], ref[k]))
+
+
 def test_two_step_nl_dot_product():
     """<F m, d> = <m, F^T d> with two fine steps per launch both ways (FP32 bar 1e-5)."""
     w, g, med, spos, rpos = _case(nz=120, nx=100, nts=80, sub=1, seed=97)
