@@ -13,6 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libeio_hip.so")
 
 EIO_DEVICE_PTRS = 1
+VARIANT_AUTO = -1  # EIO_VARIANT_AUTO: two-step tile width chosen per call
 GRID_OF_COMP = (1, 2, 0, 0, 3)
 
 
@@ -193,7 +194,8 @@ class Context:
         return out
 
     def set_time_block(self, steps, variant=0):
-        """Fine steps per launch of the FP32 NL forward (1 or 2) and the two-step variant."""
+        """Fine steps per launch of the FP32 forward passes (1 or 2) and the two-step variant;
+        VARIANT_AUTO chooses the tile width per call from the batch's size."""
         check(lib().eio_ctx_set_time_block(self.h, int(steps), int(variant)))
 
     def set_adjoint_time_block(self, steps, variant=0):

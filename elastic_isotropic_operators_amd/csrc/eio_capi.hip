@@ -131,7 +131,7 @@ struct Ctx {
     int use_graph = 0;
     int variant = 1;  // best measured FP32 variant on the 64-shot layered batch
     int time_block = 1;  // fine steps per launch of the FP32 NL forward (1 or 2)
-    int variant2 = 0;    // two-step kernel variant
+    int variant2 = 0;    // two-step kernel variant, or EIO_VARIANT_AUTO (chosen per call)
     int adj_time_block = 1;  // reverse fine steps per launch of the FP32 Born adjoint (1 or 2)
     int variant2a = 0;       // two-step adjoint kernel variant
     int step_limit = 0;  // > 0: NL loops stop after this many fine steps (tuning runs)
@@ -332,9 +332,9 @@ void build_entries2(const ModelBase& m, const int* const pos[4], const int nreg[
 enum class PlanKind { ONE, FWD2, ADJ2 };
 
 template <typename T>
-Plan make_plan(Ctx* c, int nshot, const eio_shot* shots, Op op, PlanKind kind = PlanKind::ONE) {
+Plan make_plan(Ctx* c, int nshot, const eio_shot* shots, Op op, PlanKind kind = PlanKind::ONE, int v2 = 0) {
     const bool two_step = kind != PlanKind::ONE;
-    const int TX = kind == PlanKind::FWD2   ? F32_VARIANTS2[c->variant2].R2 * F32_VARIANTS2[c->variant2].WX
+    const int TX = kind == PlanKind::FWD2   ? F32_VARIANTS2[v2].R2 * F32_VARIANTS2[v2].WX
                    : kind == PlanKind::ADJ2 ? F32_VARIANTS2A[c->variant2a].R2 * F32_VARIANTS2A[c->variant2a].WX
                                             : tile_x<T>(c->variant);
     const int TZ = two_step ? eio::TZ2 : eio::TZ;
@@ -388,6 +388,7 @@ Plan make_plan(Ctx* c, int nshot, const eio_shot* shots, Op op, PlanKind kind = 
 template <typename T>
 struct Run {
     Ctx* c;
+    int v2 = 0;  // two-step forward variant of this call's FWD2 plans
     Plan plan;
     std::vector<eio::ShotDev<T>> sd;
     eio::ShotDev<T>* d_shots = nullptr;
@@ -411,6 +412,23 @@ struct Run {
     std::vector<Out> outs;
 };
 
+// The two-step forward variant of a call.  EIO_VARIANT_AUTO: a batch that fills the chip with
+// fewer than AUTO_SMALL_TILES tiles of 56 x 32 (four per CU) takes 56 x 8 tiles (variant 3),
+// which spread its launches over every CU; larger batches take 56 x 32 (variant 4), whose
+// wider tiles re-read less halo.  Measured: 8 Born shots of 280 x 472 (600 wide tiles) run
+// their source and scattered passes 11 % faster on 56 x 8; the 64-shot layered batch runs
+// 19 % slower on it.
+constexpr int AUTO_SMALL_TILES = 1024;
+int variant2_for(Ctx* c, int nshot, const eio_shot* shots) {
+    if (c->variant2 != EIO_VARIANT_AUTO) return c->variant2;
+    long tiles = 0;
+    for (int i = 0; i < nshot; ++i) {
+        const ModelBase& m = get_model(c, shots[i].model);
+        tiles += long((m.g.nz - 2 * eio::FAT + eio::TZ2 - 1) / eio::TZ2) * ((m.g.nx - 2 * eio::FAT + 31) / 32);
+    }
+    return tiles < AUTO_SMALL_TILES ? 3 : 4;
+}
+
 // Builds the plan, the per-shot descriptors and the device tables of one call, uploads
 // input traces and zeroes the state and the outputs.  FWI calls bring their own coarse
 // storage ring (`ring`, `wcap` slices per component and shot).  `two_step`: the NL forward
@@ -424,9 +442,10 @@ Run<T> prepare(Ctx* c, int nshot, const eio_shot* shots, Op op, int flags,
     if (!shots) fail("shots is NULL");
     Run<T> run;
     run.c = c;
-    run.plan = make_plan<T>(c, nshot, shots, op, two_step ? PlanKind::FWD2 : PlanKind::ONE);
+    run.v2 = variant2_for(c, nshot, shots);
+    run.plan = make_plan<T>(c, nshot, shots, op, two_step ? PlanKind::FWD2 : PlanKind::ONE, run.v2);
     if (adj2) run.plan2 = make_plan<T>(c, nshot, shots, op, PlanKind::ADJ2);
-    if (src2) run.plan3 = make_plan<T>(c, nshot, shots, op, PlanKind::FWD2);
+    if (src2) run.plan3 = make_plan<T>(c, nshot, shots, op, PlanKind::FWD2, run.v2);
     const int nlev = (two_step || adj2 || src2) ? 4 : 2;  // state levels per shot
     const bool born = (op == Op::BORN_FWD || op == Op::BORN_ADJ);
     const bool fwi = (op == Op::FWI_FWD || op == Op::FWI_GRAD);
@@ -642,7 +661,7 @@ void launch_fwd2(Run<float>& run, int ntile, int k, int par, int dlo = 0, int dh
 #define EIO_L2(R_, WX_, OCC_, NT_)                                                                         \
     hipLaunchKernelGGL((eio::k_step2_fwd<float, R_, WX_, OCC_, NT_, DEC, FINE, SS, NOREC>), dim3(ntile),     \
                        dim3(64 * WX_), 0, s, sh, tt, te, ntile, k, par, dlo, dhi, sspar)
-    switch (run.c->variant2) {
+    switch (run.v2) {
         case 0: EIO_L2(2, 8, 2, 0); break;
         case 1: EIO_L2(2, 8, 4, 0); break;
         case 2: EIO_L2(4, 4, 2, 0); break;
@@ -1517,7 +1536,8 @@ int eio_ctx_num_variants(void) { return N_F32_VARIANTS; }
 int eio_ctx_set_time_block(eio_ctx* ctx, int steps, int variant) {
     return guarded([&]() {
         if (steps != 1 && steps != 2) fail("time block must be 1 or 2 fine steps");
-        if (variant < 0 || variant >= N_F32_VARIANTS2) fail("two-step variant out of range");
+        if (variant != EIO_VARIANT_AUTO && (variant < 0 || variant >= N_F32_VARIANTS2))
+            fail("two-step variant out of range");
         Ctx* c = as_ctx(ctx);
         c->time_block = steps;
         c->variant2 = variant;

@@ -278,6 +278,63 @@ def test_two_step_nl_forward_matches_oracle(v, sub_fd):
         assert rel_l2(out[k], ref[k]) < 1e-5, (v, k, rel_l2(out[k], ref[k]))
 
 
+def test_auto_time_block_variant_matches_explicit():
+    """EIO_VARIANT_AUTO chooses the two-step tile width per call: a one-shot 150 x 130 batch
+    (6 tiles of 56 x 32, under the threshold of 1024) equals the oracle and the explicit 56 x 8
+    variant; a batch of 4 shots of 600 x 1600 (2200 tiles) equals the explicit 56 x 32
+    variant, for the NL forward and the Born forward."""
+    w = W.small(nz=150, nx=130, nts=30, sub=1, seed=170)
+    g = oracle_grid(w)
+    med = O.media(g, w["elastic"], np.float64)
+    rng = np.random.default_rng(171)
+    spos = scattered_positions(rng, g.nz, g.nx, 20)
+    rpos = scattered_positions(rng, g.nz, g.nx, 40)
+    src = random_traces(rng, spos, g.nts, np.float64)
+    ref = O.nl_fwd_reg(g, med, spos, src, rpos, np.float64)
+    ctx = _capi.Context(0, np.float32)
+    mid = _model(ctx, w, g, med)
+    out = {}
+    for v in (3, _capi.VARIANT_AUTO):
+        ctx.set_time_block(2, v)
+        out[v] = _gpu_nl_fwd(ctx, mid, spos, rpos, src, g.nts, np.float32)
+    for k in range(5):
+        assert rel_l2(out[_capi.VARIANT_AUTO][k], ref[k]) < 1e-5, (k, rel_l2(out[_capi.VARIANT_AUTO][k], ref[k]))
+        assert rel_l2(out[_capi.VARIANT_AUTO][k], out[3][k]) < 1e-6
+    # above the threshold: 4 shots of 600 x 1600, a few coarse samples
+    wb = W.small(nz=600, nx=1600, nts=6, sub=1, seed=172)
+    gb = oracle_grid(wb)
+    medb = O.media(gb, wb["elastic"], np.float64)
+    midb = _model(ctx, wb, gb, medb)
+    ntw = (gb.nts - 1) * gb.sub + 1
+    shots, keep, recs = {}, [], {}
+    dm5 = rng.standard_normal((5, gb.nx, gb.nz)) * 1e-3
+    for v in (4, _capi.VARIANT_AUTO):
+        ctx.set_time_block(2, v)
+        for kind in ("nl", "born"):
+            rng_v = np.random.default_rng(173)
+            batch, rec_all = [], []
+            for e in range(4):
+                sp = scattered_positions(rng_v, gb.nz, gb.nx, 10)
+                rp = scattered_positions(rng_v, gb.nz, gb.nx, 30)
+                tr = random_traces(rng_v, sp, ntw if kind == "born" else gb.nts, np.float32)
+                rec = [np.zeros((rp[GRID_OF_COMP[k]].size, gb.nts), np.float32) for k in range(5)]
+                shot, kp = ctx.make_shot(midb, sp, rp, tr, rec)
+                batch.append(shot)
+                keep.append(kp)
+                rec_all.append(rec)
+            if kind == "nl":
+                ctx.nl_forward(batch)
+            else:
+                ctx.born_forward(batch, dm5)
+            recs[(v, kind)] = rec_all
+    for kind in ("nl", "born"):
+        for e in range(4):
+            for k in range(5):
+                a, b = recs[(_capi.VARIANT_AUTO, kind)][e][k], recs[(4, kind)][e][k]
+                assert np.linalg.norm(b) > 0
+                assert rel_l2(a, b) < 1e-6, (kind, e, k, rel_l2(a, b))
+
+
 @pytest.mark.parametrize("va", range(10))
 @pytest.mark.parametrize("sub_fd", [2, 4, 3])
 def test_two_step_born_adjoint_matches_oracle(va, sub_fd):
