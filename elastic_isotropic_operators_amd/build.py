@@ -39,7 +39,8 @@ def _pybind_flags():
     import pybind11
     inc = [pybind11.get_include(), sysconfig.get_paths()["include"], os.path.join(ROOT, "include"),
            os.path.join(CSRC, "host")]
-    return ["-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden"] + ["-I" + i for i in inc]
+    # OpenMP: the host-side trace interpolation runs per (shot, component) on all host cores
+    return ["-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden", "-fopenmp"] + ["-I" + i for i in inc]
 
 
 def build_pybind():

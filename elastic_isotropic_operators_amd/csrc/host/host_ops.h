@@ -532,29 +532,25 @@ class NonlinearShots {
     void forward(bool add, const T* model, int nModelShots, int nSrc, T* data, int nRec, bool save = false) {
         const int nts = fd_->nts;
         if (nModelShots != 1 && nModelShots != nExp_) throw std::runtime_error("model shot axis must be 1 or nExp");
-        std::vector<eio_shot> shots(nExp_);
-        std::vector<std::vector<T>> srcReg(size_t(nExp_) * 5), recReg(size_t(nExp_) * 5);
-        for (int e = 0; e < nExp_; ++e) {
-            const InterpVec& S = src_[e];
-            const InterpVec& Rv = rec_.size() == 1 ? rec_[0] : rec_[e];
-            check_counts(S, nSrc, Rv, nRec);
-            const T* m = model + size_t(nModelShots == 1 ? 0 : e) * 5 * nSrc * nts;
-            eio_shot& s = shots[e];
-            std::memset(&s, 0, sizeof(s));
-            s.model = model_;
-            fill_positions(s, S, Rv);
+        std::vector<eio_shot> shots = shot_table();
+        for (int e = 0; e < nExp_; ++e) check_counts(src_[e], nSrc, rec_of(e), nRec);
+        std::vector<std::vector<T>>& srcReg = srcReg_;
+        std::vector<std::vector<T>>& recReg = recReg_;
+        // per (shot, component): disjoint buffers, so the interpolation runs on all host cores
+#pragma omp parallel for collapse(2) schedule(dynamic)
+        for (int e = 0; e < nExp_; ++e)
             for (int k = 0; k < 5; ++k) {
-                const SpaceInterp& ip = *S[GRID_OF_COMP[k]];
+                const T* m = model + size_t(nModelShots == 1 ? 0 : e) * 5 * nSrc * nts;
+                const SpaceInterp& ip = *src_[e][GRID_OF_COMP[k]];
                 std::vector<T>& r = srcReg[size_t(e) * 5 + k];
-                r.assign(size_t(ip.nReg()) * nts, T(0));
+                r.resize(size_t(ip.nReg()) * nts);
                 ip.adjoint(r.data(), m + size_t(k) * nSrc * nts);
                 scale_sources(*fd_, ip, k, r.data());
                 std::vector<T>& d = recReg[size_t(e) * 5 + k];
-                d.assign(size_t(Rv[GRID_OF_COMP[k]]->nReg()) * nts, T(0));
-                s.src_tr[k] = r.data();
-                s.rec_tr[k] = d.data();
+                d.assign(size_t(rec_of(e)[GRID_OF_COMP[k]]->nReg()) * nts, T(0));
+                shots[e].src_tr[k] = r.data();
+                shots[e].rec_tr[k] = d.data();
             }
-        }
         if (save_shot(save) >= 0) {
             wavefield_.assign(size_t(nts) * 5 * fd_->ncell(), T(0));
             check_eio(eio_nl_forward_wavefield(dev_->ctx, nExp_, shots.data(), 0, save_shot(save), wavefield_.data()));
@@ -562,38 +558,32 @@ class NonlinearShots {
             check_eio(eio_nl_forward(dev_->ctx, nExp_, shots.data(), 0));
         }
         if (!add) std::fill(data, data + size_t(nExp_) * 5 * nRec * nts, T(0));
-        for (int e = 0; e < nExp_; ++e) {
-            const InterpVec& Rv = rec_.size() == 1 ? rec_[0] : rec_[e];
+#pragma omp parallel for collapse(2) schedule(dynamic)
+        for (int e = 0; e < nExp_; ++e)
             for (int k = 0; k < 5; ++k)
-                Rv[GRID_OF_COMP[k]]->forward_add(recReg[size_t(e) * 5 + k].data(),
-                                                 data + (size_t(e) * 5 + k) * nRec * nts);
-        }
+                rec_of(e)[GRID_OF_COMP[k]]->forward_add(recReg[size_t(e) * 5 + k].data(),
+                                                        data + (size_t(e) * 5 + k) * nRec * nts);
     }
 
     void adjoint(bool add, T* model, int nModelShots, int nSrc, const T* data, int nRec, bool save = false) {
         const int nts = fd_->nts;
         if (nModelShots != 1 && nModelShots != nExp_) throw std::runtime_error("model shot axis must be 1 or nExp");
-        std::vector<eio_shot> shots(nExp_);
-        std::vector<std::vector<T>> srcReg(size_t(nExp_) * 5), recReg(size_t(nExp_) * 5);
-        for (int e = 0; e < nExp_; ++e) {
-            const InterpVec& S = src_[e];
-            const InterpVec& Rv = rec_.size() == 1 ? rec_[0] : rec_[e];
-            check_counts(S, nSrc, Rv, nRec);
-            eio_shot& s = shots[e];
-            std::memset(&s, 0, sizeof(s));
-            s.model = model_;
-            fill_positions(s, S, Rv);
+        std::vector<eio_shot> shots = shot_table();
+        for (int e = 0; e < nExp_; ++e) check_counts(src_[e], nSrc, rec_of(e), nRec);
+        std::vector<std::vector<T>>& srcReg = srcReg_;
+        std::vector<std::vector<T>>& recReg = recReg_;
+#pragma omp parallel for collapse(2) schedule(dynamic)
+        for (int e = 0; e < nExp_; ++e)
             for (int k = 0; k < 5; ++k) {
-                const SpaceInterp& rp = *Rv[GRID_OF_COMP[k]];
+                const SpaceInterp& rp = *rec_of(e)[GRID_OF_COMP[k]];
                 std::vector<T>& d = recReg[size_t(e) * 5 + k];
-                d.assign(size_t(rp.nReg()) * nts, T(0));
+                d.resize(size_t(rp.nReg()) * nts);
                 rp.adjoint(d.data(), data + (size_t(e) * 5 + k) * nRec * nts);
                 std::vector<T>& r = srcReg[size_t(e) * 5 + k];
-                r.assign(size_t(S[GRID_OF_COMP[k]]->nReg()) * nts, T(0));
-                s.src_tr[k] = r.data();
-                s.rec_tr[k] = d.data();
+                r.assign(size_t(src_[e][GRID_OF_COMP[k]]->nReg()) * nts, T(0));
+                shots[e].src_tr[k] = r.data();
+                shots[e].rec_tr[k] = d.data();
             }
-        }
         if (save_shot(save) >= 0) {
             const size_t n = fd_->ncell();
             wavefield_.assign(size_t(nts) * 5 * n, T(0));
@@ -615,15 +605,20 @@ class NonlinearShots {
             check_eio(eio_nl_adjoint(dev_->ctx, nExp_, shots.data(), 0));
         }
         if (!add) std::fill(model, model + size_t(nModelShots) * 5 * nSrc * nts, T(0));
-        for (int e = 0; e < nExp_; ++e) {
-            const InterpVec& S = src_[e];
-            T* m = model + size_t(nModelShots == 1 ? 0 : e) * 5 * nSrc * nts;
-            for (int k = 0; k < 5; ++k) {
-                const SpaceInterp& ip = *S[GRID_OF_COMP[k]];
-                std::vector<T>& r = srcReg[size_t(e) * 5 + k];
-                scale_sources(*fd_, ip, k, r.data());  // same diagonal scaling (nonlinearPropElasticGpu.cpp:406-430)
-                ip.forward_add(r.data(), m + size_t(k) * nSrc * nts);
-            }
+        // one signal for all shots: every shot adds into the same slot, so shots stay in order
+        // and only the components run in parallel
+        const int nGroup = nModelShots == 1 ? 1 : nExp_;
+        for (int e0 = 0; e0 < nExp_; e0 += nGroup) {
+            const int e1 = std::min(nExp_, e0 + nGroup);
+#pragma omp parallel for collapse(2) schedule(dynamic)
+            for (int e = e0; e < e1; ++e)
+                for (int k = 0; k < 5; ++k) {
+                    T* m = model + size_t(nModelShots == 1 ? 0 : e) * 5 * nSrc * nts;
+                    const SpaceInterp& ip = *src_[e][GRID_OF_COMP[k]];
+                    std::vector<T>& r = srcReg[size_t(e) * 5 + k];
+                    scale_sources(*fd_, ip, k, r.data());  // same diagonal scaling (nonlinearPropElasticGpu.cpp:406-430)
+                    ip.forward_add(r.data(), m + size_t(k) * nSrc * nts);
+                }
         }
     }
 
@@ -639,6 +634,20 @@ class NonlinearShots {
     }
 
    private:
+    const InterpVec& rec_of(int e) const { return rec_.size() == 1 ? rec_[0] : rec_[e]; }
+    // the call's shot descriptors (positions, model id; traces are set by the caller) and
+    // one regular-trace buffer per shot and component, kept between calls
+    std::vector<eio_shot> shot_table() {
+        std::vector<eio_shot> shots(nExp_);
+        for (int e = 0; e < nExp_; ++e) {
+            std::memset(&shots[e], 0, sizeof(eio_shot));
+            shots[e].model = model_;
+            fill_positions(shots[e], src_[e], rec_of(e));
+        }
+        srcReg_.resize(size_t(nExp_) * 5);
+        recReg_.resize(size_t(nExp_) * 5);
+        return shots;
+    }
     // shot whose wavefield a saving call keeps, or -1
     int save_shot(bool save) const {
         if (!save || par_.getInt("saveWavefield", 0) == 0) return -1;
@@ -673,6 +682,7 @@ class NonlinearShots {
     std::unique_ptr<FdParam<T>> fd_;
     int model_ = -1;
     std::vector<T> wavefield_;
+    std::vector<std::vector<T>> srcReg_, recReg_;  // regular traces per (shot, component)
 };
 
 // ---------------------------------------------------------------- Born shot operator
@@ -723,23 +733,20 @@ class BornShots {
         stagger_x_adj(fd_->nz, fd_->nx, model + 2 * n, tmp.data());
         stagger_z_adj(fd_->nz, fd_->nx, tmp.data(), sl.data() + 4 * n);
         scale_slices(sl);
-        std::vector<std::vector<T>> fine, recReg;
-        std::vector<eio_shot> shots = build_shots(fine, recReg, nRec, nullptr);
+        std::vector<eio_shot> shots = build_shots(nRec, nullptr);
         const void* dm[5] = {sl.data(), sl.data() + n, sl.data() + 2 * n, sl.data() + 3 * n, sl.data() + 4 * n};
         check_eio(eio_born_forward(dev_->ctx, nExp_, shots.data(), dm, 0));
         if (!add) std::fill(data, data + size_t(nExp_) * 5 * nRec * nts, T(0));
-        for (int e = 0; e < nExp_; ++e) {
-            const InterpVec& Rv = rec_.size() == 1 ? rec_[0] : rec_[e];
+#pragma omp parallel for collapse(2) schedule(dynamic)
+        for (int e = 0; e < nExp_; ++e)
             for (int k = 0; k < 5; ++k)
-                Rv[GRID_OF_COMP[k]]->forward_add(recReg[size_t(e) * 5 + k].data(),
-                                                 data + (size_t(e) * 5 + k) * nRec * nts);
-        }
+                rec_of(e)[GRID_OF_COMP[k]]->forward_add(recReg_[size_t(e) * 5 + k].data(),
+                                                        data + (size_t(e) * 5 + k) * nRec * nts);
     }
 
     void adjoint(bool add, T* model, const T* data, int nRec) {
         const size_t n = fd_->ncell();
-        std::vector<std::vector<T>> fine, recReg;
-        std::vector<eio_shot> shots = build_shots(fine, recReg, nRec, data);
+        std::vector<eio_shot> shots = build_shots(nRec, data);
         std::vector<T> acc(5 * n, T(0));
         void* dm[5] = {acc.data(), acc.data() + n, acc.data() + 2 * n, acc.data() + 3 * n, acc.data() + 4 * n};
         check_eio(eio_born_adjoint(dev_->ctx, nExp_, shots.data(), dm, 0));
@@ -773,40 +780,50 @@ class BornShots {
     }
 
     // Born sources: interpolate, scale and fine-sample (seismicElasticOperator2D.cpp:27-119)
-    std::vector<eio_shot> build_shots(std::vector<std::vector<T>>& fine, std::vector<std::vector<T>>& recReg,
-                                      int nRec, const T* data) {
+    const InterpVec& rec_of(int e) const { return rec_.size() == 1 ? rec_[0] : rec_[e]; }
+
+    // The call's shot descriptors with fine-sampled source traces and regular receiver
+    // traces (interpolated from `data`, or zero), in buffers kept between calls.
+    std::vector<eio_shot> build_shots(int nRec, const T* data) {
         const int nts = fd_->nts, ntw = fd_->ntw;
         std::vector<eio_shot> shots(nExp_);
-        fine.assign(size_t(nExp_) * 5, {});
-        recReg.assign(size_t(nExp_) * 5, {});
         for (int e = 0; e < nExp_; ++e) {
-            const InterpVec& S = src_[e];
-            const InterpVec& Rv = rec_.size() == 1 ? rec_[0] : rec_[e];
             const std::vector<T>& sig = sig_.size() == 1 ? sig_[0] : sig_[e];
-            const int nSrc = S[0]->nDeviceIrreg;
+            const int nSrc = src_[e][0]->nDeviceIrreg;
             if (sig.size() != size_t(5) * nSrc * nts) throw std::runtime_error("source signal size not consistent");
             for (int g = 0; g < 4; ++g)
-                if (Rv[g]->nDeviceIrreg != nRec) throw std::runtime_error("receiver count not consistent with the data vector");
-            eio_shot& s = shots[e];
-            std::memset(&s, 0, sizeof(s));
-            s.model = model_;
-            fill_positions(s, S, Rv);
+                if (rec_of(e)[g]->nDeviceIrreg != nRec)
+                    throw std::runtime_error("receiver count not consistent with the data vector");
+            std::memset(&shots[e], 0, sizeof(eio_shot));
+            shots[e].model = model_;
+            fill_positions(shots[e], src_[e], rec_of(e));
+        }
+        fine_.resize(size_t(nExp_) * 5);
+        recReg_.resize(size_t(nExp_) * 5);
+        // per (shot, component): disjoint buffers, so the interpolation runs on all host cores
+#pragma omp parallel for collapse(2) schedule(dynamic)
+        for (int e = 0; e < nExp_; ++e)
             for (int k = 0; k < 5; ++k) {
-                const SpaceInterp& ip = *S[GRID_OF_COMP[k]];
-                std::vector<T> reg(size_t(ip.nReg()) * nts, T(0));
+                const std::vector<T>& sig = sig_.size() == 1 ? sig_[0] : sig_[e];
+                const int nSrc = src_[e][0]->nDeviceIrreg;
+                const SpaceInterp& ip = *src_[e][GRID_OF_COMP[k]];
+                std::vector<T> reg(size_t(ip.nReg()) * nts);
                 ip.adjoint(reg.data(), sig.data() + size_t(k) * nSrc * nts);
                 scale_sources(*fd_, ip, k, reg.data());
-                std::vector<T>& f = fine[size_t(e) * 5 + k];
-                f.assign(size_t(ip.nReg()) * ntw, T(0));
+                std::vector<T>& f = fine_[size_t(e) * 5 + k];
+                f.resize(size_t(ip.nReg()) * ntw);
                 time_interp_fwd(nts, fd_->dts, fd_->ots, fd_->sub, ip.nReg(), reg.data(), f.data());
-                s.src_tr[k] = f.data();
-                const SpaceInterp& rp = *Rv[GRID_OF_COMP[k]];
-                std::vector<T>& d = recReg[size_t(e) * 5 + k];
-                d.assign(size_t(rp.nReg()) * nts, T(0));
-                if (data) rp.adjoint(d.data(), data + (size_t(e) * 5 + k) * nRec * nts);
-                s.rec_tr[k] = d.data();
+                shots[e].src_tr[k] = f.data();
+                const SpaceInterp& rp = *rec_of(e)[GRID_OF_COMP[k]];
+                std::vector<T>& d = recReg_[size_t(e) * 5 + k];
+                if (data) {
+                    d.resize(size_t(rp.nReg()) * nts);
+                    rp.adjoint(d.data(), data + (size_t(e) * 5 + k) * nRec * nts);
+                } else {
+                    d.assign(size_t(rp.nReg()) * nts, T(0));
+                }
+                shots[e].rec_tr[k] = d.data();
             }
-        }
         return shots;
     }
 
@@ -829,6 +846,7 @@ class BornShots {
     std::unique_ptr<Device> dev_;
     std::unique_ptr<FdParam<T>> fd_;
     int model_ = -1;
+    std::vector<std::vector<T>> fine_, recReg_;  // source (fine) and receiver traces per (shot, component)
 };
 
 }  // namespace eioh
