@@ -791,9 +791,12 @@ class BornShots {
             const std::vector<T>& sig = sig_.size() == 1 ? sig_[0] : sig_[e];
             const int nSrc = src_[e][0]->nDeviceIrreg;
             if (sig.size() != size_t(5) * nSrc * nts) throw std::runtime_error("source signal size not consistent");
-            for (int g = 0; g < 4; ++g)
+            for (int g = 0; g < 4; ++g) {
                 if (rec_of(e)[g]->nDeviceIrreg != nRec)
                     throw std::runtime_error("receiver count not consistent with the data vector");
+                if (src_[e][g]->nt != nts || rec_of(e)[g]->nt != nts)
+                    throw std::runtime_error("nts not consistent with parfile");
+            }
             std::memset(&shots[e], 0, sizeof(eio_shot));
             shots[e].model = model_;
             fill_positions(shots[e], src_[e], rec_of(e));
