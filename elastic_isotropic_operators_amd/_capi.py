@@ -199,8 +199,8 @@ class Context:
         check(lib().eio_ctx_set_time_block(self.h, int(steps), int(variant)))
 
     def set_adjoint_time_block(self, steps, variant=0):
-        """Reverse fine steps per launch of the FP32 Born receiver pass (1 or 2) and the
-        two-step adjoint variant."""
+        """Reverse fine steps per launch of the FP32 adjoint passes (1 or 2) and the two-step
+        adjoint variant; VARIANT_AUTO chooses the tile width per call from the batch's size."""
         check(lib().eio_ctx_set_adjoint_time_block(self.h, int(steps), int(variant)))
 
     @staticmethod

@@ -75,7 +75,7 @@ struct Variant2A {
 };
 constexpr Variant2A F32_VARIANTS2A[] = {{2, 8, 2, 0, 1}, {4, 8, 1, 0, 1}, {2, 8, 4, 0, 1}, {4, 4, 2, 0, 1},
                                         {2, 8, 4, 1, 1}, {2, 8, 2, 1, 1}, {4, 8, 1, 1, 1}, {4, 8, 1, 1, 0},
-                                        {4, 8, 1, 0, 0}, {2, 8, 2, 1, 0}};
+                                        {4, 8, 1, 0, 0}, {2, 8, 2, 1, 0}, {1, 8, 4, 1, 0}, {1, 8, 4, 0, 1}};
 constexpr int N_F32_VARIANTS2A = int(sizeof(F32_VARIANTS2A) / sizeof(F32_VARIANTS2A[0]));
 
 template <typename T>
@@ -133,7 +133,7 @@ struct Ctx {
     int time_block = 1;  // fine steps per launch of the FP32 NL forward (1 or 2)
     int variant2 = 0;    // two-step kernel variant, or EIO_VARIANT_AUTO (chosen per call)
     int adj_time_block = 1;  // reverse fine steps per launch of the FP32 Born adjoint (1 or 2)
-    int variant2a = 0;       // two-step adjoint kernel variant
+    int variant2a = 0;       // two-step adjoint kernel variant, or EIO_VARIANT_AUTO
     int step_limit = 0;  // > 0: NL loops stop after this many fine steps (tuning runs)
     std::map<int, FwiStore> fwi;
     int next_fwi = 1;
@@ -335,7 +335,7 @@ template <typename T>
 Plan make_plan(Ctx* c, int nshot, const eio_shot* shots, Op op, PlanKind kind = PlanKind::ONE, int v2 = 0) {
     const bool two_step = kind != PlanKind::ONE;
     const int TX = kind == PlanKind::FWD2   ? F32_VARIANTS2[v2].R2 * F32_VARIANTS2[v2].WX
-                   : kind == PlanKind::ADJ2 ? F32_VARIANTS2A[c->variant2a].R2 * F32_VARIANTS2A[c->variant2a].WX
+                   : kind == PlanKind::ADJ2 ? F32_VARIANTS2A[v2].R2 * F32_VARIANTS2A[v2].WX
                                             : tile_x<T>(c->variant);
     const int TZ = two_step ? eio::TZ2 : eio::TZ;
     Plan p;
@@ -388,7 +388,8 @@ Plan make_plan(Ctx* c, int nshot, const eio_shot* shots, Op op, PlanKind kind = 
 template <typename T>
 struct Run {
     Ctx* c;
-    int v2 = 0;  // two-step forward variant of this call's FWD2 plans
+    int v2 = 0;   // two-step forward variant of this call's FWD2 plans
+    int v2a = 0;  // two-step adjoint variant of this call's ADJ2 plan
     Plan plan;
     std::vector<eio::ShotDev<T>> sd;
     eio::ShotDev<T>* d_shots = nullptr;
@@ -429,6 +430,18 @@ int variant2_for(Ctx* c, int nshot, const eio_shot* shots) {
     return tiles < AUTO_SMALL_TILES ? 3 : 4;
 }
 
+// The two-step adjoint variant of a call: EIO_VARIANT_AUTO takes 56 x 8 tiles (variant 10)
+// under the same threshold, else 56 x 32 (variant 7).
+int variant2a_for(Ctx* c, int nshot, const eio_shot* shots) {
+    if (c->variant2a != EIO_VARIANT_AUTO) return c->variant2a;
+    long tiles = 0;
+    for (int i = 0; i < nshot; ++i) {
+        const ModelBase& m = get_model(c, shots[i].model);
+        tiles += long((m.g.nz - 2 * eio::FAT + eio::TZ2 - 1) / eio::TZ2) * ((m.g.nx - 2 * eio::FAT + 31) / 32);
+    }
+    return tiles < AUTO_SMALL_TILES ? 10 : 7;
+}
+
 // Builds the plan, the per-shot descriptors and the device tables of one call, uploads
 // input traces and zeroes the state and the outputs.  FWI calls bring their own coarse
 // storage ring (`ring`, `wcap` slices per component and shot).  `two_step`: the NL forward
@@ -443,8 +456,9 @@ Run<T> prepare(Ctx* c, int nshot, const eio_shot* shots, Op op, int flags,
     Run<T> run;
     run.c = c;
     run.v2 = variant2_for(c, nshot, shots);
+    run.v2a = variant2a_for(c, nshot, shots);
     run.plan = make_plan<T>(c, nshot, shots, op, two_step ? PlanKind::FWD2 : PlanKind::ONE, run.v2);
-    if (adj2) run.plan2 = make_plan<T>(c, nshot, shots, op, PlanKind::ADJ2);
+    if (adj2) run.plan2 = make_plan<T>(c, nshot, shots, op, PlanKind::ADJ2, run.v2a);
     if (src2) run.plan3 = make_plan<T>(c, nshot, shots, op, PlanKind::FWD2, run.v2);
     const int nlev = (two_step || adj2 || src2) ? 4 : 2;  // state levels per shot
     const bool born = (op == Op::BORN_FWD || op == Op::BORN_ADJ);
@@ -684,7 +698,7 @@ void launch_adj2(Run<float>& run, int k, int sspar) {
 #define EIO_L2A(R_, WX_, OCC_, LATE_, PS_)                                                              \
     hipLaunchKernelGGL((eio::k_step2_adj<float, R_, WX_, OCC_, LATE_, PS_, MODE>), dim3(ntile), dim3(64 * WX_), 0, \
                        s, run.d_shots, run.d_tiles2, run.d_ents2, ntile, k, par, sspar)
-    switch (run.c->variant2a) {
+    switch (run.v2a) {
         case 0: EIO_L2A(2, 8, 2, false, true); break;
         case 1: EIO_L2A(4, 8, 1, false, true); break;
         case 2: EIO_L2A(2, 8, 4, false, true); break;
@@ -694,7 +708,9 @@ void launch_adj2(Run<float>& run, int k, int sspar) {
         case 6: EIO_L2A(4, 8, 1, true, true); break;
         case 7: EIO_L2A(4, 8, 1, true, false); break;
         case 8: EIO_L2A(4, 8, 1, false, false); break;
-        default: EIO_L2A(2, 8, 2, true, false); break;
+        case 9: EIO_L2A(2, 8, 2, true, false); break;
+        case 10: EIO_L2A(1, 8, 4, true, false); break;
+        default: EIO_L2A(1, 8, 4, false, true); break;
     }
 #undef EIO_L2A
 }
@@ -1558,7 +1574,8 @@ int eio_ctx_num_time_block_variants(void) { return N_F32_VARIANTS2; }
 int eio_ctx_set_adjoint_time_block(eio_ctx* ctx, int steps, int variant) {
     return guarded([&]() {
         if (steps != 1 && steps != 2) fail("adjoint time block must be 1 or 2 fine steps");
-        if (variant < 0 || variant >= N_F32_VARIANTS2A) fail("two-step adjoint variant out of range");
+        if (variant != EIO_VARIANT_AUTO && (variant < 0 || variant >= N_F32_VARIANTS2A))
+            fail("two-step adjoint variant out of range");
         Ctx* c = as_ctx(ctx);
         c->adj_time_block = steps;
         c->variant2a = variant;

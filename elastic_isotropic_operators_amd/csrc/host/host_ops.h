@@ -436,12 +436,12 @@ struct Device {
         id = device;
         check_eio(eio_ctx_create(device, prec, &ctx));
         if (prec == 4) {
-            // FP32: two fine steps per launch for the forward passes (tile width chosen per
-            // call by the batch size) and the Born receiver pass (56 x 32); passes these
-            // kernels do not cover (wavefield saving, free surface, odd fine sub-sampling) fall
-            // back to one step per launch by themselves
+            // FP32: two fine steps per launch for the forward and the adjoint passes, tile
+            // width chosen per call by the batch size; passes these kernels do not cover
+            // (wavefield saving, free surface, odd fine sub-sampling) fall back to one step
+            // per launch by themselves
             check_eio(eio_ctx_set_time_block(ctx, 2, EIO_VARIANT_AUTO));
-            check_eio(eio_ctx_set_adjoint_time_block(ctx, 2, 7));
+            check_eio(eio_ctx_set_adjoint_time_block(ctx, 2, EIO_VARIANT_AUTO));
         }
     }
     ~Device() {

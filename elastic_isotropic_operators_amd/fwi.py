@@ -159,13 +159,13 @@ class ElasticFwi:
     time_block: (fine steps per launch, two-step kernel variant) of the FP32 forwards;
              variant _capi.VARIANT_AUTO picks the tile width per call from the batch size
     adj_time_block: (reverse fine steps per launch, two-step adjoint variant) of the FP32
-             gradient's receiver pass
+             gradient's receiver pass (VARIANT_AUTO as for time_block)
     """
 
     def __init__(self, grid, src_z, src_x, sources, rec_z, rec_x, device=0, dtype=torch.float32,
                  segment=0, variant=None, src_interp=("linear", 1), rec_interp=("linear", 1),
                  time_block=(2, _capi.VARIANT_AUTO),
-                 adj_time_block=(2, 7)):
+                 adj_time_block=(2, _capi.VARIANT_AUTO)):
         if not torch.cuda.is_available():
             if _capi._lib is not None:
                 raise RuntimeError("torch sees no GPU because libeio_hip.so was loaded before torch: "

@@ -109,7 +109,8 @@ int eio_ctx_set_time_block(eio_ctx* ctx, int steps, int variant);
  * step per launch, default) or 2 (both steps of a pair in one launch, four state levels per
  * shot; the Born adjoint fields of the pair are extracted with at most one read-modify-write
  * of each coarse slot), and the variant of the two-step adjoint kernel (owned tile
- * 56 x WX*R2).  FP64 contexts ignore it.  No counterpart in the reference (its receiver pass,
+ * 56 x WX*R2), or EIO_VARIANT_AUTO: per call, 56 x 8 tiles (variant 10) for a batch of fewer
+ * than 1024 tiles of 56 x 32, else 56 x 32 (variant 7).  FP64 contexts ignore it.  No counterpart in the reference (its receiver pass,
  * BornElasticGpuFunctions.cu:262-370, and its NL adjoint take one step per launch). */
 int eio_ctx_num_adjoint_time_block_variants(void);
 int eio_ctx_adjoint_time_block_info(int variant, int* R2, int* WX, int* occ);

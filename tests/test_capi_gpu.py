@@ -281,8 +281,8 @@ def test_two_step_nl_forward_matches_oracle(v, sub_fd):
 def test_auto_time_block_variant_matches_explicit():
     """EIO_VARIANT_AUTO chooses the two-step tile width per call: a one-shot 150 x 130 batch
     (6 tiles of 56 x 32, under the threshold of 1024) equals the oracle and the explicit 56 x 8
-    variant; a batch of 4 shots of 600 x 1600 (2200 tiles) equals the explicit 56 x 32
-    variant, for the NL forward and the Born forward."""
+    variants (NL forward and NL adjoint); a batch of 4 shots of 600 x 1600 (2200 tiles) equals
+    the explicit 56 x 32 variants (NL forward, Born forward, Born adjoint)."""
     w = W.small(nz=150, nx=130, nts=30, sub=1, seed=170)
     g = oracle_grid(w)
     med = O.media(g, w["elastic"], np.float64)
@@ -300,6 +300,16 @@ def test_auto_time_block_variant_matches_explicit():
     for k in range(5):
         assert rel_l2(out[_capi.VARIANT_AUTO][k], ref[k]) < 1e-5, (k, rel_l2(out[_capi.VARIANT_AUTO][k], ref[k]))
         assert rel_l2(out[_capi.VARIANT_AUTO][k], out[3][k]) < 1e-6
+    # the adjoint's automatic choice on the same small batch: 56 x 8 (variant 10)
+    dat = random_traces(rng, rpos, g.nts, np.float64)
+    ref_a = O.nl_adj_reg(g, med, spos, rpos, dat, np.float64)
+    adj = {}
+    for va in (10, _capi.VARIANT_AUTO):
+        ctx.set_adjoint_time_block(2, va)
+        adj[va] = _gpu_nl_adj(ctx, mid, spos, rpos, dat, g.nts, np.float32)
+    for k in range(5):
+        assert rel_l2(adj[_capi.VARIANT_AUTO][k], ref_a[k]) < 1e-5, (k, rel_l2(adj[_capi.VARIANT_AUTO][k], ref_a[k]))
+        assert rel_l2(adj[_capi.VARIANT_AUTO][k], adj[10][k]) < 1e-6
     # above the threshold: 4 shots of 600 x 1600, a few coarse samples
     wb = W.small(nz=600, nx=1600, nts=6, sub=1, seed=172)
     gb = oracle_grid(wb)
@@ -333,9 +343,27 @@ def test_auto_time_block_variant_matches_explicit():
                 a, b = recs[(_capi.VARIANT_AUTO, kind)][e][k], recs[(4, kind)][e][k]
                 assert np.linalg.norm(b) > 0
                 assert rel_l2(a, b) < 1e-6, (kind, e, k, rel_l2(a, b))
+    # the Born adjoint of the large batch: 56 x 32 (variant 7)
+    img = {}
+    for va in (7, _capi.VARIANT_AUTO):
+        ctx.set_adjoint_time_block(2, va)
+        rng_v = np.random.default_rng(174)
+        batch = []
+        for e in range(4):
+            sp = scattered_positions(rng_v, gb.nz, gb.nx, 10)
+            rp = scattered_positions(rng_v, gb.nz, gb.nx, 30)
+            tr = random_traces(rng_v, sp, ntw, np.float32)
+            dd = random_traces(rng_v, rp, gb.nts, np.float32)
+            shot, kp = ctx.make_shot(midb, sp, rp, tr, dd)
+            batch.append(shot)
+            keep.append(kp)
+        img[va] = np.zeros((5, gb.nx, gb.nz), np.float32)
+        ctx.born_adjoint(batch, img[va])
+    assert np.linalg.norm(img[7]) > 0
+    assert rel_l2(img[_capi.VARIANT_AUTO], img[7]) < 1e-6, rel_l2(img[_capi.VARIANT_AUTO], img[7])
 
 
-@pytest.mark.parametrize("va", range(10))
+@pytest.mark.parametrize("va", range(12))
 @pytest.mark.parametrize("sub_fd", [2, 4, 3])
 def test_two_step_born_adjoint_matches_oracle(va, sub_fd):
     """The Born adjoint with two reverse fine steps per launch of the receiver pass (four state
@@ -370,7 +398,7 @@ def test_two_step_born_adjoint_matches_oracle(va, sub_fd):
     assert np.array_equal(again, two)
 
 
-@pytest.mark.parametrize("va", range(10))
+@pytest.mark.parametrize("va", range(12))
 def test_two_step_nl_adjoint_matches_oracle(va):
     """The NL adjoint with two reverse fine steps per launch (data injected on the step-1 halo,
     source traces extracted from u1 and u2 at owned source cells) against the FP64 oracle and
