@@ -251,7 +251,7 @@ def test_born_adjoint_with_reused_storage_of_another_shape():
     assert rel_l2(out, ref) < 1e-4, rel_l2(out, ref)
 
 
-@pytest.mark.parametrize("v", range(5))
+@pytest.mark.parametrize("v", range(8))
 @pytest.mark.parametrize("sub_fd", [2, 3])
 def test_two_step_nl_forward_matches_oracle(v, sub_fd):
     """The temporally blocked NL forward (two fine steps per launch, step 1 recomputed on the
