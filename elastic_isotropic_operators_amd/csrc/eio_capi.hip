@@ -420,26 +420,25 @@ struct Run {
 // their source and scattered passes 11 % faster on 56 x 8; the 64-shot layered batch runs
 // 19 % slower on it.
 constexpr int AUTO_SMALL_TILES = 1024;
-int variant2_for(Ctx* c, int nshot, const eio_shot* shots) {
-    if (c->variant2 != EIO_VARIANT_AUTO) return c->variant2;
+// tiles of 56 x 32 over the update regions of a batch
+long wide_tiles(Ctx* c, int nshot, const eio_shot* shots) {
     long tiles = 0;
     for (int i = 0; i < nshot; ++i) {
         const ModelBase& m = get_model(c, shots[i].model);
         tiles += long((m.g.nz - 2 * eio::FAT + eio::TZ2 - 1) / eio::TZ2) * ((m.g.nx - 2 * eio::FAT + 31) / 32);
     }
-    return tiles < AUTO_SMALL_TILES ? 3 : 4;
+    return tiles;
+}
+int variant2_for(Ctx* c, int nshot, const eio_shot* shots) {
+    if (c->variant2 != EIO_VARIANT_AUTO) return c->variant2;
+    return wide_tiles(c, nshot, shots) < AUTO_SMALL_TILES ? 3 : 4;
 }
 
 // The two-step adjoint variant of a call: EIO_VARIANT_AUTO takes 56 x 8 tiles (variant 10)
 // under the same threshold, else 56 x 32 (variant 7).
 int variant2a_for(Ctx* c, int nshot, const eio_shot* shots) {
     if (c->variant2a != EIO_VARIANT_AUTO) return c->variant2a;
-    long tiles = 0;
-    for (int i = 0; i < nshot; ++i) {
-        const ModelBase& m = get_model(c, shots[i].model);
-        tiles += long((m.g.nz - 2 * eio::FAT + eio::TZ2 - 1) / eio::TZ2) * ((m.g.nx - 2 * eio::FAT + 31) / 32);
-    }
-    return tiles < AUTO_SMALL_TILES ? 10 : 7;
+    return wide_tiles(c, nshot, shots) < AUTO_SMALL_TILES ? 10 : 7;
 }
 
 // Builds the plan, the per-shot descriptors and the device tables of one call, uploads
