@@ -209,6 +209,89 @@ def test_invalid_positions_rejected():
         ctx.nl_forward([shot])
 
 
+def test_empty_batch_rejected():
+    """A call with no shots is an argument error."""
+    ctx = _ctx(np.float32)
+    with pytest.raises(_capi.EioError):
+        ctx.nl_forward([])
+    with pytest.raises(_capi.EioError):
+        ctx.born_forward([], np.zeros((5, 4, 4), np.float32))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_shots_without_sources_or_receivers(dtype):
+    """A batch of a normal shot, a shot with no receivers and a shot with no sources (empty
+    position lists and traces): the normal shot equals the oracle, the source-less shot records
+    exactly zero, for the NL forward (FP32: two steps per launch, automatic tile width), the NL
+    adjoint and the Born forward."""
+    w = W.small(nz=90, nx=110, nts=40, sub=1, seed=180)
+    g = oracle_grid(w)
+    med = O.media(g, w["elastic"], np.float64)
+    rng = np.random.default_rng(181)
+    spos = scattered_positions(rng, g.nz, g.nx, 6)
+    rpos = scattered_positions(rng, g.nz, g.nx, 20)
+    none = [np.zeros(0, np.int32)] * 4
+    ctx = _capi.Context(0, dtype)
+    if dtype == np.float32:
+        ctx.set_time_block(2, _capi.VARIANT_AUTO)
+        ctx.set_adjoint_time_block(2, _capi.VARIANT_AUTO)
+    mid = _model(ctx, w, g, med)
+
+    def traces(pos, nt):
+        return [np.zeros((pos[GRID_OF_COMP[k]].size, nt), dtype) for k in range(5)]
+
+    # NL forward
+    src = random_traces(rng, spos, g.nts, dtype)
+    ref = O.nl_fwd_reg(g, med, spos, [s.astype(np.float64) for s in src], rpos, np.float64)
+    rec_a, rec_c = traces(rpos, g.nts), traces(rpos, g.nts)
+    for k in range(5):
+        rec_c[k][...] = 1.0  # must be overwritten with zeros
+    shots, keep = [], []
+    for sp, rp, sr, rc in ((spos, rpos, src, rec_a), (spos, none, src, traces(none, g.nts)),
+                           (none, rpos, traces(none, g.nts), rec_c)):
+        sh, kp = ctx.make_shot(mid, sp, rp, sr, rc)
+        shots.append(sh)
+        keep.append(kp)
+    ctx.nl_forward(shots)
+    for k in range(5):
+        assert rel_l2(rec_a[k], ref[k]) < TOL[dtype], (k, rel_l2(rec_a[k], ref[k]))
+        assert not np.any(rec_c[k])
+    # NL adjoint: the receiver-less shot extracts exactly zero
+    dat = random_traces(rng, rpos, g.nts, dtype)
+    ref_a = O.nl_adj_reg(g, med, spos, rpos, [d.astype(np.float64) for d in dat], np.float64)
+    out_a, out_b = traces(spos, g.nts), traces(spos, g.nts)
+    for k in range(5):
+        out_b[k][...] = 1.0
+    shots, keep = [], []
+    for sp, rp, sr, rc in ((spos, rpos, out_a, dat), (spos, none, out_b, traces(none, g.nts)),
+                           (none, rpos, traces(none, g.nts), dat)):
+        sh, kp = ctx.make_shot(mid, sp, rp, sr, rc)
+        shots.append(sh)
+        keep.append(kp)
+    ctx.nl_adjoint(shots)
+    for k in range(5):
+        assert rel_l2(out_a[k], ref_a[k]) < TOL[dtype], (k, rel_l2(out_a[k], ref_a[k]))
+        assert not np.any(out_b[k])
+    # Born forward
+    src_fine, dm5 = _born_inputs(w, g, spos, 182)
+    ref_b = O.born_fwd_reg(g, med, spos, src_fine, dm5, rpos, np.float64)
+    ntw = (g.nts - 1) * g.sub + 1
+    rec_a, rec_c = traces(rpos, g.nts), traces(rpos, g.nts)
+    for k in range(5):
+        rec_c[k][...] = 1.0
+    shots, keep = [], []
+    sf = [np.ascontiguousarray(s, dtype) for s in src_fine]
+    for sp, rp, sr, rc in ((spos, rpos, sf, rec_a), (spos, none, sf, traces(none, g.nts)),
+                           (none, rpos, traces(none, ntw), rec_c)):
+        sh, kp = ctx.make_shot(mid, sp, rp, sr, rc)
+        shots.append(sh)
+        keep.append(kp)
+    ctx.born_forward(shots, dm5)
+    for k in range(5):
+        assert rel_l2(rec_a[k], ref_b[k]) < TOL[dtype], (k, rel_l2(rec_a[k], ref_b[k]))
+        assert not np.any(rec_c[k])
+
+
 @pytest.mark.parametrize("v", range(6))
 def test_all_fp32_variants_forward_adjoint(v):
     """Every FP32 step-kernel variant (tile shape / occupancy) gives the same NL forward and
