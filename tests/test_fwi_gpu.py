@@ -376,7 +376,7 @@ def test_grouped_full_storage_gradient_equals_checkpointed(tdtype):
     tol = 1e-10 if dtype == torch.float64 else 1e-5
     for group in (1, 2, None):
         phi, grad = eng.misfit_and_gradient(m.detach(), datcomp(d_obs, comps), comps, group=group)
-        assert abs(phi - float(loss)) / float(loss) < tol
+        assert abs(phi - float(loss.detach())) / float(loss.detach()) < tol
         assert float(torch.linalg.norm(grad.double() - ref) / torch.linalg.norm(ref)) < tol, group
     assert eng.last_store_info[1] == 0          # one segment: no checkpoints
     # the stores borrow the context's workspace; freeing it between calls changes nothing
