@@ -51,9 +51,12 @@ def lib():
             ("eio_ctx_destroy", [vp]),
             ("eio_ctx_set_stream", [vp, vp]),
             ("eio_ctx_set_graph", [vp, ctypes.c_int]),
+            ("eio_ctx_graph_info", [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
+                                    ctypes.POINTER(ctypes.c_longlong)]),
             ("eio_ctx_last_timing", [vp, ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(ctypes.c_longlong),
                                      ctypes.POINTER(ctypes.c_double)]),
+            ("eio_grid_check", [ctypes.POINTER(Grid), ctypes.c_int]),
             ("eio_model_add", [vp, ctypes.POINTER(Grid), ctypes.POINTER(vp), ctypes.c_int,
                                ctypes.POINTER(ctypes.c_int)]),
             ("eio_model_set_media", [vp, ctypes.c_int, ctypes.POINTER(vp), ctypes.c_int]),
@@ -101,7 +104,7 @@ def lib():
 
 
 EXPORTED = ("eio_last_error", "eio_ctx_create", "eio_ctx_destroy", "eio_ctx_set_stream",
-            "eio_ctx_set_graph", "eio_ctx_last_timing", "eio_model_add", "eio_model_set_media",
+            "eio_ctx_set_graph", "eio_ctx_graph_info", "eio_ctx_last_timing", "eio_grid_check", "eio_model_add", "eio_model_set_media",
             "eio_model_remove", "eio_nl_forward", "eio_nl_adjoint", "eio_born_forward",
             "eio_born_adjoint", "eio_device_alloc", "eio_device_free", "eio_memcpy",
             "eio_synchronize", "eio_copy_probe", "eio_ctx_set_variant", "eio_ctx_variant_info",
@@ -120,6 +123,16 @@ def check(rc):
 
 def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data) if a is not None else ctypes.c_void_p(0)
+
+
+def grid_check(nz, nx, nts, sub_fd, dz, dx, dts, minPad, precision, alphaCos=0.99, surface=0):
+    """eio_grid_check: the grid checks of eio_model_add on the host (raises EioError)."""
+    g = Grid(int(nz), int(nx), int(nts), int(sub_fd), int(minPad), float(dz), float(dx),
+             float(dts), float(alphaCos), int(surface))
+    rc = lib().eio_grid_check(ctypes.byref(g), int(precision))
+    if rc != 0:
+        raise EioError("eio error %d: %s" % (rc, lib().eio_last_error().decode()))
+    return rc
 
 
 class Context:
@@ -229,6 +242,12 @@ class Context:
 
     def set_graph(self, on):
         check(lib().eio_ctx_set_graph(self.h, int(bool(on))))
+
+    def graph_info(self):
+        """(graphs built, graph launches, kernel nodes updated before a replay)."""
+        b, l, u = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+        check(lib().eio_ctx_graph_info(self.h, ctypes.byref(b), ctypes.byref(l), ctypes.byref(u)))
+        return b.value, l.value, u.value
 
     def last_timing(self):
         t, n, s = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_double()

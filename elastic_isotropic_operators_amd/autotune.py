@@ -5,7 +5,8 @@ waves per SIMD) and every two-step variant (owned tile 56 x WX*R2, two fine step
 launch).  Each candidate runs the caller's own batch for a bounded number of fine steps
 (``eio_ctx_set_step_limit``) and is timed with the HIP events of the context
 (``eio_ctx_last_timing``); the fastest per fine step wins and is set on the context.
-Results are cached per batch signature (grid sizes, step counts, device counts) in
+Results are cached per batch signature (grid sizes, step counts, device counts, the variant
+tables and a hash of the built library) in
 ``$XDG_CACHE_HOME/eio/autotune.json`` (or ``~/.cache``), so later runs of the same batch
 skip the sweep.  The reference has no tuning (fixed 16 x 16 CUDA blocks, varDeclare.h:5).
 """
@@ -21,10 +22,28 @@ def _cache_path():
     return os.path.join(base, "eio", "autotune.json")
 
 
+_LIB_HASH = []
+
+
+def _library_hash():
+    """Hash of the built libeio_hip.so: a rebuilt library (its variant tables may differ, or
+    its kernels run at other speeds) never reuses an older entry."""
+    if not _LIB_HASH:
+        h = hashlib.sha1()
+        with open(_capi.LIB_PATH, "rb") as f:
+            for chunk in iter(lambda: f.read(1 << 20), b""):
+                h.update(chunk)
+        _LIB_HASH.append(h.hexdigest()[:16])
+    return _LIB_HASH[0]
+
+
 def signature(grids):
-    """Batch key: sorted (nz, nx, ntw-1, n_src, n_rec) of the shots."""
+    """Batch key: sorted (nz, nx, ntw-1, n_src, n_rec) of the shots, the single-step and
+    two-step variant tables and the library build."""
     items = sorted(tuple(int(x) for x in g) for g in grids)
-    return hashlib.sha1(json.dumps(items).encode()).hexdigest()[:16]
+    key = {"shots": items, "variants": _capi.Context.variants(),
+           "time_block_variants": _capi.Context.time_block_variants(), "lib": _library_hash()}
+    return hashlib.sha1(json.dumps(key).encode()).hexdigest()[:16]
 
 
 def candidates():

@@ -20,6 +20,9 @@
 #include <memory>
 #include <numeric>
 #include <string>
+#include <tuple>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../../include/eio.h"
@@ -117,6 +120,25 @@ struct FwiStore {
     size_t bytes = 0;
 };
 
+// One kernel launch of a time loop, recorded instead of launched (graph mode): the kernel,
+// its shape and its arguments packed as the kernel takes them.
+struct LaunchRec {
+    const void* fn = nullptr;
+    dim3 grid, block;
+    std::vector<unsigned char> blob;
+    std::vector<unsigned> offs;
+};
+
+// An executable graph of one launch sequence shape (kernels and shapes in order); its kernel
+// nodes' arguments are updated in place when a later call has the same shape.
+struct GraphEntry {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipGraphNode_t> nodes;
+    std::vector<LaunchRec> log;
+    unsigned long long last_use = 0;
+};
+
 struct Ctx {
     int device = 0;
     int prec = 4;
@@ -129,6 +151,11 @@ struct Ctx {
     double last_total_ms = 0.0, last_step_ms = 0.0;
     long long last_nstep = 0;
     int use_graph = 0;
+    bool recording = false;                 // graph mode: launches go to `log`
+    std::vector<LaunchRec> log;
+    std::map<unsigned long long, GraphEntry> graphs;
+    unsigned long long graph_clock = 0;
+    long long graph_builds = 0, graph_launches = 0, graph_node_updates = 0;
     int variant = 1;  // best measured FP32 variant on the 64-shot layered batch
     int time_block = 1;  // fine steps per launch of the FP32 NL forward (1 or 2)
     int variant2 = 0;    // two-step kernel variant, or EIO_VARIANT_AUTO (chosen per call)
@@ -166,12 +193,50 @@ struct DeviceGuard {
     }
 };
 
+// ------------------------------------------------------------------ kernel launches
+// Every launch of a time loop goes through klaunch: launched at once on the context stream,
+// or, while the context records (graph mode), appended to its log with the arguments
+// converted to the kernel's parameter types and packed as the kernel-node parameters.
+template <typename Tup, size_t... I>
+void tuple_ptrs(Tup& t, void** p, std::index_sequence<I...>) {
+    ((p[I] = static_cast<void*>(&std::get<I>(t))), ...);
+}
+
+template <typename... P, typename... A>
+void klaunch(Ctx* c, void (*fn)(P...), dim3 grid, dim3 block, A&&... a) {
+    static_assert(sizeof...(P) == sizeof...(A), "kernel argument count");
+    static_assert((std::is_trivially_copyable<P>::value && ...), "kernel arguments must be trivially copyable");
+    std::tuple<P...> t(std::forward<A>(a)...);
+    if (!c->recording) {
+        void* ptrs[sizeof...(P)];
+        tuple_ptrs(t, ptrs, std::index_sequence_for<P...>{});
+        EIO_CHECK_HIP(hipLaunchKernel(reinterpret_cast<const void*>(fn), grid, block, ptrs, 0, c->stream));
+        return;
+    }
+    LaunchRec r;
+    r.fn = reinterpret_cast<const void*>(fn);
+    r.grid = grid;
+    r.block = block;
+    void* ptrs[sizeof...(P)];
+    tuple_ptrs(t, ptrs, std::index_sequence_for<P...>{});
+    constexpr size_t sizes[] = {sizeof(P)...};
+    constexpr size_t aligns[] = {alignof(P)...};
+    size_t off = 0;
+    for (size_t i = 0; i < sizeof...(P); ++i) {
+        off = (off + aligns[i] - 1) / aligns[i] * aligns[i];
+        r.offs.push_back(unsigned(off));
+        off += sizes[i];
+    }
+    r.blob.resize(off);
+    for (size_t i = 0; i < sizeof...(P); ++i) std::memcpy(r.blob.data() + r.offs[i], ptrs[i], sizes[i]);
+    c->log.push_back(std::move(r));
+}
+
 template <typename T>
 void fill0(Ctx* c, T* p, long n) {
     if (n <= 0) return;
     const long nb = (n + eio::NTHR - 1) / eio::NTHR;
-    hipLaunchKernelGGL(eio::k_fill0<T>, dim3(unsigned(nb)), dim3(eio::NTHR), 0, c->stream, p, n);
-    EIO_CHECK_HIP(hipGetLastError());
+    klaunch(c, eio::k_fill0<T>, dim3(unsigned(nb)), dim3(eio::NTHR), p, n);
 }
 
 // ------------------------------------------------------------------ model registration
@@ -187,20 +252,30 @@ void upload_model(Ctx* c, ModelBase& m, const void* const media[5], int flags) {
     EIO_CHECK_HIP(hipStreamSynchronize(c->stream));
 }
 
-template <typename T>
-int model_add(Ctx* c, const eio_grid* g, const void* const media[5], int flags) {
+// Grid checks of eio_model_add (host only; no device call).
+void check_grid(const eio_grid* g, int prec) {
     if (!g) fail("grid is NULL");
+    if (prec != 4 && prec != 8) fail("precision must be 4 or 8");
     if (g->nz < 2 * eio::FAT + 1 || g->nx < 2 * eio::FAT + 1) fail("nz and nx must exceed 2*FAT");
     if (g->nts < 2) fail("nts must be >= 2");
     if (g->sub_fd < 1) fail("sub_fd must be >= 1");
     if (g->sub_fd > SUB_MAX) fail("Subsampling parameter too high");  // fdParamElastic.cpp:29-32
     if (g->minPad < 0 || g->minPad >= PAD_MAX) fail("Padding value is too high");  // NLF:191-194
     if (!(g->dz > 0) || !(g->dx > 0) || !(g->dts > 0)) fail("dz, dx, dts must be > 0");
-    if (long(g->nz) * g->nx >= (1L << 31)) fail("grid too large for 32-bit tile indexing");
+    // the kernels address a 5-field buffer (state level, media) through one buffer descriptor
+    // with 32-bit byte offsets: its size must stay below 2^31 bytes
+    if (5LL * g->nz * g->nx * prec >= (1LL << 31))
+        fail("grid too large: 5*nz*nx*" + std::to_string(prec) +
+             " bytes must stay below 2^31 (32-bit buffer offsets)");
     if (g->surface != 0 && g->surface != 1) fail("surface must be 0 or 1");
     // the free-surface adjoint needs rows up to 2*FAT+4 damped independently of z
     if (g->surface == 1 && g->nz - 1 - eio::FAT - (eio::SURF + eio::FAT) < g->minPad)
         fail("free surface: nz too small for the bottom sponge");
+}
+
+template <typename T>
+int model_add(Ctx* c, const eio_grid* g, const void* const media[5], int flags) {
+    check_grid(g, int(sizeof(T)));
     ModelBase m;
     m.g = *g;
     m.n = long(g->nz) * g->nx;
@@ -452,6 +527,14 @@ Run<T> prepare(Ctx* c, int nshot, const eio_shot* shots, Op op, int flags,
                bool adj2 = false, bool src2 = false) {
     if (nshot <= 0) fail("nshot must be > 0");
     if (!shots) fail("shots is NULL");
+    // the learned boundary is part of the NL forward only: the adjoint, Born and FWI passes
+    // would silently run another operator than the forward they are paired with
+    if (op != Op::NL_FWD)
+        for (int i = 0; i < nshot; ++i)
+            if (get_model(c, shots[i].model).abc_on)
+                fail("a learned boundary is attached to model " + std::to_string(shots[i].model) +
+                     ": it applies to the nonlinear forward only; detach it (eio_model_set_abc with on = 0) "
+                     "for adjoint, Born and FWI calls");
     Run<T> run;
     run.c = c;
     run.v2 = variant2_for(c, nshot, shots);
@@ -636,11 +719,10 @@ inline int active_tiles(const Plan& p, int k) {
 template <typename T, int MODE>
 void launch_fwd(Run<T>& run, int ntile, int k, int sspar, int dlo = 0, int dhi = -1,
                 const eio::ShotDev<T>* dsh = nullptr) {
-    hipStream_t s = run.c->stream;
     const eio::ShotDev<T>* tab = dsh ? dsh : run.d_shots;
-#define EIO_L(R_, WX_, OCC_)                                                                 \
-    hipLaunchKernelGGL((eio::k_step_fwd<T, R_, WX_, OCC_, MODE>), dim3(ntile), dim3(64 * WX_), 0, \
-                       s, tab, run.d_tiles, run.d_ents, ntile, k, sspar, dlo, dhi)
+#define EIO_L(R_, WX_, OCC_)                                                                  \
+    klaunch(run.c, eio::k_step_fwd<T, R_, WX_, OCC_, MODE>, dim3(ntile), dim3(64 * WX_), tab, run.d_tiles, \
+            run.d_ents, ntile, k, sspar, dlo, dhi)
     if constexpr (sizeof(T) == 8) {
         EIO_L(2, 4, 1);
     } else {
@@ -650,10 +732,9 @@ void launch_fwd(Run<T>& run, int ntile, int k, int sspar, int dlo = 0, int dhi =
 }
 template <typename T, int MODE>
 void launch_adj(Run<T>& run, int ntile, int k, int sspar) {
-    hipStream_t s = run.c->stream;
-#define EIO_L(R_, WX_, OCC_)                                                                 \
-    hipLaunchKernelGGL((eio::k_step_adj<T, R_, WX_, OCC_, MODE>), dim3(ntile), dim3(64 * WX_), 0, \
-                       s, run.d_shots, run.d_tiles, run.d_ents, ntile, k, sspar)
+#define EIO_L(R_, WX_, OCC_)                                                                  \
+    klaunch(run.c, eio::k_step_adj<T, R_, WX_, OCC_, MODE>, dim3(ntile), dim3(64 * WX_), run.d_shots, \
+            run.d_tiles, run.d_ents, ntile, k, sspar)
     if constexpr (sizeof(T) == 8) {
         EIO_L(2, 4, 1);
     } else {
@@ -667,13 +748,12 @@ template <bool DEC = false, bool FINE = false, bool SS = false, bool NOREC = fal
 void launch_fwd2(Run<float>& run, int ntile, int k, int par, int dlo = 0, int dhi = -1,
                  const eio::TileDev* tiles = nullptr, const eio::Entry* ents = nullptr, int sspar = 0,
                  const eio::ShotDev<float>* dsh = nullptr) {
-    hipStream_t s = run.c->stream;
     const eio::TileDev* tt = tiles ? tiles : run.d_tiles;
     const eio::Entry* te = ents ? ents : run.d_ents;
     const eio::ShotDev<float>* sh = dsh ? dsh : run.d_shots;
-#define EIO_L2(R_, WX_, OCC_, NT_)                                                                         \
-    hipLaunchKernelGGL((eio::k_step2_fwd<float, R_, WX_, OCC_, NT_, DEC, FINE, SS, NOREC>), dim3(ntile),     \
-                       dim3(64 * WX_), 0, s, sh, tt, te, ntile, k, par, dlo, dhi, sspar)
+#define EIO_L2(R_, WX_, OCC_, NT_)                                                                    \
+    klaunch(run.c, eio::k_step2_fwd<float, R_, WX_, OCC_, NT_, DEC, FINE, SS, NOREC>, dim3(ntile),      \
+            dim3(64 * WX_), sh, tt, te, ntile, k, par, dlo, dhi, sspar)
     switch (run.v2) {
         case 0: EIO_L2(2, 8, 2, 0); break;
         case 1: EIO_L2(2, 8, 4, 0); break;
@@ -691,12 +771,11 @@ void launch_fwd2(Run<float>& run, int ntile, int k, int par, int dlo = 0, int dh
 // launch_adj, sspar: slot of coarse step its) or of the NL adjoint.
 template <int MODE = eio::M_BADJ>
 void launch_adj2(Run<float>& run, int k, int sspar) {
-    hipStream_t s = run.c->stream;
     const int ntile = int(run.plan2.tiles.size());
     const int par = (k >> 1) & 1;
-#define EIO_L2A(R_, WX_, OCC_, LATE_, PS_)                                                              \
-    hipLaunchKernelGGL((eio::k_step2_adj<float, R_, WX_, OCC_, LATE_, PS_, MODE>), dim3(ntile), dim3(64 * WX_), 0, \
-                       s, run.d_shots, run.d_tiles2, run.d_ents2, ntile, k, par, sspar)
+#define EIO_L2A(R_, WX_, OCC_, LATE_, PS_)                                                               \
+    klaunch(run.c, eio::k_step2_adj<float, R_, WX_, OCC_, LATE_, PS_, MODE>, dim3(ntile), dim3(64 * WX_),   \
+            run.d_shots, run.d_tiles2, run.d_ents2, ntile, k, par, sspar)
     switch (run.v2a) {
         case 0: EIO_L2A(2, 8, 2, false, true); break;
         case 1: EIO_L2A(4, 8, 1, false, true); break;
@@ -730,10 +809,9 @@ void adjoint_interval(Run<T>& run, int ntile, bool two, int its, int nts, int su
 
 template <typename T, bool ADJ>
 void launch_img(Run<T>& run, int ntile, int its, int which) {
-    hipStream_t s = run.c->stream;
-#define EIO_L(R_, WX_, OCC_)                                                                 \
-    hipLaunchKernelGGL((eio::k_imaging<T, R_, WX_, ADJ>), dim3(ntile), dim3(64 * WX_), 0, s,   \
-                       run.d_shots, run.d_tiles, ntile, its, which)
+#define EIO_L(R_, WX_, OCC_)                                                                  \
+    klaunch(run.c, eio::k_imaging<T, R_, WX_, ADJ>, dim3(ntile), dim3(64 * WX_), run.d_shots, run.d_tiles, \
+            ntile, its, which)
     if constexpr (sizeof(T) == 8) {
         EIO_L(2, 4, 1);
     } else {
@@ -742,23 +820,135 @@ void launch_img(Run<T>& run, int ntile, int its, int which) {
 #undef EIO_L
 }
 
-// Runs `body` (a sequence of launches) either directly or captured into one hipGraph.
+// ------------------------------------------------------------------ graph mode
+// A time loop is recorded as a list of launches, then run from an executable hipGraph (a
+// chain of kernel nodes) cached per launch-sequence shape: the first call of a shape builds
+// and instantiates it; later calls with the same kernels, grids and argument sizes replay it
+// after updating only the kernel nodes whose arguments changed.  Device tables (shot, tile
+// and entry tables) are uploaded by each call before the graph launch, on the same stream, so
+// a replay reads the call's own tables.  No memset nodes: buffers are cleared by kernels.
+constexpr size_t GRAPH_CACHE_MAX = 4;
+
+unsigned long long log_shape(const std::vector<LaunchRec>& log) {
+    unsigned long long h = 1469598103934665603ULL;
+    auto mix = [&](unsigned long long v) {
+        h ^= v;
+        h *= 1099511628211ULL;
+    };
+    mix(log.size());
+    for (const LaunchRec& r : log) {
+        mix(reinterpret_cast<unsigned long long>(r.fn));
+        mix(r.grid.x), mix(r.grid.y), mix(r.grid.z), mix(r.block.x), mix(r.block.y), mix(r.block.z);
+        mix(r.blob.size());
+    }
+    return h;
+}
+
+bool same_shape(const std::vector<LaunchRec>& a, const std::vector<LaunchRec>& b) {
+    if (a.size() != b.size()) return false;
+    for (size_t i = 0; i < a.size(); ++i) {
+        const LaunchRec &x = a[i], &y = b[i];
+        if (x.fn != y.fn || x.blob.size() != y.blob.size() || x.grid.x != y.grid.x || x.grid.y != y.grid.y ||
+            x.grid.z != y.grid.z || x.block.x != y.block.x || x.block.y != y.block.y || x.block.z != y.block.z)
+            return false;
+    }
+    return true;
+}
+
+void graph_free(GraphEntry& e) {
+    if (e.exec) (void)hipGraphExecDestroy(e.exec);
+    if (e.graph) (void)hipGraphDestroy(e.graph);
+    e.exec = nullptr;
+    e.graph = nullptr;
+    e.nodes.clear();
+}
+
+hipKernelNodeParams node_params(LaunchRec& r, std::vector<void*>& ptrs) {
+    ptrs.resize(r.offs.size());
+    for (size_t i = 0; i < r.offs.size(); ++i) ptrs[i] = r.blob.data() + r.offs[i];
+    hipKernelNodeParams p{};
+    p.func = const_cast<void*>(r.fn);
+    p.gridDim = r.grid;
+    p.blockDim = r.block;
+    p.sharedMemBytes = 0;
+    p.kernelParams = ptrs.data();
+    p.extra = nullptr;
+    return p;
+}
+
+void graph_launch(Ctx* c) {
+    std::vector<LaunchRec> log;
+    log.swap(c->log);
+    if (log.empty()) return;
+    const unsigned long long key = log_shape(log);
+    auto it = c->graphs.find(key);
+    if (it != c->graphs.end() && !same_shape(it->second.log, log)) {  // hash collision
+        graph_free(it->second);
+        c->graphs.erase(it);
+        it = c->graphs.end();
+    }
+    std::vector<void*> ptrs;
+    if (it == c->graphs.end()) {
+        if (c->graphs.size() >= GRAPH_CACHE_MAX) {  // evict the least recently used graph
+            auto lru = c->graphs.begin();
+            for (auto j = c->graphs.begin(); j != c->graphs.end(); ++j)
+                if (j->second.last_use < lru->second.last_use) lru = j;
+            graph_free(lru->second);
+            c->graphs.erase(lru);
+        }
+        GraphEntry e;
+        e.log.swap(log);
+        try {
+            EIO_CHECK_HIP(hipGraphCreate(&e.graph, 0));
+            e.nodes.resize(e.log.size());
+            for (size_t i = 0; i < e.log.size(); ++i) {
+                const hipKernelNodeParams p = node_params(e.log[i], ptrs);
+                EIO_CHECK_HIP(hipGraphAddKernelNode(&e.nodes[i], e.graph, i ? &e.nodes[i - 1] : nullptr, i ? 1 : 0,
+                                                    &p));
+            }
+            EIO_CHECK_HIP(hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0));
+        } catch (...) {
+            graph_free(e);
+            throw;
+        }
+        ++c->graph_builds;
+        it = c->graphs.emplace(key, std::move(e)).first;
+    } else {
+        GraphEntry& e = it->second;
+        for (size_t i = 0; i < log.size(); ++i) {
+            if (log[i].blob == e.log[i].blob) continue;
+            e.log[i].blob.swap(log[i].blob);
+            const hipKernelNodeParams p = node_params(e.log[i], ptrs);
+            EIO_CHECK_HIP(hipGraphExecKernelNodeSetParams(e.exec, e.nodes[i], &p));
+            ++c->graph_node_updates;
+        }
+    }
+    it->second.last_use = ++c->graph_clock;
+    EIO_CHECK_HIP(hipGraphLaunch(it->second.exec, c->stream));
+    ++c->graph_launches;
+}
+
+// Runs `body` (a sequence of launches) directly, or, in graph mode, records it and runs it
+// from the context's cached executable graph.  `graph_ok` = false for bodies that issue
+// anything but kernel launches (they always run directly).
 template <typename F>
-void run_loop(Ctx* c, F&& body) {
-    if (!c->use_graph) {
+void run_loop(Ctx* c, F&& body, bool graph_ok = true) {
+    if (!c->use_graph || !graph_ok) {
         body();
         EIO_CHECK_HIP(hipGetLastError());
         return;
     }
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    EIO_CHECK_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    body();
-    EIO_CHECK_HIP(hipStreamEndCapture(c->stream, &graph));
-    EIO_CHECK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    EIO_CHECK_HIP(hipGraphLaunch(exec, c->stream));
-    EIO_CHECK_HIP(hipGraphExecDestroy(exec));
-    EIO_CHECK_HIP(hipGraphDestroy(graph));
+    c->log.clear();
+    c->recording = true;
+    try {
+        body();
+    } catch (...) {
+        c->recording = false;
+        c->log.clear();
+        throw;
+    }
+    c->recording = false;
+    graph_launch(c);
 }
 
 template <typename T>
@@ -868,8 +1058,7 @@ void launch_abc(Run<T>& run, const AbcRun& a, int k) {
     if constexpr (sizeof(T) == 4) {
         const int nt = a.active(k);
         if (nt <= 0) return;
-        hipLaunchKernelGGL((eio::k_abc_cnn<float>), dim3(nt), dim3(eio::ABC_NTHR), 0, run.c->stream, run.d_shots,
-                           a.d_par, a.d_tiles, nt, k);
+        klaunch(run.c, eio::k_abc_cnn<float>, dim3(nt), dim3(eio::ABC_NTHR), run.d_shots, a.d_par, a.d_tiles, nt, k);
     }
 }
 
@@ -942,7 +1131,7 @@ void nl_forward(Ctx* c, int nshot, const eio_shot* shots, int flags, int wshot =
                 ++sn.next;
             }
         }
-    });
+    }, nsnap == 0);  // snapshot copies run directly
     EIO_CHECK_HIP(hipEventRecord(c->ev[3], c->stream));
     timing_end<T>(c, nlaunch);
     finish(run);
@@ -1261,8 +1450,7 @@ template <typename T>
 void copy_dev(Ctx* c, T* dst, const T* src, size_t n) {
     if (!n) return;
     const unsigned nb = unsigned(std::min<size_t>((n + eio::NTHR - 1) / eio::NTHR, 256 * 64));
-    hipLaunchKernelGGL(eio::k_copy<T>, dim3(nb), dim3(eio::NTHR), 0, c->stream, dst, src, long(n));
-    EIO_CHECK_HIP(hipGetLastError());
+    klaunch(c, eio::k_copy<T>, dim3(nb), dim3(eio::NTHR), dst, src, long(n));
 }
 
 inline int seg_begin(const FwiStore& f, int j) { return j * f.L; }
@@ -1511,6 +1699,7 @@ int eio_ctx_destroy(eio_ctx* ctx) {
         (void)hipStreamSynchronize(c->stream);
         for (auto& kv : c->models) model_free(kv.second);
         for (auto& kv : c->fwi) fwi_free(c, kv.second);
+        for (auto& kv : c->graphs) graph_free(kv.second);
         for (auto& kv : c->ws)
             if (kv.second.p) (void)hipFree(kv.second.p);
         for (auto& e : c->ev) (void)hipEventDestroy(e);
@@ -1527,7 +1716,18 @@ int eio_ctx_set_stream(eio_ctx* ctx, void* s) {
 }
 
 int eio_ctx_set_graph(eio_ctx* ctx, int use_graph) {
-    return guarded([&]() { as_ctx(ctx)->use_graph = use_graph ? 1 : 0; });
+    return guarded([&]() {
+        as_ctx(ctx)->use_graph = use_graph ? 1 : 0;
+    });
+}
+
+int eio_ctx_graph_info(eio_ctx* ctx, long long* builds, long long* launches, long long* node_updates) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        if (builds) *builds = c->graph_builds;
+        if (launches) *launches = c->graph_launches;
+        if (node_updates) *node_updates = c->graph_node_updates;
+    });
 }
 
 int eio_ctx_set_variant(eio_ctx* ctx, int variant) {
@@ -1606,6 +1806,10 @@ int eio_ctx_last_timing(eio_ctx* ctx, double* total_ms, long long* n_step, doubl
         if (n_step) *n_step = c->last_nstep;
         if (step_ms) *step_ms = c->last_step_ms;
     });
+}
+
+int eio_grid_check(const eio_grid* grid, int precision) {
+    return guarded([&]() { check_grid(grid, precision); });
 }
 
 int eio_model_add(eio_ctx* ctx, const eio_grid* grid, const void* const media[5], int flags,

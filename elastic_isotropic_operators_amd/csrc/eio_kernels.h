@@ -108,6 +108,8 @@ struct Entry {
 // ------------------------------------------------------------------ buffer access
 typedef __amdgpu_buffer_rsrc_t rsrc_t;
 
+// `bytes` < 2^31: eio_model_add rejects grids whose 5-field buffers reach 2^31 bytes, so
+// every descriptor size and lane offset of the kernels fits a non-negative int.
 __device__ __forceinline__ rsrc_t make_rsrc(const void* p, unsigned bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, int(bytes), 0x00020000);
 }

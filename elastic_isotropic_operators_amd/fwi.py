@@ -143,6 +143,21 @@ class _Store:
             pass
 
 
+def group_and_segment(group, fit, nshot, nts, segment, checkpoint=None):
+    """Shots per group and the store's segment length (coarse intervals) of one
+    misfit_and_gradient call.  fit: shots whose whole coarse Vx/Vz record fits in memory;
+    group: the caller's group size (None: `fit`); checkpoint: None chooses (full storage when
+    the group fits, else checkpoints), else a checkpoint interval that is always used (0:
+    ceil(sqrt(nts-1))).  A group that does not fit takes checkpoints (interval `segment`,
+    0: ceil(sqrt(nts-1))) instead of failing for memory."""
+    g = fit if group is None else int(group)
+    if checkpoint is not None:
+        return max(1, min(nshot if group is None else g, nshot)), int(checkpoint)
+    if 1 <= g <= fit:
+        return min(g, nshot), nts - 1
+    return max(1, min(g, nshot)), int(segment)
+
+
 class ElasticFwi:
     """Shots of one model grid, resident on one GPU.
 
@@ -351,25 +366,22 @@ class ElasticFwi:
         per_shot = (2 * self.nts + 60) * self.nz * self.nx * torch.tensor([], dtype=self.dtype).element_size()
         return int(margin * free // per_shot)
 
-    def misfit_and_gradient(self, model, d_obs, components=None, group=None):
+    def misfit_and_gradient(self, model, d_obs, components=None, group=None, checkpoint=None):
         """1/2 || C F(model) - d_obs ||^2 and its gradient with respect to `model` (rho, lambda,
         mu), shot group by shot group: each group's forward keeps the whole coarse Vx/Vz
         record (one segment, no checkpoints, the reference's full storage,
         BornElasticGpuFunctions.cu:1128-1191, sized to the device) and its gradient follows at
         once, so no segment is recomputed.  The misfit is a sum over shots, so the gradient of
         each group needs only its own residual.  group: shots per group (default: as many as
-        fit in memory; falls back to checkpoints when even one does not)."""
+        fit in memory); a group that does not fit, or checkpoint = an interval (0: ceil(sqrt(
+        nts-1)), all shots in one group unless `group` is given), takes checkpoints instead."""
         model = model.detach().to(device=self.device, dtype=self.dtype)
         d_obs = torch.as_tensor(d_obs, device=self.device, dtype=self.dtype)
         media = scaled_media(model, self.dtw)
-        if group is None:
-            # sized once: later calls reuse the context's workspace, which free memory no longer shows
-            if getattr(self, "_group", None) is None:
-                self._group = self.full_storage_group()
-            group = self._group
-        g = group
-        segment = self.nts - 1 if g >= 1 else self.segment
-        g = max(1, min(g, self.nshot))
+        # sized once: later calls reuse the context's workspace, which free memory no longer shows
+        if getattr(self, "_group", None) is None:
+            self._group = self.full_storage_group()
+        g, segment = group_and_segment(group, self._group, self.nshot, self.nts, self.segment, checkpoint)
         grad = torch.zeros((3, self.nx, self.nz), dtype=self.dtype, device=self.device)
         phi = 0.0
         for s0 in range(0, self.nshot, g):

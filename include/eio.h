@@ -84,8 +84,18 @@ int eio_ctx_set_stream(eio_ctx* ctx, void* hip_stream);
  * stencil launch [ms] measured over a bracketed run of them. */
 int eio_ctx_last_timing(eio_ctx* ctx, double* total_ms, long long* n_step_launches,
                         double* step_ms);
-/* 0 = launch each fine step (default), 1 = replay each batch of fine steps from a captured hipGraph. */
+/* 0 = launch each fine step (default); 1 = run each call's time loop from an executable
+ * hipGraph (a chain of kernel nodes) kept by the context per launch-sequence shape: built and
+ * instantiated by the first call of a shape, replayed by later ones after updating only the
+ * kernel nodes whose arguments changed (the 4 most recently used shapes are kept, until the
+ * context is destroyed; every argument of every node is compared, so a kept graph stays valid
+ * when workspace buffers move).  No
+ * counterpart in the reference, which launches every kernel of every fine step
+ * (nonlinearPropElasticGpuFunctions.cu:588-617). */
 int eio_ctx_set_graph(eio_ctx* ctx, int use_graph);
+/* Graph-mode counters of the context: graphs built (instantiated), graph launches, kernel
+ * nodes updated in place before a replay. */
+int eio_ctx_graph_info(eio_ctx* ctx, long long* builds, long long* launches, long long* node_updates);
 
 /* FP32 step-kernel variant (tile of 64 x WX*R cells, WX waves per workgroup, R x-cells per
  * lane, requested occupancy) used by the next calls.  FP64 contexts ignore it.  No counterpart
@@ -118,6 +128,12 @@ int eio_ctx_set_adjoint_time_block(eio_ctx* ctx, int steps, int variant);
 /* Tuning runs: NL forward and adjoint loops stop after `max_steps` fine steps (0: no limit).
  * Outputs of a limited call are partial.  No counterpart in the reference. */
 int eio_ctx_set_step_limit(eio_ctx* ctx, int max_steps);
+
+/* The grid checks of eio_model_add, on the host only (no device, no context): sizes, sampling,
+ * sponge width (the reference's messages, fdParamElastic.cpp:29-32,
+ * nonlinearPropElasticGpuFunctions.cu:191-194), and 5*nz*nx*precision < 2^31 bytes (the
+ * kernels address each 5-field buffer with 32-bit offsets).  EIO_ERR_ARG when a check fails. */
+int eio_grid_check(const eio_grid* grid, int precision);
 
 /* Register a model: grid constants (FD coefficients, time filter, damping table) and the
  * five scaled media rhoxDtw, rhozDtw, lamb2MuDtw, lambDtw, muxzDtw, each nz*nx.

@@ -649,7 +649,7 @@ void media_scaled(const Grid& g, const T* elastic, T* out) {
     }
 }
 
-// spaceInterpGpu.cpp:78-138 (linear, no dipole) + :288-311 unique list
+// spaceInterpGpu.cpp:78-138 (linear, with the dipole of :108-137) + :288-311 unique list
 struct Interp {
     int nIrreg;
     int nf = 4;                // regular points per device
@@ -665,13 +665,16 @@ struct Interp {
 // separable weights normalised to sum 1:
 //   sinc:  sin(pi u)/(pi u) * sin(pi v)/(pi v)      u, v = device - point, in cells
 //   gauss: exp(-u^2/8) * exp(-v^2/8)                 (the pdf constants cancel in the normalisation)
+// Dipole (linear only, spaceInterpGpu.cpp:108-137): 4 more points per device, the bilinear
+// stencil of the position shifted by (zsh dz, xsh dx) with negated weights (entries 4..7).
 // Computed here in double from the device offsets.
 Interp make_interp(int nz, double oz, double dz, double ox, double dx, int nDev, const double* zc,
-                   const double* xc, int method = 0, int nfilt = 1) {
+                   const double* xc, int method = 0, int nfilt = 1, int dipole = 0, double zsh = 0.0,
+                   double xsh = 0.0) {
     Interp ip;
     ip.nIrreg = nDev;
     const int n2 = 2 * nfilt;
-    ip.nf = method == 0 ? 4 : n2 * n2;
+    ip.nf = method == 0 ? (dipole == 1 ? 8 : 4) : n2 * n2;
     ip.idx.resize(size_t(ip.nf) * nDev);
     ip.w.resize(size_t(ip.nf) * nDev);
     for (int d = 0; d < nDev; ++d) {
@@ -687,6 +690,17 @@ Interp make_interp(int nz, double oz, double dz, double ox, double dx, int nDev,
             id[1] = b + 1;      ww[1] = (1.0 - wz) * wx;
             id[2] = b + nz;     ww[2] = wz * (1.0 - wx);
             id[3] = b + nz + 1; ww[3] = (1.0 - wz) * (1.0 - wx);
+            if (dipole == 1) {
+                double vz = (zc[d] + zsh * dz - oz) / dz, vx = (xc[d] + xsh * dx - ox) / dx;
+                const int qz = int(vz), qx = int(vx);
+                vz = 1.0 - (vz - qz);
+                vx = 1.0 - (vx - qx);
+                const int q = qx * nz + qz;
+                id[4] = q;          ww[4] = -vz * vx;
+                id[5] = q + 1;      ww[5] = -(1.0 - vz) * vx;
+                id[6] = q + nz;     ww[6] = -vz * (1.0 - vx);
+                id[7] = q + nz + 1; ww[7] = -(1.0 - vz) * (1.0 - vx);
+            }
             continue;
         }
         auto kern = [method](double u) {
@@ -1134,6 +1148,44 @@ static Grid to_grid(const OracleGrid* g) {
     void oracle_time_interp_##SUF(int nts, double dts, int sub, int nDev, const T* coarse,         \
                                   T* fine) {                                                       \
         time_interp_fwd<T>(nts, dts, sub, nDev, coarse, fine);                                     \
+    }                                                                                              \
+    /* test hook: `nsteps` fine steps of the NL loop body without sources or receivers (step,      \
+       damping of both levels, swap; nonlinearPropElasticGpuFunctions.cu:588-617) from the state   \
+       pair state[0] = older, state[1] = current ([2][5][n]), in place */                          \
+    void oracle_free_steps_##SUF(const OracleGrid* g, const T* media, T* state, int nsteps) {     \
+        Grid gr = to_grid(g);                                                                      \
+        Consts<T> c = make_consts<T>(gr);                                                          \
+        const size_t n = size_t(gr.nz) * gr.nx;                                                    \
+        Media<T> m = {media, media + n, media + 2 * n, media + 3 * n, media + 4 * n};              \
+        Fields<T> a(n), b(n);                                                                      \
+        for (int k = 0; k < 5; ++k) {                                                              \
+            std::copy(state + k * n, state + (k + 1) * n, a.f[k].begin());                         \
+            std::copy(state + (5 + k) * n, state + (6 + k) * n, b.f[k].begin());                   \
+        }                                                                                          \
+        Fields<T>*p0 = &a, *p1 = &b;                                                               \
+        for (int it = 0; it < nsteps; ++it) {                                                      \
+            if (c.surface) step_fwd_fs(c, m, *p0, *p1);                                            \
+            else step_fwd(c, m, *p0, *p1);                                                         \
+            damp_both(c, *p0, *p1);                                                                \
+            std::swap(p0, p1);                                                                     \
+        }                                                                                          \
+        for (int k = 0; k < 5; ++k) {                                                              \
+            std::copy(p0->f[k].begin(), p0->f[k].end(), state + k * n);                            \
+            std::copy(p1->f[k].begin(), p1->f[k].end(), state + (5 + k) * n);                      \
+        }                                                                                          \
+    }                                                                                              \
+    /* test hook: the factor damp_both applies to each cell (fields of ones after it) */           \
+    void oracle_damp_field_##SUF(const OracleGrid* g, T* out) {                                    \
+        Grid gr = to_grid(g);                                                                      \
+        Consts<T> c = make_consts<T>(gr);                                                          \
+        const size_t n = size_t(gr.nz) * gr.nx;                                                    \
+        Fields<T> a(n), b(n);                                                                      \
+        for (int k = 0; k < 5; ++k) {                                                              \
+            std::fill(a.f[k].begin(), a.f[k].end(), T(1));                                         \
+            std::fill(b.f[k].begin(), b.f[k].end(), T(1));                                         \
+        }                                                                                          \
+        damp_both(c, a, b);                                                                        \
+        std::copy(a.f[0].begin(), a.f[0].end(), out);                                              \
     }
 
 EIO_ORACLE_TYPED(f64, double)
@@ -1168,6 +1220,15 @@ int oracle_interp(int nz, double oz, double dz, double ox, double dx, int nDev, 
 int oracle_interp_m(int nz, double oz, double dz, double ox, double dx, int nDev, const double* zc,
                     const double* xc, int method, int nfilt, int* idx, double* w, int* uniq) {
     Interp ip = make_interp(nz, oz, dz, ox, dx, nDev, zc, xc, method, nfilt);
+    std::copy(ip.idx.begin(), ip.idx.end(), idx);
+    std::copy(ip.w.begin(), ip.w.end(), w);
+    std::copy(ip.uniq.begin(), ip.uniq.end(), uniq);
+    return int(ip.uniq.size());
+}
+// Bilinear with a dipole (8 entries per device; shifts in cells).
+int oracle_interp_dipole(int nz, double oz, double dz, double ox, double dx, int nDev, const double* zc,
+                         const double* xc, double zsh, double xsh, int* idx, double* w, int* uniq) {
+    Interp ip = make_interp(nz, oz, dz, ox, dx, nDev, zc, xc, 0, 1, 1, zsh, xsh);
     std::copy(ip.idx.begin(), ip.idx.end(), idx);
     std::copy(ip.w.begin(), ip.w.end(), w);
     std::copy(ip.uniq.begin(), ip.uniq.end(), uniq);

@@ -105,12 +105,26 @@ def time_interp(coarse, dts, sub_fd, dtype=np.float64):
 _METHODS = {"linear": 0, "sinc": 1, "gauss": 2}
 
 
-def interp(nz, oz, dz, ox, dx, zc, xc, method="linear", nfilt=1):
+def interp(nz, oz, dz, ox, dx, zc, xc, method="linear", nfilt=1, dipole=0, zshift=0.0, xshift=0.0):
     """Spatial interpolation weights of one staggered grid: (idx[nDev,nf], w[nDev,nf], unique
-    positions), nf = 4 (linear) or (2 nfilt)^2 (sinc, gauss)."""
+    positions), nf = 4 (linear), 8 (linear with dipole=1: the stencil of the position shifted
+    by (zshift dz, xshift dx) with negated weights as entries 4..7) or (2 nfilt)^2 (sinc,
+    gauss)."""
     zc = _c(zc, np.float64)
     xc = _c(xc, np.float64)
     nd = zc.size
+    if dipole == 1:
+        if method != "linear":
+            raise ValueError("dipole acquisition is implemented for linear interpolation only")
+        idx = np.zeros((nd, 8), np.int32)
+        w = np.zeros((nd, 8), np.float64)
+        uniq = np.zeros(8 * nd, np.int32)
+        f = lib().oracle_interp_dipole
+        f.restype = ctypes.c_int
+        nu = f(nz, ctypes.c_double(oz), ctypes.c_double(dz), ctypes.c_double(ox), ctypes.c_double(dx), nd,
+               _ptr(zc), _ptr(xc), ctypes.c_double(zshift), ctypes.c_double(xshift), _ptr(idx), _ptr(w),
+               _ptr(uniq))
+        return idx, w, uniq[:nu].copy()
     m = _METHODS[method]
     nf = 4 if m == 0 else (2 * nfilt) ** 2
     idx = np.zeros((nd, nf), np.int32)
@@ -123,11 +137,11 @@ def interp(nz, oz, dz, ox, dx, zc, xc, method="linear", nfilt=1):
     return idx, w, uniq[:nu].copy()
 
 
-def staggered_interp(g, zc, xc, method="linear", nfilt=1):
+def staggered_interp(g, zc, xc, method="linear", nfilt=1, dipole=0, zshift=0.0, xshift=0.0):
     """interp on the four staggered grids of g (centre, X, Z, XZ)."""
     origins = [(g.oz, g.ox), (g.oz, g.ox - 0.5 * g.dx), (g.oz - 0.5 * g.dz, g.ox),
                (g.oz - 0.5 * g.dz, g.ox - 0.5 * g.dx)]
-    return [interp(g.nz, o[0], g.dz, o[1], g.dx, zc, xc, method, nfilt) for o in origins]
+    return [interp(g.nz, o[0], g.dz, o[1], g.dx, zc, xc, method, nfilt, dipole, zshift, xshift) for o in origins]
 
 
 def irreg_to_reg(ip, irr):
@@ -286,6 +300,24 @@ def born_adj_shots(g, elastic, src_z, src_x, src_sig, rec_z, rec_x, data, dtype=
         ctypes.byref(g), _ptr(el), nshot, nsrc, _ptr(sz), _ptr(sx), _ptr(sig), rz.size, _ptr(rz),
         _ptr(rx), _ptr(dm), _ptr(da))
     return dm
+
+
+def free_steps(g, med, older, current, nsteps, dtype=np.float64):
+    """`nsteps` fine steps of the NL loop body (step, damping of both levels, swap) without
+    sources or receivers from the levels (older, current), each (5, nx, nz); returns the new
+    (older, current)."""
+    st = np.ascontiguousarray(np.stack([older, current]), dtype=dtype)
+    m = _c(med, dtype)
+    getattr(lib(), "oracle_free_steps_" + _suffix(dtype))(ctypes.byref(g), _ptr(m), _ptr(st), int(nsteps))
+    return st[0], st[1]
+
+
+def damp_field(g, dtype=np.float64):
+    """(nx, nz): the factor the sponge damping applies to each cell (1 outside the sponge and in
+    the FAT halo)."""
+    out = np.zeros((g.nx, g.nz), dtype)
+    getattr(lib(), "oracle_damp_field_" + _suffix(dtype))(ctypes.byref(g), _ptr(out))
+    return out
 
 
 def set_ftz(on=True):

@@ -114,7 +114,7 @@ def oracle_nl_irregular(w, g, src_z, src_x, rec_z, rec_x, src_model, src_interp=
     2 dtw, and 1/(dx dz): nonlinearPropElasticGpu.cpp:109-144) -> oracle regular-level NL
     forward -> irregular receivers.  Returns (5, nRec, nts) in FP64."""
     med = O.media(g, w["elastic"])
-    sips = O.staggered_interp(g, src_z, src_x, *src_interp)
+    sips = O.staggered_interp(g, src_z, src_x, *src_interp)   # (method, nfilt[, dipole, zshift, xshift])
     rips = O.staggered_interp(g, rec_z, rec_x, *rec_interp)
     dtw = g.dts / g.sub
     area = 1.0 / (g.dx * g.dz)

@@ -18,6 +18,9 @@ pytestmark = pytest.mark.gpu
 
 TOL = {np.float32: 1e-5, np.float64: 1e-11}
 DOT_TOL = {np.float32: 1e-5, np.float64: 1e-12}
+# Born adjoint (image) bars: the imaging sum accumulates over every coarse sample; FP32 reaches
+# 1e-6 .. 1e-5 on these cases
+BADJ_TOL = {np.float32: 2e-5, np.float64: 1e-10}
 
 
 def _case(nz=40, nx=56, nts=120, sub=1, seed=0):
@@ -150,9 +153,9 @@ def test_born_adjoint_parity(dtype):
     ctx = _ctx(dtype)
     mid = _model(ctx, w, g, med)
     out = _gpu_born_adj(ctx, mid, spos, rpos, src_fine, dat, g.nx, g.nz, dtype)
-    assert rel_l2(out, ref) < TOL[dtype] * 10, rel_l2(out, ref)
+    assert rel_l2(out, ref) < BADJ_TOL[dtype], rel_l2(out, ref)
     for k in range(5):
-        assert rel_l2(out[k], ref[k]) < TOL[dtype] * 10, (k, rel_l2(out[k], ref[k]))
+        assert rel_l2(out[k], ref[k]) < BADJ_TOL[dtype], (k, rel_l2(out[k], ref[k]))
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
@@ -331,7 +334,7 @@ def test_born_adjoint_with_reused_storage_of_another_shape():
     ref = O.born_adj_reg(g, med, spos, src_fine, rpos, dat, np.float64)
     mid = _model(ctx, w, g, med)
     out = _gpu_born_adj(ctx, mid, spos, rpos, src_fine, dat, g.nx, g.nz, dtype)
-    assert rel_l2(out, ref) < 1e-4, rel_l2(out, ref)
+    assert rel_l2(out, ref) < 2e-5, rel_l2(out, ref)
 
 
 @pytest.mark.parametrize("v", range(8))
@@ -474,7 +477,7 @@ def test_two_step_born_adjoint_matches_oracle(va, sub_fd):
     two = _gpu_born_adj(ctx, mid, spos, rpos, src_fine, dat, g.nx, g.nz, np.float32)
     for k in range(5):
         assert np.linalg.norm(ref[k]) > 0
-        assert rel_l2(two[k], ref[k]) < 1e-4, (va, k, rel_l2(two[k], ref[k]))
+        assert rel_l2(two[k], ref[k]) < 2e-5, (va, k, rel_l2(two[k], ref[k]))
         assert rel_l2(two[k], one[k]) < 1e-5, (va, k, rel_l2(two[k], one[k]))
     # a second call reuses the four levels and the slots: same result
     again = _gpu_born_adj(ctx, mid, spos, rpos, src_fine, dat, g.nx, g.nz, np.float32)
@@ -573,7 +576,7 @@ def test_two_step_born_source_pass_matches_oracle(v):
         assert np.linalg.norm(ref_f[k]) > 0
         assert rel_l2(two_f[k], ref_f[k]) < 1e-5, (v, k, rel_l2(two_f[k], ref_f[k]))
         assert rel_l2(two_f[k], one_f[k]) < 1e-5, (v, k, rel_l2(two_f[k], one_f[k]))
-    assert rel_l2(two_a, ref_a) < 1e-4, rel_l2(two_a, ref_a)
+    assert rel_l2(two_a, ref_a) < 2e-5, rel_l2(two_a, ref_a)
     assert rel_l2(two_a, one_a) < 1e-5, rel_l2(two_a, one_a)
 
 
@@ -600,7 +603,7 @@ def test_born_with_odd_fine_subsampling_and_time_block_two():
     out_a = _gpu_born_adj(ctx, mid, spos, rpos, src_fine, dat, g.nx, g.nz, np.float32)
     for k in range(5):
         assert rel_l2(out_f[k], ref_f[k]) < 1e-5, (k, rel_l2(out_f[k], ref_f[k]))
-    assert rel_l2(out_a, ref_a) < 1e-4, rel_l2(out_a, ref_a)
+    assert rel_l2(out_a, ref_a) < 2e-5, rel_l2(out_a, ref_a)
 
 
 def test_two_step_born_adjoint_dot_product():
@@ -823,9 +826,86 @@ def test_graph_mode_matches_eager_on_repeated_calls():
     eager = {k: f() for k, f in ops.items()}
     names = list(ops)
     for i, k in enumerate(names):
-        for rep in range(2):
+        for rep in range(3):
             ctx.set_graph(1)
             out = ops[k]()                       # each call synchronises its stream on return
             ctx.set_graph(0)
             ops[names[(i + 1) % len(names)]]()   # eager work between the graph calls
             assert all(np.array_equal(a, b) for a, b in zip(out, eager[k])), (k, rep)
+    # one graph built per operator, replayed by its later calls
+    builds, launches, updates = ctx.graph_info()
+    assert (builds, launches) == (4, 12), (builds, launches, updates)
+
+
+def test_graph_replay_of_a_batch_with_changing_inputs():
+    """Graph mode on a two-shot batch: the first call builds the graph, later calls with new
+    source traces (same device buffers) replay it unchanged and give the eager results."""
+    cases = [_case(nz=60, nx=80, nts=60, sub=2, seed=120), _case(nz=40, nx=100, nts=50, sub=2, seed=121)]
+    rng = np.random.default_rng(122)
+    ctx = _capi.Context(0, np.float32)
+    ctx.set_time_block(2, _capi.VARIANT_AUTO)
+    shots, srcs, recs, keep = [], [], [], []
+    for (w, g, med, spos, rpos) in cases:
+        mid = _model(ctx, w, g, med)
+        src = [np.zeros((spos[GRID_OF_COMP[k]].size, g.nts), np.float32) for k in range(5)]
+        rec = [np.zeros((rpos[GRID_OF_COMP[k]].size, g.nts), np.float32) for k in range(5)]
+        shot, kp = ctx.make_shot(mid, spos, rpos, src, rec)
+        shots.append(shot)
+        srcs.append(src)
+        recs.append(rec)
+        keep.append(kp)
+    for rep in range(4):
+        for (w, g, med, spos, rpos), src in zip(cases, srcs):
+            new = random_traces(rng, spos, g.nts, np.float32)
+            for k in range(5):
+                src[k][...] = new[k]
+        ctx.set_graph(0)
+        ctx.nl_forward(shots)
+        eager = [[r.copy() for r in rec] for rec in recs]
+        ctx.set_graph(1)
+        ctx.nl_forward(shots)
+        for rec, ref in zip(recs, eager):
+            assert all(np.array_equal(a, b) for a, b in zip(rec, ref)), rep
+    builds, launches, updates = ctx.graph_info()
+    assert (builds, launches) == (1, 4), (builds, launches, updates)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_grid_over_the_32bit_byte_limit_rejected(dtype):
+    """A grid whose 5-field buffers reach 2^31 bytes is an argument error at model registration,
+    before any allocation or launch (the kernels use 32-bit buffer offsets)."""
+    isz = np.dtype(dtype).itemsize
+    nz = 16384
+    nx = (1 << 31) // (5 * nz * isz) + 1        # just over the limit
+    media = np.empty((5, nx, nz), dtype)           # never touched: virtual memory only
+    ctx = _capi.Context(0, dtype)
+    with pytest.raises(_capi.EioError, match="2\\^31") as ei:
+        ctx.model_add(nz, nx, 100, 2, 10.0, 10.0, 0.001, 20, media)
+    assert "error 1:" in str(ei.value)          # EIO_ERR_ARG
+    ctx.close()
+
+
+def test_learned_boundary_rejected_outside_nl_forward():
+    """With a learned boundary attached, the NL adjoint, Born and FWI calls refuse to run (their
+    operator would not be the forward's adjoint); the NL forward runs; detached, all run."""
+    from elastic_isotropic_operators_amd import abc_learn as A
+    w, g, med, spos, rpos = _case(nts=40, sub=2, seed=130)
+    ctx = _capi.Context(0, np.float32)
+    mid = _model(ctx, w, g, med)
+    rng = np.random.default_rng(131)
+    src = [np.ascontiguousarray(x, np.float32) for x in random_traces(rng, spos, g.nts, np.float64)]
+    rec = [np.zeros((rpos[GRID_OF_COMP[k]].size, g.nts), np.float32) for k in range(5)]
+    shot, keep = ctx.make_shot(mid, spos, rpos, src, rec)
+    net = A.AbcNet("cuda", seed=3)
+    ctx.set_abc(mid, 1, net.flat(), np.ones(10, np.float32))
+    ctx.nl_forward([shot])
+    dm = np.zeros((5, g.nx, g.nz), np.float32)
+    for call in (lambda: ctx.nl_adjoint([shot]), lambda: ctx.born_adjoint([shot], dm),
+                 lambda: ctx.born_forward([shot], dm), lambda: ctx.fwi_forward([shot], 0)):
+        with pytest.raises(_capi.EioError, match="learned boundary"):
+            call()
+    ctx.set_abc(mid, 0)
+    ctx.nl_adjoint([shot])
+    h = ctx.fwi_forward([shot], 0)
+    ctx.fwi_release(h)
+    ctx.close()

@@ -314,30 +314,38 @@ def test_autograd_chain_rule_through_conversion_and_datcomp():
 
 
 def test_full_size_gradient_fp32_within_1e4_of_fp64():
-    """One shot of the 32-shot FWI workload at full size (600x1592 padded, nts=3000, sub=2,
-    55 checkpoints): the FP32 checkpointed gradient is within 1e-4 relative L2 of the FP64
-    one (the FP64 GPU path matches the FP64 oracle to 1e-10 on the small cases above)."""
+    """Two shots (the first and the last) of the 32-shot FWI workload at full size (600x1592
+    padded, nts=3000, sub=2) through the autograd path with the forward wavefield checkpointed
+    every ceil(sqrt(nts-1)) = 55 coarse intervals (54 checkpoints, each segment recomputed in the
+    backward): the FP32 gradient is within 1e-4 relative L2 of the FP64 one (the FP64 GPU path
+    matches the FP64 oracle to 1e-10 on the small cases above)."""
     import torch
     from elastic_isotropic_operators_amd import workloads as W
     from elastic_isotropic_operators_amd.fwi import ElasticFwi, datcomp
     wt, w0 = W.fwi_c3(nshots=32)
-    e = slice(16, 17)
+    e = [0, 31]
+    seg = int(np.ceil(np.sqrt(w0["nts"] - 1)))
+    assert seg == 55
     grid = dict(nz=w0["nz"], nx=w0["nx"], dz=w0["dz"], dx=w0["dx"], oz=w0["oz"], ox=w0["ox"], nts=w0["nts"],
                 dts=w0["dts"], sub=w0["sub"], minPad=W.min_pad(w0), alphaCos=w0["alphaCos"])
     grads = {}
     d_obs = None
     for dt in (torch.float64, torch.float32):
-        eng = ElasticFwi(grid, w0["src_z"][e], w0["src_x"][e], w0["model"], w0["rec_z"], w0["rec_x"], dtype=dt)
+        eng = ElasticFwi(grid, w0["src_z"][e], w0["src_x"][e], w0["model"], w0["rec_z"], w0["rec_x"], dtype=dt,
+                         segment=seg)
         if d_obs is None:
             with torch.no_grad():
                 d_obs = datcomp(eng.forward_data(torch.tensor(wt["elastic"], dtype=dt, device="cuda"))[0])
         m = torch.tensor(w0["elastic"], dtype=dt, device="cuda", requires_grad=True)
         eng.misfit(m, d_obs.to(dt), components="vx,vz,p").backward()
+        assert eng.last_store_info[:2] == (seg, 54), eng.last_store_info
         grads[dt] = m.grad.double().cpu().numpy()
         del eng, m
         torch.cuda.empty_cache()
     assert np.linalg.norm(grads[torch.float64]) > 0
-    assert rel_l2(grads[torch.float32], grads[torch.float64]) < 1e-4, rel_l2(grads[torch.float32], grads[torch.float64])
+    err = rel_l2(grads[torch.float32], grads[torch.float64])
+    print("C3 checkpointed gradient FP32 vs FP64 (shots 0, 31): rel L2 %.3e" % err, flush=True)
+    assert err < 1e-4, err
 
 
 def test_autograd_gradient_is_deterministic():

@@ -27,6 +27,38 @@ def test_c_abi_exports_every_declared_symbol():
     assert set(_capi.EXPORTED) <= names
 
 
+def test_grid_check_rejects_grids_over_the_32bit_byte_limit():
+    """eio_grid_check (no device): 5*nz*nx*precision must stay below 2^31 bytes; the grid just
+    under the limit passes, the one just over it is EIO_ERR_ARG, in FP32 and FP64."""
+    for prec in (4, 8):
+        nz = 16384
+        nx_max = ((1 << 31) - 1) // (5 * nz * prec)
+        assert _capi.grid_check(nz, nx_max, 100, 2, 10.0, 10.0, 0.001, 20, prec) == 0
+        with pytest.raises(_capi.EioError, match=r"error 1: grid too large"):
+            _capi.grid_check(nz, nx_max + 1, 100, 2, 10.0, 10.0, 0.001, 20, prec)
+    with pytest.raises(_capi.EioError, match="Subsampling parameter too high"):
+        _capi.grid_check(100, 100, 100, 101, 10.0, 10.0, 0.001, 20, 4)
+    with pytest.raises(_capi.EioError, match="Padding value is too high"):
+        _capi.grid_check(100, 100, 100, 2, 10.0, 10.0, 0.001, 500, 4)
+
+
+def test_fwi_group_and_segment_choice():
+    """misfit_and_gradient's group size and store segment: full coarse storage (one segment of
+    nts-1 intervals) only for a group that fits; a caller's group that does not fit, or no room
+    for even one shot, takes checkpoints (interval `segment`, 0 = ceil(sqrt(nts-1)));
+    checkpoint=L forces checkpoints (all shots in one group unless a group is given)."""
+    from elastic_isotropic_operators_amd.fwi import group_and_segment as gs
+    nts = 3000
+    assert gs(None, 7, 32, nts, 0) == (7, nts - 1)          # default: as many as fit
+    assert gs(4, 7, 32, nts, 0) == (4, nts - 1)             # smaller group than fits
+    assert gs(10, 7, 32, nts, 0) == (10, 0)                 # caller's group too big: checkpoints
+    assert gs(10, 7, 32, nts, 55) == (10, 55)
+    assert gs(None, 0, 32, nts, 0) == (1, 0)                # not even one shot fits
+    assert gs(None, 40, 32, nts, 0) == (32, nts - 1)        # clipped to the shot count
+    assert gs(None, 7, 32, nts, 0, checkpoint=0) == (32, 0)
+    assert gs(8, 7, 32, nts, 0, checkpoint=55) == (8, 55)
+
+
 def _conv_elastic(args):
     par = genericIO.io(params=args)
     el = genericIO.defaultIO.getVector(par.getString("elasticParam"), ndims=3)
@@ -139,6 +171,27 @@ def test_sinc_gauss_interp_matches_oracle(method, nfilt):
     a = float(np.sum(npy.forward(reg).astype(np.float64) * irr))
     b = float(np.sum(reg.astype(np.float64) * npy.adjoint(irr)))
     assert abs(a - b) < 1e-5 * abs(a)
+
+
+@pytest.mark.parametrize("zsh,xsh", [(1.0, 0.0), (-0.5, 1.5)])
+def test_dipole_interp_matches_oracle(zsh, xsh):
+    """The C++ interpolator with dipole=1 (pybind spaceInterpGpu, spaceInterpGpu.cpp:108-137)
+    against the oracle's dipole stencil: same 8 points per device, unique list, weights to
+    float rounding."""
+    from elastic_isotropic_operators_amd import pyElastic_iso_float_nl as nl
+    from elastic_isotropic_operators_amd.sep import Hypercube
+    rng = np.random.default_rng(19)
+    nz, oz, dz, ox, dx = 70, -40.0, 10.0, -45.0, 12.5
+    z = (150.0 + 300.0 * rng.random(20)).astype(np.float32)
+    x = (150.0 + 400.0 * rng.random(20)).astype(np.float32)
+    h = Hypercube.hypercube(axes=[Hypercube.axis(n=nz, o=oz, d=dz), Hypercube.axis(n=80, o=ox, d=dx)])
+    cpp = nl.spaceInterpGpu(z, x, h, 10, "linear", 1, 1, zsh, xsh)
+    idx, w, uniq = O.interp(nz, oz, dz, ox, dx, z.astype(np.float64), x.astype(np.float64), "linear", 1, 1,
+                            zsh, xsh)
+    assert list(cpp.getRegPosUnique()) == list(uniq)
+    assert np.array_equal(np.array(cpp.getRegPos()).reshape(-1, 8), idx)
+    wc = np.array(cpp.getWeights(), np.float64).reshape(-1, 8)
+    assert np.abs(wc - w).max() < 1e-5
 
 
 def test_duplicate_gpu_ids_rejected_before_device_use(tmp_path):

@@ -7,7 +7,9 @@ transposed operand, wrong scaling) fails it:
   * staggering, spatial and time interpolation: transposition (dot) tests and exact cases;
   * propagators: FP64 dot-product tests (NL and Born) at 1e-12;
   * physics: P-wave moveout in a homogeneous medium (speed of the discrete wave equation),
-    Born operator vs. finite differences of the NL operator (ties Born to NL), sponge decay;
+    Born operator vs. finite differences of the NL operator (ties Born to NL); the energy,
+    sponge, plane-wave and two-layer pins are in test_oracle_physics_cpu.py;
+  * dipole acquisition: difference of two bilinear stencils, exact on linear fields;
   * FP32 twin vs FP64 within 1e-5 (the FP32 parity bar is reachable).
 """
 import json
@@ -157,6 +159,37 @@ def test_sinc_gauss_interp_closed_forms():
         errs.append(abs(w[0] @ f - np.cos(kz * z + kx * x)))
         errs_t.append(abs(wt @ f - np.cos(kz * z + kx * x)))
     assert np.mean(errs) < 8e-3 and max(errs) < 2e-2 and np.mean(errs_t) > 1.5 * np.mean(errs)
+
+
+@pytest.mark.parametrize("zsh,xsh", [(1.0, 0.0), (0.0, 2.5), (-1.3, 0.7), (0.4, 0.3)])
+def test_dipole_weights_difference_of_two_bilinear_stencils(zsh, xsh):
+    """dipole=1 (spaceInterpGpu.cpp:108-137): entries 0..3 are the device's bilinear stencil,
+    4..7 the stencil of the position shifted by (zsh dz, xsh dx) with negated weights, so the
+    dipole reads f(p) - f(p + shift); on a linear field that is exactly -(grad f . shift)."""
+    rng = np.random.default_rng(5)
+    nz, oz, dz, ox, dx = 60, -40.0, 10.0, -45.0, 12.5
+    z = 150.0 + 300.0 * rng.random(25)
+    x = 150.0 + 400.0 * rng.random(25)
+    idx, w, uniq = O.interp(nz, oz, dz, ox, dx, z, x, "linear", 1, 1, zsh, xsh)
+    i0, w0, _ = O.interp(nz, oz, dz, ox, dx, z, x)
+    i1, w1, _ = O.interp(nz, oz, dz, ox, dx, z + zsh * dz, x + xsh * dx)
+    assert np.array_equal(idx[:, :4], i0) and np.array_equal(idx[:, 4:], i1)
+    assert np.allclose(w[:, :4], w0, rtol=0, atol=1e-15) and np.allclose(w[:, 4:], -w1, rtol=0, atol=1e-15)
+    assert np.allclose(w.sum(axis=1), 0.0, atol=1e-14)
+    # unique list: first-seen order over the 8 entries of every device
+    seen = []
+    for p in idx.ravel():
+        if p not in seen:
+            seen.append(int(p))
+    assert list(uniq) == seen
+    # exact on a linear field
+    nx = 80
+    iz = np.arange(nz)[None, :]
+    ix = np.arange(nx)[:, None]
+    a, b = 0.37, -1.9
+    f = (a * (oz + iz * dz) + b * (ox + ix * dx) + 5.0).ravel()
+    got = np.sum(w * f[idx], axis=1)
+    assert np.allclose(got, -(a * zsh * dz + b * xsh * dx), rtol=1e-12, atol=1e-12)
 
 
 def test_irregular_pipeline_composition_matches_shot_oracle():
