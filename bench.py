@@ -122,6 +122,85 @@ def cpu_baseline(shots_w, target_s=12.0):
                 host_cpu=model, host_nproc=os.cpu_count())
 
 
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    O.build()
+    O.set_ftz(True)
+    return O
+
+
+def _oracle_grid(O, w, nts=None):
+    return O.grid(w["nz"], w["nx"], w["nts"] if nts is None else nts, 2 * w["sub"], w["dz"], w["dx"], w["dts"],
+                  W.min_pad(w), w["alphaCos"], w["oz"], w["ox"])
+
+
+def c0_leg(args):
+    """Unbatched leg (workload C0: one 280x472 shot, nts 1000, sub_fd 2): the two-step NL
+    forward with one launch per pair of fine steps, the same loop replayed from the context's
+    cached hipGraph, and the whole loop in one persistent launch; device time per call (HIP
+    events) after warm-up calls, best of `args.c0_reps`; plus the FP32 oracle on the host cores
+    (the whole shot)."""
+    w = W.gaussian_c0()
+    cells = W.cell_updates(w)
+    ctx = _capi.Context(0, np.float32)
+    ctx.set_time_block(2, _capi.VARIANT_AUTO)
+    shots, keep, _, _ = prepare_shots(ctx, [w])
+    out = {"workload": w["name"], "cell_updates": cells, "dtype": "f32"}
+    for mode in ("launched", "graph", "persistent"):
+        ctx.set_graph(mode == "graph")
+        ctx.set_persistent(1 if mode == "persistent" else 0)
+        for _ in range(2):
+            ctx.nl_forward(shots, device_ptrs=True)
+        ms = []
+        for _ in range(args.c0_reps):
+            ctx.nl_forward(shots, device_ptrs=True)
+            tot, nl, _ = ctx.last_timing()
+            ms.append(tot)
+        best = float(min(ms))
+        out[mode] = {"ms": round(best, 3), "gcell_per_s": round(cells / best * 1e-6, 2), "launches": int(nl)}
+        log("c0 %s: %.3f ms per propagation (%d launches), %.2f GCell/s" % (mode, best, nl, cells / best * 1e-6))
+    b, l, u = ctx.graph_info()
+    out["graph"]["builds"], out["graph"]["replays"] = b, l - b
+    ctx.set_graph(0)
+    ctx.set_persistent(0)
+    ctx.close()
+    if not args.no_cpu:
+        O = _oracle()
+        t0 = time.perf_counter()
+        O.nl_fwd_shot(_oracle_grid(O, w), w["elastic"], w["src_z"][0], w["src_x"][0], w["rec_z"], w["rec_x"],
+                      w["model"], np.float32)
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(cells / el / 1e9, 3), "unit": "GCell/s", "cores": O.num_threads(),
+                               "kind": "port", "sample": "the whole shot, FP32 oracle with flush-to-zero, %.1f s" % el}
+        log("c0 cpu: %.1f s, %.3f GCell/s" % (el, cells / el / 1e9))
+    return out
+
+
+def fwi_cpu_baseline(w0, target_s=10.0):
+    """FP32 oracle FWI gradient of shot 0 on the host cores (NL forward of the predicted data +
+    Born adjoint of the residual: the reference's three propagation passes), trace length cut to
+    about `target_s` seconds and extrapolated linearly in nts to the full length."""
+    O = _oracle()
+
+    def run(nts):
+        g = _oracle_grid(O, w0, nts)
+        m = np.ascontiguousarray(w0["model"][:, :, :nts])
+        t0 = time.perf_counter()
+        d = O.nl_fwd_shot(g, w0["elastic"], w0["src_z"][0], w0["src_x"][0], w0["rec_z"], w0["rec_x"], m, np.float32)
+        O.born_adj_shots(g, w0["elastic"], w0["src_z"][:1], w0["src_x"][:1], m, w0["rec_z"], w0["rec_x"],
+                         d[None].astype(np.float32), np.float32)
+        return time.perf_counter() - t0
+
+    el = run(21)
+    n = int(min(w0["nts"], max(21, 1 + 20 * target_s / max(el, 1e-3))))
+    el = run(n)
+    s_shot = el * (w0["nts"] - 1) / (n - 1)
+    return {"value": round(s_shot, 2), "unit": "s/shot", "cores": O.num_threads(), "kind": "port",
+            "sample": "shot 0, %d of %d coarse samples (%.1f s), extrapolated linearly in nts; FP32 oracle, "
+                      "NL forward + Born adjoint" % (n, w0["nts"], el)}
+
+
 def fwi_child(args):
     """FWI gradient leg (child process): torch first, then the propagator library."""
     import torch
@@ -181,6 +260,36 @@ def fwi_child(args):
     rel = float(torch.linalg.norm(gs[torch.float32] - gs[torch.float64]) / torch.linalg.norm(gs[torch.float64]))
     log("fwi: FP32 vs FP64 gradient on shots %s: rel L2 %.2e" % (sel, rel))
     t = float(np.mean(times))
+    # checkpointed form (the configuration's sqrt(nts) snapshots): the autograd path over all
+    # shots at once, torch.autograd.Function -> eio_fwi_forward / eio_fwi_gradient
+    ck = None
+    if args.fwi_segment >= 0:
+        eng.ctx.release_workspace()
+        seg = args.fwi_segment or int(np.ceil(np.sqrt(w0["nts"] - 1)))
+        engc = ElasticFwi(grid, w0["src_z"], w0["src_x"], w0["model"], w0["rec_z"], w0["rec_x"],
+                          dtype=torch.float32, segment=seg, adj_time_block=adj)
+
+        def grad_ck():
+            mm = torch.tensor(w0["elastic"], device="cuda", requires_grad=True)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            engc.misfit(mm, d_obs, components=comps).backward()
+            torch.cuda.synchronize()
+            return time.perf_counter() - t1, mm.grad.double()
+
+        tw, _ = grad_ck()
+        log("fwi checkpointed warmup: %.2f s" % tw)
+        tc, gck = grad_ck()
+        Lc, nckc, nbc = engc.last_store_info
+        rel_ck = float(torch.linalg.norm(gck - g32) / torch.linalg.norm(g32))
+        log("fwi checkpointed (interval %d, %d checkpoints, %.1f GB): %.2f s (%.3f s/shot), rel L2 vs full "
+            "storage %.2e" % (Lc, nckc, nbc / 1e9, tc, tc / engc.nshot, rel_ck))
+        ck = {"s_per_shot": round(tc / engc.nshot, 4), "s_per_gradient": round(tc, 3), "checkpoint_interval": Lc,
+              "checkpoints": nckc, "store_bytes": nbc, "rel_l2_vs_full_storage_fp32": rel_ck,
+              "gcell_per_s": round(eng.nshot * per_interval * (2 * (w0["nts"] - 1) + Lc * nckc) / tc / 1e9, 2),
+              "method": "torch.autograd.Function over all shots, forward wavefield checkpointed"}
+        engc.ctx.release_workspace()
+        del engc
     out = {"s_per_shot": round(t / eng.nshot, 4), "s_per_gradient": round(t, 3), "shots": eng.nshot,
            "workload": w0["name"], "dtype": "f32", "steps": args.fwi_steps, "warmup": args.fwi_warmup,
            "gcell_per_s": round(grad_cells / t / 1e9, 2), "rel_l2_vs_fp64": rel, "fp64_check_shots": sel,
@@ -188,7 +297,11 @@ def fwi_child(args):
            "method": "shot groups with full coarse Vx/Vz storage sized to HBM, gradient per group",
            "adjoint_time_block": list(adj),
            "full_storage_bytes": 2 * 4 * w0["nz"] * w0["nx"] * w0["nts"] * eng.nshot,
-           "grad_norm": float(torch.linalg.norm(g32))}
+           "grad_norm": float(torch.linalg.norm(g32)), "checkpointed": ck}
+    if not args.no_cpu:
+        t0 = time.perf_counter()
+        out["cpu_baseline"] = fwi_cpu_baseline(w0)
+        log("fwi cpu baseline: %.1f s/shot (%.1f s)" % (out["cpu_baseline"]["value"], time.perf_counter() - t0))
     print("FWI_JSON " + json.dumps(out), flush=True)
 
 
@@ -196,7 +309,9 @@ def run_fwi_leg(args):
     import subprocess
     cmd = [sys.executable, os.path.abspath(__file__), "--fwi-child", "--fwi-shots", str(args.fwi_shots),
            "--fwi-steps", str(args.fwi_steps), "--fwi-warmup", str(args.fwi_warmup),
-           "--fwi-scale", str(args.fwi_scale), "--fwi-adj", args.fwi_adj]
+           "--fwi-scale", str(args.fwi_scale), "--fwi-adj", args.fwi_adj, "--fwi-segment", str(args.fwi_segment)]
+    if args.no_cpu:
+        cmd.append("--no-cpu")
     r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=900)
     for line in r.stdout.splitlines():
         if line.startswith("FWI_JSON "):
@@ -250,6 +365,26 @@ def born_child(args):
             ta = time.perf_counter() - t0
             res[name] = (tf, ta, data.getNdArray().astype(np.float64), img.getNdArray().astype(np.float64))
             log("born %s: forward %.3f s, adjoint %.3f s" % (name, tf, ta))
+    if not args.no_cpu:
+        # FP32 oracle Born forward + adjoint on the host cores, shot by shot until ~10 s,
+        # extrapolated to the 8 shots
+        O = _oracle()
+        g = _oracle_grid(O, w)
+        sz, sx = W.source_coords(w, 3, 26, nExp=8, spacingShots=50)
+        rz, rx = W.receiver_coords(w, 200, 1, 1, 2)
+        dsh = res["f32"][2]
+        el, n = 0.0, 0
+        while n < 8 and el < 10.0:
+            t0 = time.perf_counter()
+            O.born_fwd_shot(g, w["elastic"], sz[n], sx[n], w["model"], rz, rx, dm, np.float32)
+            O.born_adj_shots(g, w["elastic"], sz[n:n + 1], sx[n:n + 1], w["model"], rz, rx,
+                             dsh[n:n + 1].astype(np.float32), np.float32)
+            el += time.perf_counter() - t0
+            n += 1
+        cpu = {"value": round(el / n * 8, 2), "unit": "s per Born forward + adjoint of the 8 shots",
+               "cores": O.num_threads(), "kind": "port",
+               "sample": "%d of 8 shots (%.1f s), extrapolated; FP32 oracle" % (n, el)}
+        log("born cpu: %.2f s per forward + adjoint of 8 shots (%d shots timed)" % (cpu["value"], n))
     ntw = (w["nts"] - 1) * 2 * w["sub"] + 1
     cells = 8 * (w["nz"] - 8) * (w["nx"] - 8) * (ntw - 1)   # one pass of the 8 shots
     rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
@@ -261,12 +396,14 @@ def born_child(args):
            "f64_forward_s": round(res["f64"][0], 4), "f64_adjoint_s": round(res["f64"][1], 4),
            "rel_l2_data_f32_vs_f64": rel(d32, res["f64"][2]), "rel_l2_image_f32_vs_f64": rel(i32, res["f64"][3]),
            "timing": "wall time of the operator call (host interpolation and transfers included)"}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu
     print("BORN_JSON " + json.dumps(out), flush=True)
 
 
 def run_born_leg(args):
     import subprocess
-    cmd = [sys.executable, os.path.abspath(__file__), "--born-child"]
+    cmd = [sys.executable, os.path.abspath(__file__), "--born-child"] + (["--no-cpu"] if args.no_cpu else [])
     r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=600)
     for line in r.stdout.splitlines():
         if line.startswith("BORN_JSON "):
@@ -327,6 +464,10 @@ def main():
     ap.add_argument("--fwi-adj", default="2:7",
                     help="reverse fine steps per launch of the gradient's receiver pass and the two-step "
                          "adjoint variant, as steps:variant")
+    ap.add_argument("--fwi-segment", type=int, default=0,
+                    help="checkpoint interval of the checkpointed FWI gradient (0: ceil(sqrt(nts-1)); -1: skip it)")
+    ap.add_argument("--c0", type=int, default=1, help="run the unbatched C0 leg (launched, graph, persistent)")
+    ap.add_argument("--c0-reps", type=int, default=5)
     ap.add_argument("--fwi-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--abc", type=int, default=1, help="run the learned-boundary leg (needs its weights)")
     ap.add_argument("--born", type=int, default=1, help="run the Born operator leg (workload C1)")
@@ -437,9 +578,13 @@ def main():
     }
     if args.copy_probe:
         nb = 2 << 30
-        cms = ctx.copy_probe(nb, 10)
+        cms = ctx.copy_probe(nb, 10, kind=1)
         res["roofline"]["copy_probe_GBs"] = round(2 * nb / (cms / 1e3) / 1e9, 1)
-        log("copy probe: %.3f ms for 2 x %d bytes -> %.0f GB/s" % (cms, nb, 2 * nb / cms / 1e6))
+        log("copy probe (one float4 per thread, nontemporal stores): %.3f ms for 2 x %d bytes -> %.0f GB/s" %
+            (cms, nb, 2 * nb / cms / 1e6))
+        cms0 = ctx.copy_probe(nb, 10, kind=0)
+        res["roofline"]["copy_probe_gridstride_GBs"] = round(2 * nb / (cms0 / 1e3) / 1e9, 1)
+        log("copy probe (grid-stride float4): %.3f ms -> %.0f GB/s" % (cms0, 2 * nb / cms0 / 1e6))
     if os.path.exists(TRAFFIC_FILE) and args.workload == "c2":
         # measured HBM bytes per cell-update of this kernel (rocprofv3 FETCH_SIZE/WRITE_SIZE,
         # copy-kernel calibrated) times the cell-updates of this run's average launch
@@ -452,6 +597,8 @@ def main():
     # the secondary legs run in child processes; one that fails is reported in its own field
     # and does not cost the headline line
     legs = []
+    if args.c0:
+        legs.append(("unbatched_c0", "unbatched C0", c0_leg))
     if args.fwi:
         legs.append(("fwi_gradient", "fwi", run_fwi_leg))
     if args.born:

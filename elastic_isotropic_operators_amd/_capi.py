@@ -71,7 +71,7 @@ def lib():
             ("eio_device_free", [vp, vp]),
             ("eio_memcpy", [vp, vp, vp, ctypes.c_ulonglong, ctypes.c_int]),
             ("eio_synchronize", [vp]),
-            ("eio_copy_probe", [vp, vp, vp, ctypes.c_ulonglong, ctypes.c_int,
+            ("eio_copy_probe", [vp, vp, vp, ctypes.c_ulonglong, ctypes.c_int, ctypes.c_int,
                                 ctypes.POINTER(ctypes.c_double)]),
             ("eio_ctx_set_variant", [vp, ctypes.c_int]),
             ("eio_ctx_variant_info", [ctypes.c_int, ctypes.POINTER(ctypes.c_int),
@@ -93,6 +93,7 @@ def lib():
                                                  ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
             ("eio_ctx_set_adjoint_time_block", [vp, ctypes.c_int, ctypes.c_int]),
             ("eio_ctx_set_step_limit", [vp, ctypes.c_int]),
+            ("eio_ctx_set_persistent", [vp, ctypes.c_int, ctypes.c_int]),
             ("eio_nl_forward_wavefield", [vp, ctypes.c_int, ctypes.POINTER(Shot), ctypes.c_int, ctypes.c_int, vp]),
             ("eio_nl_adjoint_wavefield", [vp, ctypes.c_int, ctypes.POINTER(Shot), ctypes.c_int, ctypes.c_int, vp]),
         ]:
@@ -110,7 +111,7 @@ EXPORTED = ("eio_last_error", "eio_ctx_create", "eio_ctx_destroy", "eio_ctx_set_
             "eio_synchronize", "eio_copy_probe", "eio_ctx_set_variant", "eio_ctx_variant_info",
             "eio_ctx_num_variants", "eio_fwi_forward", "eio_fwi_gradient", "eio_fwi_info",
             "eio_fwi_release", "eio_ctx_num_time_block_variants", "eio_ctx_time_block_info",
-            "eio_ctx_set_time_block", "eio_ctx_set_step_limit", "eio_nl_forward_wavefield",
+            "eio_ctx_set_time_block", "eio_ctx_set_step_limit", "eio_ctx_set_persistent", "eio_nl_forward_wavefield",
             "eio_nl_adjoint_wavefield", "eio_model_set_abc", "eio_nl_forward_snapshots",
             "eio_nl_step_states", "eio_ctx_release_workspace", "eio_ctx_num_adjoint_time_block_variants",
             "eio_ctx_adjoint_time_block_info", "eio_ctx_set_adjoint_time_block")
@@ -178,13 +179,14 @@ class Context:
         check(lib().eio_memcpy(self.h, ctypes.c_void_p(arr.ctypes.data), ctypes.c_void_p(dptr),
                                arr.nbytes, 2))
 
-    def copy_probe(self, nbytes, reps=10):
-        """Mean time [ms] of a plain device copy of nbytes (two buffers allocated here)."""
+    def copy_probe(self, nbytes, reps=10, kind=1):
+        """Mean time [ms] of a device copy of nbytes (two buffers allocated here): kind 1 one
+        16-byte vector per thread with nontemporal stores, kind 0 a grid-stride copy."""
         a, b = self.alloc(nbytes), self.alloc(nbytes)
         ms = ctypes.c_double()
         try:
             check(lib().eio_copy_probe(self.h, ctypes.c_void_p(b), ctypes.c_void_p(a), int(nbytes),
-                                       int(reps), ctypes.byref(ms)))
+                                       int(reps), int(kind), ctypes.byref(ms)))
         finally:
             self.free(a)
             self.free(b)
@@ -225,6 +227,11 @@ class Context:
             check(lib().eio_ctx_adjoint_time_block_info(v, ctypes.byref(r), ctypes.byref(wx), ctypes.byref(oc)))
             out.append((r.value, wx.value, oc.value))
         return out
+
+    def set_persistent(self, mode, spin_limit=0):
+        """Two-step NL forward in one persistent launch: 0 off, 1 on, 2 when every tile gets its own
+        co-resident workgroup; spin_limit > 0 bounds each neighbour wait (polls)."""
+        check(lib().eio_ctx_set_persistent(self.h, int(mode), int(spin_limit)))
 
     def set_step_limit(self, max_steps):
         """Stop NL loops after max_steps fine steps (0: no limit); for tuning runs."""

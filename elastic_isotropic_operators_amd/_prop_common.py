@@ -65,8 +65,8 @@ def make(real):
         coords, (n, o, d) = G.parfile_source_coords(parObject, oz, dz, ox, dx, genericIO.defaultIO.getVector)
         sourceAxis = Hypercube.axis(n=n, o=o, d=d)
         for zarr, xarr in coords:
-            zc = SepVector.getSepVector(ns=[len(zarr)])
-            xc = SepVector.getSepVector(ns=[len(xarr)])
+            zc = SepVector.getSepVector(ns=[len(zarr)], storage=storage)
+            xc = SepVector.getSepVector(ns=[len(xarr)], storage=storage)
             zc.getNdArray()[:] = zarr
             xc.getNdArray()[:] = xarr
             for l, s in zip(lists, _interp4(zc, xc, hypers, nts, method, nfilt, dipole,
@@ -88,8 +88,8 @@ def make(real):
         zarr, xarr, (n, o, d) = G.parfile_receiver_coords(parObject, oz, dz, ox, dx,
                                                           genericIO.defaultIO.getVector)
         receiverAxis = Hypercube.axis(n=n, o=o, d=d)
-        zc = SepVector.getSepVector(ns=[len(zarr)])
-        xc = SepVector.getSepVector(ns=[len(xarr)])
+        zc = SepVector.getSepVector(ns=[len(zarr)], storage=storage)
+        xc = SepVector.getSepVector(ns=[len(xarr)], storage=storage)
         zc.getNdArray()[:] = zarr
         xc.getNdArray()[:] = xarr
         grids = _interp4(zc, xc, hypers, nts, method, nfilt, dipole, zDipoleShift, xDipoleShift)

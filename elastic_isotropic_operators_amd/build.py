@@ -55,7 +55,7 @@ def build_pybind():
             cmd = (["g++"] + _pybind_flags() +
                    ["-DEIO_REAL=%s" % ctype, "-DEIO_MODULE=%s" % name,
                     "-DEIO_KIND_%s=1" % kind.upper(), "host/pymodule.cpp"] + srcs +
-                   (["-DEIO_DEFINE_INTERP=1"] if (prec, kind) == ("float", "nl") else []) +
+                   (["-DEIO_DEFINE_INTERP=1"] if kind == "nl" else []) +
                    ["-o", out + ".tmp", "-L" + HERE, "-leio_hip", "-Wl,-rpath,$ORIGIN"])
             jobs.append((cmd, out))
     procs = [(subprocess.Popen(cmd, cwd=CSRC, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True), out)

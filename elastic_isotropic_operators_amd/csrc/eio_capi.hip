@@ -162,6 +162,8 @@ struct Ctx {
     int adj_time_block = 1;  // reverse fine steps per launch of the FP32 Born adjoint (1 or 2)
     int variant2a = 0;       // two-step adjoint kernel variant, or EIO_VARIANT_AUTO
     int step_limit = 0;  // > 0: NL loops stop after this many fine steps (tuning runs)
+    int persistent = 0;  // two-step NL forward in one persistent launch: 0 off, 1 on, 2 when it fits
+    int spin_limit = 1 << 20;  // polls per neighbour wait of the persistent launch before it aborts
     std::map<int, FwiStore> fwi;
     int next_fwi = 1;
     bool fwi_ws_busy = false;  // a store holds the workspace's "fwi_ring" / "fwi_ck" buffers
@@ -1062,6 +1064,99 @@ void launch_abc(Run<T>& run, const AbcRun& a, int k) {
     }
 }
 
+// ------------------------------------------------------------------ persistent launch
+// Neighbour table of a FWD2 plan: per tile the (up to 8) adjacent tiles of the same shot (the
+// halo of a two-step tile is 8 cells, no wider than a tile), -1 where there is none.
+std::vector<int> plan_neighbours(Ctx* c, const Plan& p, const eio_shot* shots, int TX) {
+    std::vector<int> nb(p.tiles.size() * 8, -1);
+    int base = 0;
+    for (size_t oi = 0; oi < p.order.size(); ++oi) {
+        const ModelBase& m = get_model(c, shots[p.order[oi]].model);
+        const int nbz = (m.g.nz - 2 * eio::FAT + eio::TZ2 - 1) / eio::TZ2;
+        const int nbx = (m.g.nx - 2 * eio::FAT + TX - 1) / TX;
+        for (int bx = 0; bx < nbx; ++bx)
+            for (int bz = 0; bz < nbz; ++bz) {
+                int* o = nb.data() + size_t(base + bx * nbz + bz) * 8;
+                int q = 0;
+                for (int dx = -1; dx <= 1; ++dx)
+                    for (int dz = -1; dz <= 1; ++dz) {
+                        if (!dx && !dz) continue;
+                        const int x = bx + dx, z = bz + dz;
+                        if (x >= 0 && x < nbx && z >= 0 && z < nbz) o[q] = base + x * nbz + z;
+                        ++q;
+                    }
+            }
+        base = p.tile_end[oi];
+    }
+    return nb;
+}
+
+// Co-resident workgroups of a kernel on the context's device.
+int coresident_blocks(Ctx* c, const void* fn, int threads) {
+    int per_cu = 0, ncu = 0;
+    EIO_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0));
+    EIO_CHECK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+    return per_cu * ncu;
+}
+
+// The persistent kernel of two-step variant v (same tile width; the variants with nontemporal
+// stores map to their cached-store twins: results are re-read by the next pair).
+template <typename F>
+void with_persist_kernel(int v, F&& f) {
+    switch (v) {
+        case 0: case 1: case 5: case 6: f(eio::k_step2_fwd_persist<float, 2, 8, 2>, 8); break;  // 2,8,4 spills
+        case 2: f(eio::k_step2_fwd_persist<float, 4, 4, 2>, 4); break;
+        case 3: f(eio::k_step2_fwd_persist<float, 1, 8, 4>, 8); break;
+        default: f(eio::k_step2_fwd_persist<float, 4, 8, 1>, 8); break;
+    }
+}
+
+// Whether the two-step NL forward of this run takes the persistent launch.
+bool use_persistent(Ctx* c, const Run<float>& run) {
+    if (c->persistent == 0) return false;
+    if (c->persistent == 1) return true;
+    bool fits = false;
+    with_persist_kernel(run.v2, [&](auto kern, int wx) {
+        fits = long(run.plan.tiles.size()) <= coresident_blocks(c, reinterpret_cast<const void*>(kern), 64 * wx);
+    });
+    return fits;
+}
+
+// The whole time loop of a two-step NL forward run in one cooperative launch; returns after
+// checking the abort flag.
+void launch_persistent(Run<float>& run, const eio_shot* shots, int kmax) {
+    Ctx* c = run.c;
+    const Plan& p = run.plan;
+    const int ntile = int(p.tiles.size());
+    const int TX = F32_VARIANTS2[run.v2].R2 * F32_VARIANTS2[run.v2].WX;
+    const std::vector<int> nb = plan_neighbours(c, p, shots, TX);
+    int* d_nb = static_cast<int*>(ws_get(c, "persist_nbr", nb.size() * sizeof(int)));
+    int* d_done = static_cast<int*>(ws_get(c, "persist_done", (size_t(ntile) + 64) * sizeof(int)));
+    int* d_abort = d_done + ntile;
+    EIO_CHECK_HIP(hipMemcpyAsync(d_nb, nb.data(), nb.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    fill0<int>(c, d_done, long(ntile) + 1);
+    const int npair = (kmax + 1) / 2;
+    with_persist_kernel(run.v2, [&](auto kern, int wx) {
+        const int cap = coresident_blocks(c, reinterpret_cast<const void*>(kern), 64 * wx);
+        if (cap < 1) fail("persistent launch: the kernel does not fit on a CU");
+        int G = std::min(ntile, cap);
+        if (G < ntile && G >= 8) G -= G % 8;  // several tiles per workgroup: keep them on one XCD
+        const eio::ShotDev<float>* sh = run.d_shots;
+        const eio::TileDev* tt = run.d_tiles;
+        const eio::Entry* te = run.d_ents;
+        int nt = ntile, np = npair, sl = c->spin_limit;
+        void* args[] = {&sh, &tt, &te, &nt, &d_nb, &d_done, &d_abort, &np, &sl};
+        EIO_CHECK_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kern), dim3(G), dim3(64 * wx), args, 0,
+                                                 c->stream));
+    });
+    int aborted = 0;
+    EIO_CHECK_HIP(hipMemcpyAsync(&aborted, d_abort, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    EIO_CHECK_HIP(hipStreamSynchronize(c->stream));
+    if (aborted)
+        throw EioError{EIO_ERR_RUNTIME, "persistent launch: a tile waited longer than the spin limit for a "
+                                        "neighbour; the launch was abandoned (outputs are incomplete)"};
+}
+
 // Fine-step snapshots of one shot: after step steps[i] (after the boundary correction) the
 // older then the newer level, [nsnap][2][5][nx*nz].
 template <typename T>
@@ -1106,6 +1201,17 @@ void nl_forward(Ctx* c, int nshot, const eio_shot* shots, int flags, int wshot =
         }
     }
     long long nlaunch = 0;
+    if constexpr (sizeof(T) == 4) {
+        if (two && use_persistent(c, run)) {
+            timing_begin<T>(c);
+            EIO_CHECK_HIP(hipEventRecord(c->ev[2], c->stream));
+            launch_persistent(run, shots, kmax);
+            EIO_CHECK_HIP(hipEventRecord(c->ev[3], c->stream));
+            timing_end<T>(c, 1);
+            finish(run);
+            return;
+        }
+    }
     timing_begin<T>(c);
     EIO_CHECK_HIP(hipEventRecord(c->ev[2], c->stream));
     run_loop(c, [&]() {
@@ -1792,6 +1898,16 @@ int eio_ctx_adjoint_time_block_info(int variant, int* R2, int* WX, int* occ) {
 
 int eio_ctx_num_adjoint_time_block_variants(void) { return N_F32_VARIANTS2A; }
 
+int eio_ctx_set_persistent(eio_ctx* ctx, int mode, int spin_limit) {
+    return guarded([&]() {
+        if (mode < 0 || mode > 2) fail("persistent mode must be 0, 1 or 2");
+        if (spin_limit < 0) fail("spin limit must be >= 0");
+        Ctx* c = as_ctx(ctx);
+        c->persistent = mode;
+        if (spin_limit > 0) c->spin_limit = spin_limit;
+    });
+}
+
 int eio_ctx_set_step_limit(eio_ctx* ctx, int max_steps) {
     return guarded([&]() {
         if (max_steps < 0) fail("step limit must be >= 0");
@@ -2026,18 +2142,25 @@ int eio_memcpy(eio_ctx* ctx, void* dst, const void* src, unsigned long long byte
     });
 }
 
-int eio_copy_probe(eio_ctx* ctx, void* dst, const void* src, unsigned long long bytes, int reps,
+int eio_copy_probe(eio_ctx* ctx, void* dst, const void* src, unsigned long long bytes, int reps, int kind,
                    double* ms) {
     return guarded([&]() {
+        if (kind != 0 && kind != 1) fail("copy probe kind must be 0 or 1");
         Ctx* c = as_ctx(ctx);
         if (!dst || !src || bytes % 16 || reps < 1) fail("invalid copy probe arguments");
         DeviceGuard dg(c->device);
         const long n16 = long(bytes / 16);
-        const unsigned nb = unsigned(std::min<long>((n16 + eio::NTHR - 1) / eio::NTHR, 256L * 64));
+        const long nb = (n16 + eio::NTHR - 1) / eio::NTHR;
+        if (nb >= (1L << 31)) fail("copy probe too large");
         EIO_CHECK_HIP(hipEventRecord(c->ev[0], c->stream));
-        for (int r = 0; r < reps; ++r)
-            hipLaunchKernelGGL(eio::k_copy16, dim3(nb), dim3(eio::NTHR), 0, c->stream,
-                               static_cast<float4*>(dst), static_cast<const float4*>(src), n16);
+        for (int r = 0; r < reps; ++r) {
+            if (kind == 1)
+                hipLaunchKernelGGL(eio::k_copy16_nt, dim3(unsigned(nb)), dim3(eio::NTHR), 0, c->stream,
+                                   static_cast<float4*>(dst), static_cast<const float4*>(src), n16);
+            else
+                hipLaunchKernelGGL(eio::k_copy16, dim3(unsigned(std::min<long>(nb, 256L * 64))), dim3(eio::NTHR), 0,
+                                   c->stream, static_cast<float4*>(dst), static_cast<const float4*>(src), n16);
+        }
         EIO_CHECK_HIP(hipGetLastError());
         EIO_CHECK_HIP(hipEventRecord(c->ev[1], c->stream));
         EIO_CHECK_HIP(hipEventSynchronize(c->ev[1]));

@@ -844,6 +844,19 @@ __global__ __launch_bounds__(NTHR) void k_copy16(float4* __restrict__ dst,
     for (long i = long(blockIdx.x) * NTHR + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
 }
 
+// achievable-bandwidth probe: one 16-byte vector per thread (no loop), nontemporal stores
+__global__ __launch_bounds__(NTHR) void k_copy16_nt(float4* __restrict__ dst,
+                                                    const float4* __restrict__ src, long n16) {
+    const long i = long(blockIdx.x) * NTHR + threadIdx.x;
+    if (i < n16) {
+        const float4 v = src[i];
+        __builtin_nontemporal_store(v.x, &dst[i].x);
+        __builtin_nontemporal_store(v.y, &dst[i].y);
+        __builtin_nontemporal_store(v.z, &dst[i].z);
+        __builtin_nontemporal_store(v.w, &dst[i].w);
+    }
+}
+
 // element copy (checkpoint save/restore; any size and alignment)
 template <typename T>
 __global__ __launch_bounds__(NTHR) void k_copy(T* __restrict__ dst, const T* __restrict__ src, long n) {

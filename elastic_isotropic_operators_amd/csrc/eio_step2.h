@@ -64,12 +64,11 @@ __device__ __forceinline__ void bst_pol(rsrc_t r, T v, unsigned voff, unsigned s
 // ss[sspar] (coarse its) and ss[sspar^1] (its+1) with the weights of its own fine step
 // (kernelsGpuElastic.cu:1363-1368); both steps lie in one coarse interval, so they share the
 // two slots.  NOREC: no receiver recording (the segment recompute of the FWI gradient).
-template <typename T, int R2, int WX, int OCC, int NTS, bool DEC, bool FINE = false, bool SS = false,
-          bool NOREC = false>
-__global__ __launch_bounds__(64 * WX) __attribute__((amdgpu_waves_per_eu(OCC)))
-void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict__ tiles,
-                 const Entry* __restrict__ ents, int ntiles, int k, int par, int dlo, int dhi,
-                 int sspar) {
+// The two steps of one tile (every thread of the workgroup calls it with the same tile).
+template <typename T, int R2, int WX, bool NTS, bool DEC, bool FINE, bool SS, bool NOREC>
+__device__ __forceinline__ void step2_fwd_tile(const ShotDev<T>* __restrict__ shots, const TileDev t,
+                                               const Entry* __restrict__ ents, int k, int par, int dlo,
+                                               int dhi, int sspar) {
     static_assert(8 % WX == 0, "WX must divide 8");
     static_assert(sizeof(T) == 4, "the two-step forward is FP32");
     constexpr int NT = 64 * WX;
@@ -81,7 +80,6 @@ void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
     __shared__ __attribute__((aligned(16))) T lc[5][TXC][ZC];
     __shared__ T l1[5][TX1][Z1];
 
-    const TileDev t = tiles[xcd_remap(blockIdx.x, ntiles)];
     const ShotDev<T>& S = shots[t.shot];
     const int nz = S.nz, nx = S.nx;
     const unsigned n = unsigned(nz) * unsigned(nx);
@@ -438,6 +436,78 @@ void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
                 bst(rwz0, za, vo, 0u);
                 bst(rwx1, xb, vo, 0u);
                 bst(rwz1, zb, vo, 0u);
+            }
+        }
+    }
+}
+
+template <typename T, int R2, int WX, int OCC, int NTS, bool DEC, bool FINE = false, bool SS = false,
+          bool NOREC = false>
+__global__ __launch_bounds__(64 * WX) __attribute__((amdgpu_waves_per_eu(OCC)))
+void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict__ tiles,
+                 const Entry* __restrict__ ents, int ntiles, int k, int par, int dlo, int dhi,
+                 int sspar) {
+    step2_fwd_tile<T, R2, WX, NTS != 0, DEC, FINE, SS, NOREC>(shots, tiles[xcd_remap(blockIdx.x, ntiles)], ents, k,
+                                                               par, dlo, dhi, sspar);
+}
+
+// ------------------------------------------------------------------ persistent two-step forward
+// The whole NL forward time loop of a batch in ONE launch (for batches whose launches are too
+// short to hide the launch gap).  The grid is co-resident (cooperative launch); workgroup b owns
+// the tiles xcd_remap(b + i*G) (G a multiple of 8, so a workgroup's tiles lie in the tile range
+// of its own XCD) and runs them pair after pair.  Instead of a grid barrier between pairs, each
+// tile waits only for its (up to 8) neighbours: before pair p it needs done[nb] >= p for every
+// neighbour nb, i.e. the neighbours have written pair p-1's levels that this tile's stage reads
+// (8-cell halo: only adjacent tiles) and have finished reading the levels this tile overwrites.
+// A tile publishes done[t] = p+1 after its stores: every wave waits for its stores (vmcnt 0),
+// the workgroup barrier, then one thread writes the L2 back (agent-scope release) and sets the
+// flag.  A waiting thread polls with relaxed agent-scope loads and takes an agent-scope acquire
+// (L1/L2 invalidate) before the barrier that releases the workgroup.  Deadlock-free because all
+// workgroups are resident and each processes its items in (pair, tile) order; every wait is
+// bounded by `spin_limit` polls, after which the workgroup sets *abort and returns, and every
+// other workgroup returns at its next wait, so the grid always drains.
+template <typename T, int R2, int WX, int OCC>
+__global__ __launch_bounds__(64 * WX) __attribute__((amdgpu_waves_per_eu(OCC)))
+void k_step2_fwd_persist(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict__ tiles,
+                         const Entry* __restrict__ ents, int ntiles, const int* __restrict__ nbr,
+                         int* done, int* abort_flag, int npair, int spin_limit) {
+    __shared__ int s_stop;
+    const int G = int(gridDim.x);
+    for (int p = 0; p < npair; ++p) {
+        for (int v = int(blockIdx.x); v < ntiles; v += G) {
+            const int ti = xcd_remap(v, ntiles);
+            const TileDev t = tiles[ti];
+            if (2 * p >= shots[t.shot].ntw - 1) continue;   // the tile's shot has finished (uniform)
+            if (p > 0) {
+                if (threadIdx.x < 8) {
+                    const int nb = nbr[ti * 8 + threadIdx.x];
+                    int stop = 0;
+                    if (nb >= 0) {
+                        int spins = 0;
+                        while (__hip_atomic_load(&done[nb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < p) {
+                            if (++spins > spin_limit ||
+                                __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                                __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                stop = 1;
+                                break;
+                            }
+                            __builtin_amdgcn_s_sleep(1);
+                        }
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    // lanes 0..7 of wave 0: any lane that stopped stops the workgroup
+                    const bool any = __any(stop);
+                    if (threadIdx.x == 0) s_stop = any ? 1 : 0;
+                }
+                __syncthreads();
+                if (s_stop) return;
+            }
+            step2_fwd_tile<T, R2, WX, false, false, false, false, false>(shots, t, ents, 2 * p, p & 1, 0, -1, 0);
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's stores have reached L2
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                __hip_atomic_store(&done[ti], p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
     }

@@ -4,7 +4,7 @@
 // (paths relative to elastic_iso_lib/elastic_iso_float/prop):
 //   FdParam           src/fdParamElastic.cpp:12-157, checks :256-324
 //   stagger helpers   src/stagger.cpp:18-114
-//   SpaceInterp       src/spaceInterpGpu.cpp:11-138 (linear + dipole), :139-207 (sinc, gauss), :288-376,
+//   SpaceInterpT      src/spaceInterpGpu.cpp:11-138 (linear + dipole), :139-207 (sinc, gauss), :288-376,
 //                     checks :378-403
 //   time_interp_fwd   src/interpTimeLinTbb.cpp:30-58
 //   NonlinearShots    src/nonlinearPropElasticShotsGpu.cpp:80-422 + src/nonlinearPropElasticGpu.cpp:43-473
@@ -134,7 +134,8 @@ struct FdParam {
             throw std::runtime_error("**** ERROR: 2*_sub is greater than the allowed SUB_MAX value. " +
                                      std::to_string(sub) + " > " + std::to_string(SUB_MAX) + " ****");
         ntw = (nts - 1) * sub + 1;
-        dtw = double(float(dts) / float(sub));
+        // fdParamElastic.cpp:34 in the library's real type (float: float dts / float sub)
+        dtw = double(T(dts) / T(sub));
         otw = ots;
         nz = par.getInt("nz");
         zPadPlus = par.getInt("zPadPlus");
@@ -229,19 +230,22 @@ struct FdParam {
 };
 
 // ---------------------------------------------------------------- spaceInterpGpu
-struct SpaceInterp {
-    std::vector<float> zCoord, xCoord;
+// R: the library's real type, in which coordinates and weights are computed (the FP32 library
+// in float, its FP64 copy in double: elastic_iso_double/prop/src/spaceInterpGpu.cpp:83-84).
+template <typename R>
+struct SpaceInterpT {
+    std::vector<R> zCoord, xCoord;
     std::vector<Axis> hyper;
     int nt = 0, nFilt = 1, nFiltTotal = 4, dipole = 0, nDeviceIrreg = 0, nz = 0;
-    float zDipoleShift = 0.f, xDipoleShift = 0.f;
+    R zDipoleShift = 0, xDipoleShift = 0;
     std::string method;
     std::vector<int> gridPointIndex;       // nFiltTotal per device
-    std::vector<float> weight;
+    std::vector<R> weight;
     std::vector<int> uniq;                 // unique regular points, first-seen order
     std::vector<int> map;                  // per (device, filter) -> unique index
 
-    SpaceInterp(std::vector<float> z, std::vector<float> x, std::vector<Axis> h, int nt_, std::string m,
-                int nf, int dip, float zs, float xs)
+    SpaceInterpT(std::vector<R> z, std::vector<R> x, std::vector<Axis> h, int nt_, std::string m,
+                 int nf, int dip, R zs, R xs)
         : zCoord(std::move(z)), xCoord(std::move(x)), hyper(std::move(h)), nt(nt_), method(std::move(m)) {
         if (zCoord.size() != xCoord.size()) throw std::runtime_error("zCoord and xCoord sizes differ");
         if (hyper.size() < 2) throw std::runtime_error("spaceInterpGpu needs a 2-D (or 3-D) hypercube");
@@ -250,8 +254,8 @@ struct SpaceInterp {
         if (method != "linear" && nf < 1) throw std::runtime_error("**** ERROR: nFilt must be >= 1 ****");
         nFilt = method == "linear" ? 1 : nf;
         dipole = dip;
-        zDipoleShift = float(zs * hyper[0].d);
-        xDipoleShift = float(xs * hyper[1].d);
+        zDipoleShift = R(zs * hyper[0].d);
+        xDipoleShift = R(xs * hyper[1].d);
         nDeviceIrreg = int(zCoord.size());
         nz = hyper[0].n;
         check_out_of_bounds();
@@ -269,10 +273,10 @@ struct SpaceInterp {
     }
 
     void check_out_of_bounds() const {  // spaceInterpGpu.cpp:378-403
-        const float zMin = float(hyper[0].o), xMin = float(hyper[1].o);
-        const float zMax = zMin + (hyper[0].n - 1) * float(hyper[0].d);
-        const float xMax = xMin + (hyper[1].n - 1) * float(hyper[1].d);
-        const float zb = float(hyper[0].d) * nFilt, xb = float(hyper[1].d) * nFilt;
+        const R zMin = R(hyper[0].o), xMin = R(hyper[1].o);
+        const R zMax = zMin + (hyper[0].n - 1) * R(hyper[0].d);
+        const R xMax = xMin + (hyper[1].n - 1) * R(hyper[1].d);
+        const R zb = R(hyper[0].d) * nFilt, xb = R(hyper[1].d) * nFilt;
         for (int i = 0; i < nDeviceIrreg; ++i) {
             if (zCoord[i] >= zMax - zb || zCoord[i] <= zMin + zb)
                 throw std::runtime_error("**** ERROR: One of the device is out of bounds in the z direction ****");
@@ -283,26 +287,27 @@ struct SpaceInterp {
 
     void calc_linear_weights() {  // spaceInterpGpu.cpp:78-138
         gridPointIndex.assign(size_t(nFiltTotal) * nDeviceIrreg, 0);
-        weight.assign(size_t(nFiltTotal) * nDeviceIrreg, 0.f);
-        const float oz = float(hyper[0].o), dz = float(hyper[0].d), ox = float(hyper[1].o), dx = float(hyper[1].d);
+        weight.assign(size_t(nFiltTotal) * nDeviceIrreg, R(0));
+        const R oz = R(hyper[0].o), dz = R(hyper[0].d), ox = R(hyper[1].o), dx = R(hyper[1].d);
+        const R one = R(1);
         for (int d = 0; d < nDeviceIrreg; ++d) {
             const int i1 = d * nFiltTotal;
             for (int pole = 0; pole < (dipole == 1 ? 2 : 1); ++pole) {
-                float wz = (zCoord[d] + (pole ? zDipoleShift : 0.f) - oz) / dz;
-                float wx = (xCoord[d] + (pole ? xDipoleShift : 0.f) - ox) / dx;
+                R wz = (zCoord[d] + (pole ? zDipoleShift : R(0)) - oz) / dz;
+                R wx = (xCoord[d] + (pole ? xDipoleShift : R(0)) - ox) / dx;
                 const int zReg = int(wz), xReg = int(wx);
-                wz = 1.0f - (wz - zReg);
-                wx = 1.0f - (wx - xReg);
-                const float sgn = pole ? -1.0f : 1.0f;
+                wz = one - (wz - zReg);
+                wx = one - (wx - xReg);
+                const R sgn = pole ? -one : one;
                 const int b = i1 + 4 * pole;
                 gridPointIndex[b] = xReg * nz + zReg;
                 weight[b] = sgn * wz * wx;
                 gridPointIndex[b + 1] = gridPointIndex[b] + 1;
-                weight[b + 1] = sgn * (1.0f - wz) * wx;
+                weight[b + 1] = sgn * (one - wz) * wx;
                 gridPointIndex[b + 2] = gridPointIndex[b] + nz;
-                weight[b + 2] = sgn * wz * (1.0f - wx);
+                weight[b + 2] = sgn * wz * (one - wx);
                 gridPointIndex[b + 3] = gridPointIndex[b] + nz + 1;
-                weight[b + 3] = sgn * (1.0f - wz) * (1.0f - wx);
+                weight[b + 3] = sgn * (one - wz) * (one - wx);
             }
         }
     }
@@ -323,34 +328,34 @@ struct SpaceInterp {
     // spaceInterpGpu.cpp:139-207: per device the (2 nFilt)^2 regular points whose indices run
     // from int(position) - nFilt + 1 to int(position) + nFilt, x-major, with separable sinc
     // weights (sinc(pi dz_cells) sinc(pi dx_cells)) or the product of two normal densities of
-    // standard deviation 2 cells, normalised to sum 1.  Coordinates are float and the
-    // kernels double, then rounded to float, as there.  The arrays hold the whole stencil:
+    // standard deviation 2 cells, normalised to sum 1.  Coordinates are in R and the
+    // kernels double, then rounded to R, as there.  The arrays hold the whole stencil:
     // the reference's constructor resets its per-device count to 4 after choosing nFilt
     // (spaceInterpGpu.cpp:38-41), which is consistent only for nFilt = 1.
     void calc_filter_weights(bool sinc) {
         const int n2 = 2 * nFilt;
         nFiltTotal = n2 * n2;
         gridPointIndex.assign(size_t(nFiltTotal) * nDeviceIrreg, 0);
-        weight.assign(size_t(nFiltTotal) * nDeviceIrreg, 0.f);
-        const float oz = float(hyper[0].o), dz = float(hyper[0].d), ox = float(hyper[1].o), dx = float(hyper[1].d);
+        weight.assign(size_t(nFiltTotal) * nDeviceIrreg, R(0));
+        const R oz = R(hyper[0].o), dz = R(hyper[0].d), ox = R(hyper[1].o), dx = R(hyper[1].d);
         for (int d = 0; d < nDeviceIrreg; ++d) {
             const size_t off = size_t(d) * nFiltTotal;
-            const float zf = zCoord[d], xf = xCoord[d];
+            const R zf = zCoord[d], xf = xCoord[d];
             const int zr = int((zf - oz) / dz), xr = int((xf - ox) / dx);
-            float sum = 0.f;
+            R sum = 0;
             for (int ix = 0; ix < n2; ++ix)
                 for (int iz = 0; iz < n2; ++iz) {
                     const int px = xr + ix - nFilt + 1, pz = zr + iz - nFilt + 1;
-                    const float xcur = float(px) * dx + ox, zcur = float(pz) * dz + oz;
+                    const R xcur = R(px) * dx + ox, zcur = R(pz) * dz + oz;
                     double v;
                     if (sinc) {
-                        const float xin = (xf - xcur) / dx, zin = (zf - zcur) / dz;
+                        const R xin = (xf - xcur) / dx, zin = (zf - zcur) / dz;
                         v = sinc_pi(M_PI * zin) * sinc_pi(M_PI * xin);
                     } else {
-                        v = normal_pdf(xcur, xf, double(2.f * dx)) * normal_pdf(zcur, zf, double(2.f * dz));
+                        v = normal_pdf(xcur, xf, double(R(2) * dx)) * normal_pdf(zcur, zf, double(R(2) * dz));
                     }
                     const size_t k = off + size_t(n2) * ix + iz;
-                    weight[k] = float(v);
+                    weight[k] = R(v);
                     gridPointIndex[k] = nz * px + pz;
                     sum += weight[k];
                 }
@@ -408,23 +413,25 @@ struct SpaceInterp {
 };
 
 // ---------------------------------------------------------------- interpTimeLinTbb::forward
+// Times, weights and scale in T (the FP32 library in float, its FP64 copy in double:
+// elastic_iso_double/prop/src/interpTimeLinTbb.cpp:49-55).
 template <typename T>
 void time_interp_fwd(int nts, double dts, double ots, int sub, int nDev, const T* coarse, T* fine) {
     const int ntw = (nts - 1) * sub + 1;
-    const float dtw = float(dts) / float(sub);
-    float scale = float(ntw) / float(nts);
-    scale = float(1.0 / std::sqrt(double(scale)));
+    const T dtw = T(dts) / T(sub);
+    T scale = T(ntw) / T(nts);
+    scale = T(1.0 / std::sqrt(double(scale)));
     for (int d = 0; d < nDev; ++d) {
         const T* m = coarse + size_t(d) * nts;
         T* o = fine + size_t(d) * ntw;
         for (int itw = 0; itw < ntw - 1; ++itw) {
-            const float tw = float(ots) + itw * dtw;
-            float w = (tw - float(ots)) / float(dts);
+            const T tw = T(ots) + itw * dtw;
+            T w = (tw - T(ots)) / T(dts);
             const int lo = int(w);
             w = w - lo;
-            o[itw] = (m[lo] * T(1.0 - w) + m[lo + 1] * T(w)) * T(scale);
+            o[itw] = (m[lo] * T(1.0 - w) + m[lo + 1] * T(w)) * scale;
         }
-        o[ntw - 1] = T(scale) * m[nts - 1];
+        o[ntw - 1] = scale * m[nts - 1];
     }
 }
 
@@ -466,10 +473,11 @@ inline int pick_device(const Params& par) {  // createGpuIdList (nonlinearPropEl
     return 0;  // all shots run on one device; see the batched C ABI
 }
 
-using InterpVec = std::vector<std::shared_ptr<SpaceInterp>>;
+template <typename T>
+using InterpVec = std::vector<std::shared_ptr<SpaceInterpT<T>>>;
 
 template <typename T>
-void scale_sources(const FdParam<T>& fd, const SpaceInterp& ip, int k, T* reg) {
+void scale_sources(const FdParam<T>& fd, const SpaceInterpT<T>& ip, int k, T* reg) {
     // nonlinearPropElasticGpu.cpp:116-144: stresses *2dtw, vx *rhoxDtw, vz *rhozDtw, all /(dx dz)
     const size_t n = fd.ncell();
     const T two = T(2.0 * fd.dtw);
@@ -484,8 +492,9 @@ void scale_sources(const FdParam<T>& fd, const SpaceInterp& ip, int k, T* reg) {
     }
 }
 
-inline void fill_positions(eio_shot& s, const InterpVec& src4, const InterpVec& rec4) {
-    // InterpVec order: centre, X, Z, XZ
+template <typename T>
+inline void fill_positions(eio_shot& s, const InterpVec<T>& src4, const InterpVec<T>& rec4) {
+    // order of the four interpolators: centre, X, Z, XZ
     for (int g = 0; g < 4; ++g) {
         s.src_pos[g] = src4[g]->uniq.data();
         s.n_src[g] = src4[g]->nReg();
@@ -498,8 +507,8 @@ inline void fill_positions(eio_shot& s, const InterpVec& src4, const InterpVec& 
 template <typename T>
 class NonlinearShots {
    public:
-    NonlinearShots(const T* elastic, std::vector<Axis> ax, Params par, std::vector<InterpVec> sources,
-                   std::vector<InterpVec> receivers)
+    NonlinearShots(const T* elastic, std::vector<Axis> ax, Params par, std::vector<InterpVec<T>> sources,
+                   std::vector<InterpVec<T>> receivers)
         : ax_(std::move(ax)), par_(std::move(par)), src_(std::move(sources)), rec_(std::move(receivers)) {
         nExp_ = par_.getInt("nExp");
         fd_ = std::make_unique<FdParam<T>>(elastic, ax_, par_);
@@ -541,7 +550,7 @@ class NonlinearShots {
         for (int e = 0; e < nExp_; ++e)
             for (int k = 0; k < 5; ++k) {
                 const T* m = model + size_t(nModelShots == 1 ? 0 : e) * 5 * nSrc * nts;
-                const SpaceInterp& ip = *src_[e][GRID_OF_COMP[k]];
+                const SpaceInterpT<T>& ip = *src_[e][GRID_OF_COMP[k]];
                 std::vector<T>& r = srcReg[size_t(e) * 5 + k];
                 r.resize(size_t(ip.nReg()) * nts);
                 ip.adjoint(r.data(), m + size_t(k) * nSrc * nts);
@@ -575,7 +584,7 @@ class NonlinearShots {
 #pragma omp parallel for collapse(2) schedule(dynamic)
         for (int e = 0; e < nExp_; ++e)
             for (int k = 0; k < 5; ++k) {
-                const SpaceInterp& rp = *rec_of(e)[GRID_OF_COMP[k]];
+                const SpaceInterpT<T>& rp = *rec_of(e)[GRID_OF_COMP[k]];
                 std::vector<T>& d = recReg[size_t(e) * 5 + k];
                 d.resize(size_t(rp.nReg()) * nts);
                 rp.adjoint(d.data(), data + (size_t(e) * 5 + k) * nRec * nts);
@@ -614,7 +623,7 @@ class NonlinearShots {
             for (int e = e0; e < e1; ++e)
                 for (int k = 0; k < 5; ++k) {
                     T* m = model + size_t(nModelShots == 1 ? 0 : e) * 5 * nSrc * nts;
-                    const SpaceInterp& ip = *src_[e][GRID_OF_COMP[k]];
+                    const SpaceInterpT<T>& ip = *src_[e][GRID_OF_COMP[k]];
                     std::vector<T>& r = srcReg[size_t(e) * 5 + k];
                     scale_sources(*fd_, ip, k, r.data());  // same diagonal scaling (nonlinearPropElasticGpu.cpp:406-430)
                     ip.forward_add(r.data(), m + size_t(k) * nSrc * nts);
@@ -634,7 +643,7 @@ class NonlinearShots {
     }
 
    private:
-    const InterpVec& rec_of(int e) const { return rec_.size() == 1 ? rec_[0] : rec_[e]; }
+    const InterpVec<T>& rec_of(int e) const { return rec_.size() == 1 ? rec_[0] : rec_[e]; }
     // the call's shot descriptors (positions, model id; traces are set by the caller) and
     // one regular-trace buffer per shot and component, kept between calls
     std::vector<eio_shot> shot_table() {
@@ -665,7 +674,7 @@ class NonlinearShots {
                                 fd_->media.data() + 3 * n, fd_->media.data() + 4 * n};
         check_eio(eio_model_add(dev_->ctx, &g, media, 0, &model_));
     }
-    void check_counts(const InterpVec& S, int nSrc, const InterpVec& Rv, int nRec) const {
+    void check_counts(const InterpVec<T>& S, int nSrc, const InterpVec<T>& Rv, int nRec) const {
         if (S.size() != 4 || Rv.size() != 4) throw std::runtime_error("geometry must have 4 grids");
         for (int g = 0; g < 4; ++g) {
             if (S[g]->nDeviceIrreg != nSrc) throw std::runtime_error("source count not consistent with the model vector");
@@ -676,7 +685,7 @@ class NonlinearShots {
 
     std::vector<Axis> ax_;
     Params par_;
-    std::vector<InterpVec> src_, rec_;
+    std::vector<InterpVec<T>> src_, rec_;
     int nExp_ = 1;
     std::unique_ptr<Device> dev_;
     std::unique_ptr<FdParam<T>> fd_;
@@ -691,7 +700,7 @@ class BornShots {
    public:
     // sourcesSignals: list of [5][nSrc][nts] (1 = same for all shots, or nExp)
     BornShots(const T* elastic, std::vector<Axis> ax, Params par, std::vector<std::vector<T>> sourcesSignals,
-              std::vector<InterpVec> sources, std::vector<InterpVec> receivers)
+              std::vector<InterpVec<T>> sources, std::vector<InterpVec<T>> receivers)
         : ax_(std::move(ax)), par_(std::move(par)), sig_(std::move(sourcesSignals)), src_(std::move(sources)),
           rec_(std::move(receivers)) {
         nExp_ = par_.getInt("nExp");
@@ -780,7 +789,7 @@ class BornShots {
     }
 
     // Born sources: interpolate, scale and fine-sample (seismicElasticOperator2D.cpp:27-119)
-    const InterpVec& rec_of(int e) const { return rec_.size() == 1 ? rec_[0] : rec_[e]; }
+    const InterpVec<T>& rec_of(int e) const { return rec_.size() == 1 ? rec_[0] : rec_[e]; }
 
     // The call's shot descriptors with fine-sampled source traces and regular receiver
     // traces (interpolated from `data`, or zero), in buffers kept between calls.
@@ -809,7 +818,7 @@ class BornShots {
             for (int k = 0; k < 5; ++k) {
                 const std::vector<T>& sig = sig_.size() == 1 ? sig_[0] : sig_[e];
                 const int nSrc = src_[e][0]->nDeviceIrreg;
-                const SpaceInterp& ip = *src_[e][GRID_OF_COMP[k]];
+                const SpaceInterpT<T>& ip = *src_[e][GRID_OF_COMP[k]];
                 std::vector<T> reg(size_t(ip.nReg()) * nts);
                 ip.adjoint(reg.data(), sig.data() + size_t(k) * nSrc * nts);
                 scale_sources(*fd_, ip, k, reg.data());
@@ -817,7 +826,7 @@ class BornShots {
                 f.resize(size_t(ip.nReg()) * ntw);
                 time_interp_fwd(nts, fd_->dts, fd_->ots, fd_->sub, ip.nReg(), reg.data(), f.data());
                 shots[e].src_tr[k] = f.data();
-                const SpaceInterp& rp = *rec_of(e)[GRID_OF_COMP[k]];
+                const SpaceInterpT<T>& rp = *rec_of(e)[GRID_OF_COMP[k]];
                 std::vector<T>& d = recReg_[size_t(e) * 5 + k];
                 if (data) {
                     d.resize(size_t(rp.nReg()) * nts);
@@ -844,7 +853,7 @@ class BornShots {
     std::vector<Axis> ax_;
     Params par_;
     std::vector<std::vector<T>> sig_;
-    std::vector<InterpVec> src_, rec_;
+    std::vector<InterpVec<T>> src_, rec_;
     int nExp_ = 1;
     std::unique_ptr<Device> dev_;
     std::unique_ptr<FdParam<T>> fd_;

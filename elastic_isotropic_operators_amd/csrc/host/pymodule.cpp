@@ -71,20 +71,22 @@ Params params_of(const py::object& p) {
     throw std::runtime_error("parameter object must provide a mapping of key -> value");
 }
 
-std::vector<float> coords_of(const py::object& vec) {
+using SpaceInterp = SpaceInterpT<real>;
+
+std::vector<real> coords_of(const py::object& vec) {
     py::object v = cpp_of(vec);
     py::object a = py::hasattr(v, "getNdArray") ? v.attr("getNdArray")() : v;
-    py::array_t<float, py::array::c_style | py::array::forcecast> arr(a);
-    return std::vector<float>(arr.data(), arr.data() + arr.size());
+    py::array_t<real, py::array::c_style | py::array::forcecast> arr(a);
+    return std::vector<real>(arr.data(), arr.data() + arr.size());
 }
 
-std::vector<InterpVec> interp_lists(const std::vector<std::shared_ptr<SpaceInterp>>& c,
-                                    const std::vector<std::shared_ptr<SpaceInterp>>& x,
-                                    const std::vector<std::shared_ptr<SpaceInterp>>& z,
-                                    const std::vector<std::shared_ptr<SpaceInterp>>& xz) {
+std::vector<InterpVec<real>> interp_lists(const std::vector<std::shared_ptr<SpaceInterp>>& c,
+                                          const std::vector<std::shared_ptr<SpaceInterp>>& x,
+                                          const std::vector<std::shared_ptr<SpaceInterp>>& z,
+                                          const std::vector<std::shared_ptr<SpaceInterp>>& xz) {
     if (c.size() != x.size() || c.size() != z.size() || c.size() != xz.size())
         throw std::runtime_error("geometry lists of the four grids differ in length");
-    std::vector<InterpVec> out(c.size());
+    std::vector<InterpVec<real>> out(c.size());
     for (size_t i = 0; i < c.size(); ++i) out[i] = {c[i], x[i], z[i], xz[i]};
     return out;
 }
@@ -145,7 +147,7 @@ PYBIND11_MODULE(EIO_MODULE, m) {
 #if defined(EIO_DEFINE_INTERP)
     py::class_<SpaceInterp, std::shared_ptr<SpaceInterp>>(m, "spaceInterpGpu")
         .def(py::init([](py::object zCoord, py::object xCoord, py::object hyper, int nt, std::string method,
-                         int nFilt, int dipole, float zShift, float xShift) {
+                         int nFilt, int dipole, real zShift, real xShift) {
                  return std::make_shared<SpaceInterp>(coords_of(zCoord), coords_of(xCoord),
                                                       axes_of(cpp_of(hyper)), nt, method, nFilt, dipole, zShift,
                                                       xShift);
@@ -160,9 +162,12 @@ PYBIND11_MODULE(EIO_MODULE, m) {
         .def("getNt", [](const SpaceInterp& s) { return s.nt; })
         .def("getSizePosUnique", [](const SpaceInterp& s) { return int(s.uniq.size()); });
 #else
-    // one registration of the geometry class, shared by the NL and Born modules of both precisions
+    // one registration of the geometry class per precision (in its NL module), shared with the
+    // Born module of that precision
     m.attr("spaceInterpGpu") =
-        py::module_::import("elastic_isotropic_operators_amd.pyElastic_iso_float_nl").attr("spaceInterpGpu");
+        py::module_::import(sizeof(real) == 4 ? "elastic_isotropic_operators_amd.pyElastic_iso_float_nl"
+                                              : "elastic_isotropic_operators_amd.pyElastic_iso_double_nl")
+            .attr("spaceInterpGpu");
 #endif
 
 #if defined(EIO_KIND_NL)

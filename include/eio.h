@@ -125,6 +125,14 @@ int eio_ctx_set_time_block(eio_ctx* ctx, int steps, int variant);
 int eio_ctx_num_adjoint_time_block_variants(void);
 int eio_ctx_adjoint_time_block_info(int variant, int* R2, int* WX, int* occ);
 int eio_ctx_set_adjoint_time_block(eio_ctx* ctx, int steps, int variant);
+/* The FP32 two-step NL forward (the time block 2 path) in ONE persistent launch per call
+ * instead of one launch per pair of fine steps: mode 0 = off (default), 1 = on, 2 = when every
+ * tile of the batch gets its own co-resident workgroup.  The grid is launched cooperatively;
+ * tiles wait for their neighbours' previous pair through per-tile flags (no grid barrier).
+ * spin_limit > 0 replaces the number of polls a wait may take before the launch is abandoned
+ * with EIO_ERR_RUNTIME (default 2^20).  No counterpart in the reference, which launches every
+ * kernel of every fine step (nonlinearPropElasticGpuFunctions.cu:588-617). */
+int eio_ctx_set_persistent(eio_ctx* ctx, int mode, int spin_limit);
 /* Tuning runs: NL forward and adjoint loops stop after `max_steps` fine steps (0: no limit).
  * Outputs of a limited call are partial.  No counterpart in the reference. */
 int eio_ctx_set_step_limit(eio_ctx* ctx, int max_steps);
@@ -226,11 +234,12 @@ int eio_fwi_gradient(eio_ctx* ctx, int handle, int nshot, const eio_shot* shots,
 int eio_fwi_info(eio_ctx* ctx, int handle, int* segment, int* nckpt, unsigned long long* bytes);
 int eio_fwi_release(eio_ctx* ctx, int handle);
 
-/* Diagnostics: copy `bytes` (multiple of 16) from src to dst (device pointers) `reps` times with
- * a plain 16-byte-vector copy kernel on the context stream; *ms = mean kernel time.  Used to
- * measure the achievable HBM copy bandwidth and to calibrate HBM byte counters.  No counterpart
- * in the reference. */
-int eio_copy_probe(eio_ctx* ctx, void* dst, const void* src, unsigned long long bytes, int reps,
+/* Diagnostics: copy `bytes` (multiple of 16) from src to dst (device pointers) `reps` times on
+ * the context stream; *ms = mean kernel time.  kind 1: one 16-byte vector per thread with
+ * nontemporal stores (the achievable HBM copy bandwidth); kind 0: a grid-stride 16-byte copy
+ * (the reference kernel of the HBM byte-counter calibration).  No counterpart in the
+ * reference. */
+int eio_copy_probe(eio_ctx* ctx, void* dst, const void* src, unsigned long long bytes, int reps, int kind,
                    double* ms);
 
 /* Device memory helpers for callers without their own allocator (bytes). */

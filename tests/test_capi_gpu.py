@@ -909,3 +909,96 @@ def test_learned_boundary_rejected_outside_nl_forward():
     h = ctx.fwi_forward([shot], 0)
     ctx.fwi_release(h)
     ctx.close()
+
+
+def _persistent_batch(ctx, cases, seed):
+    rng = np.random.default_rng(seed)
+    shots, recs, keep = [], [], []
+    for (w, g, med, spos, rpos) in cases:
+        mid = _model(ctx, w, g, med)
+        src = [np.ascontiguousarray(x, np.float32) for x in random_traces(rng, spos, g.nts, np.float64)]
+        rec = [np.zeros((rpos[GRID_OF_COMP[k]].size, g.nts), np.float32) for k in range(5)]
+        shot, kp = ctx.make_shot(mid, spos, rpos, src, rec)
+        shots.append(shot)
+        recs.append(rec)
+        keep.append((kp, src))
+    return shots, recs, keep
+
+
+@pytest.mark.parametrize("v2", [_capi.VARIANT_AUTO, 0, 4])
+def test_persistent_launch_equals_launched_pairs(v2):
+    """The whole two-step NL forward in one persistent launch (tiles wait for their neighbours'
+    previous pair through flags) gives the launched form's traces bit for bit, on an uneven
+    batch (grids, trace lengths, sub-samplings, odd step counts) with scattered sources and
+    receivers, and on a batch with more tiles than co-resident workgroups (several tiles per
+    workgroup)."""
+    rng = np.random.default_rng(140)
+    cases = []
+    for i, (nz, nx, nts, sub) in enumerate([(90, 130, 70, 1), (150, 90, 50, 2), (60, 200, 61, 1)]):
+        w = W.small(nz=nz, nx=nx, nts=nts, sub=sub, seed=141 + i)
+        g = oracle_grid(w)
+        med = O.media(g, w["elastic"], np.float64)
+        cases.append((w, g, med, scattered_positions(rng, g.nz, g.nx, 12), scattered_positions(rng, g.nz, g.nx, 30)))
+    big = W.small(nz=1400, nx=2400, nts=12, sub=1, seed=145)   # ~1000 tiles of 56 x 32
+    gb = oracle_grid(big)
+    cases.append((big, gb, O.media(gb, big["elastic"], np.float64), scattered_positions(rng, gb.nz, gb.nx, 40),
+                  scattered_positions(rng, gb.nz, gb.nx, 80)))
+    ctx = _capi.Context(0, np.float32)
+    ctx.set_time_block(2, v2)
+    shots, recs, keep = _persistent_batch(ctx, cases, 146)
+    ctx.nl_forward(shots)
+    ref = [[r.copy() for r in rec] for rec in recs]
+    ctx.set_persistent(1)
+    for rep in range(2):
+        for rec in recs:
+            for r in rec:
+                r[...] = np.nan
+        ctx.nl_forward(shots)
+        total_ms, nlaunch, _ = ctx.last_timing()
+        assert nlaunch == 1
+        for i, (rec, rr) in enumerate(zip(recs, ref)):
+            assert all(np.array_equal(a, b) for a, b in zip(rec, rr)), (v2, rep, i)
+    ctx.close()
+
+
+def test_persistent_launch_c0_vs_fp64_oracle():
+    """Workload C0 (one 280x472 shot, nts 1000, sub_fd 2) in one persistent launch against the
+    FP64 oracle: receivers within 1e-5."""
+    w = W.gaussian_c0()
+    g = oracle_grid(w)
+    med = O.media(g, w["elastic"], np.float64)
+    spos = reg_positions(w, w["src_z"][0], w["src_x"][0])
+    rpos = reg_positions(w, w["rec_z"], w["rec_x"])
+    src = random_traces(np.random.default_rng(150), spos, g.nts, np.float64)
+    O.set_ftz(True)
+    ref = O.nl_fwd_reg(g, med, spos, src, rpos, np.float64)
+    ctx = _capi.Context(0, np.float32)
+    ctx.set_time_block(2, _capi.VARIANT_AUTO)
+    ctx.set_persistent(2)
+    mid = _model(ctx, w, g, med)
+    out = _gpu_nl_fwd(ctx, mid, spos, rpos, src, g.nts, np.float32)
+    assert ctx.last_timing()[1] == 1           # one launch: the batch fits co-resident
+    O.set_ftz(False)
+    for k in range(5):
+        assert rel_l2(out[k], ref[k]) < 1e-5, (k, rel_l2(out[k], ref[k]))
+    ctx.close()
+
+
+def test_persistent_launch_abort_drains_and_reports():
+    """A spin limit of one poll forces the persistent launch's bounded neighbour waits to give
+    up: every workgroup returns, the call reports EIO_ERR_RUNTIME, and the context runs the
+    next call normally."""
+    w, g, med, spos, rpos = _case(nz=300, nx=400, nts=200, sub=1, seed=160)
+    ctx = _capi.Context(0, np.float32)
+    ctx.set_time_block(2, _capi.VARIANT_AUTO)
+    mid = _model(ctx, w, g, med)
+    rng = np.random.default_rng(161)
+    src = random_traces(rng, spos, g.nts, np.float64)
+    ref = _gpu_nl_fwd(ctx, mid, spos, rpos, src, g.nts, np.float32)
+    ctx.set_persistent(1, 1)
+    with pytest.raises(_capi.EioError, match="spin limit"):
+        _gpu_nl_fwd(ctx, mid, spos, rpos, src, g.nts, np.float32)
+    ctx.set_persistent(1, 1 << 20)
+    out = _gpu_nl_fwd(ctx, mid, spos, rpos, src, g.nts, np.float32)
+    assert all(np.array_equal(a, b) for a, b in zip(out, ref))
+    ctx.close()

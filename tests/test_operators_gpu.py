@@ -177,7 +177,10 @@ def test_forward_and_adjoint_wavefield(tmp_path):
     d = data.getNdArray()[1]
     d2 = data.clone()
     op.forward(False, model, d2)
-    assert np.array_equal(d2.getNdArray(), data.getNdArray())
+    # the plain forward takes two fine steps per launch, the wavefield-saving one one step: the
+    # same arithmetic per cell, equal to FP32 rounding (the compiler may contract the two
+    # kernels' multiply-adds differently)
+    assert rel_l2(d2.getNdArray(), data.getNdArray()) < 1e-6
     ri = G.staggered_interps(w["nz"], w["oz"], w["dz"], w["ox"], w["dx"], geo["rz"], geo["rx"])
     for k, gi in enumerate(G.GRID_OF_COMP):
         reg = wf[:, k].reshape(w["nts"], -1)[:, ri[gi].uniq].T
