@@ -62,6 +62,7 @@ def lib():
             ("eio_model_set_media", [vp, ctypes.c_int, ctypes.POINTER(vp), ctypes.c_int]),
             ("eio_model_remove", [vp, ctypes.c_int]),
             ("eio_nl_forward", [vp, ctypes.c_int, ctypes.POINTER(Shot), ctypes.c_int]),
+            ("eio_nl_forward_noalloc", [vp, ctypes.c_int, ctypes.POINTER(Shot), ctypes.c_int]),
             ("eio_nl_adjoint", [vp, ctypes.c_int, ctypes.POINTER(Shot), ctypes.c_int]),
             ("eio_born_forward", [vp, ctypes.c_int, ctypes.POINTER(Shot), ctypes.POINTER(vp),
                                   ctypes.c_int]),
@@ -106,7 +107,7 @@ def lib():
 
 EXPORTED = ("eio_last_error", "eio_ctx_create", "eio_ctx_destroy", "eio_ctx_set_stream",
             "eio_ctx_set_graph", "eio_ctx_graph_info", "eio_ctx_last_timing", "eio_grid_check", "eio_model_add", "eio_model_set_media",
-            "eio_model_remove", "eio_nl_forward", "eio_nl_adjoint", "eio_born_forward",
+            "eio_model_remove", "eio_nl_forward", "eio_nl_forward_noalloc", "eio_nl_adjoint", "eio_born_forward",
             "eio_born_adjoint", "eio_device_alloc", "eio_device_free", "eio_memcpy",
             "eio_synchronize", "eio_copy_probe", "eio_ctx_set_variant", "eio_ctx_variant_info",
             "eio_ctx_num_variants", "eio_fwi_forward", "eio_fwi_gradient", "eio_fwi_info",
@@ -323,6 +324,12 @@ class Context:
     def nl_forward(self, shots, device_ptrs=False):
         arr = self._shots(shots)
         check(lib().eio_nl_forward(self.h, len(shots), arr, EIO_DEVICE_PTRS if device_ptrs else 0))
+
+    def nl_forward_noalloc(self, shots):
+        """eio_nl_forward without allocation (device traces; workspace from the library's
+        64 MiB per-GPU scratch array)."""
+        arr = self._shots(shots)
+        check(lib().eio_nl_forward_noalloc(self.h, len(shots), arr, EIO_DEVICE_PTRS))
 
     def nl_adjoint(self, shots, device_ptrs=False):
         arr = self._shots(shots)

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <tuple>
@@ -29,6 +30,14 @@
 #include "eio_kernels.h"
 #include "eio_step2.h"
 #include "eio_abc.h"
+
+// Workspace of eio_nl_forward_noalloc: device memory the library never allocates at call
+// time.  Each GPU holds its own copy of this array for the life of the process; in host code
+// its name is not a device address, so host code reaches it only through the pointer that
+// hipGetSymbolAddress returns on that device (g_scratch_ptr, taken once per device when a
+// context is created there).
+constexpr size_t EIO_SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[EIO_SCRATCH_BYTES];
 
 namespace {
 
@@ -47,6 +56,23 @@ struct EioError {
     } while (0)
 
 void fail(const std::string& m) { throw EioError{EIO_ERR_ARG, m}; }
+
+// Device address of the scratch array on each device, taken with that device current the
+// first time a context is created on it.
+std::mutex g_scratch_mu;
+std::vector<unsigned char*> g_scratch_ptr;
+
+unsigned char* scratch_address(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (int(g_scratch_ptr.size()) <= device) g_scratch_ptr.resize(size_t(device) + 1, nullptr);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        EIO_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw EioError{EIO_ERR_RUNTIME, "hipGetSymbolAddress returned no address for the scratch array"};
+        g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+    }
+    return g_scratch_ptr[device];
+}
 
 constexpr double FD_C[4] = {1.196289062541883, -0.079752604188901, 0.009570312506634,
                             -6.975446437140719e-04};  // nonlinearPropElasticGpuFunctions.cu:159-167
@@ -142,12 +168,16 @@ struct GraphEntry {
 struct Ctx {
     int device = 0;
     int prec = 4;
+    unsigned char* scratch_dev = nullptr;  // this device's scratch array (scratch_address)
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev[4];
     std::map<int, ModelBase> models;
     int next_id = 1;
     std::map<std::string, DevBuf> ws;
+    // while a no-allocation call runs: its workspace is carved from the device's scratch array
+    unsigned char* bump = nullptr;
+    size_t bump_cap = 0, bump_off = 0;
     double last_total_ms = 0.0, last_step_ms = 0.0;
     long long last_nstep = 0;
     int use_graph = 0;
@@ -170,6 +200,15 @@ struct Ctx {
 };
 
 void* ws_get(Ctx* c, const std::string& name, size_t bytes) {
+    if (c->bump) {  // no-allocation call: 256-byte aligned pieces of the scratch array
+        const size_t off = (c->bump_off + 255) / 256 * 256;
+        if (off > c->bump_cap || bytes > c->bump_cap - off)
+            fail("eio_nl_forward_noalloc: the call needs more than the " + std::to_string(c->bump_cap >> 20) +
+                 " MiB scratch workspace (buffer \"" + name + "\" of " + std::to_string(bytes) +
+                 " bytes at offset " + std::to_string(off) + "); use eio_nl_forward");
+        c->bump_off = off + bytes;
+        return c->bump + off;
+    }
     DevBuf& b = c->ws[name];
     if (b.bytes < bytes) {
         if (b.p) {
@@ -1243,6 +1282,31 @@ void nl_forward(Ctx* c, int nshot, const eio_shot* shots, int flags, int wshot =
     finish(run);
 }
 
+// eio_nl_forward whose workspace (state levels, shot/tile/entry tables, the persistent
+// launch's neighbour table and flags) is carved from this device's scratch array instead of
+// the context's hipMalloc'd workspace; the loop runs without a graph (instantiating one
+// allocates).  The state is cleared by a fill kernel each call; calls that overlap on one
+// device share the array and race.
+template <typename T>
+void nl_forward_noalloc(Ctx* c, int nshot, const eio_shot* shots, int flags) {
+    if (!(flags & EIO_DEVICE_PTRS)) fail("eio_nl_forward_noalloc needs EIO_DEVICE_PTRS (device traces)");
+    if (!c->scratch_dev) fail("eio_nl_forward_noalloc: no scratch array on this device");
+    struct Restore {
+        Ctx* c;
+        int graph;
+        ~Restore() {
+            c->bump = nullptr;
+            c->bump_cap = c->bump_off = 0;
+            c->use_graph = graph;
+        }
+    } restore{c, c->use_graph};
+    c->use_graph = 0;
+    c->bump = c->scratch_dev;
+    c->bump_cap = EIO_SCRATCH_BYTES;
+    c->bump_off = 0;
+    nl_forward<T>(c, nshot, shots, flags);
+}
+
 // One fine step (no sources or receivers) of `nstate` states of one model, in place:
 // states [nstate][2][5][nx*nz] hold (older, newer); afterwards (newest, newer).  The
 // model's learned boundary, when attached, corrects the new level as in a propagation.
@@ -1791,6 +1855,12 @@ int eio_ctx_create(int device, int precision, eio_ctx** out) {
         auto* c = new Ctx();
         c->device = device;
         c->prec = precision;
+        try {
+            c->scratch_dev = scratch_address(device);
+        } catch (...) {
+            delete c;
+            throw;
+        }
         EIO_CHECK_HIP(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
         c->stream = c->own;
         for (auto& e : c->ev) EIO_CHECK_HIP(hipEventCreate(&e));
@@ -1968,6 +2038,15 @@ int eio_nl_forward(eio_ctx* ctx, int nshot, const eio_shot* shots, int flags) {
         DeviceGuard dg(c->device);
         if (c->prec == 4) nl_forward<float>(c, nshot, shots, flags);
         else nl_forward<double>(c, nshot, shots, flags);
+    });
+}
+
+int eio_nl_forward_noalloc(eio_ctx* ctx, int nshot, const eio_shot* shots, int flags) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        DeviceGuard dg(c->device);
+        if (c->prec == 4) nl_forward_noalloc<float>(c, nshot, shots, flags);
+        else nl_forward_noalloc<double>(c, nshot, shots, flags);
     });
 }
 

@@ -156,6 +156,15 @@ int eio_model_remove(eio_ctx* ctx, int model_id);
 /* Nonlinear forward of nshot shots in one batched time loop; rec_tr is overwritten.
  * Replaces propShotsElasticFwdGpu (nonlinearPropElasticGpuFunctions.h:20-26, .cu:561-648). */
 int eio_nl_forward(eio_ctx* ctx, int nshot, const eio_shot* shots, int flags);
+/* eio_nl_forward that allocates nothing: no counterpart in the reference; it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The call's workspace (state levels, shot/tile/entry
+ * tables) comes from a 64 MiB device array of the library (one per GPU, held until the
+ * process ends) instead of hipMalloc; a batch needing more is rejected with EIO_ERR_ARG
+ * before any launch.  EIO_DEVICE_PTRS is required (traces on the device); no graph is used.
+ * Models must be registered beforehand (eio_model_add allocates).  Every call on a GPU shares
+ * that GPU's array: calls that overlap on one GPU (other threads or contexts) race. */
+int eio_nl_forward_noalloc(eio_ctx* ctx, int nshot, const eio_shot* shots, int flags);
 /* Nonlinear adjoint; src_tr is overwritten.
  * Replaces propShotsElasticAdjGpu (nonlinearPropElasticGpuFunctions.h:28-34, .cu:932-1028). */
 int eio_nl_adjoint(eio_ctx* ctx, int nshot, const eio_shot* shots, int flags);
