@@ -546,7 +546,8 @@ def main():
     achieved = BYTES_PER_CELL["nl_fwd"] * cells / loop_s / 1e9
     if args.time_block == 2:
         R2, WX2, OCC2 = _capi.Context.time_block_variants()[args.variant2]
-        kname = "k_step2_fwd<float,%d,%d,%d> (2 fine steps per launch)" % (R2, WX2, OCC2)
+        kname = "k_step2_fwd<float,%d,%d,%d%s> (2 fine steps per launch)" % (
+            R2, WX2, OCC2, ",FS" if _capi.Context.time_block_streamed(args.variant2) else "")
     else:
         R, WX, OCC = _capi.Context.variants()[args.variant if args.variant >= 0 else 1]
         kname = "k_step_fwd<float,%d,%d,%d,M_NL>" % (R, WX, OCC)

@@ -89,6 +89,7 @@ def lib():
             ("eio_ctx_time_block_info", [ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
             ("eio_ctx_set_time_block", [vp, ctypes.c_int, ctypes.c_int]),
+            ("eio_ctx_time_block_streamed", [ctypes.c_int]),
             ("eio_ctx_num_adjoint_time_block_variants", []),
             ("eio_ctx_adjoint_time_block_info", [ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                                  ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
@@ -112,7 +113,7 @@ EXPORTED = ("eio_last_error", "eio_ctx_create", "eio_ctx_destroy", "eio_ctx_set_
             "eio_synchronize", "eio_copy_probe", "eio_ctx_set_variant", "eio_ctx_variant_info",
             "eio_ctx_num_variants", "eio_fwi_forward", "eio_fwi_gradient", "eio_fwi_info",
             "eio_fwi_release", "eio_ctx_num_time_block_variants", "eio_ctx_time_block_info",
-            "eio_ctx_set_time_block", "eio_ctx_set_step_limit", "eio_ctx_set_persistent", "eio_nl_forward_wavefield",
+            "eio_ctx_set_time_block", "eio_ctx_time_block_streamed", "eio_ctx_set_step_limit", "eio_ctx_set_persistent", "eio_nl_forward_wavefield",
             "eio_nl_adjoint_wavefield", "eio_model_set_abc", "eio_nl_forward_snapshots",
             "eio_nl_step_states", "eio_ctx_release_workspace", "eio_ctx_num_adjoint_time_block_variants",
             "eio_ctx_adjoint_time_block_info", "eio_ctx_set_adjoint_time_block")
@@ -247,6 +248,12 @@ class Context:
             check(lib().eio_ctx_time_block_info(v, ctypes.byref(r), ctypes.byref(wx), ctypes.byref(oc)))
             out.append((r.value, wx.value, oc.value))
         return out
+
+    @staticmethod
+    def time_block_streamed(variant):
+        """True when two-step variant `variant` streams the current level through two LDS field
+        buffers (two workgroups per CU)."""
+        return lib().eio_ctx_time_block_streamed(int(variant)) == 1
 
     def set_graph(self, on):
         check(lib().eio_ctx_set_graph(self.h, int(bool(on))))

@@ -91,12 +91,16 @@ constexpr int N_F32_VARIANTS = int(sizeof(F32_VARIANTS) / sizeof(F32_VARIANTS[0]
 constexpr Variant F64_VARIANT = {2, 4, 1};
 
 // Two-step (temporally blocked) NL forward variants: R2 owned x-cells per lane, WX waves,
-// OCC waves per SIMD; owned tile = 56 x WX*R2.  FP32 only.
+// OCC waves per SIMD; owned tile = 56 x WX*R2.  NTS: nontemporal stores; FS: u_cur streamed
+// through two LDS field buffers instead of staged whole (78.8 KB instead of 120 KB at 56 x 32:
+// two workgroups per CU), for the NL forward and the recompute (other passes of an FS
+// variant run variant 4, the same tile width).  FP32 only.
 struct Variant2 {
-    int R2, WX, OCC, NTS;
+    int R2, WX, OCC, NTS, FS;
 };
-constexpr Variant2 F32_VARIANTS2[] = {{2, 8, 2, 0}, {2, 8, 4, 0}, {4, 4, 2, 0}, {1, 8, 4, 0},
-                                      {4, 8, 1, 0}, {2, 8, 4, 1}, {2, 8, 2, 1}, {4, 8, 1, 1}};
+constexpr Variant2 F32_VARIANTS2[] = {{2, 8, 2, 0, 0}, {2, 8, 4, 0, 0}, {4, 4, 2, 0, 0}, {1, 8, 4, 0, 0},
+                                      {4, 8, 1, 0, 0}, {2, 8, 4, 1, 0}, {2, 8, 2, 1, 0}, {4, 8, 1, 1, 0},
+                                      {4, 8, 4, 0, 1}, {8, 4, 2, 0, 1}, {2, 8, 6, 0, 1}};
 constexpr int N_F32_VARIANTS2 = int(sizeof(F32_VARIANTS2) / sizeof(F32_VARIANTS2[0]));
 // Two-step adjoint (Born receiver pass, FWI gradient) variants: R2, WX, OCC as above.
 struct Variant2A {
@@ -795,6 +799,12 @@ void launch_fwd2(Run<float>& run, int ntile, int k, int par, int dlo = 0, int dh
 #define EIO_L2(R_, WX_, OCC_, NT_)                                                                    \
     klaunch(run.c, eio::k_step2_fwd<float, R_, WX_, OCC_, NT_, DEC, FINE, SS, NOREC>, dim3(ntile),      \
             dim3(64 * WX_), sh, tt, te, ntile, k, par, dlo, dhi, sspar)
+#define EIO_L2FS(R_, WX_, OCC_)                                                                       \
+    if constexpr (!DEC && !SS)                                                                        \
+        klaunch(run.c, eio::k_step2_fwd<float, R_, WX_, OCC_, 0, false, FINE, false, NOREC, true>,      \
+                dim3(ntile), dim3(64 * WX_), sh, tt, te, ntile, k, par, dlo, dhi, sspar);             \
+    else                                                                                              \
+        EIO_L2(4, 8, 1, 0)
     switch (run.v2) {
         case 0: EIO_L2(2, 8, 2, 0); break;
         case 1: EIO_L2(2, 8, 4, 0); break;
@@ -803,8 +813,12 @@ void launch_fwd2(Run<float>& run, int ntile, int k, int par, int dlo = 0, int dh
         case 4: EIO_L2(4, 8, 1, 0); break;
         case 5: EIO_L2(2, 8, 4, 1); break;
         case 6: EIO_L2(2, 8, 2, 1); break;
-        default: EIO_L2(4, 8, 1, 1); break;
+        case 7: EIO_L2(4, 8, 1, 1); break;
+        case 8: EIO_L2FS(4, 8, 4); break;
+        case 9: EIO_L2FS(8, 4, 2); break;
+        default: EIO_L2FS(2, 8, 6); break;
     }
+#undef EIO_L2FS
 #undef EIO_L2
 }
 
@@ -1146,6 +1160,9 @@ void with_persist_kernel(int v, F&& f) {
         case 0: case 1: case 5: case 6: f(eio::k_step2_fwd_persist<float, 2, 8, 2>, 8); break;  // 2,8,4 spills
         case 2: f(eio::k_step2_fwd_persist<float, 4, 4, 2>, 4); break;
         case 3: f(eio::k_step2_fwd_persist<float, 1, 8, 4>, 8); break;
+        case 8: f(eio::k_step2_fwd_persist<float, 4, 8, 4, true>, 8); break;
+        case 9: f(eio::k_step2_fwd_persist<float, 8, 4, 2, true>, 4); break;
+        case 10: f(eio::k_step2_fwd_persist<float, 2, 8, 6, true>, 8); break;
         default: f(eio::k_step2_fwd_persist<float, 4, 8, 1>, 8); break;
     }
 }
@@ -1945,6 +1962,10 @@ int eio_ctx_time_block_info(int variant, int* R2, int* WX, int* occ) {
 }
 
 int eio_ctx_num_time_block_variants(void) { return N_F32_VARIANTS2; }
+
+int eio_ctx_time_block_streamed(int variant) {
+    return variant >= 0 && variant < N_F32_VARIANTS2 ? F32_VARIANTS2[variant].FS : -1;
+}
 
 int eio_ctx_set_adjoint_time_block(eio_ctx* ctx, int steps, int variant) {
     return guarded([&]() {

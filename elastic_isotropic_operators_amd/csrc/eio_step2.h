@@ -51,6 +51,15 @@ __device__ __forceinline__ void bld4(rsrc_t r, unsigned voff, unsigned soff, flo
     v[3] = q.w;
 }
 
+// Forms the values of v at this point of the program: an empty asm statement that reads and
+// writes them keeps the compiler from sinking the arithmetic that produces them past later
+// barriers (which would keep every LDS value it reads live across the barrier).
+template <typename T, int N>
+__device__ __forceinline__ void pin_regs(T (&v)[N]) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) asm volatile("" : "+v"(v[i]));
+}
+
 template <typename T>
 __device__ __forceinline__ void bst_pol(rsrc_t r, T v, unsigned voff, unsigned soff, bool nt) {
     if (nt) bst_nt(r, v, voff, soff);
@@ -65,19 +74,21 @@ __device__ __forceinline__ void bst_pol(rsrc_t r, T v, unsigned voff, unsigned s
 // (kernelsGpuElastic.cu:1363-1368); both steps lie in one coarse interval, so they share the
 // two slots.  NOREC: no receiver recording (the segment recompute of the FWI gradient).
 // The two steps of one tile (every thread of the workgroup calls it with the same tile).
-template <typename T, int R2, int WX, bool NTS, bool DEC, bool FINE, bool SS, bool NOREC>
+template <typename T, int R2, int WX, bool NTS, bool DEC, bool FINE, bool SS, bool NOREC, bool FS = false>
 __device__ __forceinline__ void step2_fwd_tile(const ShotDev<T>* __restrict__ shots, const TileDev t,
                                                const Entry* __restrict__ ents, int k, int par, int dlo,
                                                int dhi, int sspar) {
     static_assert(8 % WX == 0, "WX must divide 8");
     static_assert(sizeof(T) == 4, "the two-step forward is FP32");
+    static_assert(!FS || (!DEC && !SS), "the field-streamed stage serves the NL forward and the recompute");
     constexpr int NT = 64 * WX;
     constexpr int TX = WX * R2;        // owned columns
     constexpr int R1 = R2 + 8 / WX;    // step-1 columns per wave
     constexpr int TX1 = TX + 8;        // step-1 columns
     constexpr int TXC = TX + 16;       // staged u_cur columns
     constexpr unsigned SZ = sizeof(T);
-    __shared__ __attribute__((aligned(16))) T lc[5][TXC][ZC];
+    // u_cur stage: all five fields, or (FS) two buffers the fields stream through
+    __shared__ __attribute__((aligned(16))) T lc[FS ? 2 : 5][TXC][ZC];
     __shared__ T l1[5][TX1][Z1];
 
     const ShotDev<T>& S = shots[t.shot];
@@ -107,84 +118,13 @@ __device__ __forceinline__ void step2_fwd_tile(const ShotDev<T>* __restrict__ sh
     // that are masked, so their loads are pointed at row FAT instead.
     const int izl = iz < 0 ? 0 : iz;
 
-    // ---- u_cur on (TX+16) x 72 cells from (x0-8, z0-8) (cells off the grid are 0), four rows
-    // per 16-byte load: all loads are issued first, then the older level and media of the
-    // step-1 cells, and only then are the staged values written to LDS, so every load of the
-    // tile is in flight at once.  Row quads start at z0-8+4q = 56 bz-4+4q, a multiple of 4:
-    // a quad is wholly above row 0 or wholly below it, and only the bottom edge cuts one.
-    constexpr int ZQ = ZC / 4;
-    constexpr int NFILL = (TXC * ZQ + NT - 1) / NT;
-    T hf[NFILL][5][4];
-    unsigned hm[NFILL];  // bit k: row k of the quad is on the grid
-#pragma unroll
-    for (int it = 0; it < NFILL; ++it) {
-        const int e = threadIdx.x + it * NT;
-        const int c = e / ZQ, q = e - c * ZQ;
-        const int gx = t.x0 - 2 * FAT + c, gz = t.z0 - 2 * FAT + 4 * q;
-        const bool in = e < TXC * ZQ && gx >= 0 && gx < nx && gz >= 0 && gz < nz;
-        const int nin = in ? (nz - gz < 4 ? nz - gz : 4) : 0;
-        hm[it] = (1u << nin) - 1u;
-        const unsigned hv = in ? (unsigned(gx) * unsigned(nz) + unsigned(gz)) * SZ : 0u;
-#pragma unroll
-        for (int f = 0; f < 5; ++f) bld4(rcur, hv, unsigned(f) * fbytes, hf[it][f]);
-    }
-    T o[5][R1], m[5][R1];
-#pragma unroll
-    for (int r = 0; r < R1; ++r) {
-        const int ix = ix1 + r;
-        const unsigned vo = (unsigned(ix < 0 ? 0 : ix) * unsigned(nz) + unsigned(izl)) * SZ;
-#pragma unroll
-        for (int f = 0; f < 5; ++f) o[f][r] = bld<T>(rold, vo, unsigned(f) * fbytes);
-#pragma unroll
-        for (int f = 0; f < 5; ++f) m[f][r] = bld<T>(rmed, vo, unsigned(f) * fbytes);
-    }
-    // decimation slots its (x0, z0) and its+1 (x1, z1) of the owned cells, with the stage
-    const int j2 = w * R2;
+    const T* cz = S.cz;
+    const T* cx = S.cx;
+    const int j2 = w * R2;                                   // first owned column of step 2 (rel x0)
     const int ix2 = t.x0 + j2;
     rsrc_t rwx0, rwx1, rwz0, rwz1;
     T f1a = T(0), f1b = T(0), f2a = T(0), f2b = T(0);
     T px0[R2], pz0[R2], px1[R2], pz1[R2];
-    if constexpr (DEC) {
-        const size_t s0 = size_t(st1.its % S.wcap) * n, s1 = size_t((st1.its + 1) % S.wcap) * n;
-        rwx0 = make_rsrc(S.Wvx + s0, fbytes);
-        rwx1 = make_rsrc(S.Wvx + s1, fbytes);
-        rwz0 = make_rsrc(S.Wvz + s0, fbytes);
-        rwz1 = make_rsrc(S.Wvz + s1, fbytes);
-        f1a = S.filt[st1.it2];
-        f1b = S.filt[S.h + st1.it2];
-        f2a = S.filt[st2.it2];
-        f2b = S.filt[S.h + st2.it2];
-        const bool sta = st1.it2 == 1 && st1.its == dlo;
-        const bool stb = st1.it2 == 1 && st1.its + 1 != dhi;
-#pragma unroll
-        for (int r = 0; r < R2; ++r) {
-            const unsigned vo = (unsigned(ix2 + r) * unsigned(nz) + unsigned(izl)) * SZ;
-            px0[r] = sta ? T(0) : bld<T>(rwx0, vo, 0);
-            pz0[r] = sta ? T(0) : bld<T>(rwz0, vo, 0);
-            px1[r] = stb ? T(0) : bld<T>(rwx1, vo, 0);
-            pz1[r] = stb ? T(0) : bld<T>(rwz1, vo, 0);
-        }
-    }
-#pragma unroll
-    for (int it = 0; it < NFILL; ++it) {
-        const int e = threadIdx.x + it * NT;
-        if (e < TXC * ZQ) {
-            const int c = e / ZQ, q = e - c * ZQ;
-#pragma unroll
-            for (int f = 0; f < 5; ++f) {
-                float4 v;
-                v.x = (hm[it] & 1u) ? hf[it][f][0] : 0.0f;
-                v.y = (hm[it] & 2u) ? hf[it][f][1] : 0.0f;
-                v.z = (hm[it] & 4u) ? hf[it][f][2] : 0.0f;
-                v.w = (hm[it] & 8u) ? hf[it][f][3] : 0.0f;
-                *reinterpret_cast<float4*>(&lc[f][c][4 * q]) = v;
-            }
-        }
-    }
-    __syncthreads();
-
-    const T* cz = S.cz;
-    const T* cx = S.cx;
     // SS: secondary-source slots and the weights of the two steps (sampled at it2-1)
     rsrc_t rssl, rssr;
     T ssf1a = T(0), ssf1b = T(0), ssf2a = T(0), ssf2b = T(0);
@@ -196,62 +136,273 @@ __device__ __forceinline__ void step2_fwd_tile(const ShotDev<T>* __restrict__ sh
         ssf2a = S.filt[st2.it2 - 1];
         ssf2b = S.filt[S.h + st2.it2 - 1];
     }
-    // ---- step 1 on columns j1 .. j1+R1-1 (rel x0-4) = lc columns j1+4 .. ; lane row = lc row lane+4
-    {
-        T dx_sxx[R1], dx_sxz[R1], dx_vx[R1], dx_vz[R1];
-        {
+    if constexpr (FS) {
+        // ---- u_cur streamed through LDS one field at a time (two buffers of (TX+16) x 72
+        // cells from (x0-8, z0-8), the same 16-byte row-quad loads as the five-field stage):
+        // a field is staged while the loads of the next one are in flight, and each of its
+        // derivatives at the step-1 cells goes straight into the sum the update takes
+        // (S0 = dx sxx + dz sxz, S1 = dz szz + dx sxz, dx vx, dz vz, S4 = dz vx + dx vz; the
+        // order sxx, szz, sxz, vx, vz keeps each sum's operand order).  LDS per tile:
+        // 2 x (TX+16) x 72 + 5 x (TX+8) x 64 floats (78.8 KB at TX = 32), so two workgroups
+        // share a CU.  The older level and media of the step-1 cells are loaded two phases
+        // before the update that uses them (vx and vz of the older level first).
+        constexpr int ZQ = ZC / 4;
+        constexpr int NFILL = (TXC * ZQ + NT - 1) / NT;
+        unsigned hv[NFILL];  // byte offset of each quad
+#pragma unroll
+        for (int it = 0; it < NFILL; ++it) {
+            const int e = threadIdx.x + it * NT;
+            const int c = e / ZQ, q = e - c * ZQ;
+            const int gx = t.x0 - 2 * FAT + c, gz = t.z0 - 2 * FAT + 4 * q;
+            const bool in = e < TXC * ZQ && gx >= 0 && gx < nx && gz >= 0 && gz < nz;
+            hv[it] = in ? (unsigned(gx) * unsigned(nz) + unsigned(gz)) * SZ : 0u;
+        }
+        auto load_field = [&](int f, T (&v)[NFILL][4]) {
+#pragma unroll
+            for (int it = 0; it < NFILL; ++it) bld4(rcur, hv[it], unsigned(f) * fbytes, v[it]);
+        };
+        auto stage_field = [&](int b, const T (&v)[NFILL][4]) {
+#pragma unroll
+            for (int it = 0; it < NFILL; ++it) {
+                const int e = threadIdx.x + it * NT;
+                if (e < TXC * ZQ) {
+                    float4 x;
+                    x.x = v[it][0];
+                    x.y = v[it][1];
+                    x.z = v[it][2];
+                    x.w = v[it][3];
+                    *reinterpret_cast<float4*>(&lc[b][0][4 * e]) = x;
+                }
+            }
+        };
+        T o[5][R1], m[5][R1];
+        auto load_om = [&](int f) {
+#pragma unroll
+            for (int r = 0; r < R1; ++r) {
+                const int ix = ix1 + r;
+                const unsigned vo = (unsigned(ix < 0 ? 0 : ix) * unsigned(nz) + unsigned(izl)) * SZ;
+                o[f][r] = bld<T>(rold, vo, unsigned(f) * fbytes);
+                m[f][r] = bld<T>(rmed, vo, unsigned(f) * fbytes);
+            }
+        };
+        // x derivatives D+ / D- of buffer b at the step-1 columns (register window of R1+8)
+        auto dxp = [&](int b, T (&out)[R1]) {
             T q[R1 + 8];
 #pragma unroll
-            for (int j = 0; j < R1 + 8; ++j) q[j] = lc[2][j1 + j][lane + FAT];
+            for (int j = 0; j < R1 + 8; ++j) q[j] = lc[b][j1 + j][lane + FAT];
 #pragma unroll
-            for (int r = 0; r < R1; ++r) dx_sxx[r] = dm_q(q, r + 4, cx);
+            for (int r = 0; r < R1; ++r) out[r] = dp_q(q, r + 4, cx);
+        };
+        auto dxm = [&](int b, T (&out)[R1]) {
+            T q[R1 + 8];
 #pragma unroll
-            for (int j = 0; j < R1 + 8; ++j) q[j] = lc[4][j1 + j][lane + FAT];
+            for (int j = 0; j < R1 + 8; ++j) q[j] = lc[b][j1 + j][lane + FAT];
 #pragma unroll
-            for (int r = 0; r < R1; ++r) dx_sxz[r] = dp_q(q, r + 4, cx);
+            for (int r = 0; r < R1; ++r) out[r] = dm_q(q, r + 4, cx);
+        };
+        // the sums of a phase are formed before its closing barrier (pin_regs)
+        T sa[NFILL][4], sb[NFILL][4];
+        T s0[R1], s1[R1], dxvx[R1], dzvz[R1], s4[R1], dv[R1];
+        load_field(2, sa);  // sxx
+        load_field(3, sb);  // szz
+        load_om(0);
+        load_om(1);
+        stage_field(0, sa);
+        __syncthreads();
+        load_field(4, sa);  // sxz
+        stage_field(1, sb);
+        load_field(0, sb);  // vx
+        dxm(0, s0);         // sxx: D- along x
+        pin_regs(s0);
+        __syncthreads();
+        stage_field(0, sa);
+        load_field(1, sa);  // vz
 #pragma unroll
-            for (int j = 0; j < R1 + 8; ++j) q[j] = lc[0][j1 + j][lane + FAT];
+        for (int r = 0; r < R1; ++r) s1[r] = dm_q(&lc[1][j1 + r + FAT][lane + FAT], 0, cz);  // szz: D- along z
+        pin_regs(s1);
+        __syncthreads();
+        stage_field(1, sb);
+        {  // sxz: D+ along x into S1, D+ along z into S0
+            T tx[R1];
+            dxp(0, tx);
 #pragma unroll
-            for (int r = 0; r < R1; ++r) dx_vx[r] = dp_q(q, r + 4, cx);
-#pragma unroll
-            for (int j = 0; j < R1 + 8; ++j) q[j] = lc[1][j1 + j][lane + FAT];
-#pragma unroll
-            for (int r = 0; r < R1; ++r) dx_vz[r] = dm_q(q, r + 4, cx);
+            for (int r = 0; r < R1; ++r) {
+                s1[r] = s1[r] + tx[r];
+                s0[r] = s0[r] + dp_q(&lc[0][j1 + r + FAT][lane + FAT], 0, cz);
+            }
         }
+        load_om(2);
 #pragma unroll
-        for (int r = 0; r < R1; ++r) {
+        for (int r = 0; r < R1; ++r) {  // vx and vz of u1
             const int ix = ix1 + r;
             const bool ok = zin && ix >= FAT && ix < nx - FAT;
-            const int c = j1 + r + FAT;
-            const T dz_sxz = dp_q(&lc[4][c][lane + FAT], 0, cz);
-            const T dz_szz = dm_q(&lc[3][c][lane + FAT], 0, cz);
-            const T dz_vz = dp_q(&lc[1][c][lane + FAT], 0, cz);
-            const T dz_vx = dm_q(&lc[0][c][lane + FAT], 0, cz);
             const T d = ok ? damp_at(S, iz, ix) : T(1);
-            T a0 = d * o[0][r] + m[0][r] * (dx_sxx[r] + dz_sxz);
-            T a1 = d * o[1][r] + m[1][r] * (dz_szz + dx_sxz[r]);
-            T a2 = d * o[2][r] + (m[2][r] * dx_vx[r] + m[3][r] * dz_vz);
-            T a3 = d * o[3][r] + (m[3][r] * dx_vx[r] + m[2][r] * dz_vz);
-            T a4 = d * o[4][r] + m[4][r] * (dz_vx + dx_vz[r]);
-            if constexpr (SS) {
-                const unsigned vo = (unsigned(ix < 0 ? 0 : ix) * unsigned(nz) + unsigned(izl)) * SZ;
-                T sl[5], sr[5];
-#pragma unroll
-                for (int f = 0; f < 5; ++f) {
-                    sl[f] = bld<T>(rssl, vo, unsigned(f) * fbytes);
-                    sr[f] = bld<T>(rssr, vo, unsigned(f) * fbytes);
-                }
-                a0 += sl[0] * ssf1a + sr[0] * ssf1b;
-                a1 += sl[1] * ssf1a + sr[1] * ssf1b;
-                a2 += sl[2] * ssf1a + sr[2] * ssf1b;
-                a3 += sl[3] * ssf1a + sr[3] * ssf1b;
-                a4 += sl[4] * ssf1a + sr[4] * ssf1b;
-            }
+            dv[r] = d;
+            const T a0 = d * o[0][r] + m[0][r] * s0[r];
+            const T a1 = d * o[1][r] + m[1][r] * s1[r];
             l1[0][j1 + r][lane] = ok ? d * a0 : T(0);
             l1[1][j1 + r][lane] = ok ? d * a1 : T(0);
+        }
+        __syncthreads();
+        stage_field(0, sa);
+        load_om(3);
+        load_om(4);
+        dxp(1, dxvx);  // vx: D+ along x, D- along z
+#pragma unroll
+        for (int r = 0; r < R1; ++r) s4[r] = dm_q(&lc[1][j1 + r + FAT][lane + FAT], 0, cz);
+        pin_regs(dxvx);
+        pin_regs(s4);
+        __syncthreads();
+        {  // vz: D- along x into S4, D+ along z
+            T tx[R1];
+            dxm(0, tx);
+#pragma unroll
+            for (int r = 0; r < R1; ++r) {
+                s4[r] = s4[r] + tx[r];
+                dzvz[r] = dp_q(&lc[0][j1 + r + FAT][lane + FAT], 0, cz);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {  // the stresses of u1
+            const int ix = ix1 + r;
+            const bool ok = zin && ix >= FAT && ix < nx - FAT;
+            const T d = dv[r];
+            const T a2 = d * o[2][r] + (m[2][r] * dxvx[r] + m[3][r] * dzvz[r]);
+            const T a3 = d * o[3][r] + (m[3][r] * dxvx[r] + m[2][r] * dzvz[r]);
+            const T a4 = d * o[4][r] + m[4][r] * s4[r];
             l1[2][j1 + r][lane] = ok ? d * a2 : T(0);
             l1[3][j1 + r][lane] = ok ? d * a3 : T(0);
             l1[4][j1 + r][lane] = ok ? d * a4 : T(0);
+        }
+    } else {
+        // ---- u_cur on (TX+16) x 72 cells from (x0-8, z0-8) (cells off the grid are 0), four rows
+        // per 16-byte load: all loads are issued first, then the older level and media of the
+        // step-1 cells, and only then are the staged values written to LDS, so every load of the
+        // tile is in flight at once.  Row quads start at z0-8+4q = 56 bz-4+4q, a multiple of 4:
+        // a quad is wholly above row 0 or wholly below it, and only the bottom edge cuts one.
+        constexpr int ZQ = ZC / 4;
+        constexpr int NFILL = (TXC * ZQ + NT - 1) / NT;
+        T hf[NFILL][5][4];
+        unsigned hm[NFILL];  // bit k: row k of the quad is on the grid
+#pragma unroll
+        for (int it = 0; it < NFILL; ++it) {
+            const int e = threadIdx.x + it * NT;
+            const int c = e / ZQ, q = e - c * ZQ;
+            const int gx = t.x0 - 2 * FAT + c, gz = t.z0 - 2 * FAT + 4 * q;
+            const bool in = e < TXC * ZQ && gx >= 0 && gx < nx && gz >= 0 && gz < nz;
+            const int nin = in ? (nz - gz < 4 ? nz - gz : 4) : 0;
+            hm[it] = (1u << nin) - 1u;
+            const unsigned hv = in ? (unsigned(gx) * unsigned(nz) + unsigned(gz)) * SZ : 0u;
+#pragma unroll
+            for (int f = 0; f < 5; ++f) bld4(rcur, hv, unsigned(f) * fbytes, hf[it][f]);
+        }
+        T o[5][R1], m[5][R1];
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {
+            const int ix = ix1 + r;
+            const unsigned vo = (unsigned(ix < 0 ? 0 : ix) * unsigned(nz) + unsigned(izl)) * SZ;
+#pragma unroll
+            for (int f = 0; f < 5; ++f) o[f][r] = bld<T>(rold, vo, unsigned(f) * fbytes);
+#pragma unroll
+            for (int f = 0; f < 5; ++f) m[f][r] = bld<T>(rmed, vo, unsigned(f) * fbytes);
+        }
+        // decimation slots its (x0, z0) and its+1 (x1, z1) of the owned cells, with the stage
+        if constexpr (DEC) {
+            const size_t s0 = size_t(st1.its % S.wcap) * n, s1 = size_t((st1.its + 1) % S.wcap) * n;
+            rwx0 = make_rsrc(S.Wvx + s0, fbytes);
+            rwx1 = make_rsrc(S.Wvx + s1, fbytes);
+            rwz0 = make_rsrc(S.Wvz + s0, fbytes);
+            rwz1 = make_rsrc(S.Wvz + s1, fbytes);
+            f1a = S.filt[st1.it2];
+            f1b = S.filt[S.h + st1.it2];
+            f2a = S.filt[st2.it2];
+            f2b = S.filt[S.h + st2.it2];
+            const bool sta = st1.it2 == 1 && st1.its == dlo;
+            const bool stb = st1.it2 == 1 && st1.its + 1 != dhi;
+#pragma unroll
+            for (int r = 0; r < R2; ++r) {
+                const unsigned vo = (unsigned(ix2 + r) * unsigned(nz) + unsigned(izl)) * SZ;
+                px0[r] = sta ? T(0) : bld<T>(rwx0, vo, 0);
+                pz0[r] = sta ? T(0) : bld<T>(rwz0, vo, 0);
+                px1[r] = stb ? T(0) : bld<T>(rwx1, vo, 0);
+                pz1[r] = stb ? T(0) : bld<T>(rwz1, vo, 0);
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < NFILL; ++it) {
+            const int e = threadIdx.x + it * NT;
+            if (e < TXC * ZQ) {
+                const int c = e / ZQ, q = e - c * ZQ;
+#pragma unroll
+                for (int f = 0; f < 5; ++f) {
+                    float4 v;
+                    v.x = (hm[it] & 1u) ? hf[it][f][0] : 0.0f;
+                    v.y = (hm[it] & 2u) ? hf[it][f][1] : 0.0f;
+                    v.z = (hm[it] & 4u) ? hf[it][f][2] : 0.0f;
+                    v.w = (hm[it] & 8u) ? hf[it][f][3] : 0.0f;
+                    *reinterpret_cast<float4*>(&lc[f][c][4 * q]) = v;
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- step 1 on columns j1 .. j1+R1-1 (rel x0-4) = lc columns j1+4 .. ; lane row = lc row lane+4
+        {
+            T dx_sxx[R1], dx_sxz[R1], dx_vx[R1], dx_vz[R1];
+            {
+                T q[R1 + 8];
+#pragma unroll
+                for (int j = 0; j < R1 + 8; ++j) q[j] = lc[2][j1 + j][lane + FAT];
+#pragma unroll
+                for (int r = 0; r < R1; ++r) dx_sxx[r] = dm_q(q, r + 4, cx);
+#pragma unroll
+                for (int j = 0; j < R1 + 8; ++j) q[j] = lc[4][j1 + j][lane + FAT];
+#pragma unroll
+                for (int r = 0; r < R1; ++r) dx_sxz[r] = dp_q(q, r + 4, cx);
+#pragma unroll
+                for (int j = 0; j < R1 + 8; ++j) q[j] = lc[0][j1 + j][lane + FAT];
+#pragma unroll
+                for (int r = 0; r < R1; ++r) dx_vx[r] = dp_q(q, r + 4, cx);
+#pragma unroll
+                for (int j = 0; j < R1 + 8; ++j) q[j] = lc[1][j1 + j][lane + FAT];
+#pragma unroll
+                for (int r = 0; r < R1; ++r) dx_vz[r] = dm_q(q, r + 4, cx);
+            }
+#pragma unroll
+            for (int r = 0; r < R1; ++r) {
+                const int ix = ix1 + r;
+                const bool ok = zin && ix >= FAT && ix < nx - FAT;
+                const int c = j1 + r + FAT;
+                const T dz_sxz = dp_q(&lc[4][c][lane + FAT], 0, cz);
+                const T dz_szz = dm_q(&lc[3][c][lane + FAT], 0, cz);
+                const T dz_vz = dp_q(&lc[1][c][lane + FAT], 0, cz);
+                const T dz_vx = dm_q(&lc[0][c][lane + FAT], 0, cz);
+                const T d = ok ? damp_at(S, iz, ix) : T(1);
+                T a0 = d * o[0][r] + m[0][r] * (dx_sxx[r] + dz_sxz);
+                T a1 = d * o[1][r] + m[1][r] * (dz_szz + dx_sxz[r]);
+                T a2 = d * o[2][r] + (m[2][r] * dx_vx[r] + m[3][r] * dz_vz);
+                T a3 = d * o[3][r] + (m[3][r] * dx_vx[r] + m[2][r] * dz_vz);
+                T a4 = d * o[4][r] + m[4][r] * (dz_vx + dx_vz[r]);
+                if constexpr (SS) {
+                    const unsigned vo = (unsigned(ix < 0 ? 0 : ix) * unsigned(nz) + unsigned(izl)) * SZ;
+                    T sl[5], sr[5];
+#pragma unroll
+                    for (int f = 0; f < 5; ++f) {
+                        sl[f] = bld<T>(rssl, vo, unsigned(f) * fbytes);
+                        sr[f] = bld<T>(rssr, vo, unsigned(f) * fbytes);
+                    }
+                    a0 += sl[0] * ssf1a + sr[0] * ssf1b;
+                    a1 += sl[1] * ssf1a + sr[1] * ssf1b;
+                    a2 += sl[2] * ssf1a + sr[2] * ssf1b;
+                    a3 += sl[3] * ssf1a + sr[3] * ssf1b;
+                    a4 += sl[4] * ssf1a + sr[4] * ssf1b;
+                }
+                l1[0][j1 + r][lane] = ok ? d * a0 : T(0);
+                l1[1][j1 + r][lane] = ok ? d * a1 : T(0);
+                l1[2][j1 + r][lane] = ok ? d * a2 : T(0);
+                l1[3][j1 + r][lane] = ok ? d * a3 : T(0);
+                l1[4][j1 + r][lane] = ok ? d * a4 : T(0);
+            }
         }
     }
     __syncthreads();
@@ -333,6 +484,12 @@ __device__ __forceinline__ void step2_fwd_tile(const ShotDev<T>* __restrict__ sh
 #pragma unroll
             for (int r = 0; r < R2; ++r) dx_vz[r] = dm_q(q, r + 4, cx);
         }
+        if constexpr (FS) {
+            pin_regs(dx_sxx);
+            pin_regs(dx_sxz);
+            pin_regs(dx_vx);
+            pin_regs(dx_vz);
+        }
 #pragma unroll
         for (int r = 0; r < R2; ++r) {
             const int ix = ix2 + r;
@@ -344,8 +501,13 @@ __device__ __forceinline__ void step2_fwd_tile(const ShotDev<T>* __restrict__ sh
             for (int f = 0; f < 5; ++f) mm[f] = bld<T>(rmed, vo, unsigned(f) * fbytes);
             // older level of step 2 = u_cur at this cell (lc column j2+r+8, row lane+4)
             T oo[5];
+            if constexpr (FS) {  // re-read (the stage held one field at a time; an L2 hit)
 #pragma unroll
-            for (int f = 0; f < 5; ++f) oo[f] = lc[f][c + FAT][lz2 + FAT];
+                for (int f = 0; f < 5; ++f) oo[f] = bld<T>(rcur, vo, unsigned(f) * fbytes);
+            } else {
+#pragma unroll
+                for (int f = 0; f < 5; ++f) oo[f] = lc[f][c + FAT][lz2 + FAT];
+            }
             const T dz_sxz = dp_q(&l1[4][c][lz2], 0, cz);
             const T dz_szz = dm_q(&l1[3][c][lz2], 0, cz);
             const T dz_vz = dp_q(&l1[1][c][lz2], 0, cz);
@@ -374,6 +536,12 @@ __device__ __forceinline__ void step2_fwd_tile(const ShotDev<T>* __restrict__ sh
             v2[2][r] = d * a2;
             v2[3][r] = d * a3;
             v2[4][r] = d * a4;
+            // FS (a 128-VGPR budget): one column's loads and LDS reads at a time
+            if constexpr (FS) {
+#pragma unroll
+                for (int f = 0; f < 5; ++f) asm volatile("" : "+v"(v2[f][r]));
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
     }
     if (has_src || has_rec) {  // stage u2 in the l1 area (uniform per tile)
@@ -442,13 +610,13 @@ __device__ __forceinline__ void step2_fwd_tile(const ShotDev<T>* __restrict__ sh
 }
 
 template <typename T, int R2, int WX, int OCC, int NTS, bool DEC, bool FINE = false, bool SS = false,
-          bool NOREC = false>
+          bool NOREC = false, bool FS = false>
 __global__ __launch_bounds__(64 * WX) __attribute__((amdgpu_waves_per_eu(OCC)))
 void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict__ tiles,
                  const Entry* __restrict__ ents, int ntiles, int k, int par, int dlo, int dhi,
                  int sspar) {
-    step2_fwd_tile<T, R2, WX, NTS != 0, DEC, FINE, SS, NOREC>(shots, tiles[xcd_remap(blockIdx.x, ntiles)], ents, k,
-                                                               par, dlo, dhi, sspar);
+    step2_fwd_tile<T, R2, WX, NTS != 0, DEC, FINE, SS, NOREC, FS>(shots, tiles[xcd_remap(blockIdx.x, ntiles)], ents,
+                                                                   k, par, dlo, dhi, sspar);
 }
 
 // ------------------------------------------------------------------ persistent two-step forward
@@ -466,7 +634,7 @@ void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
 // workgroups are resident and each processes its items in (pair, tile) order; every wait is
 // bounded by `spin_limit` polls, after which the workgroup sets *abort and returns, and every
 // other workgroup returns at its next wait, so the grid always drains.
-template <typename T, int R2, int WX, int OCC>
+template <typename T, int R2, int WX, int OCC, bool FS = false>
 __global__ __launch_bounds__(64 * WX) __attribute__((amdgpu_waves_per_eu(OCC)))
 void k_step2_fwd_persist(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict__ tiles,
                          const Entry* __restrict__ ents, int ntiles, const int* __restrict__ nbr,
@@ -502,7 +670,7 @@ void k_step2_fwd_persist(const ShotDev<T>* __restrict__ shots, const TileDev* __
                 __syncthreads();
                 if (s_stop) return;
             }
-            step2_fwd_tile<T, R2, WX, false, false, false, false, false>(shots, t, ents, 2 * p, p & 1, 0, -1, 0);
+            step2_fwd_tile<T, R2, WX, false, false, false, false, false, FS>(shots, t, ents, 2 * p, p & 1, 0, -1, 0);
             __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's stores have reached L2
             __syncthreads();
             if (threadIdx.x == 0) {

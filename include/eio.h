@@ -114,6 +114,12 @@ int eio_ctx_set_variant(eio_ctx* ctx, int variant);
 int eio_ctx_num_time_block_variants(void);
 int eio_ctx_time_block_info(int variant, int* R2, int* WX, int* occ);
 int eio_ctx_set_time_block(eio_ctx* ctx, int steps, int variant);
+/* 1 when two-step variant `variant` streams the current level through two LDS field buffers
+ * (78.8 KB of LDS per 56 x 32 tile instead of 120 KB: two workgroups per CU) for the NL
+ * forward and the FWI segment recompute (its other passes run variant 4, the same tile
+ * width), 0 when it stages all five fields, -1 for an unknown variant.  No counterpart in the
+ * reference. */
+int eio_ctx_time_block_streamed(int variant);
 /* Reverse fine steps per launch of the FP32 adjoint passes (eio_born_adjoint,
  * eio_fwi_gradient and eio_nl_adjoint without free surface or saved wavefield): 1 (one fused
  * step per launch, default) or 2 (both steps of a pair in one launch, four state levels per

@@ -337,7 +337,7 @@ def test_born_adjoint_with_reused_storage_of_another_shape():
     assert rel_l2(out, ref) < 2e-5, rel_l2(out, ref)
 
 
-@pytest.mark.parametrize("v", range(8))
+@pytest.mark.parametrize("v", range(11))
 @pytest.mark.parametrize("sub_fd", [2, 3])
 def test_two_step_nl_forward_matches_oracle(v, sub_fd):
     """The temporally blocked NL forward (two fine steps per launch, step 1 recomputed on the
@@ -925,7 +925,7 @@ def _persistent_batch(ctx, cases, seed):
     return shots, recs, keep
 
 
-@pytest.mark.parametrize("v2", [_capi.VARIANT_AUTO, 0, 4])
+@pytest.mark.parametrize("v2", [_capi.VARIANT_AUTO, 0, 4, 8, 10])
 def test_persistent_launch_equals_launched_pairs(v2):
     """The whole two-step NL forward in one persistent launch (tiles wait for their neighbours'
     previous pair through flags) gives the launched form's traces bit for bit, on an uneven
