@@ -583,6 +583,9 @@ def main():
         res["roofline"]["copy_probe_GBs"] = round(2 * nb / (cms / 1e3) / 1e9, 1)
         log("copy probe (one float4 per thread, nontemporal stores): %.3f ms for 2 x %d bytes -> %.0f GB/s" %
             (cms, nb, 2 * nb / cms / 1e6))
+        cms4 = ctx.copy_probe(nb, 10, kind=2)
+        res["roofline"]["copy_probe_word_GBs"] = round(2 * nb / (cms4 / 1e3) / 1e9, 1)
+        log("copy probe (one 4-byte word per thread): %.3f ms -> %.0f GB/s" % (cms4, 2 * nb / cms4 / 1e6))
         cms0 = ctx.copy_probe(nb, 10, kind=0)
         res["roofline"]["copy_probe_gridstride_GBs"] = round(2 * nb / (cms0 / 1e3) / 1e9, 1)
         log("copy probe (grid-stride float4): %.3f ms -> %.0f GB/s" % (cms0, 2 * nb / cms0 / 1e6))

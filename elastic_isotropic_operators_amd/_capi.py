@@ -183,7 +183,8 @@ class Context:
 
     def copy_probe(self, nbytes, reps=10, kind=1):
         """Mean time [ms] of a device copy of nbytes (two buffers allocated here): kind 1 one
-        16-byte vector per thread with nontemporal stores, kind 0 a grid-stride copy."""
+        16-byte vector per thread with nontemporal stores, kind 0 a grid-stride copy, kind 2 one
+        4-byte word per thread."""
         a, b = self.alloc(nbytes), self.alloc(nbytes)
         ms = ctypes.c_double()
         try:

@@ -2245,16 +2245,20 @@ int eio_memcpy(eio_ctx* ctx, void* dst, const void* src, unsigned long long byte
 int eio_copy_probe(eio_ctx* ctx, void* dst, const void* src, unsigned long long bytes, int reps, int kind,
                    double* ms) {
     return guarded([&]() {
-        if (kind != 0 && kind != 1) fail("copy probe kind must be 0 or 1");
+        if (kind < 0 || kind > 2) fail("copy probe kind must be 0, 1 or 2");
         Ctx* c = as_ctx(ctx);
         if (!dst || !src || bytes % 16 || reps < 1) fail("invalid copy probe arguments");
         DeviceGuard dg(c->device);
         const long n16 = long(bytes / 16);
         const long nb = (n16 + eio::NTHR - 1) / eio::NTHR;
-        if (nb >= (1L << 31)) fail("copy probe too large");
+        const long nb4 = (4 * n16 + eio::NTHR - 1) / eio::NTHR;
+        if (nb4 >= (1L << 31)) fail("copy probe too large");
         EIO_CHECK_HIP(hipEventRecord(c->ev[0], c->stream));
         for (int r = 0; r < reps; ++r) {
-            if (kind == 1)
+            if (kind == 2)
+                hipLaunchKernelGGL(eio::k_copy4, dim3(unsigned(nb4)), dim3(eio::NTHR), 0, c->stream,
+                                   static_cast<float*>(dst), static_cast<const float*>(src), 4 * n16);
+            else if (kind == 1)
                 hipLaunchKernelGGL(eio::k_copy16_nt, dim3(unsigned(nb)), dim3(eio::NTHR), 0, c->stream,
                                    static_cast<float4*>(dst), static_cast<const float4*>(src), n16);
             else

@@ -857,6 +857,13 @@ __global__ __launch_bounds__(NTHR) void k_copy16_nt(float4* __restrict__ dst,
     }
 }
 
+// the same probe at the access width of the stencil kernels' per-cell loads and stores: one
+// 4-byte word per thread (a wave moves 256 contiguous bytes per instruction)
+__global__ __launch_bounds__(NTHR) void k_copy4(float* __restrict__ dst, const float* __restrict__ src, long n4) {
+    const long i = long(blockIdx.x) * NTHR + threadIdx.x;
+    if (i < n4) dst[i] = src[i];
+}
+
 // element copy (checkpoint save/restore; any size and alignment)
 template <typename T>
 __global__ __launch_bounds__(NTHR) void k_copy(T* __restrict__ dst, const T* __restrict__ src, long n) {

@@ -252,8 +252,9 @@ int eio_fwi_release(eio_ctx* ctx, int handle);
 /* Diagnostics: copy `bytes` (multiple of 16) from src to dst (device pointers) `reps` times on
  * the context stream; *ms = mean kernel time.  kind 1: one 16-byte vector per thread with
  * nontemporal stores (the achievable HBM copy bandwidth); kind 0: a grid-stride 16-byte copy
- * (the reference kernel of the HBM byte-counter calibration).  No counterpart in the
- * reference. */
+ * (the reference kernel of the HBM byte-counter calibration); kind 2: one 4-byte word per
+ * thread (the access width of the stencil kernels' per-cell loads and stores).  No
+ * counterpart in the reference. */
 int eio_copy_probe(eio_ctx* ctx, void* dst, const void* src, unsigned long long bytes, int reps, int kind,
                    double* ms);
 
