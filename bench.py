@@ -358,13 +358,15 @@ def born_child(args):
             t0 = time.perf_counter()
             op.forward(False, model, data)
             tf = time.perf_counter() - t0
+            df = op.lastPropagationMs() / 1e3
             img = model.clone()
             op.adjoint(False, img, data)
             t0 = time.perf_counter()
             op.adjoint(False, img, data)
             ta = time.perf_counter() - t0
-            res[name] = (tf, ta, data.getNdArray().astype(np.float64), img.getNdArray().astype(np.float64))
-            log("born %s: forward %.3f s, adjoint %.3f s" % (name, tf, ta))
+            da = op.lastPropagationMs() / 1e3
+            res[name] = (tf, ta, data.getNdArray().astype(np.float64), img.getNdArray().astype(np.float64), df, da)
+            log("born %s: forward %.3f s (device %.3f s), adjoint %.3f s (device %.3f s)" % (name, tf, df, ta, da))
     if not args.no_cpu:
         # FP32 oracle Born forward + adjoint on the host cores, shot by shot until ~10 s,
         # extrapolated to the 8 shots
@@ -388,14 +390,18 @@ def born_child(args):
     ntw = (w["nts"] - 1) * 2 * w["sub"] + 1
     cells = 8 * (w["nz"] - 8) * (w["nx"] - 8) * (ntw - 1)   # one pass of the 8 shots
     rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
-    tf, ta, d32, i32 = res["f32"]
+    tf, ta, d32, i32, df, da = res["f32"]
     out = {"workload": w["name"], "shots": 8, "dtype": "f32", "cell_updates_per_pass": cells,
            "forward_s": round(tf, 4), "adjoint_s": round(ta, 4),
            "forward_gcell_per_s": round(2 * cells / tf / 1e9, 2),   # source + scattered pass
            "adjoint_gcell_per_s": round(2 * cells / ta / 1e9, 2),   # source + receiver pass
+           "device_forward_s": round(df, 4), "device_adjoint_s": round(da, 4),
+           "device_forward_gcell_per_s": round(2 * cells / df / 1e9, 2),
+           "device_adjoint_gcell_per_s": round(2 * cells / da / 1e9, 2),
            "f64_forward_s": round(res["f64"][0], 4), "f64_adjoint_s": round(res["f64"][1], 4),
            "rel_l2_data_f32_vs_f64": rel(d32, res["f64"][2]), "rel_l2_image_f32_vs_f64": rel(i32, res["f64"][3]),
-           "timing": "wall time of the operator call (host interpolation and transfers included)"}
+           "timing": "forward_s/adjoint_s: wall time of the operator call (host interpolation and transfers "
+                     "included); device_*: the call's device time on the context stream (HIP events)"}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu
     print("BORN_JSON " + json.dumps(out), flush=True)
@@ -458,7 +464,7 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=1, help="measure a plain copy kernel")
     ap.add_argument("--fwi", type=int, default=1, help="run the FWI gradient leg")
     ap.add_argument("--fwi-shots", type=int, default=32)
-    ap.add_argument("--fwi-steps", type=int, default=1)
+    ap.add_argument("--fwi-steps", type=int, default=2)
     ap.add_argument("--fwi-warmup", type=int, default=1)
     ap.add_argument("--fwi-scale", type=float, default=1.0)
     ap.add_argument("--fwi-adj", default="2:7",
@@ -546,8 +552,12 @@ def main():
     achieved = BYTES_PER_CELL["nl_fwd"] * cells / loop_s / 1e9
     if args.time_block == 2:
         R2, WX2, OCC2 = _capi.Context.time_block_variants()[args.variant2]
-        kname = "k_step2_fwd<float,%d,%d,%d%s> (2 fine steps per launch)" % (
-            R2, WX2, OCC2, ",FS" if _capi.Context.time_block_streamed(args.variant2) else "")
+        layout = _capi.Context.time_block_layout(args.variant2)
+        if layout == 2:
+            kname = "k_step2q_fwd<float,%d> (56x24 tile, 4-row lane quads, 2 fine steps per launch)" % OCC2
+        else:
+            kname = "k_step2_fwd<float,%d,%d,%d%s> (2 fine steps per launch)" % (
+                R2, WX2, OCC2, ",FS" if layout == 1 else "")
     else:
         R, WX, OCC = _capi.Context.variants()[args.variant if args.variant >= 0 else 1]
         kname = "k_step_fwd<float,%d,%d,%d,M_NL>" % (R, WX, OCC)

@@ -61,6 +61,7 @@ def lib():
                                ctypes.POINTER(ctypes.c_int)]),
             ("eio_model_set_media", [vp, ctypes.c_int, ctypes.POINTER(vp), ctypes.c_int]),
             ("eio_model_remove", [vp, ctypes.c_int]),
+            ("eio_model_set_damping", [vp, ctypes.c_int, vp]),
             ("eio_nl_forward", [vp, ctypes.c_int, ctypes.POINTER(Shot), ctypes.c_int]),
             ("eio_nl_forward_noalloc", [vp, ctypes.c_int, ctypes.POINTER(Shot), ctypes.c_int]),
             ("eio_nl_adjoint", [vp, ctypes.c_int, ctypes.POINTER(Shot), ctypes.c_int]),
@@ -89,7 +90,7 @@ def lib():
             ("eio_ctx_time_block_info", [ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
             ("eio_ctx_set_time_block", [vp, ctypes.c_int, ctypes.c_int]),
-            ("eio_ctx_time_block_streamed", [ctypes.c_int]),
+            ("eio_ctx_time_block_layout", [ctypes.c_int]),
             ("eio_ctx_num_adjoint_time_block_variants", []),
             ("eio_ctx_adjoint_time_block_info", [ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                                  ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
@@ -108,12 +109,12 @@ def lib():
 
 EXPORTED = ("eio_last_error", "eio_ctx_create", "eio_ctx_destroy", "eio_ctx_set_stream",
             "eio_ctx_set_graph", "eio_ctx_graph_info", "eio_ctx_last_timing", "eio_grid_check", "eio_model_add", "eio_model_set_media",
-            "eio_model_remove", "eio_nl_forward", "eio_nl_forward_noalloc", "eio_nl_adjoint", "eio_born_forward",
+            "eio_model_remove", "eio_model_set_damping", "eio_nl_forward", "eio_nl_forward_noalloc", "eio_nl_adjoint", "eio_born_forward",
             "eio_born_adjoint", "eio_device_alloc", "eio_device_free", "eio_memcpy",
             "eio_synchronize", "eio_copy_probe", "eio_ctx_set_variant", "eio_ctx_variant_info",
             "eio_ctx_num_variants", "eio_fwi_forward", "eio_fwi_gradient", "eio_fwi_info",
             "eio_fwi_release", "eio_ctx_num_time_block_variants", "eio_ctx_time_block_info",
-            "eio_ctx_set_time_block", "eio_ctx_time_block_streamed", "eio_ctx_set_step_limit", "eio_ctx_set_persistent", "eio_nl_forward_wavefield",
+            "eio_ctx_set_time_block", "eio_ctx_time_block_layout", "eio_ctx_set_step_limit", "eio_ctx_set_persistent", "eio_nl_forward_wavefield",
             "eio_nl_adjoint_wavefield", "eio_model_set_abc", "eio_nl_forward_snapshots",
             "eio_nl_step_states", "eio_ctx_release_workspace", "eio_ctx_num_adjoint_time_block_variants",
             "eio_ctx_adjoint_time_block_info", "eio_ctx_set_adjoint_time_block")
@@ -251,10 +252,10 @@ class Context:
         return out
 
     @staticmethod
-    def time_block_streamed(variant):
-        """True when two-step variant `variant` streams the current level through two LDS field
-        buffers (two workgroups per CU)."""
-        return lib().eio_ctx_time_block_streamed(int(variant)) == 1
+    def time_block_layout(variant):
+        """Lane layout of two-step variant `variant`: 0 row lanes with the whole stage, 1 row
+        lanes with the streamed stage, 2 four-row quad lanes (16-byte accesses)."""
+        return lib().eio_ctx_time_block_layout(int(variant))
 
     def set_graph(self, on):
         check(lib().eio_ctx_set_graph(self.h, int(bool(on))))
@@ -294,6 +295,11 @@ class Context:
             med = np.ascontiguousarray(media, dtype=self.dtype)
             arr = (ctypes.c_void_p * 5)(*[med[k].ctypes.data for k in range(5)])
             check(lib().eio_model_set_media(self.h, mid, arr, 0))
+
+    def set_damping(self, mid, table):
+        """Replace the model's sponge table (minPad factors in (0, 1])."""
+        t = np.ascontiguousarray(table, self.dtype)
+        check(lib().eio_model_set_damping(self.h, int(mid), t.ctypes.data_as(ctypes.c_void_p)))
 
     def model_remove(self, mid):
         check(lib().eio_model_remove(self.h, mid))

@@ -22,9 +22,11 @@ Network (per edge x-min, x-max, z-min, z-max): three "valid" 5x5 convolutions 10
 per-field scale.  Without biases the network is positively homogeneous, so its output scales
 with the wavefield.  Training: Adam on the per-sample relative squared error, each frame row
 weighted by the square of the stencil coefficient that reads it from the first update cell
-(the outermost frame rows barely matter to the stencil and are the hardest to predict).  The FP32 torch
-network here is the trainer; the GPU kernel runs the same network with BF16 weights and
-activations and FP32 accumulation, which `edge_forward(..., bf16=True)` emulates.
+(the outermost frame rows barely matter to the stencil and are the hardest to predict).  The
+network's forward and backward during training run on the library's training kernels
+(eio_abc_conv_forward / eio_abc_conv_backward: BF16 MFMA, FP32 accumulation, the inference
+kernel's rounding points; torch supplies the loss, the leaky-ReLU derivative and Adam);
+`edge_forward(..., bf16=True)` is the torch restatement the kernels are checked against.
 
 The test of a trained network is the boundary reflection energy: receivers inside the model
 record a shot with the wide sponge (the reference), the plain narrow sponge, a plain
@@ -32,6 +34,7 @@ intermediate sponge and the learned narrow boundary; the energy of each record's
 from the reference, relative to the reference's energy, measures what the boundary sends
 back.
 """
+import ctypes
 import time
 
 import numpy as np
@@ -147,6 +150,83 @@ class AbcNet:
     @classmethod
     def from_state(cls, st, device="cuda"):
         return cls(device, weights=[[st["%s_%d" % (EDGES[e], l)] for l in range(3)] for e in range(4)])
+
+
+# ------------------------------------------------------------------ training kernels (HIP)
+# One 5x5 layer of the network on the library's MFMA kernels (eio_abc_conv_forward /
+# eio_abc_conv_backward): NHWC, 16 channels, BF16 activations and weights at the inference
+# kernel's rounding points, FP32 accumulation; the backward gives the input gradient and the
+# weight gradient (summed in a fixed order).  Straight-through rounding, as torch's casts.
+_TRAIN = {}
+
+
+def _train_ctx():
+    if "ctx" not in _TRAIN:
+        L = _capi.lib()
+        vp, ci = ctypes.c_void_p, ctypes.c_int
+        L.eio_abc_conv_forward.argtypes = [vp, vp, ci, ci, ci, vp, ci, vp, vp]
+        L.eio_abc_conv_forward.restype = ci
+        L.eio_abc_conv_backward.argtypes = [vp, vp, vp, ci, ci, ci, vp, vp, vp]
+        L.eio_abc_conv_backward.restype = ci
+        _TRAIN["lib"] = L
+        _TRAIN["ctx"] = _capi.Context(torch.cuda.current_device(), np.float32)
+    return _TRAIN["lib"], _TRAIN["ctx"]
+
+
+class Conv5Layer(torch.autograd.Function):
+    """x (B, H, W, 16) FP32 holding BF16 values (W >= 20), w (16, 16, 5, 5) -> BF16(leaky(z))
+    (leaky) or z (the last layer) with z the valid 5x5 convolution, (B, H-4, W-4, 16)."""
+
+    @staticmethod
+    def forward(fctx, x, w, leaky):
+        L, ctx = _train_ctx()
+        a = x.detach().to(torch.bfloat16).contiguous()
+        wc = w.detach().float().contiguous()
+        B, H, Wd, C = a.shape
+        assert C == 16 and wc.shape == (16, 16, 5, 5)
+        z = torch.empty((B, H - 4, Wd - 4, 16), dtype=torch.float32, device=x.device)
+        h = torch.empty((B, H - 4, Wd - 4, 16), dtype=torch.bfloat16, device=x.device)
+        torch.cuda.synchronize()
+        _capi.check(L.eio_abc_conv_forward(ctx.h, a.data_ptr(), B, H, Wd, wc.data_ptr(), int(bool(leaky)),
+                                           z.data_ptr(), h.data_ptr()))
+        fctx.save_for_backward(a, wc, z)
+        fctx.leaky = bool(leaky)
+        return h.float() if leaky else z.clone()
+
+    @staticmethod
+    def backward(fctx, dh):
+        L, ctx = _train_ctx()
+        a, wc, z = fctx.saved_tensors
+        dz = dh.float()
+        if fctx.leaky:
+            dz = torch.where(z > 0, dz, LEAK * dz)
+        dz = dz.contiguous()
+        B, H, Wd, _ = a.shape
+        da = torch.empty(a.shape, dtype=torch.float32, device=dz.device) if fctx.needs_input_grad[0] else None
+        dw = torch.empty((16, 16, 5, 5), dtype=torch.float32, device=dz.device)
+        torch.cuda.synchronize()
+        _capi.check(L.eio_abc_conv_backward(ctx.h, a.data_ptr(), dz.data_ptr(), B, H, Wd, wc.data_ptr(),
+                                            None if da is None else da.data_ptr(), dw.data_ptr()))
+        return da, dw, None
+
+
+def edge_forward_hip(net, e, x):
+    """net.edge_forward(e, x, bf16=True) on the training kernels: x (B, 10, X + 12, Z + 12) ->
+    (B, 5, X, Z), differentiable in x and in the network's weights.  The long axis of a frame
+    is the kernels' W axis (z frames are transposed, and their taps with them)."""
+    h = F.pad(x, (0, 0, 0, 0, 0, 16 - x.shape[1])).permute(0, 2, 3, 1)   # (B, X+12, Z+12, 16)
+    tr = e >= 2
+    if tr:
+        h = h.transpose(1, 2)
+    for l in range(3):
+        w = net.W[e][l]
+        w = F.pad(w, (0, 0, 0, 0, 0, 16 - w.shape[1], 0, 16 - w.shape[0]))
+        if tr:
+            w = w.transpose(2, 3)
+        h = Conv5Layer.apply(h.contiguous(), w.contiguous(), l < 2)
+    if tr:
+        h = h.transpose(1, 2)
+    return h[..., :5].permute(0, 3, 1, 2)
 
 
 def save(path, net, scale, strip, on=1, info=None):
@@ -440,9 +520,11 @@ def train_rollout(net, seqs, meds, wn, scale, iters=1500, lr=2e-4, horizons=(4, 
 
 # ------------------------------------------------------------------ training
 def train(data, scale, iters=4000, lr=2e-3, batch=32, seed=0, val_frac=0.1, row_floor=0.05, diff=True, depth=4,
-          log=print):
+          log=print, engine="hip"):
     """Adam on the per-sample relative squared error of the corrections; returns the network
-    and the held-out relative L2 error per edge."""
+    and the held-out relative L2 error per edge.  engine "hip": the network's forward and
+    backward run on the library's training kernels (BF16 MFMA, the inference kernel's
+    rounding); "torch": FP32 torch convolutions."""
     torch.manual_seed(seed)
     inv = (1.0 / scale).view(1, 10, 1, 1)
     sets = []
@@ -466,7 +548,7 @@ def train(data, scale, iters=4000, lr=2e-3, batch=32, seed=0, val_frac=0.1, row_
         for e in range(4):
             Xt, Yt = sets[e][0], sets[e][1]
             idx = torch.randint(0, Xt.shape[0], (batch,), device="cuda", generator=g)
-            pred = net.edge_forward(e, Xt[idx])
+            pred = edge_forward_hip(net, e, Xt[idx]) if engine == "hip" else net.edge_forward(e, Xt[idx])
             y = Yt[idx]
             rw = row_weights(e, floor=row_floor, depth=depth)
             loss = loss + (rw * (pred - y) ** 2).sum() / (rw * y ** 2).sum().clamp_min(1e-30)
@@ -480,16 +562,25 @@ def train(data, scale, iters=4000, lr=2e-3, batch=32, seed=0, val_frac=0.1, row_
         for e in range(4):
             Xv, Yv = sets[e][2], sets[e][3]
             rw = row_weights(e, floor=row_floor, depth=depth)
-            err = sum(float((rw * (net.edge_forward(e, Xv[i:i + 64]) - Yv[i:i + 64]) ** 2).sum())
+            err = sum(float((rw * (net.edge_forward(e, Xv[i:i + 64], bf16=True) - Yv[i:i + 64]) ** 2).sum())
                       for i in range(0, Xv.shape[0], 64))
             val.append((err / float((rw * Yv ** 2).sum())) ** 0.5)
     return net, val
 
 
 # ------------------------------------------------------------------ reflection test
+def _edge_receivers(nz_i, nx_i, rec_every, inset, d=10.0):
+    """Receivers every `rec_every` cells along the four interior edges, `inset` cells in."""
+    zs = np.arange(inset, nz_i - inset, rec_every) * d
+    xs = np.arange(inset, nx_i - inset, rec_every) * d
+    rz = np.concatenate([np.full(xs.size, inset * d), np.full(xs.size, (nz_i - 1 - inset) * d), zs, zs])
+    rx = np.concatenate([xs, xs, np.full(zs.size, inset * d), np.full(zs.size, (nx_i - 1 - inset) * d)])
+    return rz, rx
+
+
 def reflection_test(vvr, net=None, scale=None, strip=10, mid=60, wide=120, nts=1500, dts=0.001, sub=2,
                     src=None, rec_every=2, inset=3, device=0, time_block=2, on=1, log=print,
-                    variants=("plain", "mid", "learned", "nosponge")):
+                    variants=("plain", "mid", "learned", "nosponge"), damping=None):
     """Shot records of the same interior model under the wide sponge (reference), the plain
     `strip` sponge, the plain `mid` sponge and (with a network) the learned `strip` boundary.
     Receivers: every `rec_every` cells along the four interior edges, `inset` cells in.
@@ -499,10 +590,7 @@ def reflection_test(vvr, net=None, scale=None, strip=10, mid=60, wide=120, nts=1
     ctx = _capi.Context(device, np.float32)
     nx_i, nz_i = vvr.shape[1], vvr.shape[2]
     d = 10.0
-    zs = np.arange(inset, nz_i - inset, rec_every) * d
-    xs = np.arange(inset, nx_i - inset, rec_every) * d
-    rz = np.concatenate([np.full(xs.size, inset * d), np.full(xs.size, (nz_i - 1 - inset) * d), zs, zs])
-    rx = np.concatenate([xs, xs, np.full(zs.size, inset * d), np.full(zs.size, (nx_i - 1 - inset) * d)])
+    rz, rx = _edge_receivers(nz_i, nx_i, rec_every, inset, d)
     if src is None:
         src = (0.3 * nz_i * d, 0.5 * nx_i * d)
     ntw = (nts - 1) * 2 * sub + 1
@@ -517,12 +605,17 @@ def reflection_test(vvr, net=None, scale=None, strip=10, mid=60, wide=120, nts=1
         runs.append(("learned%d" % strip, strip, True))
     if net is not None and "nosponge" in variants:
         runs.append(("learned%d_nosponge" % strip, strip, "nosponge"))
+    if damping is not None:
+        runs.append(("fitted%d" % strip, strip, "fitted"))
     ref = None
     for name, pad, learned in runs:
         w = padded(vvr, pad)
         if learned == "nosponge":
             w["minPad"] = 0
         shot = Shot(ctx, w, nts, dts, sub, src[0], src[1], rz, rx)
+        if learned == "fitted":
+            ctx.set_damping(shot.mid, np.asarray(damping, np.float32))
+            learned = False
         if learned:
             ctx.set_abc(shot.mid, on, net.flat(), scale)
         ctx.set_time_block(1 if learned else time_block, 0 if learned else 6)
@@ -541,6 +634,59 @@ def reflection_test(vvr, net=None, scale=None, strip=10, mid=60, wide=120, nts=1
         ctx.model_remove(shot.mid)
     ctx.close()
     return out
+
+
+# ------------------------------------------------------------------ fitted damping table
+def cosine_table(strip, alpha=0.99):
+    """The reference's sponge factors (nonlinearPropElasticGpuFunctions.cu:195-202)."""
+    j = np.arange(strip)
+    return alpha + (1.0 - alpha) * np.cos(np.pi / strip * (strip - j))
+
+
+def fit_damping(strip=10, wide=120, n_models=3, seed=55, nz_i=200, nx_i=400, nts=1500, dts=0.001, sub=2,
+                evals=400, rec_every=2, inset=3, log=print):
+    """A boundary that is stable by construction: the narrow strip's damping table itself,
+    `strip` factors in (0, 1] (d = exp(-exp(theta)), so no table can add energy), fitted to
+    wide-sponge runs that the propagator generates.  Minimises the mean reflection energy
+    (relative to the wide run's record) over n_models seeded layered models with Nelder-Mead
+    from the cosine table; returns (table, {"start": .., "best": .., "evals": ..})."""
+    from scipy.optimize import minimize
+    ctx = _capi.Context(0, np.float32)
+    ctx.set_time_block(2, _capi.VARIANT_AUTO)
+    rng = np.random.default_rng(seed)
+    rz, rx = _edge_receivers(nz_i, nx_i, rec_every, inset)
+    cases = []
+    for i in range(n_models):
+        vvr = W.layered_vvr(rng, nz_i, nx_i)
+        src = (float(rng.uniform(0.2, 0.8)) * nz_i * 10.0, float(rng.uniform(0.2, 0.8)) * nx_i * 10.0)
+        ref_shot = Shot(ctx, padded(vvr, wide), nts, dts, sub, src[0], src[1], rz, rx)
+        ref, _ = ref_shot.forward()
+        ctx.model_remove(ref_shot.mid)
+        cases.append((Shot(ctx, padded(vvr, strip), nts, dts, sub, src[0], src[1], rz, rx), ref))
+
+    def loss(theta):
+        table = np.exp(-np.exp(np.clip(theta, -30.0, 3.0))).astype(np.float32)
+        table = np.maximum(table, np.float32(1e-6))
+        r = 0.0
+        for shot, ref in cases:
+            ctx.set_damping(shot.mid, table)
+            data, _ = shot.forward()
+            e = float(((data - ref) ** 2).sum() / (ref ** 2).sum())
+            r += e if np.isfinite(e) else 1e30
+        return r / len(cases)
+
+    theta0 = np.log(-np.log(cosine_table(strip)))
+    start = loss(theta0)
+    res = minimize(loss, theta0, method="Nelder-Mead",
+                   options={"maxfev": evals, "xatol": 1e-3, "fatol": 1e-9 * start, "adaptive": True})
+    table = np.exp(-np.exp(np.clip(res.x, -30.0, 3.0)))
+    if log:
+        log("fitted damping: mean reflection %.3e (cosine table %.3e) after %d evaluations; table %s" %
+            (res.fun, start, res.nfev, " ".join("%.4f" % v for v in table)))
+    for shot, _ in cases:
+        ctx.model_remove(shot.mid)
+    ctx.close()
+    return table, {"start": start, "best": float(res.fun), "evals": int(res.nfev)}
 
 
 def with_gain(net, gamma):
@@ -601,6 +747,8 @@ def main(argv=None):
     ap.add_argument("--seq-models", type=int, default=100)
     ap.add_argument("--modes", nargs="+", default=["level"], choices=["diff", "level"])
     ap.add_argument("--depths", type=int, nargs="+", default=[4])
+    ap.add_argument("--engine", default="hip", choices=["hip", "torch"],
+                    help="training forward/backward: the library's MFMA kernels or torch FP32")
     args = ap.parse_args(argv)
 
     def log(m):
@@ -618,7 +766,8 @@ def main(argv=None):
         log("mode %s scale %s" % (mode, " ".join("%.3e" % v for v in scale.tolist())))
         for fl in args.row_floor:
           for depth in args.depths:
-            net, val = train(data, scale, iters=args.iters, row_floor=fl, diff=diff, depth=depth, log=log)
+            net, val = train(data, scale, iters=args.iters, row_floor=fl, diff=diff, depth=depth, log=log,
+                             engine=args.engine)
             log("mode %s row floor %g depth %d: held-out relative error per edge %s" % (
                 mode, fl, depth, " ".join("%.3f" % v for v in val)))
             if args.rollout:

@@ -26,10 +26,11 @@
 // rows (z frames); the workgroup stages the tile's input (+6 cells) to LDS as BF16 and
 // computes conv1 on (+4), conv2 on (+2) and conv3 on the tile, an MFMA block being 16 cells
 // along the frame (rows of one column for the x frames, columns of one row for the z frames,
-// whose LDS layout is transposed for that): for each of the 25 taps, A = 16 cells x 16 input
-// channels (read from LDS), B = 16 input x 16 output channels (the tap's weights, held in
-// VGPRs), with v_mfma_f32_16x16x16_bf16.  The network reads only update cells and writes only
-// frame cells, so tiles need no ordering among themselves: one launch per step.
+// whose LDS layout is transposed for that): the 25 taps are taken in 13 pairs (the last one
+// padded with zero weights), and for each pair A = 16 cells x 32 (two taps x 16 input
+// channels, read from LDS 16 bytes per lane), B = 32 x 16 output channels (the pair's weights,
+// held in VGPRs), with gfx950's v_mfma_f32_16x16x32_bf16.  The network reads only update cells
+// and writes only frame cells, so tiles need no ordering among themselves: one launch per step.
 #pragma once
 #include "eio_kernels.h"
 
@@ -42,8 +43,9 @@ constexpr int ABC_NTHR = 256;                       // 4 waves
 constexpr int ABC_MAXOX = 16;                       // tile columns
 constexpr int ABC_MAXOZ = 24;                       // tile rows
 constexpr int ABC_MAXR3 = ABC_MAXOZ;                // rows computed by conv3 (>= 16)
-constexpr int ABC_FRAG = 64 * 4;                    // bf16 of one 16x16 B fragment
-constexpr int ABC_WEDGE = ABC_LAYERS * ABC_TAPS * ABC_FRAG;  // bf16 per edge
+constexpr int ABC_PAIRS = (ABC_TAPS + 1) / 2;       // MFMA K steps of two taps
+constexpr int ABC_FRAG = 64 * 8;                    // bf16 of one 32x16 B fragment
+constexpr int ABC_WEDGE = ABC_LAYERS * ABC_PAIRS * ABC_FRAG;  // bf16 per edge
 constexpr float ABC_LEAK = 0.1f;
 // LDS: buffer 0 holds the input then conv2's output, buffer 1 conv1's output
 constexpr int ABC_LDS0 = (ABC_MAXOX + 12) * (ABC_MAXR3 + 12) * ABC_CH;  // bf16
@@ -58,14 +60,14 @@ struct AbcTile {
 
 // Per-shot parameters of the network (device pointers), indexed by the ordered shot.
 struct AbcShot {
-    const unsigned short* w;  // [4 edges][3 layers][25 taps][64 lanes][4] bf16, MFMA B-fragment order
+    const unsigned short* w;  // [4 edges][3 layers][13 tap pairs][64 lanes][8] bf16, MFMA B-fragment order
     float scale[5];           // per-field scale of the new level (and of the output)
     float inv_scale[10];      // inverse scales of the 10 input channels
     int diff;                 // 1: channels 5..9 are new - previous, 0: the previous level
     int depth;                // frame rows written, counted from the update region (1..4; the rest stay 0)
 };
 
-typedef short abc_s4 __attribute__((ext_vector_type(4)));
+typedef __bf16 abc_b8 __attribute__((ext_vector_type(8)));
 typedef float abc_f4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ unsigned short abc_bf16(float f) {  // round to nearest even
@@ -85,7 +87,7 @@ __device__ __forceinline__ int nx_of(unsigned n, int nz) { return int(n / unsign
 
 template <bool LAST, bool XMAJ, typename T>
 __device__ __forceinline__ void abc_layer(const unsigned short* __restrict__ in, int nA, int nB,
-                                          unsigned short* __restrict__ out, const abc_s4 (&b)[ABC_TAPS],
+                                          unsigned short* __restrict__ out, const abc_b8 (&b)[ABC_PAIRS],
                                           int lane, int wave, T* field, const AbcShot& P, unsigned n, int nz,
                                           int x0, int z0, int ox, int oz) {
     constexpr int NW = ABC_NTHR / 64;
@@ -101,16 +103,16 @@ __device__ __forceinline__ void abc_layer(const unsigned short* __restrict__ in,
         const int b1 = two ? min((blk2 - a1 * nbb) * 16, oB - 16) : b0;
         abc_f4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int dx = 0; dx < 5; ++dx) {
-#pragma unroll
-            for (int dz = 0; dz < 5; ++dz) {
-                // tap (dx, dz): cell (+dx along x, +dz along z) of the output cell
-                const int dA = XMAJ ? dz : dx, dB = XMAJ ? dx : dz;
-                const abc_s4 q0 = *reinterpret_cast<const abc_s4*>(in + ((a0 + dA) * nB + b0 + i + dB) * ABC_CH + 4 * g);
-                const abc_s4 q1 = *reinterpret_cast<const abc_s4*>(in + ((a1 + dA) * nB + b1 + i + dB) * ABC_CH + 4 * g);
-                acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(q0, b[dx * 5 + dz], acc0, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(q1, b[dx * 5 + dz], acc1, 0, 0, 0);
-            }
+        for (int p = 0; p < ABC_PAIRS; ++p) {
+            // lanes of group g < 2 read tap 2p, the others tap 2p+1 (the pad of the last pair
+            // reads tap 24 again: finite values times zero weights), input channels 8 (g & 1) ..
+            const int tap = (g < 2 || 2 * p + 1 >= ABC_TAPS) ? 2 * p : 2 * p + 1;
+            const int dx = tap / 5, dz = tap - 5 * (tap / 5);   // cell (+dx along x, +dz along z)
+            const int dA = XMAJ ? dz : dx, dB = XMAJ ? dx : dz;
+            const abc_b8 q0 = *reinterpret_cast<const abc_b8*>(in + ((a0 + dA) * nB + b0 + i + dB) * ABC_CH + 8 * (g & 1));
+            const abc_b8 q1 = *reinterpret_cast<const abc_b8*>(in + ((a1 + dA) * nB + b1 + i + dB) * ABC_CH + 8 * (g & 1));
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(q0, b[p], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(q1, b[p], acc1, 0, 0, 0);
         }
         // acc[j] = output cell (a, b + 4g + j), channel i
 #pragma unroll
@@ -142,8 +144,8 @@ __device__ __forceinline__ void abc_layer(const unsigned short* __restrict__ in,
 template <bool XMAJ, typename T>
 __device__ __forceinline__ void abc_tile(const ShotDev<T>& S, const AbcShot& P, const AbcTile& tl, int k,
                                          unsigned short* lds0, unsigned short* lds1,
-                                         const abc_s4 (&w1)[ABC_TAPS], const abc_s4 (&w2)[ABC_TAPS],
-                                         const abc_s4 (&w3)[ABC_TAPS]) {
+                                         const abc_b8 (&w1)[ABC_PAIRS], const abc_b8 (&w2)[ABC_PAIRS],
+                                         const abc_b8 (&w3)[ABC_PAIRS]) {
     const int nz = S.nz, nx = S.nx;
     const unsigned n = unsigned(nz) * unsigned(nx);
     T* nw = (k & 1) ? S.B : S.A;
@@ -199,16 +201,17 @@ __global__ __launch_bounds__(ABC_NTHR) void k_abc_cnn(const ShotDev<T>* __restri
     if (t >= ntile) return;
     const AbcTile tl = tiles[t];
     const AbcShot P = par[tl.shot];
-    // the three layers' B fragments: lane holds W[tap][cin = 4*(lane/16) + j][cout = lane%16];
-    // loaded before the staging so their latency overlaps it
+    // the three layers' B fragments: for tap pair p, lane holds W[tap 2p + (lane/32)]
+    // [cin = 8*((lane/16)&1) + j][cout = lane%16]; loaded before the staging so their latency
+    // overlaps it
     const int lane = threadIdx.x & 63;
     const unsigned short* we = P.w + size_t(tl.edge) * ABC_WEDGE;
-    abc_s4 w1[ABC_TAPS], w2[ABC_TAPS], w3[ABC_TAPS];
+    abc_b8 w1[ABC_PAIRS], w2[ABC_PAIRS], w3[ABC_PAIRS];
 #pragma unroll
-    for (int q = 0; q < ABC_TAPS; ++q) {
-        w1[q] = *reinterpret_cast<const abc_s4*>(we + (q * 64 + lane) * 4);
-        w2[q] = *reinterpret_cast<const abc_s4*>(we + ABC_TAPS * ABC_FRAG + (q * 64 + lane) * 4);
-        w3[q] = *reinterpret_cast<const abc_s4*>(we + 2 * ABC_TAPS * ABC_FRAG + (q * 64 + lane) * 4);
+    for (int q = 0; q < ABC_PAIRS; ++q) {
+        w1[q] = *reinterpret_cast<const abc_b8*>(we + (q * 64 + lane) * 8);
+        w2[q] = *reinterpret_cast<const abc_b8*>(we + ABC_PAIRS * ABC_FRAG + (q * 64 + lane) * 8);
+        w3[q] = *reinterpret_cast<const abc_b8*>(we + 2 * ABC_PAIRS * ABC_FRAG + (q * 64 + lane) * 8);
     }
     if (tl.edge >= 2) abc_tile<true, T>(shots[tl.shot], P, tl, k, lds0, lds1, w1, w2, w3);
     else abc_tile<false, T>(shots[tl.shot], P, tl, k, lds0, lds1, w1, w2, w3);

@@ -29,7 +29,9 @@
 #include "../../include/eio.h"
 #include "eio_kernels.h"
 #include "eio_step2.h"
+#include "eio_step2q.h"
 #include "eio_abc.h"
+#include "eio_abc_train.h"
 
 // Workspace of eio_nl_forward_noalloc: device memory the library never allocates at call
 // time.  Each GPU holds its own copy of this array for the life of the process; in host code
@@ -90,17 +92,19 @@ constexpr Variant F32_VARIANTS[] = {{4, 4, 2}, {2, 8, 4}, {2, 4, 4}, {4, 8, 2}, 
 constexpr int N_F32_VARIANTS = int(sizeof(F32_VARIANTS) / sizeof(F32_VARIANTS[0]));
 constexpr Variant F64_VARIANT = {2, 4, 1};
 
-// Two-step (temporally blocked) NL forward variants: R2 owned x-cells per lane, WX waves,
-// OCC waves per SIMD; owned tile = 56 x WX*R2.  NTS: nontemporal stores; FS: u_cur streamed
-// through two LDS field buffers instead of staged whole (78.8 KB instead of 120 KB at 56 x 32:
-// two workgroups per CU), for the NL forward and the recompute (other passes of an FS
-// variant run variant 4, the same tile width).  FP32 only.
+// Two-step (temporally blocked) NL forward variants: owned tile = 56 x WX*R2, WX waves, OCC
+// waves per SIMD.  LAYOUT 0: lane l owns row z0-4+l of R2 columns, u_cur staged whole in LDS;
+// 1: the same lanes, u_cur streamed through two LDS field buffers (78.8 KB instead of 120 KB
+// at 56 x 32: two workgroups per CU); 2: a lane owns a 4-row quad of one column (k_step2q_fwd:
+// 16-byte loads and stores; R2 = 3 stands for its 24 owned columns).  Layouts 1 and 2 serve
+// the NL forward and the recompute; the other passes of such a variant run a layout-0 kernel
+// of the same tile width.  NTS: nontemporal stores.  FP32 only.
 struct Variant2 {
-    int R2, WX, OCC, NTS, FS;
+    int R2, WX, OCC, NTS, LAYOUT;
 };
 constexpr Variant2 F32_VARIANTS2[] = {{2, 8, 2, 0, 0}, {2, 8, 4, 0, 0}, {4, 4, 2, 0, 0}, {1, 8, 4, 0, 0},
                                       {4, 8, 1, 0, 0}, {2, 8, 4, 1, 0}, {2, 8, 2, 1, 0}, {4, 8, 1, 1, 0},
-                                      {4, 8, 4, 0, 1}, {8, 4, 2, 0, 1}, {2, 8, 6, 0, 1}};
+                                      {4, 8, 4, 0, 1}, {8, 4, 2, 0, 1}, {2, 8, 6, 0, 1}, {3, 8, 2, 0, 2}};
 constexpr int N_F32_VARIANTS2 = int(sizeof(F32_VARIANTS2) / sizeof(F32_VARIANTS2[0]));
 // Two-step adjoint (Born receiver pass, FWI gradient) variants: R2, WX, OCC as above.
 struct Variant2A {
@@ -132,7 +136,7 @@ struct ModelBase {
     double cz[4], cx[4];
     int abc_on = 0;         // learned absorbing boundary attached: 1 difference inputs, 2 level inputs
     int abc_depth = 4;      // frame rows the network writes
-    void* abc_w = nullptr;  // [4][3][25][64][4] bf16 B fragments
+    void* abc_w = nullptr;  // [4][3][13][64][8] bf16 B fragments (tap pairs)
     float abc_scale[10] = {1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f};
 };
 
@@ -805,6 +809,12 @@ void launch_fwd2(Run<float>& run, int ntile, int k, int par, int dlo = 0, int dh
                 dim3(ntile), dim3(64 * WX_), sh, tt, te, ntile, k, par, dlo, dhi, sspar);             \
     else                                                                                              \
         EIO_L2(4, 8, 1, 0)
+#define EIO_L2Q(OCC_)                                                                                 \
+    if constexpr (!DEC && !SS && !FINE)                                                               \
+        klaunch(run.c, eio::k_step2q_fwd<float, OCC_, NOREC>, dim3(ntile), dim3(eio::QNT), sh, tt, te, ntile, \
+                k, par);                                                                              \
+    else                                                                                              \
+        EIO_L2(3, 8, 2, 0)
     switch (run.v2) {
         case 0: EIO_L2(2, 8, 2, 0); break;
         case 1: EIO_L2(2, 8, 4, 0); break;
@@ -816,8 +826,10 @@ void launch_fwd2(Run<float>& run, int ntile, int k, int par, int dlo = 0, int dh
         case 7: EIO_L2(4, 8, 1, 1); break;
         case 8: EIO_L2FS(4, 8, 4); break;
         case 9: EIO_L2FS(8, 4, 2); break;
-        default: EIO_L2FS(2, 8, 6); break;
+        case 10: EIO_L2FS(2, 8, 6); break;
+        default: EIO_L2Q(2); break;
     }
+#undef EIO_L2Q
 #undef EIO_L2FS
 #undef EIO_L2
 }
@@ -1163,6 +1175,7 @@ void with_persist_kernel(int v, F&& f) {
         case 8: f(eio::k_step2_fwd_persist<float, 4, 8, 4, true>, 8); break;
         case 9: f(eio::k_step2_fwd_persist<float, 8, 4, 2, true>, 4); break;
         case 10: f(eio::k_step2_fwd_persist<float, 2, 8, 6, true>, 8); break;
+        case 11: f(eio::k_step2_fwd_persist<float, 3, 8, 2, false, true>, 8); break;
         default: f(eio::k_step2_fwd_persist<float, 4, 8, 1>, 8); break;
     }
 }
@@ -1399,10 +1412,11 @@ void set_abc(Ctx* c, int model, int on, const float* w, const float* scale) {
                         for (int dz = 0; dz < 5; ++dz) {
                             const float v = w[wo + ((size_t(o) * cin[l] + i) * 5 + dx) * 5 + dz];
                             const int tap = dx * 5 + dz;
-                            // B fragment: lane = 16 * (i / 4) + o, element i % 4
-                            const int lane = 16 * (i / 4) + o;
-                            frag[size_t(e) * eio::ABC_WEDGE + size_t(l) * eio::ABC_TAPS * eio::ABC_FRAG +
-                                 size_t(tap) * eio::ABC_FRAG + lane * 4 + (i % 4)] = bf(v);
+                            // B fragment of tap pair tap/2 (32 x 16): K row 16 (tap % 2) + i sits in
+                            // lane 16 g + o with g = 2 (tap % 2) + i / 8, element i % 8
+                            const int lane = 16 * (2 * (tap % 2) + i / 8) + o;
+                            frag[size_t(e) * eio::ABC_WEDGE + size_t(l) * eio::ABC_PAIRS * eio::ABC_FRAG +
+                                 size_t(tap / 2) * eio::ABC_FRAG + lane * 8 + (i % 8)] = bf(v);
                         }
             wo += size_t(cout[l]) * cin[l] * 25;
         }
@@ -1963,8 +1977,8 @@ int eio_ctx_time_block_info(int variant, int* R2, int* WX, int* occ) {
 
 int eio_ctx_num_time_block_variants(void) { return N_F32_VARIANTS2; }
 
-int eio_ctx_time_block_streamed(int variant) {
-    return variant >= 0 && variant < N_F32_VARIANTS2 ? F32_VARIANTS2[variant].FS : -1;
+int eio_ctx_time_block_layout(int variant) {
+    return variant >= 0 && variant < N_F32_VARIANTS2 ? F32_VARIANTS2[variant].LAYOUT : -1;
 }
 
 int eio_ctx_set_adjoint_time_block(eio_ctx* ctx, int steps, int variant) {
@@ -2039,6 +2053,26 @@ int eio_model_set_media(eio_ctx* ctx, int model_id, const void* const media[5], 
         ModelBase& m = get_model(c, model_id);
         if (c->prec == 4) upload_model<float>(c, m, media, flags);
         else upload_model<double>(c, m, media, flags);
+    });
+}
+
+int eio_model_set_damping(eio_ctx* ctx, int model_id, const void* table) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        if (!table) fail("damping table is NULL");
+        DeviceGuard dg(c->device);
+        ModelBase& m = get_model(c, model_id);
+        const int n = m.g.minPad;
+        if (n < 1) fail("the model has no sponge (minPad = 0)");
+        std::vector<unsigned char> h(size_t(n) * c->prec);
+        std::memcpy(h.data(), table, h.size());
+        for (int j = 0; j < n; ++j) {
+            const double v = c->prec == 4 ? double(reinterpret_cast<const float*>(h.data())[j])
+                                           : reinterpret_cast<const double*>(h.data())[j];
+            if (!(v > 0.0 && v <= 1.0)) fail("damping factors must lie in (0, 1]");
+        }
+        EIO_CHECK_HIP(hipStreamSynchronize(c->stream));
+        EIO_CHECK_HIP(hipMemcpy(m.damp, h.data(), h.size(), hipMemcpyHostToDevice));
     });
 }
 
@@ -2271,6 +2305,59 @@ int eio_copy_probe(eio_ctx* ctx, void* dst, const void* src, unsigned long long 
         float t = 0.f;
         EIO_CHECK_HIP(hipEventElapsedTime(&t, c->ev[0], c->ev[1]));
         if (ms) *ms = double(t) / reps;
+    });
+}
+
+// ---- training kernels of the learned-boundary network (device pointers, synchronous)
+namespace {
+void abct_check(Ctx* c, int B, int H, int W) {
+    if (c->prec != 4) fail("the learned-boundary training kernels run in FP32 contexts");
+    if (B < 1 || H < 5 || W < 20) fail("layer input must have B >= 1, H >= 5 and W >= 20 (16 output cells)");
+    if (size_t(B) * H * W * eio::ABC_CH >= (size_t(1) << 31)) fail("layer too large");
+}
+int abct_grid(long nblk) { return int(std::min<long>((nblk + 3) / 4, 65536L)); }
+}  // namespace
+
+int eio_abc_conv_forward(eio_ctx* ctx, const void* a, int B, int H, int W, const float* weights, int leaky,
+                         float* z, void* h) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        abct_check(c, B, H, W);
+        if (!a || !weights || !z || !h) fail("NULL pointer");
+        DeviceGuard dg(c->device);
+        const long nblk = long(B) * (H - 4) * ((W - 4 + eio::AT_CELLS - 1) / eio::AT_CELLS);
+        hipLaunchKernelGGL(eio::k_abct_conv_fwd, dim3(abct_grid(nblk)), dim3(eio::AT_NTHR), 0, c->stream,
+                           static_cast<const unsigned short*>(a), B, H, W, weights, leaky, z,
+                           static_cast<unsigned short*>(h));
+        EIO_CHECK_HIP(hipGetLastError());
+        EIO_CHECK_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int eio_abc_conv_backward(eio_ctx* ctx, const void* a, const float* dz, int B, int H, int W, const float* weights,
+                          float* da, float* dweights) {
+    return guarded([&]() {
+        Ctx* c = as_ctx(ctx);
+        abct_check(c, B, H, W);
+        if (!a || !dz || !weights || !dweights) fail("NULL pointer");
+        DeviceGuard dg(c->device);
+        if (da) {
+            const long nblk = long(B) * H * ((W + eio::AT_CELLS - 1) / eio::AT_CELLS);
+            hipLaunchKernelGGL(eio::k_abct_conv_bwd_data, dim3(abct_grid(nblk)), dim3(eio::AT_NTHR), 0, c->stream,
+                               dz, B, H, W, weights, da);
+            EIO_CHECK_HIP(hipGetLastError());
+        }
+        const long nblk = long(B) * (H - 4) * ((W - 4 + 31) / 32);
+        const int nparts = int(std::min<long>(nblk, 1024L));
+        const size_t per = size_t(eio::ABC_TAPS) * eio::ABC_CH * eio::ABC_CH;
+        float* part = static_cast<float*>(ws_get(c, "abct_part", size_t(nparts) * per * sizeof(float)));
+        hipLaunchKernelGGL(eio::k_abct_conv_bwd_weight, dim3(nparts), dim3(eio::AT_NTHR), 0, c->stream,
+                           static_cast<const unsigned short*>(a), dz, B, H, W, part);
+        EIO_CHECK_HIP(hipGetLastError());
+        hipLaunchKernelGGL(eio::k_abct_reduce_weight, dim3(unsigned((per + eio::AT_NTHR - 1) / eio::AT_NTHR)),
+                           dim3(eio::AT_NTHR), 0, c->stream, part, nparts, dweights);
+        EIO_CHECK_HIP(hipGetLastError());
+        EIO_CHECK_HIP(hipStreamSynchronize(c->stream));
     });
 }
 

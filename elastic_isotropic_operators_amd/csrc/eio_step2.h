@@ -634,7 +634,12 @@ void k_step2_fwd(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
 // workgroups are resident and each processes its items in (pair, tile) order; every wait is
 // bounded by `spin_limit` polls, after which the workgroup sets *abort and returns, and every
 // other workgroup returns at its next wait, so the grid always drains.
-template <typename T, int R2, int WX, int OCC, bool FS = false>
+// the quad-lane tile of eio_step2q.h
+template <typename T, bool NOREC>
+__device__ __forceinline__ void step2q_fwd_tile(const ShotDev<T>* __restrict__ shots, const TileDev t,
+                                                const Entry* __restrict__ ents, int k, int par);
+
+template <typename T, int R2, int WX, int OCC, bool FS = false, bool QUAD = false>
 __global__ __launch_bounds__(64 * WX) __attribute__((amdgpu_waves_per_eu(OCC)))
 void k_step2_fwd_persist(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict__ tiles,
                          const Entry* __restrict__ ents, int ntiles, const int* __restrict__ nbr,
@@ -670,7 +675,8 @@ void k_step2_fwd_persist(const ShotDev<T>* __restrict__ shots, const TileDev* __
                 __syncthreads();
                 if (s_stop) return;
             }
-            step2_fwd_tile<T, R2, WX, false, false, false, false, false, FS>(shots, t, ents, 2 * p, p & 1, 0, -1, 0);
+            if constexpr (QUAD) step2q_fwd_tile<T, false>(shots, t, ents, 2 * p, p & 1);
+            else step2_fwd_tile<T, R2, WX, false, false, false, false, false, FS>(shots, t, ents, 2 * p, p & 1, 0, -1, 0);
             __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's stores have reached L2
             __syncthreads();
             if (threadIdx.x == 0) {

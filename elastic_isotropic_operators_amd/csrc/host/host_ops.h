@@ -774,6 +774,12 @@ class BornShots {
     }
 
     int nExp() const { return nExp_; }
+    // device time of the last forward or adjoint call (its passes and imaging), ms
+    double last_ms() const {
+        double t = 0;
+        eio_ctx_last_timing(dev_->ctx, &t, nullptr, nullptr);
+        return t;
+    }
 
    private:
     void scale_slices(std::vector<T>& sl) const {

@@ -326,6 +326,7 @@ PYBIND11_MODULE(EIO_MODULE, m) {
                  py::gil_scoped_release rel;
                  op.adjoint(add, mo.mutable_data(), da.data(), nr);
              })
+        .def("lastPropagationMs", &BO::last_ms)
         .def("setBackground", [](BO& op, py::object elasticParam) {
             py::array_t<real> el = array_of(elasticParam, false);
             op.setBackground(el.data());

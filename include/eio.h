@@ -114,12 +114,14 @@ int eio_ctx_set_variant(eio_ctx* ctx, int variant);
 int eio_ctx_num_time_block_variants(void);
 int eio_ctx_time_block_info(int variant, int* R2, int* WX, int* occ);
 int eio_ctx_set_time_block(eio_ctx* ctx, int steps, int variant);
-/* 1 when two-step variant `variant` streams the current level through two LDS field buffers
- * (78.8 KB of LDS per 56 x 32 tile instead of 120 KB: two workgroups per CU) for the NL
- * forward and the FWI segment recompute (its other passes run variant 4, the same tile
- * width), 0 when it stages all five fields, -1 for an unknown variant.  No counterpart in the
+/* Lane layout of two-step variant `variant`: 0 = lane l owns grid row z0-4+l of several
+ * columns and the current level is staged whole in LDS; 1 = the same lanes, the current level
+ * streamed through two LDS field buffers (two workgroups per CU at 56 x 32); 2 = a lane owns a
+ * 4-row quad of one column (16-byte loads and stores), streamed stage.  Layouts 1 and 2 serve
+ * the NL forward and the FWI segment recompute; the other passes of such a variant run a
+ * layout-0 kernel of the same tile width.  -1 for an unknown variant.  No counterpart in the
  * reference. */
-int eio_ctx_time_block_streamed(int variant);
+int eio_ctx_time_block_layout(int variant);
 /* Reverse fine steps per launch of the FP32 adjoint passes (eio_born_adjoint,
  * eio_fwi_gradient and eio_nl_adjoint without free surface or saved wavefield): 1 (one fused
  * step per launch, default) or 2 (both steps of a pair in one launch, four state levels per
@@ -158,6 +160,12 @@ int eio_model_add(eio_ctx* ctx, const eio_grid* grid, const void* const media[5]
 /* Replace the media of a model (setBackground, nonlinearPropElasticShotsGpu.h:61). */
 int eio_model_set_media(eio_ctx* ctx, int model_id, const void* const media[5], int flags);
 int eio_model_remove(eio_ctx* ctx, int model_id);
+/* Replace the model's sponge table (minPad factors in the context's precision, host pointer;
+ * entry j multiplies both time levels of the cells j cells in from the grid edge's update
+ * ring, as the cosine table of kernelsGpuElastic.cu:1192-1221 does).  Factors must lie in
+ * (0, 1], so any table only removes energy.  No counterpart in the reference, whose table is
+ * the cosine taper (used to fit a narrow sponge to wide-sponge runs). */
+int eio_model_set_damping(eio_ctx* ctx, int model_id, const void* table);
 
 /* Nonlinear forward of nshot shots in one batched time loop; rec_tr is overwritten.
  * Replaces propShotsElasticFwdGpu (nonlinearPropElasticGpuFunctions.h:20-26, .cu:561-648). */
@@ -208,6 +216,20 @@ int eio_model_set_abc(eio_ctx* ctx, int model_id, int on, const float* weights, 
  * No counterpart in the reference: training data of the learned boundary. */
 int eio_nl_forward_snapshots(eio_ctx* ctx, int nshot, const eio_shot* shots, int flags, int shot, int nsnap,
                              const int* steps, void* out);
+
+/* Training kernels of the learned-boundary network (no counterpart in the reference): one
+ * 5x5 "valid" convolution layer of 16 channels, NHWC, BF16 activations and weights, FP32
+ * accumulation on MFMA, the long axis W (>= 20).  Device pointers; the calls return when the
+ * kernels have finished.  weights: FP32 [16 out][16 in][5][5] (unused channels zero).
+ * forward:  a [B][H][W][16] BF16 -> z [B][H-4][W-4][16] FP32 and h = BF16(leaky(z)) (leaky = 1,
+ *           slope 0.1) or BF16(z) (leaky = 0), the same rounding points as the inference kernel.
+ * backward: dz [B][H-4][W-4][16] FP32 (the gradient with respect to z) -> da [B][H][W][16] FP32
+ *           (NULL: skipped) and dweights [16][16][5][5] FP32, both computed with dz rounded to
+ *           BF16; the weight gradient is summed in a fixed order (deterministic). */
+int eio_abc_conv_forward(eio_ctx* ctx, const void* a, int B, int H, int W, const float* weights, int leaky,
+                         float* z, void* h);
+int eio_abc_conv_backward(eio_ctx* ctx, const void* a, const float* dz, int B, int H, int W, const float* weights,
+                          float* da, float* dweights);
 
 /* One fine step without sources or receivers of nstate states of model_id, in place:
  * states [nstate][2][5][nx][nz] (device, EIO_DEVICE_PTRS) hold (older, newer) as the

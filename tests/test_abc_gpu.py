@@ -134,3 +134,64 @@ def test_torch_training_step_matches_the_gpu_step():
     b = ref[:, 0, :, F:nx - F, F:nz - F].double()
     assert float((a - b).norm() / b.norm()) < 1e-6
     ctx.close()
+
+
+@pytest.mark.parametrize("edge", [0, 3])
+def test_training_kernels_match_torch_autograd(edge):
+    """The network's forward and backward on the training kernels (eio_abc_conv_forward /
+    eio_abc_conv_backward, BF16 MFMA) against torch autograd of the BF16-emulating network
+    (edge_forward(bf16=True), the same rounding points): outputs, the input gradient and the
+    gradient of every layer's weights, for an x frame (4 x 90 outputs) and a z frame (90 x 4,
+    transposed for the kernels); a second backward gives the same weight gradients bit for bit."""
+    import torch
+    from elastic_isotropic_operators_amd import abc_learn as A
+    torch.manual_seed(7)
+    net = A.AbcNet("cuda", seed=11)
+    X, Z = (4, 90) if edge < 2 else (90, 4)
+    x = torch.randn((3, 10, X + 12, Z + 12), device="cuda")
+    gy = torch.randn((3, 5, X, Z), device="cuda")
+
+    def run(fwd):
+        for w in net.parameters():
+            w.grad = None
+        xi = x.clone().requires_grad_(True)
+        y = fwd(xi)
+        (y * gy).sum().backward()
+        return y.detach(), xi.grad.detach(), [w.grad.detach().clone() for w in net.W[edge]]
+
+    y_ref, gx_ref, gw_ref = run(lambda xi: net.edge_forward(edge, xi, bf16=True))
+    y_hip, gx_hip, gw_hip = run(lambda xi: A.edge_forward_hip(net, edge, xi))
+    err_y = rel_l2(y_hip.cpu().numpy(), y_ref.cpu().numpy())
+    err_x = rel_l2(gx_hip.cpu().numpy(), gx_ref.cpu().numpy())
+    err_w = [rel_l2(a.cpu().numpy(), b.cpu().numpy()) for a, b in zip(gw_hip, gw_ref)]
+    print("training kernels edge %d: output %.2e, input grad %.2e, weight grads %s" %
+          (edge, err_y, err_x, " ".join("%.2e" % e for e in err_w)))
+    assert err_y < 5e-3, err_y
+    assert err_x < 2e-2, err_x
+    assert max(err_w) < 2e-2, err_w
+    _, _, gw_again = run(lambda xi: A.edge_forward_hip(net, edge, xi))
+    assert all(torch.equal(a, b) for a, b in zip(gw_hip, gw_again))
+
+
+def test_set_damping_with_the_cosine_table_changes_nothing():
+    """eio_model_set_damping: the model's own cosine table written back gives the same
+    propagation bit for bit, a different table changes it, and factors outside (0, 1] are
+    rejected before anything is changed."""
+    from elastic_isotropic_operators_amd import _capi
+    from elastic_isotropic_operators_amd import abc_learn as A
+    ctx = _capi.Context(0, np.float32)
+    shot, w, _ = _small_shot(ctx)
+    base, _ = shot.forward()
+    ctx.set_damping(shot.mid, A.cosine_table(w["minPad"]).astype(np.float32))
+    same, _ = shot.forward()
+    assert np.array_equal(base, same)
+    with pytest.raises(_capi.EioError, match=r"\(0, 1\]"):
+        ctx.set_damping(shot.mid, np.full(w["minPad"], 1.5, np.float32))
+    with pytest.raises(_capi.EioError, match=r"\(0, 1\]"):
+        ctx.set_damping(shot.mid, np.zeros(w["minPad"], np.float32))
+    again, _ = shot.forward()
+    assert np.array_equal(base, again)
+    ctx.set_damping(shot.mid, np.full(w["minPad"], 0.9, np.float32))
+    other, _ = shot.forward()
+    assert not np.allclose(base, other)
+    ctx.close()

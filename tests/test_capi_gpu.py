@@ -337,7 +337,7 @@ def test_born_adjoint_with_reused_storage_of_another_shape():
     assert rel_l2(out, ref) < 2e-5, rel_l2(out, ref)
 
 
-@pytest.mark.parametrize("v", range(11))
+@pytest.mark.parametrize("v", range(12))
 @pytest.mark.parametrize("sub_fd", [2, 3])
 def test_two_step_nl_forward_matches_oracle(v, sub_fd):
     """The temporally blocked NL forward (two fine steps per launch, step 1 recomputed on the
@@ -548,7 +548,7 @@ def test_two_step_nl_dot_product():
     assert abs(a - b) / abs(b) < DOT_TOL[np.float32], (a, b)
 
 
-@pytest.mark.parametrize("v", [0, 1, 4, 7])
+@pytest.mark.parametrize("v", [0, 1, 4, 7, 8, 11])
 def test_two_step_born_source_pass_matches_oracle(v):
     """Born forward and adjoint whose source and scattered passes take two fine steps per launch
     (fine-sampled sources injected on the step-1 halo and Vx/Vz decimated into the whole
@@ -925,7 +925,7 @@ def _persistent_batch(ctx, cases, seed):
     return shots, recs, keep
 
 
-@pytest.mark.parametrize("v2", [_capi.VARIANT_AUTO, 0, 4, 8, 10])
+@pytest.mark.parametrize("v2", [_capi.VARIANT_AUTO, 0, 4, 8, 10, 11])
 def test_persistent_launch_equals_launched_pairs(v2):
     """The whole two-step NL forward in one persistent launch (tiles wait for their neighbours'
     previous pair through flags) gives the launched form's traces bit for bit, on an uneven

@@ -75,13 +75,16 @@ def test_checkpointed_gradient_matches_full_storage_oracle(dtype, segment):
         ctx.fwi_gradient(h, [shot2], out2)
 
 
+@pytest.mark.parametrize("variant2", [4, 11])
 @pytest.mark.parametrize("segment,sub", [(1, 2), (2, 1), (5, 2), (118, 1), (0, 2), (119, 2)])
-def test_two_step_checkpointed_forward_and_recompute_match_oracle(segment, sub):
+def test_two_step_checkpointed_forward_and_recompute_match_oracle(segment, sub, variant2):
     """FP32 with two fine steps per launch everywhere: the checkpointed forward (checkpoints
     taken from the level pair that holds the state at each segment start), the two-step
     segment recompute (no receiver recording) and the two-step receiver pass, against the FP64
     oracle's data and full-storage Born adjoint; every ring case (segment 1: four slices) and
-    sub_fd 2 and 4."""
+    sub_fd 2 and 4; the 56 x 32 row-lane forward (variant 4) and the 56 x 24 quad-lane one
+    (variant 11, whose last segment's decimating forward runs the row-lane kernel of its
+    width)."""
     w, g, med, spos, rpos = _reg_case(sub=sub)
     spos = spos[0]
     rng = np.random.default_rng(141)
@@ -91,7 +94,7 @@ def test_two_step_checkpointed_forward_and_recompute_match_oracle(segment, sub):
     src_fine = [O.time_interp(s, g.dts, g.sub) for s in src]
     ref_g = O.born_adj_reg(g, med, spos, src_fine, rpos, res, np.float64)
     ctx = _capi.Context(0, np.float32)
-    ctx.set_time_block(2, 4)
+    ctx.set_time_block(2, variant2)
     ctx.set_adjoint_time_block(2, 7)
     mid = ctx.model_add(g.nz, g.nx, g.nts, g.sub, g.dz, g.dx, g.dts, g.minPad, med, g.alphaCos)
     srcc = [np.ascontiguousarray(x, np.float32) for x in src]
@@ -110,7 +113,7 @@ def test_two_step_checkpointed_forward_and_recompute_match_oracle(segment, sub):
     ctx.fwi_release(h)
 
 
-@pytest.mark.parametrize("variant2", [4, 0])
+@pytest.mark.parametrize("variant2", [4, 0, 11])
 def test_two_step_forward_with_whole_record_matches_oracle(variant2):
     """FP32, one segment (the whole coarse record): the forward runs the two-step kernel with
     the decimation fused; data and gradient against the FP64 oracle's full-storage Born
