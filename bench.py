@@ -455,8 +455,9 @@ def main():
     ap.add_argument("--nshots", type=int, default=64)
     ap.add_argument("--graph", type=int, default=0)
     ap.add_argument("--variant", type=int, default=-1, help="FP32 step-kernel variant (-1: default)")
-    ap.add_argument("--time-block", type=int, default=1, help="fine steps per launch (1 or 2)")
+    ap.add_argument("--time-block", type=int, default=1, help="fine steps per launch (1, 2 or 3)")
     ap.add_argument("--variant2", type=int, default=0, help="two-step kernel variant")
+    ap.add_argument("--variant3", type=int, default=1, help="three-step kernel variant")
     ap.add_argument("--autotune", type=int, default=1,
                     help="pick time block and variant by timing every candidate (overrides the flags)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -514,14 +515,16 @@ def main():
     ctx.set_graph(args.graph)
     if args.variant >= 0:
         ctx.set_variant(args.variant)
-    ctx.set_time_block(args.time_block, args.variant2)
+    ctx.set_time_block(args.time_block, args.variant3 if args.time_block == 3 else args.variant2)
     shots, keep, cells, grids = prepare_shots(ctx, shots_w)
     log("uploaded %d shots, %.3e cell-updates per step" % (len(shots), cells))
     tuned = None
     if args.autotune:
         tuned = autotune.tune_nl_forward(ctx, shots, grids, log=log)
         args.time_block, v = tuned["choice"]
-        if args.time_block == 2:
+        if args.time_block == 3:
+            args.variant3 = v
+        elif args.time_block == 2:
             args.variant2 = v
         else:
             args.variant = v
@@ -550,7 +553,10 @@ def main():
     value = cells * args.steps / elapsed / 1e9
     loop_s = float(np.mean([s * n for s, n in zip(step_ms, nlaunch)])) / 1e3
     achieved = BYTES_PER_CELL["nl_fwd"] * cells / loop_s / 1e9
-    if args.time_block == 2:
+    if args.time_block == 3:
+        R3, WX3, OCC3 = _capi.Context.time_block3_variants()[args.variant3]
+        kname = "k_step3_fwd<float,%d,%d,%d> (3 fine steps per launch)" % (R3, WX3, OCC3)
+    elif args.time_block == 2:
         R2, WX2, OCC2 = _capi.Context.time_block_variants()[args.variant2]
         layout = _capi.Context.time_block_layout(args.variant2)
         if layout == 2:
@@ -576,7 +582,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": wl_name, "shots": len(shots), "cell_updates_per_step": cells,
                    "scale": args.scale, "graph": args.graph, "variant": args.variant,
-                   "time_block": args.time_block, "variant2": args.variant2,
+                   "time_block": args.time_block, "variant2": args.variant2, "variant3": args.variant3,
                    "autotune": None if tuned is None else {"choice": list(tuned["choice"]),
                                                            "ms_per_fine_step": tuned["table"],
                                                            "cached": tuned["cached"]}},

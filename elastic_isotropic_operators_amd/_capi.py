@@ -90,6 +90,9 @@ def lib():
             ("eio_ctx_time_block_info", [ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
             ("eio_ctx_set_time_block", [vp, ctypes.c_int, ctypes.c_int]),
+            ("eio_ctx_num_time_block3_variants", []),
+            ("eio_ctx_time_block3_info", [ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
             ("eio_ctx_time_block_layout", [ctypes.c_int]),
             ("eio_ctx_num_adjoint_time_block_variants", []),
             ("eio_ctx_adjoint_time_block_info", [ctypes.c_int, ctypes.POINTER(ctypes.c_int),
@@ -114,7 +117,8 @@ EXPORTED = ("eio_last_error", "eio_ctx_create", "eio_ctx_destroy", "eio_ctx_set_
             "eio_synchronize", "eio_copy_probe", "eio_ctx_set_variant", "eio_ctx_variant_info",
             "eio_ctx_num_variants", "eio_fwi_forward", "eio_fwi_gradient", "eio_fwi_info",
             "eio_fwi_release", "eio_ctx_num_time_block_variants", "eio_ctx_time_block_info",
-            "eio_ctx_set_time_block", "eio_ctx_time_block_layout", "eio_ctx_set_step_limit", "eio_ctx_set_persistent", "eio_nl_forward_wavefield",
+            "eio_ctx_set_time_block", "eio_ctx_num_time_block3_variants", "eio_ctx_time_block3_info",
+            "eio_ctx_time_block_layout", "eio_ctx_set_step_limit", "eio_ctx_set_persistent", "eio_nl_forward_wavefield",
             "eio_nl_adjoint_wavefield", "eio_model_set_abc", "eio_nl_forward_snapshots",
             "eio_nl_step_states", "eio_ctx_release_workspace", "eio_ctx_num_adjoint_time_block_variants",
             "eio_ctx_adjoint_time_block_info", "eio_ctx_set_adjoint_time_block")
@@ -248,6 +252,16 @@ class Context:
         for v in range(lib().eio_ctx_num_time_block_variants()):
             r, wx, oc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
             check(lib().eio_ctx_time_block_info(v, ctypes.byref(r), ctypes.byref(wx), ctypes.byref(oc)))
+            out.append((r.value, wx.value, oc.value))
+        return out
+
+    @staticmethod
+    def time_block3_variants():
+        """[(R3, WX, occ)] of the three-step NL forward kernel variants (owned tile 48 x WX*R3)."""
+        out = []
+        for v in range(lib().eio_ctx_num_time_block3_variants()):
+            r, wx, oc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+            check(lib().eio_ctx_time_block3_info(v, ctypes.byref(r), ctypes.byref(wx), ctypes.byref(oc)))
             out.append((r.value, wx.value, oc.value))
         return out
 

@@ -42,13 +42,15 @@ def signature(grids):
     two-step variant tables and the library build."""
     items = sorted(tuple(int(x) for x in g) for g in grids)
     key = {"shots": items, "variants": _capi.Context.variants(),
-           "time_block_variants": _capi.Context.time_block_variants(), "lib": _library_hash()}
+           "time_block_variants": _capi.Context.time_block_variants(),
+           "time_block3_variants": _capi.Context.time_block3_variants(), "lib": _library_hash()}
     return hashlib.sha1(json.dumps(key).encode()).hexdigest()[:16]
 
 
 def candidates():
     out = [(1, v) for v in range(len(_capi.Context.variants()))]
     out += [(2, v) for v in range(len(_capi.Context.time_block_variants()))]
+    out += [(3, v) for v in range(len(_capi.Context.time_block3_variants()))]
     return out
 
 
@@ -58,7 +60,7 @@ def _apply(ctx, cand):
         ctx.set_time_block(1, 0)
         ctx.set_variant(v)
     else:
-        ctx.set_time_block(2, v)
+        ctx.set_time_block(tb, v)
 
 
 def tune_nl_forward(ctx, shots, grids, steps=192, device_ptrs=True, use_cache=True, log=None):

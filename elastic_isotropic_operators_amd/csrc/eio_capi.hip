@@ -30,6 +30,7 @@
 #include "eio_kernels.h"
 #include "eio_step2.h"
 #include "eio_step2q.h"
+#include "eio_step3.h"
 #include "eio_abc.h"
 #include "eio_abc_train.h"
 
@@ -106,6 +107,13 @@ constexpr Variant2 F32_VARIANTS2[] = {{2, 8, 2, 0, 0}, {2, 8, 4, 0, 0}, {4, 4, 2
                                       {4, 8, 1, 0, 0}, {2, 8, 4, 1, 0}, {2, 8, 2, 1, 0}, {4, 8, 1, 1, 0},
                                       {4, 8, 4, 0, 1}, {8, 4, 2, 0, 1}, {2, 8, 6, 0, 1}, {3, 8, 2, 0, 2}};
 constexpr int N_F32_VARIANTS2 = int(sizeof(F32_VARIANTS2) / sizeof(F32_VARIANTS2[0]));
+// Three-step NL forward variants (k_step3_fwd): R3 owned x-cells per lane, WX waves, OCC waves
+// per SIMD; owned tile = 48 x WX*R3.
+struct Variant3 {
+    int R3, WX, OCC;
+};
+constexpr Variant3 F32_VARIANTS3[] = {{3, 8, 1}, {4, 8, 1}};
+constexpr int N_F32_VARIANTS3 = int(sizeof(F32_VARIANTS3) / sizeof(F32_VARIANTS3[0]));
 // Two-step adjoint (Born receiver pass, FWI gradient) variants: R2, WX, OCC as above.
 struct Variant2A {
     int R2, WX, OCC, LATE, PS;  // LATE: per-cell loads after the stage is written; PS: slots with the stage
@@ -195,7 +203,8 @@ struct Ctx {
     unsigned long long graph_clock = 0;
     long long graph_builds = 0, graph_launches = 0, graph_node_updates = 0;
     int variant = 1;  // best measured FP32 variant on the 64-shot layered batch
-    int time_block = 1;  // fine steps per launch of the FP32 NL forward (1 or 2)
+    int time_block = 1;  // fine steps per launch of the FP32 NL forward (1, 2 or 3)
+    int variant3 = 0;    // three-step kernel variant, or EIO_VARIANT_AUTO
     int variant2 = 0;    // two-step kernel variant, or EIO_VARIANT_AUTO (chosen per call)
     int adj_time_block = 1;  // reverse fine steps per launch of the FP32 Born adjoint (1 or 2)
     int variant2a = 0;       // two-step adjoint kernel variant, or EIO_VARIANT_AUTO
@@ -449,19 +458,61 @@ void build_entries2(const ModelBase& m, const int* const pos[4], const int nreg[
     }
 }
 
+// Entries of a three-step tile: sources of the step-1 region [x0-8, x0+TX+8) x [z0-8, z0+56)
+// (every tile whose region holds the source gets it), receivers of the owned cells only;
+// lcell relative to (z0-8, x0-8).
+void build_entries3(const ModelBase& m, const int* const pos[4], const int nreg[4], int TX, bool halo,
+                    std::vector<std::vector<eio::Entry>>& per_tile, int nbz, int nbx, const char* what) {
+    const int nz = m.g.nz, nx = m.g.nx;
+    constexpr int O = eio::H3, TZ = eio::TZ3;
+    for (int k = 0; k < 5; ++k) {
+        const int gi = GRID_OF_COMP[k];
+        const int cnt = nreg[gi];
+        if (cnt < 0) fail(std::string(what) + ": negative device count");
+        if (cnt > 0 && !pos[gi]) fail(std::string(what) + ": NULL position list");
+        if (cnt >= (1 << 24)) fail(std::string(what) + ": too many devices");
+        for (int u = 0; u < cnt; ++u) {
+            const int p = pos[gi][u];
+            if (p < 0 || long(p) >= m.n) fail(std::string(what) + ": device position out of the grid");
+            const int iz = p % nz, ix = p / nz;
+            if (iz < eio::FAT || iz >= nz - eio::FAT || ix < eio::FAT || ix >= nx - eio::FAT)
+                fail(std::string(what) +
+                     ": device inside the FAT halo (only update-region positions are supported)");
+            const int h = halo ? O : 0;
+            // candidate tiles (a superset of those whose region [FAT + bz TZ - h,
+            // FAT + (bz+1) TZ + h) holds iz, likewise along x); the exact test follows
+            const int bz_lo = std::max(0, (iz - eio::FAT - h) / TZ - 1);
+            const int bz_hi = std::min(nbz - 1, (iz - eio::FAT + h) / TZ);
+            const int bx_lo = std::max(0, (ix - eio::FAT - h) / TX - 1);
+            const int bx_hi = std::min(nbx - 1, (ix - eio::FAT + h) / TX);
+            for (int bx = bx_lo; bx <= bx_hi; ++bx)
+                for (int bz = bz_lo; bz <= bz_hi; ++bz) {
+                    const int z0 = eio::FAT + bz * TZ, x0 = eio::FAT + bx * TX;
+                    const int lz = iz - (z0 - O), lx = ix - (x0 - O);
+                    if (lz < O - h || lz >= O + TZ + h || lx < O - h || lx >= O + TX + h) continue;
+                    eio::Entry e;
+                    e.lcell = lz | (lx << 8);
+                    e.crow = u | (k << 24);
+                    per_tile[size_t(bx) * nbz + bz].push_back(e);
+                }
+        }
+    }
+}
+
 // Tile plans: one-step kernels (64 x TX tiles), the two-step forward (56 x TX tiles, source
 // entries on the step-1 region, receivers owned-only) and the two-step adjoint (56 x TX
 // tiles, receiver entries on the step-1 region: they are injected in both steps; source
 // entries owned-only: the NL adjoint reads them in both steps).
-enum class PlanKind { ONE, FWD2, ADJ2 };
+enum class PlanKind { ONE, FWD2, ADJ2, FWD3 };
 
 template <typename T>
 Plan make_plan(Ctx* c, int nshot, const eio_shot* shots, Op op, PlanKind kind = PlanKind::ONE, int v2 = 0) {
-    const bool two_step = kind != PlanKind::ONE;
+    const bool two_step = kind == PlanKind::FWD2 || kind == PlanKind::ADJ2;
     const int TX = kind == PlanKind::FWD2   ? F32_VARIANTS2[v2].R2 * F32_VARIANTS2[v2].WX
                    : kind == PlanKind::ADJ2 ? F32_VARIANTS2A[v2].R2 * F32_VARIANTS2A[v2].WX
+                   : kind == PlanKind::FWD3 ? F32_VARIANTS3[v2].R3 * F32_VARIANTS3[v2].WX
                                             : tile_x<T>(c->variant);
-    const int TZ = two_step ? eio::TZ2 : eio::TZ;
+    const int TZ = kind == PlanKind::FWD3 ? eio::TZ3 : two_step ? eio::TZ2 : eio::TZ;
     Plan p;
     p.order.resize(nshot);
     std::iota(p.order.begin(), p.order.end(), 0);
@@ -474,7 +525,10 @@ Plan make_plan(Ctx* c, int nshot, const eio_shot* shots, Op op, PlanKind kind = 
         const int nbz = (m.g.nz - 2 * eio::FAT + TZ - 1) / TZ;
         const int nbx = (m.g.nx - 2 * eio::FAT + TX - 1) / TX;
         std::vector<std::vector<eio::Entry>> src_t(size_t(nbz) * nbx), rec_t(size_t(nbz) * nbx);
-        if (kind == PlanKind::FWD2) {
+        if (kind == PlanKind::FWD3) {
+            build_entries3(m, s.src_pos, s.n_src, TX, true, src_t, nbz, nbx, "source");
+            build_entries3(m, s.rec_pos, s.n_rec, TX, false, rec_t, nbz, nbx, "receiver");
+        } else if (kind == PlanKind::FWD2) {
             build_entries2(m, s.src_pos, s.n_src, TX, true, src_t, nbz, nbx, "source");
             build_entries2(m, s.rec_pos, s.n_rec, TX, false, rec_t, nbz, nbx, "receiver");
         } else if (kind == PlanKind::ADJ2) {
@@ -514,6 +568,7 @@ struct Run {
     Ctx* c;
     int v2 = 0;   // two-step forward variant of this call's FWD2 plans
     int v2a = 0;  // two-step adjoint variant of this call's ADJ2 plan
+    int v3 = 0;   // three-step forward variant of this call's FWD3 plan
     Plan plan;
     std::vector<eio::ShotDev<T>> sd;
     eio::ShotDev<T>* d_shots = nullptr;
@@ -565,6 +620,13 @@ int variant2a_for(Ctx* c, int nshot, const eio_shot* shots) {
     return wide_tiles(c, nshot, shots) < AUTO_SMALL_TILES ? 10 : 7;
 }
 
+// The three-step forward variant of a call: EIO_VARIANT_AUTO takes 48 x 24 tiles (variant 0)
+// for a batch of fewer than AUTO_SMALL_TILES tiles of 56 x 32, else 48 x 32 (variant 1).
+int variant3_for(Ctx* c, int nshot, const eio_shot* shots) {
+    if (c->variant3 != EIO_VARIANT_AUTO) return c->variant3;
+    return wide_tiles(c, nshot, shots) < AUTO_SMALL_TILES ? 0 : 1;
+}
+
 // Builds the plan, the per-shot descriptors and the device tables of one call, uploads
 // input traces and zeroes the state and the outputs.  FWI calls bring their own coarse
 // storage ring (`ring`, `wcap` slices per component and shot).  `two_step`: the NL forward
@@ -573,7 +635,7 @@ int variant2a_for(Ctx* c, int nshot, const eio_shot* shots) {
 template <typename T>
 Run<T> prepare(Ctx* c, int nshot, const eio_shot* shots, Op op, int flags,
                const void* const dm_fwd[5], T* ring = nullptr, int wcap = 0, bool two_step = false,
-               bool adj2 = false, bool src2 = false) {
+               bool adj2 = false, bool src2 = false, bool three_step = false) {
     if (nshot <= 0) fail("nshot must be > 0");
     if (!shots) fail("shots is NULL");
     // the learned boundary is part of the NL forward only: the adjoint, Born and FWI passes
@@ -588,10 +650,12 @@ Run<T> prepare(Ctx* c, int nshot, const eio_shot* shots, Op op, int flags,
     run.c = c;
     run.v2 = variant2_for(c, nshot, shots);
     run.v2a = variant2a_for(c, nshot, shots);
-    run.plan = make_plan<T>(c, nshot, shots, op, two_step ? PlanKind::FWD2 : PlanKind::ONE, run.v2);
+    run.v3 = variant3_for(c, nshot, shots);
+    run.plan = three_step ? make_plan<T>(c, nshot, shots, op, PlanKind::FWD3, run.v3)
+                          : make_plan<T>(c, nshot, shots, op, two_step ? PlanKind::FWD2 : PlanKind::ONE, run.v2);
     if (adj2) run.plan2 = make_plan<T>(c, nshot, shots, op, PlanKind::ADJ2, run.v2a);
     if (src2) run.plan3 = make_plan<T>(c, nshot, shots, op, PlanKind::FWD2, run.v2);
-    const int nlev = (two_step || adj2 || src2) ? 4 : 2;  // state levels per shot
+    const int nlev = (two_step || three_step || adj2 || src2) ? 4 : 2;  // state levels per shot
     const bool born = (op == Op::BORN_FWD || op == Op::BORN_ADJ);
     const bool fwi = (op == Op::FWI_FWD || op == Op::FWI_GRAD);
     const bool has_ss = born || op == Op::FWI_GRAD;
@@ -642,7 +706,7 @@ Run<T> prepare(Ctx* c, int nshot, const eio_shot* shots, Op op, int flags,
         for (int k = 0; k < 4; ++k) { d.cz[k] = T(m.cz[k]); d.cx[k] = T(m.cx[k]); }
         d.A = state + so;
         d.B = state + so + 5 * size_t(m.n);
-        if (two_step || adj2 || src2) {
+        if (two_step || three_step || adj2 || src2) {
             d.C = state + so + 10 * size_t(m.n);
             d.D = state + so + 15 * size_t(m.n);
         }
@@ -832,6 +896,21 @@ void launch_fwd2(Run<float>& run, int ntile, int k, int par, int dlo = 0, int dh
 #undef EIO_L2Q
 #undef EIO_L2FS
 #undef EIO_L2
+}
+
+// Fine steps k, k+1, k+2 of the FP32 NL forward (levels A,B -> C,D when par = 0).
+void launch_fwd3(Run<float>& run, int ntile, int k, int par) {
+    if (ntile <= 0) return;
+    switch (run.v3) {
+        case 0:
+            klaunch(run.c, eio::k_step3_fwd<float, 3, 8, 1>, dim3(ntile), dim3(512), run.d_shots, run.d_tiles,
+                    run.d_ents, ntile, k, par);
+            break;
+        default:
+            klaunch(run.c, eio::k_step3_fwd<float, 4, 8, 1>, dim3(ntile), dim3(512), run.d_shots, run.d_tiles,
+                    run.d_ents, ntile, k, par);
+            break;
+    }
 }
 
 // Reverse fine steps k, k+1 (k even) of the Born receiver pass (adjoint-field slots as in
@@ -1243,8 +1322,10 @@ void nl_forward(Ctx* c, int nshot, const eio_shot* shots, int flags, int wshot =
     bool any_fs = false;
     for (int i = 0; i < nshot && shots; ++i) any_fs = any_fs || get_model(c, shots[i].model).g.surface;
     const bool abc = any_abc(c, nshot, shots);
-    const bool two = sizeof(T) == 4 && c->time_block == 2 && !wfld && !any_fs && !abc && nsnap == 0;
-    Run<T> run = prepare<T>(c, nshot, shots, Op::NL_FWD, flags, nullptr, nullptr, 0, two);
+    const bool plain = sizeof(T) == 4 && !wfld && !any_fs && !abc && nsnap == 0;
+    const bool three = plain && c->time_block == 3;
+    const bool two = plain && c->time_block == 2;
+    Run<T> run = prepare<T>(c, nshot, shots, Op::NL_FWD, flags, nullptr, nullptr, 0, two, false, false, three);
     if (wfld) attach_wavefield(run, wshot, wfld, flags);
     AbcRun ar;
     if (abc) ar = abc_prepare(run, shots);
@@ -1289,6 +1370,12 @@ void nl_forward(Ctx* c, int nshot, const eio_shot* shots, int flags, int wshot =
                 // pairs (k, k+1); a shot whose last pair has one step does only step k
                 for (int k = 0; k < kmax; k += 2, ++nlaunch)
                     launch_fwd2(run, active_tiles(run.plan, k), k, (k >> 1) & 1);
+                return;
+            }
+            if (three) {
+                // triples (k, k+1, k+2); a shot's last launch may take one or two steps
+                for (int k = 0; k < kmax; k += 3, ++nlaunch)
+                    launch_fwd3(run, active_tiles(run.plan, k), k, (k / 3) & 1);
                 return;
             }
         }
@@ -1510,7 +1597,7 @@ void born_forward(Ctx* c, int nshot, const eio_shot* shots, const void* const dm
     const void* dmp[5] = {b, b + m.n, b + 2 * m.n, b + 3 * m.n, b + 4 * m.n};
     const int kmax = m.ntw - 1, sub = m.g.sub_fd, nts = m.g.nts;
     // pairs must stay inside a coarse interval (decimation, secondary-source slots): even sub_fd
-    const bool src2 = sizeof(T) == 4 && c->time_block == 2 && sub % 2 == 0;
+    const bool src2 = sizeof(T) == 4 && c->time_block >= 2 && sub % 2 == 0;
     Run<T> run = prepare<T>(c, nshot, shots, Op::BORN_FWD, flags, dmp, nullptr, 0, false, false, src2);
     const int ntile = int(run.plan.tiles.size());
     long long nstep = 0;
@@ -1551,7 +1638,7 @@ void born_adjoint(Ctx* c, int nshot, const eio_shot* shots, void* const dm[5], i
     const int kmax = m.ntw - 1, sub = m.g.sub_fd, nts = m.g.nts;
     const bool two = sizeof(T) == 4 && c->adj_time_block == 2 && sub % 2 == 0;
     // pairs must stay inside a coarse interval (decimation, secondary-source slots): even sub_fd
-    const bool src2 = sizeof(T) == 4 && c->time_block == 2 && sub % 2 == 0;
+    const bool src2 = sizeof(T) == 4 && c->time_block >= 2 && sub % 2 == 0;
     Run<T> run = prepare<T>(c, nshot, shots, Op::BORN_ADJ, flags, nullptr, nullptr, 0, false, two, src2);
     const int ntile = int(run.plan.tiles.size());
     timing_begin<T>(c);
@@ -1702,7 +1789,7 @@ int fwi_forward(Ctx* c, int nshot, const eio_shot* shots, int flags, int segment
         // FP32: the two-step kernel (with the decimation in the last segment); pairs of fine
         // steps stay inside one coarse interval because the fine sub-sampling is even, and
         // segments start at even fine steps
-        const bool two = sizeof(T) == 4 && c->time_block == 2 && sub % 2 == 0;
+        const bool two = sizeof(T) == 4 && c->time_block >= 2 && sub % 2 == 0;
         Run<T> run = prepare<T>(c, nshot, shots, Op::FWI_FWD, flags, nullptr, static_cast<T*>(f.ring), f.wcap, two);
         const int ntile = int(run.plan.tiles.size());
         T* state = run.sd[0].A;  // the batch state is one contiguous buffer in plan order
@@ -1769,7 +1856,7 @@ void fwi_gradient(Ctx* c, int handle, int nshot, const eio_shot* shots, void* co
     const bool two = sizeof(T) == 4 && c->adj_time_block == 2 && f.sub % 2 == 0;
     // FP32: segments are recomputed two fine steps per launch (four levels per shot), the
     // kernel of the forward, so that a recomputed ring equals the forward's bit for bit
-    const bool rtwo = sizeof(T) == 4 && c->time_block == 2 && f.sub % 2 == 0;
+    const bool rtwo = sizeof(T) == 4 && c->time_block >= 2 && f.sub % 2 == 0;
     Run<T> run = prepare<T>(c, nshot, shots, Op::FWI_GRAD, flags, nullptr, static_cast<T*>(f.ring), f.wcap,
                             false, two, rtwo);
     const int ntile = int(run.plan.tiles.size());
@@ -1957,12 +2044,17 @@ int eio_ctx_num_variants(void) { return N_F32_VARIANTS; }
 
 int eio_ctx_set_time_block(eio_ctx* ctx, int steps, int variant) {
     return guarded([&]() {
-        if (steps != 1 && steps != 2) fail("time block must be 1 or 2 fine steps");
-        if (variant != EIO_VARIANT_AUTO && (variant < 0 || variant >= N_F32_VARIANTS2))
-            fail("two-step variant out of range");
+        if (steps < 1 || steps > 3) fail("time block must be 1, 2 or 3 fine steps");
+        const int nv = steps == 3 ? N_F32_VARIANTS3 : N_F32_VARIANTS2;
+        if (variant != EIO_VARIANT_AUTO && (variant < 0 || variant >= nv)) fail("time-block variant out of range");
         Ctx* c = as_ctx(ctx);
         c->time_block = steps;
-        c->variant2 = variant;
+        if (steps == 3) {
+            c->variant3 = variant;
+            c->variant2 = EIO_VARIANT_AUTO;   // the other forward passes take two steps per launch
+        } else {
+            c->variant2 = variant;
+        }
     });
 }
 
@@ -1976,6 +2068,17 @@ int eio_ctx_time_block_info(int variant, int* R2, int* WX, int* occ) {
 }
 
 int eio_ctx_num_time_block_variants(void) { return N_F32_VARIANTS2; }
+
+int eio_ctx_num_time_block3_variants(void) { return N_F32_VARIANTS3; }
+
+int eio_ctx_time_block3_info(int variant, int* R3, int* WX, int* occ) {
+    return guarded([&]() {
+        if (variant < 0 || variant >= N_F32_VARIANTS3) fail("three-step variant out of range");
+        if (R3) *R3 = F32_VARIANTS3[variant].R3;
+        if (WX) *WX = F32_VARIANTS3[variant].WX;
+        if (occ) *occ = F32_VARIANTS3[variant].OCC;
+    });
+}
 
 int eio_ctx_time_block_layout(int variant) {
     return variant >= 0 && variant < N_F32_VARIANTS2 ? F32_VARIANTS2[variant].LAYOUT : -1;

@@ -1002,3 +1002,60 @@ def test_persistent_launch_abort_drains_and_reports():
     out = _gpu_nl_fwd(ctx, mid, spos, rpos, src, g.nts, np.float32)
     assert all(np.array_equal(a, b) for a, b in zip(out, ref))
     ctx.close()
+
+
+@pytest.mark.parametrize("v3", [0, 1])
+@pytest.mark.parametrize("nts,sub_fd", [(60, 2), (62, 2), (60, 3)])
+def test_three_step_nl_forward_matches_oracle(v3, nts, sub_fd):
+    """Three fine steps per launch (u1 on an 8-cell and u2 on a 4-cell halo, recomputed per
+    tile) equal the oracle for both tile widths, with sources and receivers scattered over the
+    grid (many on tile edges: sources injected into every tile's copies of u1 and u2), and step
+    counts 118, 122 and 177 (the last launch takes 1, 2 and 3 steps)."""
+    w = W.small(nz=150, nx=130, nts=nts, sub=1, seed=62)
+    g = O.grid(w["nz"], w["nx"], w["nts"], sub_fd, w["dz"], w["dx"], w["dts"], W.min_pad(w),
+               w["alphaCos"], w["oz"], w["ox"])
+    med = O.media(g, w["elastic"], np.float64)
+    rng = np.random.default_rng(63 + v3)
+    spos = scattered_positions(rng, g.nz, g.nx, 40)
+    rpos = scattered_positions(rng, g.nz, g.nx, 60)
+    src = random_traces(rng, spos, g.nts, np.float64)
+    ref = O.nl_fwd_reg(g, med, spos, src, rpos, np.float64)
+    ctx = _capi.Context(0, np.float32)
+    ctx.set_time_block(3, v3)
+    mid = _model(ctx, w, g, med)
+    out = _gpu_nl_fwd(ctx, mid, spos, rpos, src, g.nts, np.float32)
+    nlaunch = ctx.last_timing()[1]
+    assert nlaunch == -(-((g.nts - 1) * g.sub) // 3), nlaunch   # ceil(fine steps / 3) launches
+    for k in range(5):
+        assert np.linalg.norm(ref[k]) > 0
+        assert rel_l2(out[k], ref[k]) < 1e-5, (v3, k, rel_l2(out[k], ref[k]))
+
+
+def test_three_step_uneven_batch_matches_oracle():
+    """An uneven batch (grids, trace lengths and sub-samplings differ, so shots finish in
+    different launches, each with its own 1-, 2- or 3-step last launch) through the three-step
+    forward equals the oracle per shot."""
+    rng = np.random.default_rng(64)
+    cases = []
+    for i, (nz, nx, nts, sub) in enumerate([(90, 130, 70, 1), (150, 90, 50, 2), (60, 200, 61, 1)]):
+        w = W.small(nz=nz, nx=nx, nts=nts, sub=sub, seed=65 + i)
+        g = oracle_grid(w)
+        med = O.media(g, w["elastic"], np.float64)
+        cases.append((w, g, med, scattered_positions(rng, g.nz, g.nx, 12), scattered_positions(rng, g.nz, g.nx, 30)))
+    ctx = _capi.Context(0, np.float32)
+    ctx.set_time_block(3, _capi.VARIANT_AUTO)
+    shots, recs, refs, keep = [], [], [], []
+    for (w, g, med, spos, rpos) in cases:
+        mid = _model(ctx, w, g, med)
+        src = random_traces(rng, spos, g.nts, np.float64)
+        refs.append(O.nl_fwd_reg(g, med, spos, src, rpos, np.float64))
+        rec = [np.zeros((rpos[GRID_OF_COMP[k]].size, g.nts), np.float32) for k in range(5)]
+        srcc = [np.ascontiguousarray(x, np.float32) for x in src]
+        shot, kp = ctx.make_shot(mid, spos, rpos, srcc, rec)
+        shots.append(shot)
+        recs.append(rec)
+        keep.append((kp, srcc))
+    ctx.nl_forward(shots)
+    for rec, ref in zip(recs, refs):
+        for k in range(5):
+            assert rel_l2(rec[k], ref[k]) < 1e-5, (k, rel_l2(rec[k], ref[k]))
