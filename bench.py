@@ -138,18 +138,19 @@ def _oracle_grid(O, w, nts=None):
 def c0_leg(args):
     """Unbatched leg (workload C0: one 280x472 shot, nts 1000, sub_fd 2): the two-step NL
     forward with one launch per pair of fine steps, the same loop replayed from the context's
-    cached hipGraph, and the whole loop in one persistent launch; device time per call (HIP
-    events) after warm-up calls, best of `args.c0_reps`; plus the FP32 oracle on the host cores
-    (the whole shot)."""
+    cached hipGraph, the whole loop in one persistent launch, and the three-step forward (one
+    launch per three fine steps); device time per call (HIP events) after warm-up calls, best
+    of `args.c0_reps`; plus the FP32 oracle on the host cores (the whole shot)."""
     w = W.gaussian_c0()
     cells = W.cell_updates(w)
     ctx = _capi.Context(0, np.float32)
     ctx.set_time_block(2, _capi.VARIANT_AUTO)
     shots, keep, _, _ = prepare_shots(ctx, [w])
     out = {"workload": w["name"], "cell_updates": cells, "dtype": "f32"}
-    for mode in ("launched", "graph", "persistent"):
+    for mode in ("launched", "graph", "persistent", "three_step"):
         ctx.set_graph(mode == "graph")
         ctx.set_persistent(1 if mode == "persistent" else 0)
+        ctx.set_time_block(3 if mode == "three_step" else 2, _capi.VARIANT_AUTO)
         for _ in range(2):
             ctx.nl_forward(shots, device_ptrs=True)
         ms = []
