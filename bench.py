@@ -474,7 +474,7 @@ def main():
                          "adjoint variant, as steps:variant")
     ap.add_argument("--fwi-segment", type=int, default=0,
                     help="checkpoint interval of the checkpointed FWI gradient (0: ceil(sqrt(nts-1)); -1: skip it)")
-    ap.add_argument("--c0", type=int, default=1, help="run the unbatched C0 leg (launched, graph, persistent)")
+    ap.add_argument("--c0", type=int, default=1, help="run the unbatched C0 leg (launched, graph, persistent, three-step)")
     ap.add_argument("--c0-reps", type=int, default=5)
     ap.add_argument("--fwi-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--abc", type=int, default=1, help="run the learned-boundary leg (needs its weights)")

@@ -112,7 +112,7 @@ constexpr int N_F32_VARIANTS2 = int(sizeof(F32_VARIANTS2) / sizeof(F32_VARIANTS2
 struct Variant3 {
     int R3, WX, OCC;
 };
-constexpr Variant3 F32_VARIANTS3[] = {{3, 8, 1}, {4, 8, 1}};
+constexpr Variant3 F32_VARIANTS3[] = {{3, 8, 1}, {4, 8, 1}, {1, 8, 4}};
 constexpr int N_F32_VARIANTS3 = int(sizeof(F32_VARIANTS3) / sizeof(F32_VARIANTS3[0]));
 // Two-step adjoint (Born receiver pass, FWI gradient) variants: R2, WX, OCC as above.
 struct Variant2A {
@@ -620,11 +620,12 @@ int variant2a_for(Ctx* c, int nshot, const eio_shot* shots) {
     return wide_tiles(c, nshot, shots) < AUTO_SMALL_TILES ? 10 : 7;
 }
 
-// The three-step forward variant of a call: EIO_VARIANT_AUTO takes 48 x 24 tiles (variant 0)
-// for a batch of fewer than AUTO_SMALL_TILES tiles of 56 x 32, else 48 x 32 (variant 1).
+// The three-step forward variant of a call: EIO_VARIANT_AUTO takes 48 x 8 tiles (variant 2,
+// two workgroups per CU) for a batch of fewer than AUTO_SMALL_TILES tiles of 56 x 32, else
+// 48 x 32 (variant 1).
 int variant3_for(Ctx* c, int nshot, const eio_shot* shots) {
     if (c->variant3 != EIO_VARIANT_AUTO) return c->variant3;
-    return wide_tiles(c, nshot, shots) < AUTO_SMALL_TILES ? 0 : 1;
+    return wide_tiles(c, nshot, shots) < AUTO_SMALL_TILES ? 2 : 1;
 }
 
 // Builds the plan, the per-shot descriptors and the device tables of one call, uploads
@@ -904,6 +905,10 @@ void launch_fwd3(Run<float>& run, int ntile, int k, int par) {
     switch (run.v3) {
         case 0:
             klaunch(run.c, eio::k_step3_fwd<float, 3, 8, 1>, dim3(ntile), dim3(512), run.d_shots, run.d_tiles,
+                    run.d_ents, ntile, k, par);
+            break;
+        case 2:
+            klaunch(run.c, eio::k_step3_fwd<float, 1, 8, 4>, dim3(ntile), dim3(512), run.d_shots, run.d_tiles,
                     run.d_ents, ntile, k, par);
             break;
         default:

@@ -117,7 +117,7 @@ int eio_ctx_set_time_block(eio_ctx* ctx, int steps, int variant);
 /* steps = 3: the FP32 NL forward (without saved wavefield, free surface, learned boundary or
  * snapshots) takes THREE fine steps per launch (k_step3_fwd: 100 B per three cell-updates,
  * owned tile 48 x WX*R3, u_cur staged with a 12-cell halo), `variant` indexing the three-step
- * table (EIO_VARIANT_AUTO: 48 x 24 tiles below 1024 tiles of 56 x 32, else 48 x 32); the
+ * table (EIO_VARIANT_AUTO: 48 x 8 tiles below 1024 tiles of 56 x 32, else 48 x 32); the
  * other forward passes then take two steps per launch with the automatic two-step variant.
  * No counterpart in the reference. */
 int eio_ctx_num_time_block3_variants(void);

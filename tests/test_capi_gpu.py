@@ -1004,11 +1004,11 @@ def test_persistent_launch_abort_drains_and_reports():
     ctx.close()
 
 
-@pytest.mark.parametrize("v3", [0, 1])
+@pytest.mark.parametrize("v3", [0, 1, 2])
 @pytest.mark.parametrize("nts,sub_fd", [(60, 2), (62, 2), (60, 3)])
 def test_three_step_nl_forward_matches_oracle(v3, nts, sub_fd):
     """Three fine steps per launch (u1 on an 8-cell and u2 on a 4-cell halo, recomputed per
-    tile) equal the oracle for both tile widths, with sources and receivers scattered over the
+    tile) equal the oracle for the three tile widths (24, 32 and 8 columns), with sources and receivers scattered over the
     grid (many on tile edges: sources injected into every tile's copies of u1 and u2), and step
     counts 118, 122 and 177 (the last launch takes 1, 2 and 3 steps)."""
     w = W.small(nz=150, nx=130, nts=nts, sub=1, seed=62)
