@@ -135,6 +135,12 @@ __device__ __forceinline__ void bst(rsrc_t r, double v, unsigned voff, unsigned 
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u64, v), r, int(voff), int(soff), 0);
 }
 
+// 16 bytes from a buffer (a 4-byte-aligned offset is enough): the LLVM intrinsic itself,
+// since this toolchain's __builtin_amdgcn_raw_buffer_load_b128 is lowered to a 32-bit load.
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+__device__ f32x4_t raw_buffer_load_f32x4(rsrc_t r, int voff, int soff, int aux)
+    __asm("llvm.amdgcn.raw.ptr.buffer.load.v4f32");
+
 // Bijective block -> tile map that gives each of the 8 XCDs a contiguous range of tiles
 // (workgroups are dispatched round-robin over XCDs), so neighbouring tiles share an L2.
 __device__ __forceinline__ int xcd_remap(int b, int n) {

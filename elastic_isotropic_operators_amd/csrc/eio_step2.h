@@ -37,12 +37,7 @@ constexpr int TZ2 = 56;       // owned z rows of a two-step tile
 constexpr int Z1 = 64;        // rows of u1 (= lanes)
 constexpr int ZC = 72;        // rows of u_cur staged in LDS
 
-// Four consecutive floats from a buffer (a 4-byte-aligned offset is enough): the LLVM
-// intrinsic itself, since this toolchain's __builtin_amdgcn_raw_buffer_load_b128 is lowered
-// to a 32-bit load.
-typedef float f32x4_t __attribute__((ext_vector_type(4)));
-__device__ f32x4_t raw_buffer_load_f32x4(rsrc_t r, int voff, int soff, int aux)
-    __asm("llvm.amdgcn.raw.ptr.buffer.load.v4f32");
+// Four consecutive floats from a buffer (raw_buffer_load_f32x4, eio_kernels.h).
 __device__ __forceinline__ void bld4(rsrc_t r, unsigned voff, unsigned soff, float v[4]) {
     const f32x4_t q = raw_buffer_load_f32x4(r, int(voff), int(soff), 0);
     v[0] = q.x;

@@ -126,6 +126,7 @@ def test_c1_born_forward_and_adjoint_vs_fp64_oracle(tmp_path):
     op, model, data, elastic, sig = _born(PF, args, PF.BornOpInitFloat)
     model.getNdArray()[...] = dm
     op.forward(False, model, data)
+    assert op.pyOp.lastPropagationMs() > 0   # device time of the call, as bench.py reads it
     g = oracle_grid(w)
     ref = np.stack([O.born_fwd_shot(g, w["elastic"], geo["sz"][e], geo["sx"][e], w["model"], geo["rz"],
                                     geo["rx"], dm) for e in range(8)])
