@@ -359,13 +359,13 @@ def born_child(args):
             t0 = time.perf_counter()
             op.forward(False, model, data)
             tf = time.perf_counter() - t0
-            df = op.lastPropagationMs() / 1e3
+            df = op.pyOp.lastPropagationMs() / 1e3   # device time of the compiled operator's last call
             img = model.clone()
             op.adjoint(False, img, data)
             t0 = time.perf_counter()
             op.adjoint(False, img, data)
             ta = time.perf_counter() - t0
-            da = op.lastPropagationMs() / 1e3
+            da = op.pyOp.lastPropagationMs() / 1e3
             res[name] = (tf, ta, data.getNdArray().astype(np.float64), img.getNdArray().astype(np.float64), df, da)
             log("born %s: forward %.3f s (device %.3f s), adjoint %.3f s (device %.3f s)" % (name, tf, df, ta, da))
     if not args.no_cpu:
@@ -430,7 +430,7 @@ def abc_child(args):
     net, scale, strip, on = A.load(ABC_WEIGHTS)
     vvr, nts = W.learned_abc_c4(scale=args.abc_scale)
     res = A.reflection_test(vvr, net, scale, strip=strip, mid=60, wide=120, nts=nts, rec_every=4, on=on,
-                            time_block=2, log=log)
+                            time_block=(1, 2), log=log)
     out = {"workload": "learned_abc_c4_%dx%d_nts%d" % (vvr.shape[2], vvr.shape[1], nts), "strip": strip,
            "runs": res, "weights": os.path.relpath(ABC_WEIGHTS, ROOT)}
     print("ABC_JSON " + json.dumps(out), flush=True)

@@ -217,8 +217,9 @@ class Context:
         return out
 
     def set_time_block(self, steps, variant=0):
-        """Fine steps per launch of the FP32 forward passes (1 or 2) and the two-step variant;
-        VARIANT_AUTO chooses the tile width per call from the batch's size."""
+        """Fine steps per launch of the FP32 NL forward (1, 2 or 3) and the variant of the
+        two- or three-step kernel (ignored for 1: the single-step kernel keeps the context's
+        own variant); VARIANT_AUTO chooses the tile width per call from the batch's size."""
         check(lib().eio_ctx_set_time_block(self.h, int(steps), int(variant)))
 
     def set_adjoint_time_block(self, steps, variant=0):
