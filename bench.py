@@ -41,7 +41,9 @@ from elastic_isotropic_operators_amd import _capi, autotune, geometry as G, work
 BYTES_PER_CELL = {"nl_fwd": 80}
 HBM_PEAK_GBS = 8000.0
 METRIC = "GCell-updates/s + FWI-gradient s/shot at \u22641e-4 rel-L2 vs FP64, 1 MI355X"
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_step2_fwd_layered_batch_c2_traffic.json")
+# measured bytes per cell-update of the dominant kernel per (time block, variant, grouping)
+TRAFFIC_FILES = [os.path.join(ROOT, "profiles", f) for f in
+                 ("k_step2_fwd_layered_batch_c2_traffic.json", "k_step_fwd_layered_batch_c2_grouped_traffic.json")]
 
 
 def log(msg):
@@ -457,6 +459,8 @@ def main():
     ap.add_argument("--graph", type=int, default=0)
     ap.add_argument("--variant", type=int, default=-1, help="FP32 step-kernel variant (-1: default)")
     ap.add_argument("--time-block", type=int, default=1, help="fine steps per launch (1, 2 or 3)")
+    ap.add_argument("--group-bytes", type=int, default=-1,
+                    help="working-set budget of the batch's shot groups (-1: automatic, 192 MiB; 0: one group)")
     ap.add_argument("--variant2", type=int, default=0, help="two-step kernel variant")
     ap.add_argument("--variant3", type=int, default=1, help="three-step kernel variant")
     ap.add_argument("--autotune", type=int, default=1,
@@ -517,11 +521,12 @@ def main():
     if args.variant >= 0:
         ctx.set_variant(args.variant)
     ctx.set_time_block(args.time_block, args.variant3 if args.time_block == 3 else args.variant2)
+    ctx.set_shot_group_bytes(args.group_bytes)
     shots, keep, cells, grids = prepare_shots(ctx, shots_w)
     log("uploaded %d shots, %.3e cell-updates per step" % (len(shots), cells))
     tuned = None
     if args.autotune:
-        tuned = autotune.tune_nl_forward(ctx, shots, grids, log=log)
+        tuned = autotune.tune_nl_forward(ctx, shots, grids, log=log, group_bytes=args.group_bytes)
         args.time_block, v = tuned["choice"]
         if args.time_block == 3:
             args.variant3 = v
@@ -568,6 +573,9 @@ def main():
     else:
         R, WX, OCC = _capi.Context.variants()[args.variant if args.variant >= 0 else 1]
         kname = "k_step_fwd<float,%d,%d,%d,M_NL>" % (R, WX, OCC)
+    groups = ctx.last_shot_groups()
+    if groups > 1:
+        kname += " in %d shot groups" % groups
     res = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -584,6 +592,7 @@ def main():
         "config": {"workload": wl_name, "shots": len(shots), "cell_updates_per_step": cells,
                    "scale": args.scale, "graph": args.graph, "variant": args.variant,
                    "time_block": args.time_block, "variant2": args.variant2, "variant3": args.variant3,
+                   "group_bytes": args.group_bytes, "shot_groups": groups,
                    "autotune": None if tuned is None else {"choice": list(tuned["choice"]),
                                                            "ms_per_fine_step": tuned["table"],
                                                            "cached": tuned["cached"]}},
@@ -606,15 +615,21 @@ def main():
         cms0 = ctx.copy_probe(nb, 10, kind=0)
         res["roofline"]["copy_probe_gridstride_GBs"] = round(2 * nb / (cms0 / 1e3) / 1e9, 1)
         log("copy probe (grid-stride float4): %.3f ms -> %.0f GB/s" % (cms0, 2 * nb / cms0 / 1e6))
-    if os.path.exists(TRAFFIC_FILE) and args.workload == "c2":
-        # measured HBM bytes per cell-update of this kernel (rocprofv3 FETCH_SIZE/WRITE_SIZE,
+    for tf in TRAFFIC_FILES:
+        # measured bytes per cell-update of this kernel (rocprofv3 FETCH_SIZE/WRITE_SIZE,
         # copy-kernel calibrated) times the cell-updates of this run's average launch
-        tr = json.load(open(TRAFFIC_FILE))
-        if tr["time_block"] == args.time_block and tr["variant2"] == args.variant2:
+        if not os.path.exists(tf) or args.workload != "c2":
+            continue
+        tr = json.load(open(tf))
+        v = args.variant2 if args.time_block == 2 else args.variant if args.time_block == 1 else args.variant3
+        if (tr["time_block"] == args.time_block and tr.get("variant", tr.get("variant2")) == v
+                and (tr.get("shot_groups", 1) > 1) == (groups > 1)):
             upd_per_launch = cells / float(nlaunch[-1])
             res["roofline"]["traffic"] = round(tr["bytes_per_cell_update"] * upd_per_launch)
             res["roofline"]["traffic_bytes_per_cell_update"] = round(tr["bytes_per_cell_update"], 2)
-            res["roofline"]["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
+            res["roofline"]["traffic_source"] = os.path.relpath(tf, ROOT)
+            if "traffic_level" in tr:
+                res["roofline"]["traffic_level"] = tr["traffic_level"]
     # the secondary legs run in child processes; one that fails is reported in its own field
     # and does not cost the headline line
     legs = []

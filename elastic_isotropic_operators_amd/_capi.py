@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libeio_hip.so")
 
 EIO_DEVICE_PTRS = 1
 VARIANT_AUTO = -1  # EIO_VARIANT_AUTO: two-step tile width chosen per call
+GROUP_AUTO = -1    # EIO_GROUP_AUTO: shot groups of a 192 MiB working set
 GRID_OF_COMP = (1, 2, 0, 0, 3)
 
 
@@ -99,6 +100,8 @@ def lib():
                                                  ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
             ("eio_ctx_set_adjoint_time_block", [vp, ctypes.c_int, ctypes.c_int]),
             ("eio_ctx_set_step_limit", [vp, ctypes.c_int]),
+            ("eio_ctx_set_shot_group_bytes", [vp, ctypes.c_longlong]),
+            ("eio_ctx_last_shot_groups", [vp, ctypes.POINTER(ctypes.c_int)]),
             ("eio_ctx_set_persistent", [vp, ctypes.c_int, ctypes.c_int]),
             ("eio_nl_forward_wavefield", [vp, ctypes.c_int, ctypes.POINTER(Shot), ctypes.c_int, ctypes.c_int, vp]),
             ("eio_nl_adjoint_wavefield", [vp, ctypes.c_int, ctypes.POINTER(Shot), ctypes.c_int, ctypes.c_int, vp]),
@@ -121,7 +124,8 @@ EXPORTED = ("eio_last_error", "eio_ctx_create", "eio_ctx_destroy", "eio_ctx_set_
             "eio_ctx_time_block_layout", "eio_ctx_set_step_limit", "eio_ctx_set_persistent", "eio_nl_forward_wavefield",
             "eio_nl_adjoint_wavefield", "eio_model_set_abc", "eio_nl_forward_snapshots",
             "eio_nl_step_states", "eio_ctx_release_workspace", "eio_ctx_num_adjoint_time_block_variants",
-            "eio_ctx_adjoint_time_block_info", "eio_ctx_set_adjoint_time_block")
+            "eio_ctx_adjoint_time_block_info", "eio_ctx_set_adjoint_time_block",
+            "eio_ctx_set_shot_group_bytes", "eio_ctx_last_shot_groups")
 
 
 def check(rc):
@@ -245,6 +249,17 @@ class Context:
     def set_step_limit(self, max_steps):
         """Stop NL loops after max_steps fine steps (0: no limit); for tuning runs."""
         check(lib().eio_ctx_set_step_limit(self.h, int(max_steps)))
+
+    def set_shot_group_bytes(self, nbytes):
+        """Working-set budget of the FP32 NL forward's shot groups: GROUP_AUTO (192 MiB, the
+        default), 0 (one group) or a byte count; each group runs all its steps before the next."""
+        check(lib().eio_ctx_set_shot_group_bytes(self.h, int(nbytes)))
+
+    def last_shot_groups(self):
+        """Number of shot groups of the last NL forward call."""
+        n = ctypes.c_int()
+        check(lib().eio_ctx_last_shot_groups(self.h, ctypes.byref(n)))
+        return n.value
 
     @staticmethod
     def time_block_variants():

@@ -209,6 +209,8 @@ struct Ctx {
     int adj_time_block = 1;  // reverse fine steps per launch of the FP32 Born adjoint (1 or 2)
     int variant2a = 0;       // two-step adjoint kernel variant, or EIO_VARIANT_AUTO
     int step_limit = 0;  // > 0: NL loops stop after this many fine steps (tuning runs)
+    long long group_bytes = EIO_GROUP_AUTO;  // shot-group working-set budget of the NL forward
+    int last_groups = 0;                     // shot groups of the last NL forward call
     int persistent = 0;  // two-step NL forward in one persistent launch: 0 off, 1 on, 2 when it fits
     int spin_limit = 1 << 20;  // polls per neighbour wait of the persistent launch before it aborts
     std::map<int, FwiStore> fwi;
@@ -818,6 +820,46 @@ inline int active_tiles(const Plan& p, int k) {
     return n;
 }
 
+// Shots [s0, s1) of a plan (in plan order) that run their steps together.
+struct ShotGroup {
+    int s0, s1;
+};
+constexpr long long GROUP_AUTO_BYTES = 192ll << 20;   // of the 256 MB Infinity Cache
+
+// Consecutive groups of the plan's shots whose working set (`per_cell` bytes per cell of the
+// grid) fits `budget` bytes (EIO_GROUP_AUTO: GROUP_AUTO_BYTES; 0: one group); a shot larger
+// than the budget is a group of its own.
+template <typename T>
+std::vector<ShotGroup> shot_groups(const Run<T>& run, long long budget, long long per_cell) {
+    const int n = int(run.plan.nsteps.size());
+    if (budget == EIO_GROUP_AUTO) budget = GROUP_AUTO_BYTES;
+    if (budget <= 0 || n == 0) return {{0, n}};
+    std::vector<ShotGroup> g;
+    int s0 = 0;
+    long long used = 0;
+    for (int i = 0; i < n; ++i) {
+        const long long b = (long long)run.sd[i].nz * run.sd[i].nx * per_cell;
+        if (i > s0 && used + b > budget) {
+            g.push_back({s0, i});
+            s0 = i;
+            used = 0;
+        }
+        used += b;
+    }
+    g.push_back({s0, n});
+    return g;
+}
+
+// tiles of group g active at step k, counted from the group's first tile (the group's shots
+// are sorted by decreasing step count, so the active ones come first)
+inline int group_tiles(const Plan& p, const ShotGroup& g, int k) {
+    const int t0 = g.s0 ? p.tile_end[g.s0 - 1] : 0;
+    int n = 0;
+    for (int i = g.s0; i < g.s1; ++i)
+        if (k < p.nsteps[i]) n = p.tile_end[i] - t0;
+    return n;
+}
+
 #define EIO_F32_VARIANT_SWITCH(V, CALL)                                                  \
     switch (V) {                                                                          \
         case 0: CALL(4, 4, 2); break;                                                     \
@@ -832,10 +874,11 @@ inline int active_tiles(const Plan& p, int k) {
 // the slot shared with the next segment, or -1); `dsh` replaces the run's shot table.
 template <typename T, int MODE>
 void launch_fwd(Run<T>& run, int ntile, int k, int sspar, int dlo = 0, int dhi = -1,
-                const eio::ShotDev<T>* dsh = nullptr) {
+                const eio::ShotDev<T>* dsh = nullptr, const eio::TileDev* tiles = nullptr) {
     const eio::ShotDev<T>* tab = dsh ? dsh : run.d_shots;
+    const eio::TileDev* tt = tiles ? tiles : run.d_tiles;
 #define EIO_L(R_, WX_, OCC_)                                                                  \
-    klaunch(run.c, eio::k_step_fwd<T, R_, WX_, OCC_, MODE>, dim3(ntile), dim3(64 * WX_), tab, run.d_tiles, \
+    klaunch(run.c, eio::k_step_fwd<T, R_, WX_, OCC_, MODE>, dim3(ntile), dim3(64 * WX_), tab, tt, \
             run.d_ents, ntile, k, sspar, dlo, dhi)
     if constexpr (sizeof(T) == 8) {
         EIO_L(2, 4, 1);
@@ -899,20 +942,22 @@ void launch_fwd2(Run<float>& run, int ntile, int k, int par, int dlo = 0, int dh
 #undef EIO_L2
 }
 
-// Fine steps k, k+1, k+2 of the FP32 NL forward (levels A,B -> C,D when par = 0).
-void launch_fwd3(Run<float>& run, int ntile, int k, int par) {
+// Fine steps k, k+1, k+2 of the FP32 NL forward (levels A,B -> C,D when par = 0); `tiles`
+// replaces the run's tile table (a shot group's first tile).
+void launch_fwd3(Run<float>& run, int ntile, int k, int par, const eio::TileDev* tiles = nullptr) {
     if (ntile <= 0) return;
+    const eio::TileDev* tt = tiles ? tiles : run.d_tiles;
     switch (run.v3) {
         case 0:
-            klaunch(run.c, eio::k_step3_fwd<float, 3, 8, 1>, dim3(ntile), dim3(512), run.d_shots, run.d_tiles,
+            klaunch(run.c, eio::k_step3_fwd<float, 3, 8, 1>, dim3(ntile), dim3(512), run.d_shots, tt,
                     run.d_ents, ntile, k, par);
             break;
         case 2:
-            klaunch(run.c, eio::k_step3_fwd<float, 1, 8, 4>, dim3(ntile), dim3(512), run.d_shots, run.d_tiles,
+            klaunch(run.c, eio::k_step3_fwd<float, 1, 8, 4>, dim3(ntile), dim3(512), run.d_shots, tt,
                     run.d_ents, ntile, k, par);
             break;
         default:
-            klaunch(run.c, eio::k_step3_fwd<float, 4, 8, 1>, dim3(ntile), dim3(512), run.d_shots, run.d_tiles,
+            klaunch(run.c, eio::k_step3_fwd<float, 4, 8, 1>, dim3(ntile), dim3(512), run.d_shots, tt,
                     run.d_ents, ntile, k, par);
             break;
     }
@@ -1367,20 +1412,33 @@ void nl_forward(Ctx* c, int nshot, const eio_shot* shots, int flags, int wshot =
             return;
         }
     }
+    // plain runs: shot groups, each through all its steps before the next
+    const std::vector<ShotGroup> groups =
+        plain ? shot_groups(run, c->group_bytes, (two || three ? 25 : 15) * (long long)sizeof(T))
+              : std::vector<ShotGroup>{{0, nshot}};
+    c->last_groups = int(groups.size());
     timing_begin<T>(c);
     EIO_CHECK_HIP(hipEventRecord(c->ev[2], c->stream));
     run_loop(c, [&]() {
         if constexpr (sizeof(T) == 4) {
-            if (two) {
-                // pairs (k, k+1); a shot whose last pair has one step does only step k
-                for (int k = 0; k < kmax; k += 2, ++nlaunch)
-                    launch_fwd2(run, active_tiles(run.plan, k), k, (k >> 1) & 1);
-                return;
-            }
-            if (three) {
-                // triples (k, k+1, k+2); a shot's last launch may take one or two steps
-                for (int k = 0; k < kmax; k += 3, ++nlaunch)
-                    launch_fwd3(run, active_tiles(run.plan, k), k, (k / 3) & 1);
+            if (plain) {
+                for (const ShotGroup& g : groups) {
+                    const eio::TileDev* tg = run.d_tiles + (g.s0 ? run.plan.tile_end[g.s0 - 1] : 0);
+                    int kg = run.plan.nsteps[g.s0];
+                    if (c->step_limit > 0) kg = std::min(kg, c->step_limit);
+                    if (two) {
+                        // pairs (k, k+1); a shot whose last pair has one step does only step k
+                        for (int k = 0; k < kg; k += 2, ++nlaunch)
+                            launch_fwd2(run, group_tiles(run.plan, g, k), k, (k >> 1) & 1, 0, -1, tg);
+                    } else if (three) {
+                        // triples (k, k+1, k+2); a shot's last launch may take one or two steps
+                        for (int k = 0; k < kg; k += 3, ++nlaunch)
+                            launch_fwd3(run, group_tiles(run.plan, g, k), k, (k / 3) & 1, tg);
+                    } else {
+                        for (int k = 0; k < kg; ++k, ++nlaunch)
+                            launch_fwd<T, eio::M_NL>(run, group_tiles(run.plan, g, k), k, 0, 0, -1, nullptr, tg);
+                    }
+                }
                 return;
             }
         }
@@ -2060,6 +2118,20 @@ int eio_ctx_set_time_block(eio_ctx* ctx, int steps, int variant) {
         } else {
             c->variant2 = variant;
         }
+    });
+}
+
+int eio_ctx_set_shot_group_bytes(eio_ctx* ctx, long long bytes) {
+    return guarded([&]() {
+        if (bytes < 0 && bytes != EIO_GROUP_AUTO) fail("shot-group budget must be >= 0 or EIO_GROUP_AUTO");
+        as_ctx(ctx)->group_bytes = bytes;
+    });
+}
+
+int eio_ctx_last_shot_groups(eio_ctx* ctx, int* groups) {
+    return guarded([&]() {
+        if (!groups) fail("groups is NULL");
+        *groups = as_ctx(ctx)->last_groups;
     });
 }
 

@@ -152,6 +152,18 @@ int eio_ctx_set_persistent(eio_ctx* ctx, int mode, int spin_limit);
 /* Tuning runs: NL forward and adjoint loops stop after `max_steps` fine steps (0: no limit).
  * Outputs of a limited call are partial.  No counterpart in the reference. */
 int eio_ctx_set_step_limit(eio_ctx* ctx, int max_steps);
+/* Shot groups of the FP32 NL forward: the ordered shots of a call are split into consecutive
+ * groups whose per-step working set (the state levels a step touches and the media: 15 values
+ * per cell with one fine step per launch, 25 with two or three) fits `bytes`, and each group
+ * runs all its steps before the next one starts, so its working set stays in the MI355X's
+ * 256 MB Infinity Cache between steps.  EIO_GROUP_AUTO (the default): 192 MiB; 0: one group
+ * (every shot advances each launch).  Results are those of the ungrouped call; the persistent
+ * launch and runs with snapshots, a saved wavefield, a free surface or a learned boundary take
+ * one group.  No counterpart in the reference (one shot per call, NLF:561-647). */
+#define EIO_GROUP_AUTO (-1LL)
+int eio_ctx_set_shot_group_bytes(eio_ctx* ctx, long long bytes);
+/* Number of shot groups of the context's last NL forward call. */
+int eio_ctx_last_shot_groups(eio_ctx* ctx, int* groups);
 
 /* The grid checks of eio_model_add, on the host only (no device, no context): sizes, sampling,
  * sponge width (the reference's messages, fdParamElastic.cpp:29-32,

@@ -1059,3 +1059,49 @@ def test_three_step_uneven_batch_matches_oracle():
     for rec, ref in zip(recs, refs):
         for k in range(5):
             assert rel_l2(rec[k], ref[k]) < 1e-5, (k, rel_l2(rec[k], ref[k]))
+
+
+@pytest.mark.parametrize("tb", [1, 2, 3])
+def test_shot_groups_equal_one_group_and_the_oracle(tb):
+    """Shot groups (each group runs all its steps before the next starts) give the receiver
+    traces of the one-group call bit for bit, and the oracle's, with one, two and three fine
+    steps per launch, on an uneven batch that a small working-set budget splits into several
+    groups (shots of different lengths inside one group finish in different launches)."""
+    rng = np.random.default_rng(71)
+    sizes = [(90, 130, 70, 1), (150, 90, 50, 2), (60, 200, 61, 1), (120, 100, 40, 2), (70, 70, 66, 1)]
+    cases = []
+    for i, (nz, nx, nts, sub) in enumerate(sizes):
+        w = W.small(nz=nz, nx=nx, nts=nts, sub=sub, seed=72 + i)
+        g = oracle_grid(w)
+        med = O.media(g, w["elastic"], np.float64)
+        cases.append((w, g, med, scattered_positions(rng, g.nz, g.nx, 8), scattered_positions(rng, g.nz, g.nx, 20)))
+    ctx = _capi.Context(0, np.float32)
+    ctx.set_time_block(tb, _capi.VARIANT_AUTO)
+    shots, recs, refs, keep = [], [], [], []
+    for (w, g, med, spos, rpos) in cases:
+        mid = _model(ctx, w, g, med)
+        src = random_traces(rng, spos, g.nts, np.float64)
+        refs.append(O.nl_fwd_reg(g, med, spos, src, rpos, np.float64))
+        rec = [np.zeros((rpos[GRID_OF_COMP[k]].size, g.nts), np.float32) for k in range(5)]
+        srcc = [np.ascontiguousarray(x, np.float32) for x in src]
+        shot, kp = ctx.make_shot(mid, spos, rpos, srcc, rec)
+        shots.append(shot)
+        recs.append(rec)
+        keep.append((kp, srcc))
+    ctx.set_shot_group_bytes(0)
+    ctx.nl_forward(shots)
+    assert ctx.last_shot_groups() == 1
+    one = [[r.copy() for r in rec] for rec in recs]
+    per_cell = (15 if tb == 1 else 25) * 4
+    ctx.set_shot_group_bytes(2 * max(c[1].nz * c[1].nx for c in cases) * per_cell)
+    for r in recs:
+        for x in r:
+            x[...] = 0
+    ctx.nl_forward(shots)
+    assert ctx.last_shot_groups() >= 2, ctx.last_shot_groups()
+    for rec, r1, ref in zip(recs, one, refs):
+        for k in range(5):
+            assert np.array_equal(rec[k], r1[k]), k
+            assert rel_l2(rec[k], ref[k]) < 1e-5, (k, rel_l2(rec[k], ref[k]))
+    ctx.set_shot_group_bytes(_capi.GROUP_AUTO)
+    ctx.close()

@@ -231,5 +231,7 @@ def test_autotune_signature_and_candidates():
     b = autotune.signature([(400, 800, 5000, 4, 800), (300, 600, 7996, 4, 600)])
     assert a == b and len(a) == 16
     assert a != autotune.signature([(300, 600, 7996, 4, 600)])
+    # the shot-group budget changes which form is fastest: it is part of the key
+    assert a != autotune.signature([(300, 600, 7996, 4, 600), (400, 800, 5000, 4, 800)], group_bytes=0)
     c = autotune.candidates()
     assert (1, 0) in c and (2, 0) in c and len(set(c)) == len(c)
