@@ -227,8 +227,8 @@ def fwi_child(args):
     torch.cuda.synchronize()
     log("fwi: workload %s, observed data in %.1f s" % (w0["name"], time.perf_counter() - t0))
     m = torch.tensor(w0["elastic"], device="cuda")
-    group = eng.full_storage_group()
-    log("fwi: %d shots per full-storage group" % group)
+    group = eng.default_group()
+    log("fwi: %d shots per full-storage group (at most %d fit in memory)" % (group, eng.full_storage_group()))
     g32 = None
 
     def gradient():
@@ -297,7 +297,7 @@ def fwi_child(args):
            "workload": w0["name"], "dtype": "f32", "steps": args.fwi_steps, "warmup": args.fwi_warmup,
            "gcell_per_s": round(grad_cells / t / 1e9, 2), "rel_l2_vs_fp64": rel, "fp64_check_shots": sel,
            "checkpoint_interval": L, "checkpoints": nck, "store_bytes": nbytes, "shots_per_group": group,
-           "method": "shot groups with full coarse Vx/Vz storage sized to HBM, gradient per group",
+           "method": "shot groups sized to the Infinity Cache (records within HBM), full coarse Vx/Vz storage, gradient per group",
            "adjoint_time_block": list(adj),
            "full_storage_bytes": 2 * 4 * w0["nz"] * w0["nx"] * w0["nts"] * eng.nshot,
            "grad_norm": float(torch.linalg.norm(g32)), "checkpointed": ck}
@@ -473,7 +473,7 @@ def main():
     ap.add_argument("--fwi-steps", type=int, default=2)
     ap.add_argument("--fwi-warmup", type=int, default=1)
     ap.add_argument("--fwi-scale", type=float, default=1.0)
-    ap.add_argument("--fwi-adj", default="2:7",
+    ap.add_argument("--fwi-adj", default="2:-1",
                     help="reverse fine steps per launch of the gradient's receiver pass and the two-step "
                          "adjoint variant, as steps:variant")
     ap.add_argument("--fwi-segment", type=int, default=0,

@@ -911,12 +911,16 @@ void launch_fwd2(Run<float>& run, int ntile, int k, int par, int dlo = 0, int dh
 #define EIO_L2(R_, WX_, OCC_, NT_)                                                                    \
     klaunch(run.c, eio::k_step2_fwd<float, R_, WX_, OCC_, NT_, DEC, FINE, SS, NOREC>, dim3(ntile),      \
             dim3(64 * WX_), sh, tt, te, ntile, k, par, dlo, dhi, sspar)
-#define EIO_L2FS(R_, WX_, OCC_)                                                                       \
+// streamed-stage variants (layout 1): the coarse-record (DEC) and scattered (SS) passes run the
+// row-lane kernel of the same tile width (R_ * WX_ = FR_ * FWX_: the plan's tiles and entries
+// were built for that width)
+#define EIO_L2FS(R_, WX_, OCC_, FR_, FWX_, FOCC_)                                                     \
+    static_assert(R_ * WX_ == FR_ * FWX_, "fallback kernel of another tile width");                 \
     if constexpr (!DEC && !SS)                                                                        \
         klaunch(run.c, eio::k_step2_fwd<float, R_, WX_, OCC_, 0, false, FINE, false, NOREC, true>,      \
                 dim3(ntile), dim3(64 * WX_), sh, tt, te, ntile, k, par, dlo, dhi, sspar);             \
     else                                                                                              \
-        EIO_L2(4, 8, 1, 0)
+        EIO_L2(FR_, FWX_, FOCC_, 0)
 #define EIO_L2Q(OCC_)                                                                                 \
     if constexpr (!DEC && !SS && !FINE)                                                               \
         klaunch(run.c, eio::k_step2q_fwd<float, OCC_, NOREC>, dim3(ntile), dim3(eio::QNT), sh, tt, te, ntile, \
@@ -932,9 +936,9 @@ void launch_fwd2(Run<float>& run, int ntile, int k, int par, int dlo = 0, int dh
         case 5: EIO_L2(2, 8, 4, 1); break;
         case 6: EIO_L2(2, 8, 2, 1); break;
         case 7: EIO_L2(4, 8, 1, 1); break;
-        case 8: EIO_L2FS(4, 8, 4); break;
-        case 9: EIO_L2FS(8, 4, 2); break;
-        case 10: EIO_L2FS(2, 8, 6); break;
+        case 8: EIO_L2FS(4, 8, 4, 4, 8, 1); break;
+        case 9: EIO_L2FS(8, 4, 2, 4, 8, 1); break;
+        case 10: EIO_L2FS(2, 8, 6, 2, 8, 4); break;
         default: EIO_L2Q(2); break;
     }
 #undef EIO_L2Q

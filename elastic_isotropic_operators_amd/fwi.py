@@ -143,6 +143,20 @@ class _Store:
             pass
 
 
+# Working-set budget of a shot group, as the library's automatic shot groups of the NL forward
+# (EIO_GROUP_AUTO: 192 MiB of the MI355X's 256 MB Infinity Cache).
+CACHE_GROUP_BYTES = 192 << 20
+# Values per grid cell that one step of the gradient's passes touches: the receiver pass's four
+# adjoint state levels and the media (25), its adjoint-field slots (10) and the record's Vx/Vz
+# slices (4), rounded up.
+GRADIENT_VALUES_PER_CELL = 40
+
+
+def cache_group(nz, nx, itemsize, budget=CACHE_GROUP_BYTES):
+    """Shots per group whose gradient passes keep their working set within `budget` (at least 1)."""
+    return max(1, int(budget // (GRADIENT_VALUES_PER_CELL * int(nz) * int(nx) * int(itemsize))))
+
+
 def group_and_segment(group, fit, nshot, nts, segment, checkpoint=None):
     """Shots per group and the store's segment length (coarse intervals) of one
     misfit_and_gradient call.  fit: shots whose whole coarse Vx/Vz record fits in memory;
@@ -366,14 +380,25 @@ class ElasticFwi:
         per_shot = (2 * self.nts + 60) * self.nz * self.nx * torch.tensor([], dtype=self.dtype).element_size()
         return int(margin * free // per_shot)
 
+    def default_group(self):
+        """Shots per group of misfit_and_gradient when the caller gives none: as many as keep
+        the passes' working set in the Infinity Cache (cache_group), at most as many as keep
+        their whole coarse record in memory; 0 when not even one record fits."""
+        if getattr(self, "_group", None) is None:
+            self._group = self.full_storage_group()
+        if self._group < 1:
+            return 0
+        return min(self._group, cache_group(self.nz, self.nx, torch.tensor([], dtype=self.dtype).element_size()))
+
     def misfit_and_gradient(self, model, d_obs, components=None, group=None, checkpoint=None):
         """1/2 || C F(model) - d_obs ||^2 and its gradient with respect to `model` (rho, lambda,
         mu), shot group by shot group: each group's forward keeps the whole coarse Vx/Vz
         record (one segment, no checkpoints, the reference's full storage,
         BornElasticGpuFunctions.cu:1128-1191, sized to the device) and its gradient follows at
         once, so no segment is recomputed.  The misfit is a sum over shots, so the gradient of
-        each group needs only its own residual.  group: shots per group (default: as many as
-        fit in memory); a group that does not fit, or checkpoint = an interval (0: ceil(sqrt(
+        each group needs only its own residual.  group: shots per group (default:
+        default_group(), the shots whose passes stay in the Infinity Cache and whose records fit
+        in memory); a group that does not fit, or checkpoint = an interval (0: ceil(sqrt(
         nts-1)), all shots in one group unless `group` is given), takes checkpoints instead."""
         model = model.detach().to(device=self.device, dtype=self.dtype)
         d_obs = torch.as_tensor(d_obs, device=self.device, dtype=self.dtype)
@@ -381,6 +406,8 @@ class ElasticFwi:
         # sized once: later calls reuse the context's workspace, which free memory no longer shows
         if getattr(self, "_group", None) is None:
             self._group = self.full_storage_group()
+        if group is None and checkpoint is None and self._group >= 1:
+            group = self.default_group()
         g, segment = group_and_segment(group, self._group, self.nshot, self.nts, self.segment, checkpoint)
         grad = torch.zeros((3, self.nx, self.nz), dtype=self.dtype, device=self.device)
         phi = 0.0

@@ -548,7 +548,7 @@ def test_two_step_nl_dot_product():
     assert abs(a - b) / abs(b) < DOT_TOL[np.float32], (a, b)
 
 
-@pytest.mark.parametrize("v", [0, 1, 4, 7, 8, 11])
+@pytest.mark.parametrize("v", [0, 1, 4, 7, 8, 9, 10, 11])
 def test_two_step_born_source_pass_matches_oracle(v):
     """Born forward and adjoint whose source and scattered passes take two fine steps per launch
     (fine-sampled sources injected on the step-1 halo and Vx/Vz decimated into the whole

@@ -75,7 +75,7 @@ def test_checkpointed_gradient_matches_full_storage_oracle(dtype, segment):
         ctx.fwi_gradient(h, [shot2], out2)
 
 
-@pytest.mark.parametrize("variant2", [4, 11])
+@pytest.mark.parametrize("variant2", [4, 10, 11])
 @pytest.mark.parametrize("segment,sub", [(1, 2), (2, 1), (5, 2), (118, 1), (0, 2), (119, 2)])
 def test_two_step_checkpointed_forward_and_recompute_match_oracle(segment, sub, variant2):
     """FP32 with two fine steps per launch everywhere: the checkpointed forward (checkpoints
@@ -113,7 +113,7 @@ def test_two_step_checkpointed_forward_and_recompute_match_oracle(segment, sub, 
     ctx.fwi_release(h)
 
 
-@pytest.mark.parametrize("variant2", [4, 0, 11])
+@pytest.mark.parametrize("variant2", [4, 0, 9, 10, 11])
 def test_two_step_forward_with_whole_record_matches_oracle(variant2):
     """FP32, one segment (the whole coarse record): the forward runs the two-step kernel with
     the decimation fused; data and gradient against the FP64 oracle's full-storage Born

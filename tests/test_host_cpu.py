@@ -59,6 +59,17 @@ def test_fwi_group_and_segment_choice():
     assert gs(8, 7, 32, nts, 0, checkpoint=55) == (8, 55)
 
 
+def test_fwi_cache_group():
+    """The default FWI shot group keeps the gradient passes' working set (40 values per cell)
+    within the 192 MiB shot-group budget: one shot of the 600 x 1592 C3 grid (153 MB), more of
+    small grids, never zero."""
+    from elastic_isotropic_operators_amd.fwi import cache_group, CACHE_GROUP_BYTES
+    assert CACHE_GROUP_BYTES == 192 << 20
+    assert cache_group(600, 1592, 4) == 1
+    assert cache_group(280, 472, 4) == 9          # 21.1 MB per shot
+    assert cache_group(5000, 5000, 8) == 1
+
+
 def _conv_elastic(args):
     par = genericIO.io(params=args)
     el = genericIO.defaultIO.getVector(par.getString("elasticParam"), ndims=3)
