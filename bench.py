@@ -559,7 +559,10 @@ def main():
     value = cells * args.steps / elapsed / 1e9
     loop_s = float(np.mean([s * n for s, n in zip(step_ms, nlaunch)])) / 1e3
     achieved = BYTES_PER_CELL["nl_fwd"] * cells / loop_s / 1e9
-    if args.time_block == 3:
+    if (args.time_block == 3 and args.variant3 < 0) or (args.time_block == 2 and args.variant2 < 0):
+        kname = "k_step%d_fwd (the library's automatic variant per call, %d fine steps per launch)" % (
+            args.time_block, args.time_block)
+    elif args.time_block == 3:
         R3, WX3, OCC3 = _capi.Context.time_block3_variants()[args.variant3]
         kname = "k_step3_fwd<float,%d,%d,%d> (3 fine steps per launch)" % (R3, WX3, OCC3)
     elif args.time_block == 2:

@@ -601,6 +601,8 @@ struct Run {
 // their source and scattered passes 11 % faster on 56 x 8; the 64-shot layered batch runs
 // 19 % slower on it.
 constexpr int AUTO_SMALL_TILES = 1024;
+// shot-group working-set budget of EIO_GROUP_AUTO, of the MI355X's 256 MB Infinity Cache
+constexpr long long GROUP_AUTO_BYTES = 192ll << 20;
 // tiles of 56 x 32 over the update regions of a batch
 long wide_tiles(Ctx* c, int nshot, const eio_shot* shots) {
     long tiles = 0;
@@ -610,8 +612,13 @@ long wide_tiles(Ctx* c, int nshot, const eio_shot* shots) {
     }
     return tiles;
 }
-int variant2_for(Ctx* c, int nshot, const eio_shot* shots) {
+// `grouped`: a plain NL forward whose shots run in more than one shot group; their data stays
+// in the Infinity Cache between launches, where 56 x 16 tiles with the streamed stage (variant
+// 10, three workgroups per CU) are the fastest form (64-shot batch: 0.520 ms per fine step
+// against 0.633 for 56 x 32).
+int variant2_for(Ctx* c, int nshot, const eio_shot* shots, bool grouped = false) {
     if (c->variant2 != EIO_VARIANT_AUTO) return c->variant2;
+    if (grouped) return 10;
     return wide_tiles(c, nshot, shots) < AUTO_SMALL_TILES ? 3 : 4;
 }
 
@@ -630,15 +637,29 @@ int variant3_for(Ctx* c, int nshot, const eio_shot* shots) {
     return wide_tiles(c, nshot, shots) < AUTO_SMALL_TILES ? 2 : 1;
 }
 
+// Whether a plain FP32 NL forward of these shots splits into more than one shot group: their
+// working set (`per_cell` bytes per cell) exceeds the context's group budget.
+bool nl_grouped(Ctx* c, int nshot, const eio_shot* shots, long long per_cell) {
+    const long long budget = c->group_bytes == EIO_GROUP_AUTO ? GROUP_AUTO_BYTES : c->group_bytes;
+    if (budget <= 0) return false;
+    long long total = 0;
+    for (int i = 0; i < nshot; ++i) {
+        const ModelBase& m = get_model(c, shots[i].model);
+        total += (long long)m.g.nz * m.g.nx * per_cell;
+    }
+    return total > budget;
+}
+
 // Builds the plan, the per-shot descriptors and the device tables of one call, uploads
 // input traces and zeroes the state and the outputs.  FWI calls bring their own coarse
 // storage ring (`ring`, `wcap` slices per component and shot).  `two_step`: the NL forward
 // runs the two-step kernel; `adj2`: the adjoint passes run the two-step adjoint kernel;
-// `src2`: the Born source pass runs the two-step forward kernel.
+// `src2`: the Born source pass runs the two-step forward kernel; `grouped`: a plain NL forward
+// that runs in more than one shot group (its automatic two-step variant differs).
 template <typename T>
 Run<T> prepare(Ctx* c, int nshot, const eio_shot* shots, Op op, int flags,
                const void* const dm_fwd[5], T* ring = nullptr, int wcap = 0, bool two_step = false,
-               bool adj2 = false, bool src2 = false, bool three_step = false) {
+               bool adj2 = false, bool src2 = false, bool three_step = false, bool grouped = false) {
     if (nshot <= 0) fail("nshot must be > 0");
     if (!shots) fail("shots is NULL");
     // the learned boundary is part of the NL forward only: the adjoint, Born and FWI passes
@@ -651,7 +672,7 @@ Run<T> prepare(Ctx* c, int nshot, const eio_shot* shots, Op op, int flags,
                      "for adjoint, Born and FWI calls");
     Run<T> run;
     run.c = c;
-    run.v2 = variant2_for(c, nshot, shots);
+    run.v2 = variant2_for(c, nshot, shots, grouped);
     run.v2a = variant2a_for(c, nshot, shots);
     run.v3 = variant3_for(c, nshot, shots);
     run.plan = three_step ? make_plan<T>(c, nshot, shots, op, PlanKind::FWD3, run.v3)
@@ -824,7 +845,6 @@ inline int active_tiles(const Plan& p, int k) {
 struct ShotGroup {
     int s0, s1;
 };
-constexpr long long GROUP_AUTO_BYTES = 192ll << 20;   // of the 256 MB Infinity Cache
 
 // Consecutive groups of the plan's shots whose working set (`per_cell` bytes per cell of the
 // grid) fits `budget` bytes (EIO_GROUP_AUTO: GROUP_AUTO_BYTES; 0: one group); a shot larger
@@ -1379,7 +1399,10 @@ void nl_forward(Ctx* c, int nshot, const eio_shot* shots, int flags, int wshot =
     const bool plain = sizeof(T) == 4 && !wfld && !any_fs && !abc && nsnap == 0;
     const bool three = plain && c->time_block == 3;
     const bool two = plain && c->time_block == 2;
-    Run<T> run = prepare<T>(c, nshot, shots, Op::NL_FWD, flags, nullptr, nullptr, 0, two, false, false, three);
+    const long long per_cell = (two || three ? 25 : 15) * (long long)sizeof(T);
+    const bool grouped = plain && nl_grouped(c, nshot, shots, per_cell);
+    Run<T> run = prepare<T>(c, nshot, shots, Op::NL_FWD, flags, nullptr, nullptr, 0, two, false, false, three,
+                            grouped);
     if (wfld) attach_wavefield(run, wshot, wfld, flags);
     AbcRun ar;
     if (abc) ar = abc_prepare(run, shots);
@@ -1418,8 +1441,7 @@ void nl_forward(Ctx* c, int nshot, const eio_shot* shots, int flags, int wshot =
     }
     // plain runs: shot groups, each through all its steps before the next
     const std::vector<ShotGroup> groups =
-        plain ? shot_groups(run, c->group_bytes, (two || three ? 25 : 15) * (long long)sizeof(T))
-              : std::vector<ShotGroup>{{0, nshot}};
+        plain ? shot_groups(run, c->group_bytes, per_cell) : std::vector<ShotGroup>{{0, nshot}};
     c->last_groups = int(groups.size());
     timing_begin<T>(c);
     EIO_CHECK_HIP(hipEventRecord(c->ev[2], c->stream));

@@ -368,7 +368,9 @@ def test_auto_time_block_variant_matches_explicit():
     """EIO_VARIANT_AUTO chooses the two-step tile width per call: a one-shot 150 x 130 batch
     (6 tiles of 56 x 32, under the threshold of 1024) equals the oracle and the explicit 56 x 8
     variants (NL forward and NL adjoint); a batch of 4 shots of 600 x 1600 (2200 tiles) equals
-    the explicit 56 x 32 variants (NL forward, Born forward, Born adjoint)."""
+    the explicit 56 x 32 variants (Born forward, Born adjoint) and, for its NL forward, which
+    runs in two shot groups and so takes 56 x 16 with the streamed stage, agrees with them to
+    FP32 rounding."""
     w = W.small(nz=150, nx=130, nts=30, sub=1, seed=170)
     g = oracle_grid(w)
     med = O.media(g, w["elastic"], np.float64)
@@ -1076,7 +1078,9 @@ def test_shot_groups_equal_one_group_and_the_oracle(tb):
         med = O.media(g, w["elastic"], np.float64)
         cases.append((w, g, med, scattered_positions(rng, g.nz, g.nx, 8), scattered_positions(rng, g.nz, g.nx, 20)))
     ctx = _capi.Context(0, np.float32)
-    ctx.set_time_block(tb, _capi.VARIANT_AUTO)
+    # one kernel for both calls (the automatic two-step choice differs between a grouped and an
+    # ungrouped call): variant 10, which the automatic choice takes for a grouped call
+    ctx.set_time_block(tb, 10 if tb == 2 else _capi.VARIANT_AUTO)
     shots, recs, refs, keep = [], [], [], []
     for (w, g, med, spos, rpos) in cases:
         mid = _model(ctx, w, g, med)
@@ -1103,5 +1107,14 @@ def test_shot_groups_equal_one_group_and_the_oracle(tb):
         for k in range(5):
             assert np.array_equal(rec[k], r1[k]), k
             assert rel_l2(rec[k], ref[k]) < 1e-5, (k, rel_l2(rec[k], ref[k]))
+    if tb == 2:
+        # the automatic variant of a grouped call is variant 10: the same traces again
+        grouped = [[r.copy() for r in rec] for rec in recs]
+        ctx.set_time_block(2, _capi.VARIANT_AUTO)
+        ctx.nl_forward(shots)
+        assert ctx.last_shot_groups() >= 2
+        for rec, r1 in zip(recs, grouped):
+            for k in range(5):
+                assert np.array_equal(rec[k], r1[k]), k
     ctx.set_shot_group_bytes(_capi.GROUP_AUTO)
     ctx.close()
