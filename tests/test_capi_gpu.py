@@ -1063,12 +1063,13 @@ def test_three_step_uneven_batch_matches_oracle():
             assert rel_l2(rec[k], ref[k]) < 1e-5, (k, rel_l2(rec[k], ref[k]))
 
 
-@pytest.mark.parametrize("tb", [1, 2, 3])
-def test_shot_groups_equal_one_group_and_the_oracle(tb):
+@pytest.mark.parametrize("tb,tiny", [(1, False), (2, False), (3, False), (2, True)])
+def test_shot_groups_equal_one_group_and_the_oracle(tb, tiny):
     """Shot groups (each group runs all its steps before the next starts) give the receiver
     traces of the one-group call bit for bit, and the oracle's, with one, two and three fine
     steps per launch, on an uneven batch that a small working-set budget splits into several
-    groups (shots of different lengths inside one group finish in different launches)."""
+    groups (shots of different lengths inside one group finish in different launches); `tiny`:
+    a budget below every shot's working set, so each shot is a group of its own."""
     rng = np.random.default_rng(71)
     sizes = [(90, 130, 70, 1), (150, 90, 50, 2), (60, 200, 61, 1), (120, 100, 40, 2), (70, 70, 66, 1)]
     cases = []
@@ -1097,12 +1098,12 @@ def test_shot_groups_equal_one_group_and_the_oracle(tb):
     assert ctx.last_shot_groups() == 1
     one = [[r.copy() for r in rec] for rec in recs]
     per_cell = (15 if tb == 1 else 25) * 4
-    ctx.set_shot_group_bytes(2 * max(c[1].nz * c[1].nx for c in cases) * per_cell)
+    ctx.set_shot_group_bytes(1 if tiny else 2 * max(c[1].nz * c[1].nx for c in cases) * per_cell)
     for r in recs:
         for x in r:
             x[...] = 0
     ctx.nl_forward(shots)
-    assert ctx.last_shot_groups() >= 2, ctx.last_shot_groups()
+    assert ctx.last_shot_groups() == len(cases) if tiny else ctx.last_shot_groups() >= 2, ctx.last_shot_groups()
     for rec, r1, ref in zip(recs, one, refs):
         for k in range(5):
             assert np.array_equal(rec[k], r1[k]), k
