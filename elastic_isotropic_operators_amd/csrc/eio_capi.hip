@@ -105,7 +105,8 @@ struct Variant2 {
 };
 constexpr Variant2 F32_VARIANTS2[] = {{2, 8, 2, 0, 0}, {2, 8, 4, 0, 0}, {4, 4, 2, 0, 0}, {1, 8, 4, 0, 0},
                                       {4, 8, 1, 0, 0}, {2, 8, 4, 1, 0}, {2, 8, 2, 1, 0}, {4, 8, 1, 1, 0},
-                                      {4, 8, 4, 0, 1}, {8, 4, 2, 0, 1}, {2, 8, 6, 0, 1}, {3, 8, 2, 0, 2}};
+                                      {4, 8, 4, 0, 1}, {8, 4, 2, 0, 1}, {2, 8, 6, 0, 1}, {3, 8, 2, 0, 2},
+                                      {1, 8, 8, 0, 1}};
 constexpr int N_F32_VARIANTS2 = int(sizeof(F32_VARIANTS2) / sizeof(F32_VARIANTS2[0]));
 // Three-step NL forward variants (k_step3_fwd): R3 owned x-cells per lane, WX waves, OCC waves
 // per SIMD; owned tile = 48 x WX*R3.
@@ -613,12 +614,13 @@ long wide_tiles(Ctx* c, int nshot, const eio_shot* shots) {
     return tiles;
 }
 // `grouped`: a plain NL forward whose shots run in more than one shot group; their data stays
-// in the Infinity Cache between launches, where 56 x 16 tiles with the streamed stage (variant
-// 10, three workgroups per CU) are the fastest form (64-shot batch: 0.520 ms per fine step
-// against 0.633 for 56 x 32).
+// in the Infinity Cache between launches, the kernel is bound by the latency of its load
+// phases, and 56 x 8 tiles with the streamed stage at 8 waves per SIMD (variant 12, four
+// workgroups per CU) are the fastest form (64-shot batch: 0.502 ms per fine step against 0.518
+// for 56 x 16 at 6 waves and 0.633 for 56 x 32).
 int variant2_for(Ctx* c, int nshot, const eio_shot* shots, bool grouped = false) {
     if (c->variant2 != EIO_VARIANT_AUTO) return c->variant2;
-    if (grouped) return 10;
+    if (grouped) return 12;
     return wide_tiles(c, nshot, shots) < AUTO_SMALL_TILES ? 3 : 4;
 }
 
@@ -959,6 +961,7 @@ void launch_fwd2(Run<float>& run, int ntile, int k, int par, int dlo = 0, int dh
         case 8: EIO_L2FS(4, 8, 4, 4, 8, 1); break;
         case 9: EIO_L2FS(8, 4, 2, 4, 8, 1); break;
         case 10: EIO_L2FS(2, 8, 6, 2, 8, 4); break;
+        case 12: EIO_L2FS(1, 8, 8, 1, 8, 4); break;
         default: EIO_L2Q(2); break;
     }
 #undef EIO_L2Q
@@ -1325,11 +1328,13 @@ void with_persist_kernel(int v, F&& f) {
         case 0: case 1: case 5: case 6: f(eio::k_step2_fwd_persist<float, 2, 8, 2>, 8); break;  // 2,8,4 spills
         case 2: f(eio::k_step2_fwd_persist<float, 4, 4, 2>, 4); break;
         case 3: f(eio::k_step2_fwd_persist<float, 1, 8, 4>, 8); break;
+        case 12: f(eio::k_step2_fwd_persist<float, 1, 8, 4, true>, 8); break;  // 8 waves/SIMD spills
         case 8: f(eio::k_step2_fwd_persist<float, 4, 8, 4, true>, 8); break;
         case 9: f(eio::k_step2_fwd_persist<float, 8, 4, 2, true>, 4); break;
         case 10: f(eio::k_step2_fwd_persist<float, 2, 8, 6, true>, 8); break;
         case 11: f(eio::k_step2_fwd_persist<float, 3, 8, 2, false, true>, 8); break;
-        default: f(eio::k_step2_fwd_persist<float, 4, 8, 1>, 8); break;
+        case 4: case 7: f(eio::k_step2_fwd_persist<float, 4, 8, 1>, 8); break;
+        default: fail("no persistent kernel for two-step variant " + std::to_string(v));
     }
 }
 

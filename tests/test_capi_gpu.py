@@ -337,7 +337,7 @@ def test_born_adjoint_with_reused_storage_of_another_shape():
     assert rel_l2(out, ref) < 2e-5, rel_l2(out, ref)
 
 
-@pytest.mark.parametrize("v", range(12))
+@pytest.mark.parametrize("v", range(13))
 @pytest.mark.parametrize("sub_fd", [2, 3])
 def test_two_step_nl_forward_matches_oracle(v, sub_fd):
     """The temporally blocked NL forward (two fine steps per launch, step 1 recomputed on the
@@ -369,7 +369,7 @@ def test_auto_time_block_variant_matches_explicit():
     (6 tiles of 56 x 32, under the threshold of 1024) equals the oracle and the explicit 56 x 8
     variants (NL forward and NL adjoint); a batch of 4 shots of 600 x 1600 (2200 tiles) equals
     the explicit 56 x 32 variants (Born forward, Born adjoint) and, for its NL forward, which
-    runs in two shot groups and so takes 56 x 16 with the streamed stage, agrees with them to
+    runs in two shot groups and so takes 56 x 8 with the streamed stage, agrees with them to
     FP32 rounding."""
     w = W.small(nz=150, nx=130, nts=30, sub=1, seed=170)
     g = oracle_grid(w)
@@ -550,7 +550,7 @@ def test_two_step_nl_dot_product():
     assert abs(a - b) / abs(b) < DOT_TOL[np.float32], (a, b)
 
 
-@pytest.mark.parametrize("v", [0, 1, 4, 7, 8, 9, 10, 11])
+@pytest.mark.parametrize("v", [0, 1, 4, 7, 8, 9, 10, 11, 12])
 def test_two_step_born_source_pass_matches_oracle(v):
     """Born forward and adjoint whose source and scattered passes take two fine steps per launch
     (fine-sampled sources injected on the step-1 halo and Vx/Vz decimated into the whole
@@ -927,7 +927,7 @@ def _persistent_batch(ctx, cases, seed):
     return shots, recs, keep
 
 
-@pytest.mark.parametrize("v2", [_capi.VARIANT_AUTO, 0, 4, 8, 10, 11])
+@pytest.mark.parametrize("v2", [_capi.VARIANT_AUTO, 0, 4, 8, 10, 11, 12])
 def test_persistent_launch_equals_launched_pairs(v2):
     """The whole two-step NL forward in one persistent launch (tiles wait for their neighbours'
     previous pair through flags) gives the launched form's traces bit for bit, on an uneven
@@ -1080,8 +1080,8 @@ def test_shot_groups_equal_one_group_and_the_oracle(tb, tiny):
         cases.append((w, g, med, scattered_positions(rng, g.nz, g.nx, 8), scattered_positions(rng, g.nz, g.nx, 20)))
     ctx = _capi.Context(0, np.float32)
     # one kernel for both calls (the automatic two-step choice differs between a grouped and an
-    # ungrouped call): variant 10, which the automatic choice takes for a grouped call
-    ctx.set_time_block(tb, 10 if tb == 2 else _capi.VARIANT_AUTO)
+    # ungrouped call): variant 12, which the automatic choice takes for a grouped call
+    ctx.set_time_block(tb, 12 if tb == 2 else _capi.VARIANT_AUTO)
     shots, recs, refs, keep = [], [], [], []
     for (w, g, med, spos, rpos) in cases:
         mid = _model(ctx, w, g, med)
@@ -1109,7 +1109,7 @@ def test_shot_groups_equal_one_group_and_the_oracle(tb, tiny):
             assert np.array_equal(rec[k], r1[k]), k
             assert rel_l2(rec[k], ref[k]) < 1e-5, (k, rel_l2(rec[k], ref[k]))
     if tb == 2:
-        # the automatic variant of a grouped call is variant 10: the same traces again
+        # the automatic variant of a grouped call is variant 12: the same traces again
         grouped = [[r.copy() for r in rec] for rec in recs]
         ctx.set_time_block(2, _capi.VARIANT_AUTO)
         ctx.nl_forward(shots)

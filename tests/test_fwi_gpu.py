@@ -113,7 +113,7 @@ def test_two_step_checkpointed_forward_and_recompute_match_oracle(segment, sub, 
     ctx.fwi_release(h)
 
 
-@pytest.mark.parametrize("variant2", [4, 0, 9, 10, 11])
+@pytest.mark.parametrize("variant2", [4, 0, 9, 10, 11, 12])
 def test_two_step_forward_with_whole_record_matches_oracle(variant2):
     """FP32, one segment (the whole coarse record): the forward runs the two-step kernel with
     the decimation fused; data and gradient against the FP64 oracle's full-storage Born
