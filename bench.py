@@ -43,7 +43,8 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "GCell-updates/s + FWI-gradient s/shot at \u22641e-4 rel-L2 vs FP64, 1 MI355X"
 # measured bytes per cell-update of the dominant kernel per (time block, variant, grouping)
 TRAFFIC_FILES = [os.path.join(ROOT, "profiles", f) for f in
-                 ("k_step2_fwd_layered_batch_c2_traffic.json", "k_step2_fwd_fs_layered_batch_c2_grouped_traffic.json")]
+                 ("k_step2_fwd_layered_batch_c2_traffic.json", "k_step2_fwd_fs_layered_batch_c2_grouped_traffic.json",
+                  "k_step2_fwd_fs56x8_layered_batch_c2_grouped_traffic.json")]
 
 
 def log(msg):
