@@ -596,11 +596,14 @@ struct Run {
 };
 
 // The two-step forward variant of a call.  EIO_VARIANT_AUTO: a batch that fills the chip with
-// fewer than AUTO_SMALL_TILES tiles of 56 x 32 (four per CU) takes 56 x 8 tiles (variant 3),
-// which spread its launches over every CU; larger batches take 56 x 32 (variant 4), whose
-// wider tiles re-read less halo.  Measured: 8 Born shots of 280 x 472 (600 wide tiles) run
-// their source and scattered passes 11 % faster on 56 x 8; the 64-shot layered batch runs
-// 19 % slower on it.
+// fewer than AUTO_SMALL_TILES tiles of 56 x 32 (four per CU) takes 56 x 8 tiles, which spread
+// its launches over every CU: the streamed stage at 8 waves per SIMD (variant 12; the
+// coarse-record and scattered passes, which have no streamed form, run variant 3's row-lane
+// 56 x 8 kernel); larger batches take 56 x 32 (variant 4), whose wider tiles re-read less
+// halo, unless they run in shot groups.  Measured: 8 Born shots of 280 x 472 (600 wide tiles)
+// run their source and scattered passes 11 % faster on 56 x 8 than on 56 x 32, and 8 NL
+// forward shots of that grid 16 % faster on variant 12 than on variant 3 (one shot: the same);
+// the 64-shot layered batch in one group runs 19 % slower on 56 x 8.
 constexpr int AUTO_SMALL_TILES = 1024;
 // shot-group working-set budget of EIO_GROUP_AUTO, of the MI355X's 256 MB Infinity Cache
 constexpr long long GROUP_AUTO_BYTES = 192ll << 20;
@@ -621,7 +624,7 @@ long wide_tiles(Ctx* c, int nshot, const eio_shot* shots) {
 int variant2_for(Ctx* c, int nshot, const eio_shot* shots, bool grouped = false) {
     if (c->variant2 != EIO_VARIANT_AUTO) return c->variant2;
     if (grouped) return 12;
-    return wide_tiles(c, nshot, shots) < AUTO_SMALL_TILES ? 3 : 4;
+    return wide_tiles(c, nshot, shots) < AUTO_SMALL_TILES ? 12 : 4;
 }
 
 // The two-step adjoint variant of a call: EIO_VARIANT_AUTO takes 56 x 8 tiles (variant 10)

@@ -106,10 +106,10 @@ int eio_ctx_set_variant(eio_ctx* ctx, int variant);
 /* Fine steps per launch of the FP32 forward passes: 1 (one fused step per launch) or 2 (two
  * steps per launch, temporally blocked: 50 instead of 80 bytes of HBM traffic per
  * cell-update, four state levels per shot), and the variant of the two-step kernel
- * (owned tile 56 x WX*R2), or EIO_VARIANT_AUTO: per call, 56 x 8 tiles for a batch of
- * fewer than 1024 tiles of 56 x 32, else 56 x 32, and 56 x 8 with the streamed stage at 8
- * waves per SIMD (variant 12) for an NL forward that runs in several shot groups
- * (eio_ctx_set_shot_group_bytes).
+ * (owned tile 56 x WX*R2), or EIO_VARIANT_AUTO: per call, 56 x 8 tiles with the streamed
+ * stage at 8 waves per SIMD (variant 12) for a batch of fewer than 1024 tiles of 56 x 32 and
+ * for an NL forward that runs in several shot groups (eio_ctx_set_shot_group_bytes), else
+ * 56 x 32 (variant 4).
  * It applies to the NL forward (without
  * saved wavefield, free surface or learned boundary), the whole-record FWI forward and the
  * Born source and scattered passes.  No counterpart in the reference. */
