@@ -598,7 +598,7 @@ struct Run {
 // The two-step forward variant of a call.  EIO_VARIANT_AUTO: a batch that fills the chip with
 // fewer than AUTO_SMALL_TILES tiles of 56 x 32 (four per CU) takes 56 x 8 tiles, which spread
 // its launches over every CU: the streamed stage at 8 waves per SIMD (variant 12; the
-// coarse-record and scattered passes, which have no streamed form, run variant 3's row-lane
+// coarse-record pass at 6, the scattered pass, which has no streamed form, variant 3's row-lane
 // 56 x 8 kernel); larger batches take 56 x 32 (variant 4), whose wider tiles re-read less
 // halo, unless they run in shot groups.  Measured: 8 Born shots of 280 x 472 (600 wide tiles)
 // run their source and scattered passes 11 % faster on 56 x 8 than on 56 x 32, and 8 NL
@@ -936,13 +936,17 @@ void launch_fwd2(Run<float>& run, int ntile, int k, int par, int dlo = 0, int dh
 #define EIO_L2(R_, WX_, OCC_, NT_)                                                                    \
     klaunch(run.c, eio::k_step2_fwd<float, R_, WX_, OCC_, NT_, DEC, FINE, SS, NOREC>, dim3(ntile),      \
             dim3(64 * WX_), sh, tt, te, ntile, k, par, dlo, dhi, sspar)
-// streamed-stage variants (layout 1): the coarse-record (DEC) and scattered (SS) passes run the
-// row-lane kernel of the same tile width (R_ * WX_ = FR_ * FWX_: the plan's tiles and entries
-// were built for that width)
-#define EIO_L2FS(R_, WX_, OCC_, FR_, FWX_, FOCC_)                                                     \
+// streamed-stage variants (layout 1): the coarse-record (DEC) pass runs the streamed kernel at
+// occupancy DOCC_ (its slot registers do not fit the plain pass's budget without spills; 0:
+// no streamed form), the scattered (SS) pass, which has none, the row-lane kernel of the same
+// tile width (R_ * WX_ = FR_ * FWX_: the plan's tiles and entries were built for that width)
+#define EIO_L2FS(R_, WX_, OCC_, DOCC_, FR_, FWX_, FOCC_)                                              \
     static_assert(R_ * WX_ == FR_ * FWX_, "fallback kernel of another tile width");                 \
     if constexpr (!DEC && !SS)                                                                        \
         klaunch(run.c, eio::k_step2_fwd<float, R_, WX_, OCC_, 0, false, FINE, false, NOREC, true>,      \
+                dim3(ntile), dim3(64 * WX_), sh, tt, te, ntile, k, par, dlo, dhi, sspar);             \
+    else if constexpr (DEC && !SS && DOCC_ > 0)                                                      \
+        klaunch(run.c, eio::k_step2_fwd<float, R_, WX_, DOCC_, 0, true, FINE, false, NOREC, true>,      \
                 dim3(ntile), dim3(64 * WX_), sh, tt, te, ntile, k, par, dlo, dhi, sspar);             \
     else                                                                                              \
         EIO_L2(FR_, FWX_, FOCC_, 0)
@@ -961,10 +965,10 @@ void launch_fwd2(Run<float>& run, int ntile, int k, int par, int dlo = 0, int dh
         case 5: EIO_L2(2, 8, 4, 1); break;
         case 6: EIO_L2(2, 8, 2, 1); break;
         case 7: EIO_L2(4, 8, 1, 1); break;
-        case 8: EIO_L2FS(4, 8, 4, 4, 8, 1); break;
-        case 9: EIO_L2FS(8, 4, 2, 4, 8, 1); break;
-        case 10: EIO_L2FS(2, 8, 6, 2, 8, 4); break;
-        case 12: EIO_L2FS(1, 8, 8, 1, 8, 4); break;
+        case 8: EIO_L2FS(4, 8, 4, 4, 4, 8, 1); break;
+        case 9: EIO_L2FS(8, 4, 2, 0, 4, 8, 1); break;
+        case 10: EIO_L2FS(2, 8, 6, 6, 2, 8, 4); break;
+        case 12: EIO_L2FS(1, 8, 8, 6, 1, 8, 4); break;
         default: EIO_L2Q(2); break;
     }
 #undef EIO_L2Q

@@ -75,7 +75,7 @@ __device__ __forceinline__ void step2_fwd_tile(const ShotDev<T>* __restrict__ sh
                                                int dhi, int sspar) {
     static_assert(8 % WX == 0, "WX must divide 8");
     static_assert(sizeof(T) == 4, "the two-step forward is FP32");
-    static_assert(!FS || (!DEC && !SS), "the field-streamed stage serves the NL forward and the recompute");
+    static_assert(!FS || !SS, "the field-streamed stage has no scattered (SS) pass");
     constexpr int NT = 64 * WX;
     constexpr int TX = WX * R2;        // owned columns
     constexpr int R1 = R2 + 8 / WX;    // step-1 columns per wave
@@ -120,6 +120,28 @@ __device__ __forceinline__ void step2_fwd_tile(const ShotDev<T>* __restrict__ sh
     rsrc_t rwx0, rwx1, rwz0, rwz1;
     T f1a = T(0), f1b = T(0), f2a = T(0), f2b = T(0);
     T px0[R2], pz0[R2], px1[R2], pz1[R2];
+    // decimation slots its (x0, z0) and its+1 (x1, z1) of the owned cells (DEC)
+    auto load_slots = [&]() {
+        const size_t s0 = size_t(st1.its % S.wcap) * n, s1 = size_t((st1.its + 1) % S.wcap) * n;
+        rwx0 = make_rsrc(S.Wvx + s0, fbytes);
+        rwx1 = make_rsrc(S.Wvx + s1, fbytes);
+        rwz0 = make_rsrc(S.Wvz + s0, fbytes);
+        rwz1 = make_rsrc(S.Wvz + s1, fbytes);
+        f1a = S.filt[st1.it2];
+        f1b = S.filt[S.h + st1.it2];
+        f2a = S.filt[st2.it2];
+        f2b = S.filt[S.h + st2.it2];
+        const bool sta = st1.it2 == 1 && st1.its == dlo;
+        const bool stb = st1.it2 == 1 && st1.its + 1 != dhi;
+#pragma unroll
+        for (int r = 0; r < R2; ++r) {
+            const unsigned vo = (unsigned(ix2 + r) * unsigned(nz) + unsigned(izl)) * SZ;
+            px0[r] = sta ? T(0) : bld<T>(rwx0, vo, 0);
+            pz0[r] = sta ? T(0) : bld<T>(rwz0, vo, 0);
+            px1[r] = stb ? T(0) : bld<T>(rwx1, vo, 0);
+            pz1[r] = stb ? T(0) : bld<T>(rwz1, vo, 0);
+        }
+    };
     // SS: secondary-source slots and the weights of the two steps (sampled at it2-1)
     rsrc_t rssl, rssr;
     T ssf1a = T(0), ssf1b = T(0), ssf2a = T(0), ssf2b = T(0);
@@ -301,28 +323,8 @@ __device__ __forceinline__ void step2_fwd_tile(const ShotDev<T>* __restrict__ sh
 #pragma unroll
             for (int f = 0; f < 5; ++f) m[f][r] = bld<T>(rmed, vo, unsigned(f) * fbytes);
         }
-        // decimation slots its (x0, z0) and its+1 (x1, z1) of the owned cells, with the stage
-        if constexpr (DEC) {
-            const size_t s0 = size_t(st1.its % S.wcap) * n, s1 = size_t((st1.its + 1) % S.wcap) * n;
-            rwx0 = make_rsrc(S.Wvx + s0, fbytes);
-            rwx1 = make_rsrc(S.Wvx + s1, fbytes);
-            rwz0 = make_rsrc(S.Wvz + s0, fbytes);
-            rwz1 = make_rsrc(S.Wvz + s1, fbytes);
-            f1a = S.filt[st1.it2];
-            f1b = S.filt[S.h + st1.it2];
-            f2a = S.filt[st2.it2];
-            f2b = S.filt[S.h + st2.it2];
-            const bool sta = st1.it2 == 1 && st1.its == dlo;
-            const bool stb = st1.it2 == 1 && st1.its + 1 != dhi;
-#pragma unroll
-            for (int r = 0; r < R2; ++r) {
-                const unsigned vo = (unsigned(ix2 + r) * unsigned(nz) + unsigned(izl)) * SZ;
-                px0[r] = sta ? T(0) : bld<T>(rwx0, vo, 0);
-                pz0[r] = sta ? T(0) : bld<T>(rwz0, vo, 0);
-                px1[r] = stb ? T(0) : bld<T>(rwx1, vo, 0);
-                pz1[r] = stb ? T(0) : bld<T>(rwz1, vo, 0);
-            }
-        }
+        // the decimation slots with the stage
+        if constexpr (DEC) load_slots();
 #pragma unroll
         for (int it = 0; it < NFILL; ++it) {
             const int e = threadIdx.x + it * NT;
@@ -401,6 +403,9 @@ __device__ __forceinline__ void step2_fwd_tile(const ShotDev<T>* __restrict__ sh
         }
     }
     __syncthreads();
+    // FS: the decimation slots after step 1 (its registers are taken during the streamed
+    // stage; step 1's sources and receivers and step 2 cover their latency)
+    if constexpr (DEC && FS) load_slots();
 
     const bool has_src = !SS && t.src_e > t.src_b;
     const bool has_rec = !FINE && !NOREC && t.rec_e > t.rec_b;
