@@ -101,6 +101,7 @@ def lib():
             ("eio_ctx_set_adjoint_time_block", [vp, ctypes.c_int, ctypes.c_int]),
             ("eio_ctx_set_step_limit", [vp, ctypes.c_int]),
             ("eio_ctx_set_shot_group_bytes", [vp, ctypes.c_longlong]),
+            ("eio_ctx_set_shot_group_streams", [vp, ctypes.c_int]),
             ("eio_ctx_last_shot_groups", [vp, ctypes.POINTER(ctypes.c_int)]),
             ("eio_ctx_set_persistent", [vp, ctypes.c_int, ctypes.c_int]),
             ("eio_nl_forward_wavefield", [vp, ctypes.c_int, ctypes.POINTER(Shot), ctypes.c_int, ctypes.c_int, vp]),
@@ -125,7 +126,7 @@ EXPORTED = ("eio_last_error", "eio_ctx_create", "eio_ctx_destroy", "eio_ctx_set_
             "eio_nl_adjoint_wavefield", "eio_model_set_abc", "eio_nl_forward_snapshots",
             "eio_nl_step_states", "eio_ctx_release_workspace", "eio_ctx_num_adjoint_time_block_variants",
             "eio_ctx_adjoint_time_block_info", "eio_ctx_set_adjoint_time_block",
-            "eio_ctx_set_shot_group_bytes", "eio_ctx_last_shot_groups")
+            "eio_ctx_set_shot_group_bytes", "eio_ctx_set_shot_group_streams", "eio_ctx_last_shot_groups")
 
 
 def check(rc):
@@ -254,6 +255,10 @@ class Context:
         """Working-set budget of the FP32 NL forward's shot groups: GROUP_AUTO (192 MiB, the
         default), 0 (one group) or a byte count; each group runs all its steps before the next."""
         check(lib().eio_ctx_set_shot_group_bytes(self.h, int(nbytes)))
+
+    def set_shot_group_streams(self, n):
+        """Streams the shot groups of a call run on (2, the default, or 1)."""
+        check(lib().eio_ctx_set_shot_group_streams(self.h, int(n)))
 
     def last_shot_groups(self):
         """Number of shot groups of the last NL forward call."""

@@ -212,6 +212,9 @@ struct Ctx {
     int step_limit = 0;  // > 0: NL loops stop after this many fine steps (tuning runs)
     long long group_bytes = EIO_GROUP_AUTO;  // shot-group working-set budget of the NL forward
     int last_groups = 0;                     // shot groups of the last NL forward call
+    int group_streams = 2;                   // streams the shot groups of a call run on (1 or 2)
+    hipStream_t side = nullptr;              // the library's second stream (default priority)
+    hipEvent_t fork = nullptr, join = nullptr;
     int persistent = 0;  // two-step NL forward in one persistent launch: 0 off, 1 on, 2 when it fits
     int spin_limit = 1 << 20;  // polls per neighbour wait of the persistent launch before it aborts
     std::map<int, FwiStore> fwi;
@@ -643,9 +646,8 @@ int variant3_for(Ctx* c, int nshot, const eio_shot* shots) {
 }
 
 // Whether a plain FP32 NL forward of these shots splits into more than one shot group: their
-// working set (`per_cell` bytes per cell) exceeds the context's group budget.
-bool nl_grouped(Ctx* c, int nshot, const eio_shot* shots, long long per_cell) {
-    const long long budget = c->group_bytes == EIO_GROUP_AUTO ? GROUP_AUTO_BYTES : c->group_bytes;
+// working set (`per_cell` bytes per cell) exceeds the budget of one group.
+bool nl_grouped(Ctx* c, int nshot, const eio_shot* shots, long long per_cell, long long budget) {
     if (budget <= 0) return false;
     long long total = 0;
     for (int i = 0; i < nshot; ++i) {
@@ -1411,8 +1413,14 @@ void nl_forward(Ctx* c, int nshot, const eio_shot* shots, int flags, int wshot =
     const bool plain = sizeof(T) == 4 && !wfld && !any_fs && !abc && nsnap == 0;
     const bool three = plain && c->time_block == 3;
     const bool two = plain && c->time_block == 2;
+    // plain runs: shot groups, each through all its steps before the next; without graph
+    // recording the groups run on two streams, each group with half the budget (the cache holds
+    // one group of each stream), so one stream's launch tails and gaps are filled by the other's
     const long long per_cell = (two || three ? 25 : 15) * (long long)sizeof(T);
-    const bool grouped = plain && nl_grouped(c, nshot, shots, per_cell);
+    const long long budget = c->group_bytes == EIO_GROUP_AUTO ? GROUP_AUTO_BYTES : c->group_bytes;
+    const int nstream = plain && budget > 0 && c->group_streams > 1 && !c->use_graph ? 2 : 1;
+    const long long gbudget = budget > 0 ? std::max(1LL, budget / nstream) : 0;
+    const bool grouped = plain && nl_grouped(c, nshot, shots, per_cell, gbudget);
     Run<T> run = prepare<T>(c, nshot, shots, Op::NL_FWD, flags, nullptr, nullptr, 0, two, false, false, three,
                             grouped);
     if (wfld) attach_wavefield(run, wshot, wfld, flags);
@@ -1451,16 +1459,46 @@ void nl_forward(Ctx* c, int nshot, const eio_shot* shots, int flags, int wshot =
             return;
         }
     }
-    // plain runs: shot groups, each through all its steps before the next
     const std::vector<ShotGroup> groups =
-        plain ? shot_groups(run, c->group_bytes, per_cell) : std::vector<ShotGroup>{{0, nshot}};
+        plain ? shot_groups(run, gbudget, per_cell) : std::vector<ShotGroup>{{0, nshot}};
     c->last_groups = int(groups.size());
+    // stream of each group: the one with less work so far (cell-updates), in plan order
+    std::vector<int> gstream(groups.size(), 0);
+    if (nstream > 1 && groups.size() > 1) {
+        long long work[2] = {0, 0};
+        for (size_t gi = 0; gi < groups.size(); ++gi) {
+            long long w = 0;
+            for (int i = groups[gi].s0; i < groups[gi].s1; ++i)
+                w += (long long)run.sd[i].nz * run.sd[i].nx * run.plan.nsteps[i];
+            gstream[gi] = work[1] < work[0] ? 1 : 0;
+            work[gstream[gi]] += w;
+        }
+    }
+    const bool side = nstream > 1 && groups.size() > 1;
     timing_begin<T>(c);
     EIO_CHECK_HIP(hipEventRecord(c->ev[2], c->stream));
     run_loop(c, [&]() {
         if constexpr (sizeof(T) == 4) {
             if (plain) {
-                for (const ShotGroup& g : groups) {
+                // the side stream starts after what the call queued so far; the call's stream
+                // waits for it at the end (also on errors, so no launch outlives the call)
+                struct SideStream {
+                    Ctx* c;
+                    hipStream_t main;
+                    bool on;
+                    ~SideStream() {
+                        c->stream = main;
+                        if (on && hipEventRecord(c->join, c->side) == hipSuccess)
+                            (void)hipStreamWaitEvent(main, c->join, 0);
+                    }
+                } ss{c, c->stream, side};
+                if (side) {
+                    EIO_CHECK_HIP(hipEventRecord(c->fork, ss.main));
+                    EIO_CHECK_HIP(hipStreamWaitEvent(c->side, c->fork, 0));
+                }
+                for (size_t gi = 0; gi < groups.size(); ++gi) {
+                    const ShotGroup& g = groups[gi];
+                    c->stream = gstream[gi] ? c->side : ss.main;
                     const eio::TileDev* tg = run.d_tiles + (g.s0 ? run.plan.tile_end[g.s0 - 1] : 0);
                     int kg = run.plan.nsteps[g.s0];
                     if (c->step_limit > 0) kg = std::min(kg, c->step_limit);
@@ -2083,6 +2121,9 @@ int eio_ctx_create(int device, int precision, eio_ctx** out) {
         EIO_CHECK_HIP(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
         c->stream = c->own;
         for (auto& e : c->ev) EIO_CHECK_HIP(hipEventCreate(&e));
+        EIO_CHECK_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+        EIO_CHECK_HIP(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
+        EIO_CHECK_HIP(hipEventCreateWithFlags(&c->join, hipEventDisableTiming));
         *out = reinterpret_cast<eio_ctx*>(c);
     });
 }
@@ -2098,6 +2139,10 @@ int eio_ctx_destroy(eio_ctx* ctx) {
         for (auto& kv : c->ws)
             if (kv.second.p) (void)hipFree(kv.second.p);
         for (auto& e : c->ev) (void)hipEventDestroy(e);
+        (void)hipStreamSynchronize(c->side);
+        (void)hipEventDestroy(c->fork);
+        (void)hipEventDestroy(c->join);
+        (void)hipStreamDestroy(c->side);
         (void)hipStreamDestroy(c->own);
         delete c;
     });
@@ -2163,6 +2208,13 @@ int eio_ctx_set_shot_group_bytes(eio_ctx* ctx, long long bytes) {
     return guarded([&]() {
         if (bytes < 0 && bytes != EIO_GROUP_AUTO) fail("shot-group budget must be >= 0 or EIO_GROUP_AUTO");
         as_ctx(ctx)->group_bytes = bytes;
+    });
+}
+
+int eio_ctx_set_shot_group_streams(eio_ctx* ctx, int streams) {
+    return guarded([&]() {
+        if (streams != 1 && streams != 2) fail("shot-group streams must be 1 or 2");
+        as_ctx(ctx)->group_streams = streams;
     });
 }
 

@@ -159,12 +159,20 @@ int eio_ctx_set_step_limit(eio_ctx* ctx, int max_steps);
  * groups whose per-step working set (the state levels a step touches and the media: 15 values
  * per cell with one fine step per launch, 25 with two or three) fits `bytes`, and each group
  * runs all its steps before the next one starts, so its working set stays in the MI355X's
- * 256 MB Infinity Cache between steps.  EIO_GROUP_AUTO (the default): 192 MiB; 0: one group
+ * 256 MB Infinity Cache between steps.  EIO_GROUP_AUTO (the default): 192 MiB (shared by the
+ * groups running at once on the call's streams, eio_ctx_set_shot_group_streams); 0: one group
  * (every shot advances each launch).  Results are those of the ungrouped call; the persistent
  * launch and runs with snapshots, a saved wavefield, a free surface or a learned boundary take
  * one group.  No counterpart in the reference (one shot per call, NLF:561-647). */
 #define EIO_GROUP_AUTO (-1LL)
 int eio_ctx_set_shot_group_bytes(eio_ctx* ctx, long long bytes);
+/* Streams the shot groups of a call run on: 2 (the default) runs the groups on the context's
+ * stream and a second stream of the library (default priority), each group with half the
+ * budget, so that one stream's launch tails and gaps are filled by the other's work (the
+ * cache holds one group of each); the call's stream waits for both before the call returns.
+ * 1: one stream.  Graph recording (eio_ctx_set_graph) always uses one.  No counterpart in the
+ * reference. */
+int eio_ctx_set_shot_group_streams(eio_ctx* ctx, int streams);
 /* Number of shot groups of the context's last NL forward call. */
 int eio_ctx_last_shot_groups(eio_ctx* ctx, int* groups);
 

@@ -1063,13 +1063,15 @@ def test_three_step_uneven_batch_matches_oracle():
             assert rel_l2(rec[k], ref[k]) < 1e-5, (k, rel_l2(rec[k], ref[k]))
 
 
-@pytest.mark.parametrize("tb,tiny", [(1, False), (2, False), (3, False), (2, True)])
-def test_shot_groups_equal_one_group_and_the_oracle(tb, tiny):
+@pytest.mark.parametrize("tb,tiny,streams", [(1, False, 2), (2, False, 2), (3, False, 2), (2, True, 2),
+                                             (2, False, 1)])
+def test_shot_groups_equal_one_group_and_the_oracle(tb, tiny, streams):
     """Shot groups (each group runs all its steps before the next starts) give the receiver
     traces of the one-group call bit for bit, and the oracle's, with one, two and three fine
     steps per launch, on an uneven batch that a small working-set budget splits into several
     groups (shots of different lengths inside one group finish in different launches); `tiny`:
-    a budget below every shot's working set, so each shot is a group of its own."""
+    a budget below every shot's working set, so each shot is a group of its own; `streams`: the
+    groups run on one stream or on two at once (each with half the budget)."""
     rng = np.random.default_rng(71)
     sizes = [(90, 130, 70, 1), (150, 90, 50, 2), (60, 200, 61, 1), (120, 100, 40, 2), (70, 70, 66, 1)]
     cases = []
@@ -1093,6 +1095,7 @@ def test_shot_groups_equal_one_group_and_the_oracle(tb, tiny):
         shots.append(shot)
         recs.append(rec)
         keep.append((kp, srcc))
+    ctx.set_shot_group_streams(streams)
     ctx.set_shot_group_bytes(0)
     ctx.nl_forward(shots)
     assert ctx.last_shot_groups() == 1
