@@ -530,7 +530,8 @@ def main():
     log("uploaded %d shots, %.3e cell-updates per step" % (len(shots), cells))
     tuned = None
     if args.autotune:
-        tuned = autotune.tune_nl_forward(ctx, shots, grids, log=log, group_bytes=args.group_bytes)
+        tuned = autotune.tune_nl_forward(ctx, shots, grids, log=log, group_bytes=args.group_bytes,
+                                         group_streams=args.group_streams)
         args.time_block, v = tuned["choice"]
         if args.time_block == 3:
             args.variant3 = v

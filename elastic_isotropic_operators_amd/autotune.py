@@ -37,11 +37,12 @@ def _library_hash():
     return _LIB_HASH[0]
 
 
-def signature(grids, group_bytes=_capi.GROUP_AUTO):
-    """Batch key: sorted (nz, nx, ntw-1, n_src, n_rec) of the shots, the shot-group budget,
-    the single-, two- and three-step variant tables and the library build."""
+def signature(grids, group_bytes=_capi.GROUP_AUTO, group_streams=2):
+    """Batch key: sorted (nz, nx, ntw-1, n_src, n_rec) of the shots, the shot-group budget and
+    streams, the single-, two- and three-step variant tables and the library build."""
     items = sorted(tuple(int(x) for x in g) for g in grids)
-    key = {"shots": items, "group_bytes": int(group_bytes), "variants": _capi.Context.variants(),
+    key = {"shots": items, "group_bytes": int(group_bytes), "group_streams": int(group_streams),
+           "variants": _capi.Context.variants(),
            "time_block_variants": _capi.Context.time_block_variants(),
            "time_block3_variants": _capi.Context.time_block3_variants(), "lib": _library_hash()}
     return hashlib.sha1(json.dumps(key).encode()).hexdigest()[:16]
@@ -64,13 +65,14 @@ def _apply(ctx, cand):
 
 
 def tune_nl_forward(ctx, shots, grids, steps=192, device_ptrs=True, use_cache=True, log=None,
-                    group_bytes=_capi.GROUP_AUTO):
+                    group_bytes=_capi.GROUP_AUTO, group_streams=2):
     """Time every candidate on `shots` (eio_shot list of the caller's batch; outputs are
     overwritten with partial data) for `steps` fine steps and set the fastest on `ctx`.
-    grids: per shot (nz, nx, ntw-1, n_src, n_rec) for the cache key; group_bytes: the
-    shot-group budget set on `ctx` (part of the key: it changes which form is fastest).
+    grids: per shot (nz, nx, ntw-1, n_src, n_rec) for the cache key; group_bytes,
+    group_streams: the shot-group settings of `ctx` (part of the key: they change which form is
+    fastest).
     Returns {"choice": (time_block, variant), "table": {cand: ms per fine step}, "cached": bool}."""
-    key = signature(grids, group_bytes)
+    key = signature(grids, group_bytes, group_streams)
     path = _cache_path()
     cache = {}
     if use_cache and os.path.exists(path):

@@ -244,5 +244,6 @@ def test_autotune_signature_and_candidates():
     assert a != autotune.signature([(300, 600, 7996, 4, 600)])
     # the shot-group budget changes which form is fastest: it is part of the key
     assert a != autotune.signature([(300, 600, 7996, 4, 600), (400, 800, 5000, 4, 800)], group_bytes=0)
+    assert a != autotune.signature([(300, 600, 7996, 4, 600), (400, 800, 5000, 4, 800)], group_streams=1)
     c = autotune.candidates()
     assert (1, 0) in c and (2, 0) in c and len(set(c)) == len(c)
