@@ -173,7 +173,7 @@ int eio_ctx_set_shot_group_bytes(eio_ctx* ctx, long long bytes);
  * 1: one stream.  Graph recording (eio_ctx_set_graph) always uses one.  No counterpart in the
  * reference. */
 int eio_ctx_set_shot_group_streams(eio_ctx* ctx, int streams);
-/* Number of shot groups of the context's last NL forward call. */
+/* Number of shot groups of the context's last NL forward call.  No counterpart in the reference. */
 int eio_ctx_last_shot_groups(eio_ctx* ctx, int* groups);
 
 /* The grid checks of eio_model_add, on the host only (no device, no context): sizes, sampling,
