@@ -721,7 +721,14 @@ void k_step2_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
     constexpr int TX1 = TX + 8;        // step-1 columns
     constexpr int TXC = TX + 16;       // staged columns
     constexpr unsigned SZ = sizeof(T);
-    __shared__ __attribute__((aligned(16))) T lc[5][TXC][ZC];  // M u_cur, then M u1 on the step-1 region
+    // M u_cur, then M u1 on the step-1 region.  Fields 0, 1, 4 (lc[0], lc[1], lc[2]) take x
+    // and z derivatives and keep the whole stage; field 2 takes x derivatives only, so lcx
+    // keeps the 64 rows of the step-1 region (lcx[c][r] = stage row r+4); field 3 takes z
+    // derivatives only, so lcz keeps the TX+8 columns of the step-1 region (lcz[c][r] = stage
+    // column c+4): 3 KB less LDS (51,968 B per 56 x 8 workgroup).
+    __shared__ __attribute__((aligned(16))) T lc[3][TXC][ZC];
+    __shared__ __attribute__((aligned(16))) T lcx[TXC][Z1];
+    __shared__ __attribute__((aligned(16))) T lcz[TX1][ZC];
     __shared__ T l1[5][TX1][Z1];       // u1, then u2 for the owned data injection
 
     const TileDev t = tiles[xcd_remap(blockIdx.x, ntiles)];
@@ -843,8 +850,11 @@ void k_step2_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
                     p[3][k2] = hm[it][3][k2] * u2 + hm[it][2][k2] * u3;
                     p[4][k2] = hm[it][4][k2] * u4;
                 }
-#pragma unroll
-                for (int f = 0; f < 5; ++f) *reinterpret_cast<f32x4_t*>(&lc[f][c][4 * q]) = p[f];
+                *reinterpret_cast<f32x4_t*>(&lc[0][c][4 * q]) = p[0];
+                *reinterpret_cast<f32x4_t*>(&lc[1][c][4 * q]) = p[1];
+                *reinterpret_cast<f32x4_t*>(&lc[2][c][4 * q]) = p[4];
+                if (q >= 1 && q <= Z1 / 4) *reinterpret_cast<f32x4_t*>(&lcx[c][4 * q - FAT]) = p[2];
+                if (c >= FAT && c < FAT + TX1) *reinterpret_cast<f32x4_t*>(&lcz[c - FAT][4 * q]) = p[3];
             }
         }
     }
@@ -862,11 +872,11 @@ void k_step2_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
         {
             T q[R1 + 8];
 #pragma unroll
-            for (int j = 0; j < R1 + 8; ++j) q[j] = lc[2][j1 + j][lane + FAT];
+            for (int j = 0; j < R1 + 8; ++j) q[j] = lcx[j1 + j][lane];
 #pragma unroll
             for (int r = 0; r < R1; ++r) dx_a[r] = dm_q(q, r + 4, cx);
 #pragma unroll
-            for (int j = 0; j < R1 + 8; ++j) q[j] = lc[4][j1 + j][lane + FAT];
+            for (int j = 0; j < R1 + 8; ++j) q[j] = lc[2][j1 + j][lane + FAT];
 #pragma unroll
             for (int r = 0; r < R1; ++r) dx_s6[r] = dp_q(q, r + 4, cx);
 #pragma unroll
@@ -883,8 +893,8 @@ void k_step2_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
             const int ix = ix1 + r;
             const bool ok = zin && ix >= FAT && ix < nx - FAT;
             const int c = j1 + r + FAT;
-            const T dz_s6 = dp_q(&lc[4][c][lane + FAT], 0, cz);
-            const T dz_b = dm_q(&lc[3][c][lane + FAT], 0, cz);
+            const T dz_s6 = dp_q(&lc[2][c][lane + FAT], 0, cz);
+            const T dz_b = dm_q(&lcz[c - FAT][lane + FAT], 0, cz);
             const T dz_s1 = dp_q(&lc[1][c][lane + FAT], 0, cz);
             const T dz_s0 = dm_q(&lc[0][c][lane + FAT], 0, cz);
             const T d = ok ? damp_at(S, iz, ix) : T(1);
@@ -941,9 +951,9 @@ void k_step2_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
         for (int f = 0; f < 5; ++f) u[f] = l1[f][j][lane];
         lc[0][j + FAT][lane + FAT] = m1[0][r] * u[0];
         lc[1][j + FAT][lane + FAT] = m1[1][r] * u[1];
-        lc[2][j + FAT][lane + FAT] = m1[2][r] * u[2] + m1[3][r] * u[3];
-        lc[3][j + FAT][lane + FAT] = m1[3][r] * u[2] + m1[2][r] * u[3];
-        lc[4][j + FAT][lane + FAT] = m1[4][r] * u[4];
+        lcx[j + FAT][lane] = m1[2][r] * u[2] + m1[3][r] * u[3];
+        lcz[j][lane + FAT] = m1[3][r] * u[2] + m1[2][r] * u[3];
+        lc[2][j + FAT][lane + FAT] = m1[4][r] * u[4];
         if (zown && j >= FAT && j < FAT + TX && ix < nx - FAT) {
             const unsigned vo = (unsigned(ix) * unsigned(nz) + unsigned(iz)) * SZ;
 #pragma unroll
@@ -968,11 +978,11 @@ void k_step2_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
         {
             T q[R2 + 8];
 #pragma unroll
-            for (int j = 0; j < R2 + 8; ++j) q[j] = lc[2][j2 + FAT + j][lz2 + FAT];
+            for (int j = 0; j < R2 + 8; ++j) q[j] = lcx[j2 + FAT + j][lz2];
 #pragma unroll
             for (int r = 0; r < R2; ++r) dx_a[r] = dm_q(q, r + 4, cx);
 #pragma unroll
-            for (int j = 0; j < R2 + 8; ++j) q[j] = lc[4][j2 + FAT + j][lz2 + FAT];
+            for (int j = 0; j < R2 + 8; ++j) q[j] = lc[2][j2 + FAT + j][lz2 + FAT];
 #pragma unroll
             for (int r = 0; r < R2; ++r) dx_s6[r] = dp_q(q, r + 4, cx);
 #pragma unroll
@@ -989,8 +999,8 @@ void k_step2_adj(const ShotDev<T>* __restrict__ shots, const TileDev* __restrict
             const int ix = ix2 + r;
             const bool ok = zown && ix < nx - FAT;
             const int c = j2 + r + 2 * FAT;
-            const T dz_s6 = dp_q(&lc[4][c][lz2 + FAT], 0, cz);
-            const T dz_b = dm_q(&lc[3][c][lz2 + FAT], 0, cz);
+            const T dz_s6 = dp_q(&lc[2][c][lz2 + FAT], 0, cz);
+            const T dz_b = dm_q(&lcz[c - FAT][lz2 + FAT], 0, cz);
             const T dz_s1 = dp_q(&lc[1][c][lz2 + FAT], 0, cz);
             const T dz_s0 = dm_q(&lc[0][c][lz2 + FAT], 0, cz);
             const T d = ok ? damp_at(S, iz, ix) : T(1);

@@ -155,7 +155,7 @@ def test_two_step_forward_with_whole_record_matches_oracle(variant2):
 
 @pytest.mark.parametrize("va,sub,segment", [(0, 1, 0), (1, 2, 59), (2, 1, 59), (3, 2, 0), (1, 1, 0), (4, 1, 0),
                                              (4, 2, 59), (5, 2, 0), (6, 1, 59), (7, 2, 59),
-                                             (8, 1, 0), (9, 2, 59)])
+                                             (8, 1, 0), (9, 2, 59), (10, 2, 59), (11, 1, 0), (10, 1, 0)])
 def test_two_step_adjoint_gradient_matches_oracle(va, sub, segment):
     """FP32 FWI gradient with two reverse fine steps per launch of the receiver pass, over the
     checkpointed store (segment 0 -> 8: recomputed segments, one-step tiles) and the whole
