@@ -103,6 +103,8 @@ def lib():
             ("eio_ctx_set_shot_group_bytes", [vp, ctypes.c_longlong]),
             ("eio_ctx_set_shot_group_streams", [vp, ctypes.c_int]),
             ("eio_ctx_last_shot_groups", [vp, ctypes.POINTER(ctypes.c_int)]),
+            ("eio_host_alloc", [ctypes.c_ulonglong, ctypes.POINTER(vp)]),
+            ("eio_host_free", [vp]),
             ("eio_ctx_set_persistent", [vp, ctypes.c_int, ctypes.c_int]),
             ("eio_nl_forward_wavefield", [vp, ctypes.c_int, ctypes.POINTER(Shot), ctypes.c_int, ctypes.c_int, vp]),
             ("eio_nl_adjoint_wavefield", [vp, ctypes.c_int, ctypes.POINTER(Shot), ctypes.c_int, ctypes.c_int, vp]),
@@ -126,7 +128,8 @@ EXPORTED = ("eio_last_error", "eio_ctx_create", "eio_ctx_destroy", "eio_ctx_set_
             "eio_nl_adjoint_wavefield", "eio_model_set_abc", "eio_nl_forward_snapshots",
             "eio_nl_step_states", "eio_ctx_release_workspace", "eio_ctx_num_adjoint_time_block_variants",
             "eio_ctx_adjoint_time_block_info", "eio_ctx_set_adjoint_time_block",
-            "eio_ctx_set_shot_group_bytes", "eio_ctx_set_shot_group_streams", "eio_ctx_last_shot_groups")
+            "eio_ctx_set_shot_group_bytes", "eio_ctx_set_shot_group_streams", "eio_ctx_last_shot_groups",
+            "eio_host_alloc", "eio_host_free")
 
 
 def check(rc):

@@ -2532,6 +2532,21 @@ int eio_device_free(eio_ctx* ctx, void* ptr) {
     });
 }
 
+int eio_host_alloc(unsigned long long bytes, void** ptr) {
+    return guarded([&]() {
+        if (!ptr) fail("ptr is NULL");
+        *ptr = nullptr;
+        if (bytes == 0) return;
+        EIO_CHECK_HIP(hipHostMalloc(ptr, bytes, hipHostMallocPortable));
+    });
+}
+
+int eio_host_free(void* ptr) {
+    return guarded([&]() {
+        if (ptr) EIO_CHECK_HIP(hipHostFree(ptr));
+    });
+}
+
 int eio_memcpy(eio_ctx* ctx, void* dst, const void* src, unsigned long long bytes, int kind) {
     return guarded([&]() {
         Ctx* c = as_ctx(ctx);

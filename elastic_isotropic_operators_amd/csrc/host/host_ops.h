@@ -75,6 +75,29 @@ inline void check_eio(int rc) {
     if (rc != EIO_OK) throw std::runtime_error(std::string("eio: ") + eio_last_error());
 }
 
+// Page-locked allocator (eio_host_alloc) for the per-call trace buffers the library copies to
+// or from the device: the copies run as direct DMA, not through a pageable staging pass.
+template <typename T>
+struct PinnedAlloc {
+    using value_type = T;
+    PinnedAlloc() = default;
+    template <typename U>
+    PinnedAlloc(const PinnedAlloc<U>&) {}
+    T* allocate(size_t n) {
+        void* p = nullptr;
+        check_eio(eio_host_alloc((unsigned long long)n * sizeof(T), &p));
+        if (!p && n) throw std::bad_alloc();
+        return static_cast<T*>(p);
+    }
+    void deallocate(T* p, size_t) { (void)eio_host_free(p); }
+    template <typename U>
+    bool operator==(const PinnedAlloc<U>&) const { return true; }
+    template <typename U>
+    bool operator!=(const PinnedAlloc<U>&) const { return false; }
+};
+template <typename T>
+using PinnedVec = std::vector<T, PinnedAlloc<T>>;
+
 // ---------------------------------------------------------------- staggering (stagger.cpp)
 template <typename T>
 void stagger_x_adj(int nz, int nx, const T* d, T* m) {  // stagger.cpp:38-57
@@ -543,20 +566,21 @@ class NonlinearShots {
         if (nModelShots != 1 && nModelShots != nExp_) throw std::runtime_error("model shot axis must be 1 or nExp");
         std::vector<eio_shot> shots = shot_table();
         for (int e = 0; e < nExp_; ++e) check_counts(src_[e], nSrc, rec_of(e), nRec);
-        std::vector<std::vector<T>>& srcReg = srcReg_;
-        std::vector<std::vector<T>>& recReg = recReg_;
+        std::vector<PinnedVec<T>>& srcReg = srcReg_;
+        std::vector<PinnedVec<T>>& recReg = recReg_;
         // per (shot, component): disjoint buffers, so the interpolation runs on all host cores
 #pragma omp parallel for collapse(2) schedule(dynamic)
         for (int e = 0; e < nExp_; ++e)
             for (int k = 0; k < 5; ++k) {
                 const T* m = model + size_t(nModelShots == 1 ? 0 : e) * 5 * nSrc * nts;
                 const SpaceInterpT<T>& ip = *src_[e][GRID_OF_COMP[k]];
-                std::vector<T>& r = srcReg[size_t(e) * 5 + k];
+                PinnedVec<T>& r = srcReg[size_t(e) * 5 + k];
                 r.resize(size_t(ip.nReg()) * nts);
                 ip.adjoint(r.data(), m + size_t(k) * nSrc * nts);
                 scale_sources(*fd_, ip, k, r.data());
-                std::vector<T>& d = recReg[size_t(e) * 5 + k];
-                d.assign(size_t(rec_of(e)[GRID_OF_COMP[k]]->nReg()) * nts, T(0));
+                // output: the library writes every regular receiver trace of the call
+                PinnedVec<T>& d = recReg[size_t(e) * 5 + k];
+                d.resize(size_t(rec_of(e)[GRID_OF_COMP[k]]->nReg()) * nts);
                 shots[e].src_tr[k] = r.data();
                 shots[e].rec_tr[k] = d.data();
             }
@@ -579,17 +603,18 @@ class NonlinearShots {
         if (nModelShots != 1 && nModelShots != nExp_) throw std::runtime_error("model shot axis must be 1 or nExp");
         std::vector<eio_shot> shots = shot_table();
         for (int e = 0; e < nExp_; ++e) check_counts(src_[e], nSrc, rec_of(e), nRec);
-        std::vector<std::vector<T>>& srcReg = srcReg_;
-        std::vector<std::vector<T>>& recReg = recReg_;
+        std::vector<PinnedVec<T>>& srcReg = srcReg_;
+        std::vector<PinnedVec<T>>& recReg = recReg_;
 #pragma omp parallel for collapse(2) schedule(dynamic)
         for (int e = 0; e < nExp_; ++e)
             for (int k = 0; k < 5; ++k) {
                 const SpaceInterpT<T>& rp = *rec_of(e)[GRID_OF_COMP[k]];
-                std::vector<T>& d = recReg[size_t(e) * 5 + k];
+                PinnedVec<T>& d = recReg[size_t(e) * 5 + k];
                 d.resize(size_t(rp.nReg()) * nts);
                 rp.adjoint(d.data(), data + (size_t(e) * 5 + k) * nRec * nts);
-                std::vector<T>& r = srcReg[size_t(e) * 5 + k];
-                r.assign(size_t(src_[e][GRID_OF_COMP[k]]->nReg()) * nts, T(0));
+                // output: the library writes every regular source trace of the call
+                PinnedVec<T>& r = srcReg[size_t(e) * 5 + k];
+                r.resize(size_t(src_[e][GRID_OF_COMP[k]]->nReg()) * nts);
                 shots[e].src_tr[k] = r.data();
                 shots[e].rec_tr[k] = d.data();
             }
@@ -624,7 +649,7 @@ class NonlinearShots {
                 for (int k = 0; k < 5; ++k) {
                     T* m = model + size_t(nModelShots == 1 ? 0 : e) * 5 * nSrc * nts;
                     const SpaceInterpT<T>& ip = *src_[e][GRID_OF_COMP[k]];
-                    std::vector<T>& r = srcReg[size_t(e) * 5 + k];
+                    PinnedVec<T>& r = srcReg[size_t(e) * 5 + k];
                     scale_sources(*fd_, ip, k, r.data());  // same diagonal scaling (nonlinearPropElasticGpu.cpp:406-430)
                     ip.forward_add(r.data(), m + size_t(k) * nSrc * nts);
                 }
@@ -691,7 +716,7 @@ class NonlinearShots {
     std::unique_ptr<FdParam<T>> fd_;
     int model_ = -1;
     std::vector<T> wavefield_;
-    std::vector<std::vector<T>> srcReg_, recReg_;  // regular traces per (shot, component)
+    std::vector<PinnedVec<T>> srcReg_, recReg_;  // regular traces per (shot, component), page-locked
 };
 
 // ---------------------------------------------------------------- Born shot operator
@@ -828,18 +853,15 @@ class BornShots {
                 std::vector<T> reg(size_t(ip.nReg()) * nts);
                 ip.adjoint(reg.data(), sig.data() + size_t(k) * nSrc * nts);
                 scale_sources(*fd_, ip, k, reg.data());
-                std::vector<T>& f = fine_[size_t(e) * 5 + k];
+                PinnedVec<T>& f = fine_[size_t(e) * 5 + k];
                 f.resize(size_t(ip.nReg()) * ntw);
                 time_interp_fwd(nts, fd_->dts, fd_->ots, fd_->sub, ip.nReg(), reg.data(), f.data());
                 shots[e].src_tr[k] = f.data();
                 const SpaceInterpT<T>& rp = *rec_of(e)[GRID_OF_COMP[k]];
-                std::vector<T>& d = recReg_[size_t(e) * 5 + k];
-                if (data) {
-                    d.resize(size_t(rp.nReg()) * nts);
-                    rp.adjoint(d.data(), data + (size_t(e) * 5 + k) * nRec * nts);
-                } else {
-                    d.assign(size_t(rp.nReg()) * nts, T(0));
-                }
+                PinnedVec<T>& d = recReg_[size_t(e) * 5 + k];
+                d.resize(size_t(rp.nReg()) * nts);
+                // forward: an output the library writes in full
+                if (data) rp.adjoint(d.data(), data + (size_t(e) * 5 + k) * nRec * nts);
                 shots[e].rec_tr[k] = d.data();
             }
         return shots;
@@ -864,7 +886,8 @@ class BornShots {
     std::unique_ptr<Device> dev_;
     std::unique_ptr<FdParam<T>> fd_;
     int model_ = -1;
-    std::vector<std::vector<T>> fine_, recReg_;  // source (fine) and receiver traces per (shot, component)
+    // source (fine) and receiver traces per (shot, component), page-locked
+    std::vector<PinnedVec<T>> fine_, recReg_;
 };
 
 }  // namespace eioh

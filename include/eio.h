@@ -317,6 +317,13 @@ int eio_device_free(eio_ctx* ctx, void* ptr);
 int eio_memcpy(eio_ctx* ctx, void* dst, const void* src, unsigned long long bytes, int kind);
 /* kind: 1 = host->device, 2 = device->host, 3 = device->device */
 int eio_synchronize(eio_ctx* ctx);
+/* Page-locked host memory (bytes; *ptr = NULL for 0 bytes) for buffers that calls with host
+ * pointers copy to or from the device every time, such as regular-grid traces: the copies
+ * are direct DMA instead of a pass through the runtime's pageable staging buffer.  Usable
+ * with every context; free with eio_host_free (which waits for the device).  No
+ * counterpart in the reference, whose host trace buffers are pageable. */
+int eio_host_alloc(unsigned long long bytes, void** ptr);
+int eio_host_free(void* ptr);
 
 #ifdef __cplusplus
 }

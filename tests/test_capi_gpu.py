@@ -1122,3 +1122,35 @@ def test_shot_groups_equal_one_group_and_the_oracle(tb, tiny, streams):
                 assert np.array_equal(rec[k], r1[k]), k
     ctx.set_shot_group_bytes(_capi.GROUP_AUTO)
     ctx.close()
+
+
+def test_pinned_host_buffers_round_trip():
+    """eio_host_alloc: page-locked host memory (the operator modules' per-call trace buffers).
+    A host -> device -> host copy through two such buffers returns the same bytes; 0 bytes
+    gives NULL and freeing NULL is allowed."""
+    import ctypes
+    L = _capi.lib()
+    p = ctypes.c_void_p(1)
+    _capi.check(L.eio_host_alloc(0, ctypes.byref(p)))
+    assert p.value is None
+    _capi.check(L.eio_host_free(None))
+    ctx = _capi.Context(0, np.float32)
+    n = 1 << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    _capi.check(L.eio_host_alloc(4 * n, ctypes.byref(a)))
+    _capi.check(L.eio_host_alloc(4 * n, ctypes.byref(b)))
+    d = ctx.alloc(4 * n)
+    try:
+        x = np.ctypeslib.as_array((ctypes.c_float * n).from_address(a.value))
+        y = np.ctypeslib.as_array((ctypes.c_float * n).from_address(b.value))
+        x[:] = np.random.default_rng(5).standard_normal(n).astype(np.float32)
+        y[:] = 0
+        _capi.check(L.eio_memcpy(ctx.h, ctypes.c_void_p(d), a, 4 * n, 1))
+        _capi.check(L.eio_memcpy(ctx.h, b, ctypes.c_void_p(d), 4 * n, 2))
+        _capi.check(L.eio_synchronize(ctx.h))
+        assert np.array_equal(x, y)
+    finally:
+        ctx.free(d)
+        _capi.check(L.eio_host_free(a))
+        _capi.check(L.eio_host_free(b))
+        ctx.close()
